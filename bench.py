@@ -1,0 +1,192 @@
+"""Benchmark of the D3Q27 cumulant hot path on one MI355X.
+
+Workload (the plain form on the GPU): uniform 256^3 box, FP32 populations, equilibrium velocity
+inlet ux = 0.05 at x = 0, zero-gradient outlet, free-slip sides, sphere of diameter 32 cells centred at
+(96, 128, 128) with halfway bounce-back, Re = u D / nu = 200 (nu = 0.008, tau = 0.524).  WALE and
+inlet noise off (the plain parity form); cumulant defaults of the reference otherwise.  A step is one
+fused stream-collide launch over the whole box; inputs are resident in HBM.
+
+The same box with FP16 shifted storage (FP32 arithmetic) is timed as the precision form and reported
+beside it.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+N = 256
+D = 32
+U_IN = 0.05
+RE = 200.0
+
+
+def workload_grid():
+    from openaerolab_alpha_01_amd.geometry import sphere_mask, uniform_grid
+    g = uniform_grid((N, N, N))
+    g.solid = sphere_mask(g, (96.0, 128.0, 128.0), D / 2)
+    return g
+
+
+def tau_of():
+    nu = U_IN * D / RE
+    return 0.5 + 3.0 * nu
+
+
+def bytes_per_lup(storage: str) -> int:
+    return 2 * 27 * {"f32": 4, "f64": 8, "f16s": 2, "bf16s": 2}[storage]
+
+
+def gpu_leg(grid, storage: str, stream: str, steps: int, warmup: int):
+    from openaerolab_alpha_01_amd import solver
+    params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
+    plan = solver.uniform_plan(grid, tau_of(), storage=storage, stream=stream, params=params, u_inlet=U_IN)
+    plan.init_equilibrium(None, None, 1.0, (U_IN, 0.0, 0.0))
+    if warmup:
+        plan.time_steps(warmup)
+    plan.sync()
+    t0 = time.perf_counter()
+    ms_total, ms_kernel = plan.time_steps(steps)
+    plan.sync()
+    wall = time.perf_counter() - t0
+    plan.level_step(0, steps + warmup, force=True)
+    F = plan.force()[0]
+    plan.guard(0.2, 0.5)
+    plan.close()
+    return ms_total, ms_kernel, wall, F
+
+
+def copy_bandwidth_gbs() -> float | None:
+    """Device-to-device copy of 4 GiB (read + write), the measured HBM ceiling of this box."""
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            return None
+        n = 1 << 30
+        a = torch.empty(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        reps = 10
+        for _ in range(reps):
+            b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        del a, b
+        torch.cuda.empty_cache()
+        return 2 * 4 * n / (ms * 1e-3) / 1e9
+    except Exception:
+        return None
+
+
+def cpu_baseline(grid, budget_s: float = 20.0):
+    """The CPU oracle (this project's C++ restatement of the reference arithmetic, OpenMP) on the same
+    workload, FP32: as many whole 256^3 steps as fit in the budget (at least one)."""
+    from oracle import oracle as O
+    G = O.Grid(grid)
+    n = grid.n_cells
+    rho = np.ones(n, dtype=np.float32)
+    u = np.zeros((3, n), dtype=np.float32)
+    u[0] = U_IN
+    f = O.init_eq("f32", G, rho, u)
+    g = f.copy()
+    del rho, u
+    p = O.make_params(tau_of(), u_inlet=U_IN)
+    done, t_used = 0, 0.0
+    while done == 0 or (t_used < budget_s and done < 20):
+        t0 = time.perf_counter()
+        O.step("f32", G, p, f, out=g)
+        t_used += time.perf_counter() - t0
+        f, g = g, f
+        done += 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    return n * done / t_used / 1e6, threads, done, t_used
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--stream", default="ab", choices=["ab", "aa"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp16", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU halo exchange is a design, not an implementation)", file=sys.stderr)
+        return 2
+    if args.steps % 2 or args.warmup % 2:
+        args.steps += args.steps % 2
+        args.warmup += args.warmup % 2
+
+    grid = workload_grid()
+    n_cells = N ** 3
+    n_fluid = int(n_cells - grid.solid.sum())
+    ms_total, ms_kernel, wall, F = gpu_leg(grid, "f32", args.stream, args.steps, args.warmup)
+    mlups = n_cells * args.steps / (ms_total * 1e-3) / 1e6
+    achieved = bytes_per_lup("f32") * n_fluid / (ms_kernel * 1e-3) / 1e9
+    area = np.pi * (D / 2) ** 2
+    cd = float(F[0] / (0.5 * U_IN ** 2 * area))
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_step_f32_256.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": "MLUPS (D3Q27 cumulant, FP32 populations, 256^3 sphere box)",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_total / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "256^3 box, inlet ux=0.05, zero-gradient outlet, free-slip sides, sphere D=32 at "
+                               "(96,128,128), halfway bounce-back, Re=200 (tau=0.524), cumulant, WALE off",
+                   "cells": n_cells, "fluid_cells": n_fluid, "stream": args.stream, "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_lup": bytes_per_lup("f32"), "kernel_ms": round(ms_kernel, 5)},
+        "cd_transient": round(cd, 5),
+    }
+    if not args.no_fp16:
+        ms_t16, ms_k16, _, F16 = gpu_leg(grid, "f16s", args.stream, args.steps, args.warmup)
+        m16 = n_cells * args.steps / (ms_t16 * 1e-3) / 1e6
+        a16 = bytes_per_lup("f16s") * n_fluid / (ms_k16 * 1e-3) / 1e9
+        out["fp16_shifted"] = {"value": round(m16, 1), "unit": "MLUPS", "ms_per_step": round(ms_t16 / args.steps, 4),
+                               "speedup_vs_f32": round(m16 / mlups, 3),
+                               "roofline": {"achieved": round(a16, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                            "frac": round(a16 / HBM_PEAK_GBS, 4),
+                                            "bytes_per_lup": bytes_per_lup("f16s")}}
+    bw = copy_bandwidth_gbs()
+    if bw is not None:
+        out["roofline"]["measured_copy_gbs"] = round(bw, 1)
+    if not args.no_cpu_baseline:
+        v, cores, done, secs = cpu_baseline(grid)
+        out["cpu_baseline"] = {"value": round(v, 2), "unit": "MLUPS", "cores": cores, "kind": "port",
+                               "sample": f"{done} FP32 steps of the same 256^3 workload in {secs:.1f} s "
+                                         "(CPU oracle: C++/OpenMP restatement of the reference arithmetic)"}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

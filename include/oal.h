@@ -1,0 +1,192 @@
+/*
+ * oal.h — C ABI of liboal, the MI355X (gfx950) D3Q27 cumulant lattice-Boltzmann hot path.
+ *
+ * Plain C: pointers, sizes and status codes only; no exceptions cross the boundary.  Every entry
+ * point returns an int status: 0 (OAL_OK) on success, a negative OAL_ERR_* otherwise;
+ * oal_strerror() names it and oal_last_error() gives the message of the last failure on the
+ * calling thread.
+ *
+ * The library owns all device memory of a plan.  Host arrays passed in use the reference's
+ * BlockLevel conventions (src/geometry/blocks.jl:16-200): populations f[x,y,z,b,k] column-major
+ * (x fastest, then y, z, block, direction), 1-based block coordinates map_x/y/z, and a
+ * column-major n_blocks x 27 neighbour table of 1-based block indices with 0 = none
+ * (src/geometry/domain_topology.jl:919-955).  Direction k follows the reference order
+ * k = (cx+1) + 3 (cy+1) + 9 (cz+1) (src/collision/lattice.jl:10-59).
+ *
+ * Streams: every call that launches work takes a hipStream_t (passed as void*, NULL = the plan's
+ * own stream) and returns without synchronising; oal_sync() waits for the plan's work.
+ */
+#ifndef OAL_H
+#define OAL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OAL_ABI_VERSION 1
+
+enum {
+    OAL_OK = 0,
+    OAL_ERR_ARG = -1,          /* invalid argument or descriptor */
+    OAL_ERR_HIP = -2,          /* a HIP runtime call failed */
+    OAL_ERR_NOMEM = -3,        /* device allocation failed */
+    OAL_ERR_UNSUPPORTED = -4,  /* a feature outside this library's scope was requested */
+    OAL_ERR_STATE = -5,        /* call out of order (e.g. no populations uploaded) */
+    OAL_ERR_NONFINITE = -6     /* the stability guard found a bad cell */
+};
+
+/* population storage of a plan */
+enum {
+    OAL_F32 = 0,     /* FP32 populations, unshifted (the reference's storage) */
+    OAL_F64 = 1,     /* FP64 populations and arithmetic (validation runs only) */
+    OAL_F16S = 2,    /* FP16 storage of f - w_k, FP32 arithmetic */
+    OAL_BF16S = 3    /* BF16 storage of f - w_k, FP32 arithmetic */
+};
+
+/* streaming pattern */
+enum {
+    OAL_STREAM_AB = 0,   /* pull from buffer A into buffer B, then swap (the reference's scheme) */
+    OAL_STREAM_AA = 1    /* in-place AA pattern, one buffer, even/odd kernels */
+};
+
+/* treatment of obstacle cells */
+enum {
+    OAL_BB_REFERENCE = 0,  /* solid cells reflect their gathered populations (kernel_cumulant.jl:405-417) */
+    OAL_BB_HALFWAY = 1     /* halfway bounce-back at the fluid cell, wall half a cell out */
+};
+
+/* limiter_type of the reference kernel (kernel_cumulant.jl:725-735, 849-884) */
+enum { OAL_LIMITER_NONE = 0, OAL_LIMITER_FACTORED = 1, OAL_LIMITER_POSITIVITY = 2 };
+
+/* Collision parameters shared by every level (the cumulant arguments of stream_collide_cumulant!,
+ * src/collision/kernel_cumulant.jl:42-76; defaults of src/io/config_loader.jl:82-88, 154-159). */
+typedef struct {
+    float omega_bulk;          /* CUMULANT_OMEGA_BULK (1.0) */
+    float omega_3;             /* CUMULANT_OMEGA_3 (1.0) */
+    float omega_4;             /* CUMULANT_OMEGA_4 (1.0) */
+    float lambda;              /* CUMULANT_LAMBDA_PARAM (1/6) */
+    int32_t adaptive_omega4;   /* CUMULANT_ADAPTIVE_OMEGA4 (1) */
+    int32_t limiter;           /* OAL_LIMITER_* (factored) */
+    int32_t compressible;      /* COMPRESSIBILITY_CORRECTION (0): O(u^4) equilibria at inlet/limiter */
+    float c_wale;              /* WALE constant; 0 disables the subgrid model */
+    float nu_sgs_background;   /* NU_SGS_BACKGROUND (0) */
+    float inlet_turbulence;    /* INLET_TURBULENCE_INTENSITY; 0 = steady inlet */
+} OalCollision;
+
+/* One domain of a level: an extent in cells with its boundary conditions.  A level holds one
+ * domain (uniform box, one octree level) or many (independent boxes advanced together). */
+typedef struct {
+    int32_t extent[3];         /* nx_global, ny_global, nz_global at this level */
+    int32_t periodic[3];       /* nonzero: the axis wraps through the neighbour table */
+    float tau;                 /* tau_molecular of this level */
+    float u_inlet;             /* equilibrium inlet velocity along +x at x = 0 */
+    float body_force[3];       /* uniform force density, lattice units of this level (Guo term) */
+    float amr_sgs_scale;       /* _amr_sgs_scale (1 on a single level) */
+    int32_t ymin_wall_bc;      /* 1: no-slip bounce-back at y_min (GROUND_ANALYSIS) */
+    int32_t zwall_bc;          /* 0: legacy mirror, 2: no-slip bounce-back; 1 is not supported */
+    float force_scale;         /* factor on this domain's momentum-exchange sum (8^(1-l) on octree levels) */
+    int32_t reserved;
+} OalDomainDesc;
+
+/* One refinement level (a BlockLevel).  Blocks of one domain must be contiguous in the list. */
+typedef struct {
+    int32_t n_blocks;
+    const int32_t* map_x;            /* [n_blocks] 1-based block coordinates (active_coords_x) */
+    const int32_t* map_y;
+    const int32_t* map_z;
+    const int32_t* neighbor_table;   /* [n_blocks * 27] column-major, 1-based, 0 = none */
+    const int32_t* block_domain;     /* [n_blocks] 0-based domain index, NULL = all domain 0 */
+    int32_t n_domains;
+    const OalDomainDesc* domains;    /* [n_domains] */
+    const uint8_t* obstacle;         /* [bs^3 * n_blocks] reference layout, nonzero = solid; NULL = none */
+} OalLevelDesc;
+
+typedef struct {
+    int32_t abi_version;       /* OAL_ABI_VERSION */
+    int32_t storage;           /* OAL_F32 / OAL_F64 / OAL_F16S / OAL_BF16S */
+    int32_t block_size;        /* cells per block edge: 16 */
+    int32_t stream_mode;       /* OAL_STREAM_AB or OAL_STREAM_AA */
+    int32_t bb_mode;           /* OAL_BB_REFERENCE or OAL_BB_HALFWAY */
+    int32_t n_levels;          /* 1 for a uniform grid */
+    const OalLevelDesc* levels;
+    OalCollision collision;
+    int32_t device;            /* HIP device ordinal */
+    int32_t reserved;
+} OalPlanDesc;
+
+typedef struct OalPlan OalPlan;
+
+/* Replaces: nothing (ABI handshake). */
+int oal_abi_version(void);
+const char* oal_strerror(int status);
+const char* oal_last_error(void);
+
+/* Builds the device state of a solver run: the BlockLevel structs and their upload
+ * (src/geometry/blocks.jl:250-507, Adapt at :203) and the lattice arrays of
+ * build_lattice_arrays_gpu_d3q27 (src/collision/timestep.jl:2005-2023). */
+int oal_plan_create(const OalPlanDesc* desc, OalPlan** plan);
+int oal_plan_destroy(OalPlan* plan);
+
+/* Number of cells (n_blocks * bs^3) of a level: the length of one direction of f. */
+int oal_level_cells(const OalPlan* plan, int level, int64_t* n_cells);
+
+/* Populations in/out in the reference layout f[x,y,z,b,k], unshifted; host_dtype OAL_F32 or
+ * OAL_F64.  The library converts to its storage (and from it).  Replaces the copyto! of
+ * BlockLevel.f / f_temp (src/geometry/blocks.jl:604-690, src/solver/solver_control.jl:830-851). */
+int oal_upload_f(OalPlan* plan, int level, const void* host_f, int host_dtype);
+int oal_download_f(OalPlan* plan, int level, void* host_f, int host_dtype);
+
+/* Equilibrium initialisation at per-cell (rho, u): init_eq! (src/main.jl:546-572).
+ * rho [n_cells] and u [3 * n_cells] in the reference layout ([x,y,z,b] and [x,y,z,b,3]), FP64;
+ * NULL fields take rho0 / u0 everywhere. */
+int oal_init_equilibrium(OalPlan* plan, int level, const double* rho, const double* u,
+                         double rho0, const double u0[3]);
+
+/* Per-step options of oal_level_step / oal_step */
+enum {
+    OAL_STEP_FORCE = 1         /* collect the momentum-exchange force of this step */
+};
+
+/* One step of one level: perform_timestep_v2! (src/collision/timestep.jl:946-2000) with the
+ * cumulant operator of get_collision_kernel (src/collision/physics_kernels.jl:50-74).
+ * time_step is the reference's time_step_seed (inlet noise). */
+int oal_level_step(OalPlan* plan, int level, int32_t time_step, int flags, void* stream);
+
+/* n coarse steps of every level with 2:1 sub-cycling and level transfers: execute_timestep_batch!
+ * (src/solver/solver_control.jl:796-824) over recursive_step! (:55-422). */
+int oal_step(OalPlan* plan, int n_steps, int32_t first_time_step, int flags, void* stream);
+
+/* The momentum-exchange force of the last step collected with OAL_STEP_FORCE, FP64, per domain of
+ * a level: F[3 * n_domains].  It sums, over every link from a fluid cell toward a solid cell, the
+ * momentum the link carries in the streaming that follows the step: (f_i* + f_opp(i)(t+1)) c_i with
+ * i toward the wall, shifted by w_i (src/octree_q19/src/stream.jl:248-256, reduce :382-389); the
+ * reference collects the same sum while gathering the next step. */
+int oal_force(OalPlan* plan, int level, double* force);
+
+/* FP64 mass and momentum over fluid cells of each domain: out[4 * n_domains] = (mass, jx, jy, jz)
+ * (src/octree_q19/src/state.jl:333-355). */
+int oal_mass(OalPlan* plan, int level, double* out, void* stream);
+
+/* rho [n_cells] and u [3 * n_cells] of the current populations, FP64 host arrays in the reference
+ * layout (materialize_current_state_for_io!, src/solver/solver_control.jl:830-851). */
+int oal_download_macro(OalPlan* plan, int level, double* rho, double* u);
+
+/* Stability guard (src/octree_q19/src/io_run.jl:31-65): counts fluid cells with rho <= rho_min,
+ * |u| >= u_max or a non-finite population.  *bad receives the count; returns OAL_ERR_NONFINITE when
+ * it is nonzero. */
+int oal_guard(OalPlan* plan, int level, float rho_min, float u_max, int64_t* bad);
+
+/* Waits for all work of the plan. */
+int oal_sync(OalPlan* plan);
+
+/* Times n steps of one level on the plan's stream with HIP events: *ms_total is the wall time of
+ * the n launches, *ms_kernel the mean duration of one step kernel. */
+int oal_time_steps(OalPlan* plan, int level, int n, int flags, double* ms_total, double* ms_kernel);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OAL_H */

@@ -1,0 +1,61 @@
+"""Build recipe of liboal.so (gfx950) and, for the test harness, the CPU oracle.
+
+`python -m openaerolab_alpha_01_amd.build` compiles in-tree; the built library sits next to this
+file so that it travels with the source tree.  hipcc cross-compiles for gfx950 without a GPU.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+LIB = PKG / "liboal.so"
+SOURCES = [CSRC / "oal_kernels.hip", CSRC / "oal_plan.hip"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# -ffp-contract=off: every multiply and add rounds separately, as in the reference arithmetic.
+# -packed-fp32-ops off: no v_pk_*_f32 instructions are generated.
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", str(ROOT / "include"),
+]
+
+
+def build_library(verbose: bool = False, extra: list[str] | None = None) -> Path:
+    """Compile liboal.so for gfx950 (always recompiles)."""
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [HIPCC, *HIP_FLAGS, *(extra or []), "-o", str(tmp), *map(str, SOURCES)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=str(CSRC))
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle(verbose: bool = False) -> Path:
+    """Compile the CPU oracle (test infrastructure) with its Makefile."""
+    oracle = ROOT / "oracle"
+    cmd = ["make", "-C", str(oracle), "-B", "liboal_oracle.so"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, stdout=None if verbose else subprocess.DEVNULL)
+    return oracle / "liboal_oracle.so"
+
+
+def main(argv: list[str]) -> int:
+    verbose = "-v" in argv
+    build_library(verbose=verbose, extra=["-Rpass-analysis=kernel-resource-usage"] if "--resources" in argv else None)
+    if "--no-oracle" not in argv:
+        build_oracle(verbose=verbose)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

@@ -1,0 +1,82 @@
+// Kernel argument structs and launchers shared by the plan code and the kernels of liboal.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "oal_device.h"
+
+namespace oal {
+
+constexpr int BS = 16;            // cells per block edge
+constexpr int BS2 = BS * BS;
+constexpr int BS3 = BS * BS * BS;
+constexpr int WG = 256;           // threads per workgroup: one 16 x 16 plane of a block
+constexpr int PLANES = BS;        // workgroups per block in the step kernels
+
+// per-block flags, computed on the host when the plan is built
+enum : uint32_t {
+    BF_EDGE = 1u,      // a source of some cell may lie outside the domain extent (domain-face rules)
+    BF_SOLID = 2u,     // the block or its one-cell halo holds a solid cell
+    BF_MISSING = 4u,   // some neighbour block is absent inside the extent (rest-weight fallback)
+    BF_PARTIAL = 8u    // some cells of the block lie beyond the extent
+};
+
+// one domain of a level on the device
+struct DevDomain {
+    int32_t ext[3];
+    int32_t per[3];
+    float tau, u_in, amr_scale, force_scale;
+    float F[3];
+    int32_t ymin_wall, zwall, forced;
+};
+
+// arguments of the fused stream-collide kernels
+struct StepArgs {
+    const void* fin;             // populations read this step
+    void* fout;                  // populations written (== fin for the AA pattern)
+    const int32_t* nbr;          // [27 * nb] dir-major, 0-based, -1 = none
+    const int32_t* borg;         // [3 * nb] block origin in cells
+    const int32_t* bdom;         // [nb] domain index
+    const uint32_t* bflags;      // [nb]
+    const uint8_t* solid;        // [ncell] or null
+    const DevDomain* doms;
+    const void* vel_in;          // [3 * ncell] WALE velocity of the previous step (T)
+    void* vel_out;               // [3 * ncell] velocity written (WALE, macro output)
+    void* rho_out;               // [ncell] density written (macro output) or null
+    double* fpart;               // [n_items * 4] per-workgroup force partials or null
+    uint32_t ncell;              // cells per direction (stride between directions)
+    uint32_t nb;                 // blocks
+    uint32_t n_items;            // workgroups launched (nb * PLANES)
+    int32_t time_step;
+    int32_t parity;              // AA pattern: 1 odd step (neighbour slots), 0 even step (own slots)
+    int32_t pad;
+    CollParams cp;
+};
+
+struct ReduceArgs {
+    const double* part;          // [n_items * 4]
+    const int32_t* dom_item0;    // [n_domains + 1] first item of each domain
+    double* out;                 // [n_domains * 4]
+    int32_t n_domains;
+    int32_t width;               // values per item used (3 force, 4 mass)
+};
+
+// launchers (oal_kernels.hip); all asynchronous on `stream`
+hipError_t launch_step(int storage, int stream_mode, bool wale, const StepArgs& a, hipStream_t stream);
+hipError_t launch_init_eq(int storage, void* f, uint32_t ncell, const double* rho, const double* u, double rho0,
+                          double ux0, double uy0, double uz0, int permute, void* vel, hipStream_t stream);
+hipError_t launch_convert_in(int storage, void* f, const void* src, int src_dtype, uint32_t ncell, int permute,
+                             hipStream_t stream);
+hipError_t launch_convert_out(int storage, const void* f, void* dst, int dst_dtype, uint32_t ncell, int permute,
+                              hipStream_t stream);
+hipError_t launch_macro(int storage, const void* f, double* rho, double* u, uint32_t ncell, int permute, void* vel,
+                        hipStream_t stream);
+hipError_t launch_mass(int storage, const void* f, const uint8_t* solid, const int32_t* borg, const int32_t* bdom,
+                       const DevDomain* doms, uint32_t nb, int permute, double* part, hipStream_t stream);
+hipError_t launch_reduce(const ReduceArgs& r, hipStream_t stream);
+hipError_t launch_guard(int storage, const void* f, const uint8_t* solid, const int32_t* borg, const int32_t* bdom,
+                        const DevDomain* doms, uint32_t nb, float rho_min, float u_max,
+                        unsigned long long* count, hipStream_t stream);
+
+}  // namespace oal

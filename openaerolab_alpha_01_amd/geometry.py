@@ -1,0 +1,151 @@
+"""Block topology and obstacle masks on the host (numpy), in the reference's BlockLevel conventions.
+
+A level is a list of 16^3-cell blocks.  Block b has 1-based block coordinates (map_x, map_y, map_z)
+(active_coords of src/geometry/blocks.jl) and a 27-entry row of the neighbour table, stored
+column-major n_blocks x 27 with 1-based entries and 0 = none (src/geometry/domain_topology.jl:919-955).
+Cell (x, y, z) of block b (0-based inside the block) is element b*bs^3 + z*bs^2 + y*bs + x of a field,
+and direction k of the populations adds k*n_cells: the column-major f[x,y,z,b,k] of the reference.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+BS = 16
+
+
+def lattice_d3q27():
+    """D3Q27 velocity set in the reference order k = (cx+1) + 3(cy+1) + 9(cz+1) (lattice.jl:10-59).
+
+    Returns (c [27,3] int, w [27] float64, opp, mirror_y, mirror_z).
+    """
+    c = np.array([(x, y, z) for z in (-1, 0, 1) for y in (-1, 0, 1) for x in (-1, 0, 1)], dtype=np.int64)
+    d2 = (c * c).sum(1)
+    w = np.select([d2 == 0, d2 == 1, d2 == 2, d2 == 3], [8 / 27, 2 / 27, 1 / 54, 1 / 216])
+    idx = {tuple(v): i for i, v in enumerate(c)}
+    opp = np.array([idx[tuple(-v)] for v in c])
+    mir_y = np.array([idx[(v[0], -v[1], v[2])] for v in c])
+    mir_z = np.array([idx[(v[0], v[1], -v[2])] for v in c])
+    return c, w, opp, mir_y, mir_z
+
+
+def build_neighbor_table(coords: np.ndarray, nblk: tuple[int, int, int],
+                         periodic: tuple[bool, bool, bool] = (False, False, False)) -> np.ndarray:
+    """27-neighbour table of a block list, restating build_neighbor_table (domain_topology.jl:919-955).
+
+    coords: [nb, 3] 1-based block coordinates; nblk: blocks per axis.  Returns int32 [nb, 27] with
+    1-based block indices and 0 where no block exists; a periodic axis wraps (the reference wraps y
+    only, under periodic_y).  The caller passes the Fortran-ordered (column-major) array to the ABI.
+    """
+    coords = np.asarray(coords, dtype=np.int64)
+    nb = coords.shape[0]
+    ptr = np.zeros(tuple(nblk), dtype=np.int32)
+    ptr[coords[:, 0] - 1, coords[:, 1] - 1, coords[:, 2] - 1] = np.arange(1, nb + 1, dtype=np.int32)
+    table = np.zeros((nb, 27), dtype=np.int32)
+    k = 0
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                n = coords + np.array([dx, dy, dz])
+                ok = np.ones(nb, dtype=bool)
+                for a in range(3):
+                    if periodic[a]:
+                        n[:, a] = (n[:, a] - 1) % nblk[a] + 1
+                    else:
+                        ok &= (n[:, a] >= 1) & (n[:, a] <= nblk[a])
+                vals = np.zeros(nb, dtype=np.int32)
+                vals[ok] = ptr[n[ok, 0] - 1, n[ok, 1] - 1, n[ok, 2] - 1]
+                table[:, k] = vals
+                k += 1
+    return table
+
+
+@dataclass
+class BlockGrid:
+    """One level of blocks covering the extent `n` (cells) of one or more domains."""
+    n: tuple[int, int, int]
+    periodic: tuple[bool, bool, bool]
+    coords: np.ndarray                 # [nb, 3] int32, 1-based block coordinates
+    neighbor: np.ndarray               # [nb, 27] int32, 1-based, 0 = none
+    solid: np.ndarray | None = None    # [nb*bs^3] uint8 or None
+    bs: int = BS
+    block_domain: np.ndarray | None = None
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def n_blocks(self) -> int:
+        return int(self.coords.shape[0])
+
+    @property
+    def n_cells(self) -> int:
+        return self.n_blocks * self.bs ** 3
+
+    def cell_coords(self) -> np.ndarray:
+        """[n_cells, 3] 0-based global cell coordinates of every stored cell."""
+        bs = self.bs
+        z, y, x = np.meshgrid(np.arange(bs), np.arange(bs), np.arange(bs), indexing="ij")
+        loc = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+        org = (self.coords.astype(np.int64) - 1) * bs
+        return (org[:, None, :] + loc[None, :, :]).reshape(-1, 3)
+
+    def inside(self) -> np.ndarray:
+        """[n_cells] bool: the cell lies inside the extent (blocks may overhang it)."""
+        g = self.cell_coords()
+        return (g[:, 0] < self.n[0]) & (g[:, 1] < self.n[1]) & (g[:, 2] < self.n[2])
+
+    def to_blocks(self, dense: np.ndarray) -> np.ndarray:
+        """Dense field [nx, ny, nz, ...] (x first) -> block layout [n_cells, ...]; overhang cells get 0."""
+        g = self.cell_coords()
+        ins = self.inside()
+        out = np.zeros((self.n_cells,) + dense.shape[3:], dtype=dense.dtype)
+        out[ins] = dense[g[ins, 0], g[ins, 1], g[ins, 2]]
+        return out
+
+    def to_dense(self, blocks: np.ndarray) -> np.ndarray:
+        """Block layout [n_cells, ...] -> dense [nx, ny, nz, ...]."""
+        g = self.cell_coords()
+        ins = self.inside()
+        out = np.zeros(tuple(self.n) + blocks.shape[1:], dtype=blocks.dtype)
+        out[g[ins, 0], g[ins, 1], g[ins, 2]] = blocks[ins]
+        return out
+
+
+def uniform_grid(n, periodic=(False, False, False), bs: int = BS) -> BlockGrid:
+    """Every block of a box of n = (nx, ny, nz) cells, block x fastest; blocks overhang a non-multiple extent."""
+    n = tuple(int(v) for v in n)
+    periodic = tuple(bool(p) for p in periodic)
+    for a in range(3):
+        if periodic[a] and n[a] % bs:
+            raise ValueError("a periodic axis needs an extent that is a multiple of the block size")
+    nblk = tuple(-(-v // bs) for v in n)
+    bz, by, bx = np.meshgrid(np.arange(nblk[2]), np.arange(nblk[1]), np.arange(nblk[0]), indexing="ij")
+    coords = np.stack([bx.ravel(), by.ravel(), bz.ravel()], axis=1).astype(np.int32) + 1
+    table = build_neighbor_table(coords, nblk, periodic)
+    return BlockGrid(n=n, periodic=periodic, coords=coords, neighbor=table, bs=bs)
+
+
+def sphere_mask(grid: BlockGrid, center, radius: float) -> np.ndarray:
+    """Solid iff the cell centre lies within the sphere: apply_analytic_sphere_obstacle!
+    (src/geometry/domain_generation.jl:111-154) in lattice units, cell i (1-based) centred at i - 1/2,
+    i.e. 0-based cell j centred at j + 1/2."""
+    g = grid.cell_coords().astype(np.float64) + 0.5
+    d = g - np.asarray(center, dtype=np.float64)[None, :]
+    m = (d * d).sum(1) <= radius * radius
+    return (m & grid.inside()).astype(np.uint8)
+
+
+def ellipsoid_mask(grid: BlockGrid, center, semi_axes, rotation: np.ndarray) -> np.ndarray:
+    """Solid iff the cell centre lies inside the ellipsoid with the given semi-axes along the columns
+    of `rotation` (a 3x3 rotation matrix)."""
+    g = grid.cell_coords().astype(np.float64) + 0.5
+    d = (g - np.asarray(center, dtype=np.float64)[None, :]) @ np.asarray(rotation, dtype=np.float64)
+    q = ((d / np.asarray(semi_axes, dtype=np.float64)[None, :]) ** 2).sum(1)
+    return ((q <= 1.0) & grid.inside()).astype(np.uint8)
+
+
+def channel_walls(grid: BlockGrid, axis: int = 1) -> np.ndarray:
+    """Solid layers at the first and last cell along `axis` (a plane channel)."""
+    g = grid.cell_coords()
+    m = (g[:, axis] == 0) | (g[:, axis] == grid.n[axis] - 1)
+    return (m & grid.inside()).astype(np.uint8)

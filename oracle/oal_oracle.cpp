@@ -1,0 +1,788 @@
+/*
+ * oal_oracle — CPU restatement of the reference D3Q27 cumulant lattice-Boltzmann update.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker of the MI355X library, never part of it:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The product
+ * (openaerolab_alpha_01_amd/liboal.so) neither links nor calls anything here.
+ *
+ * What it restates (reference paths relative to the r-acad/OpenAeroLab_alpha_01 tree):
+ *   lattice          src/collision/lattice.jl:10-59, src/collision/timestep.jl:2005-2023
+ *   equilibria       src/collision/physics_utils.jl:30-100
+ *   inlet noise      src/collision/physics_utils.jl:5-16
+ *   chimera          src/collision/kernel_cumulant.jl:4-26
+ *   step (one cell)  src/collision/kernel_cumulant.jl:173-908 (pull gather, boundary rules,
+ *                    solid reflection, WALE, cumulant relaxation, positivity limiter, Guo term)
+ *   WALE gradients   src/collision/physics_utils.jl:1981-2035, 2217-2233
+ *   init             src/main.jl:546-572 (init_eq!)
+ *   momentum exch.   src/octree_q19/src/stream.jl:248-256 (F += (f_i* + f_opp(i)(t+1)) c_i)
+ *   FP64 mass        src/octree_q19/src/state.jl:333-343
+ *
+ * Layout (the reference's BlockLevel layout, src/geometry/blocks.jl:16-200, column-major
+ * f[x,y,z,b,k]): population k of cell (x,y,z) of block b sits at
+ *   k*ncell + b*bs^3 + z*bs^2 + y*bs + x          (0-based, ncell = nb*bs^3).
+ * Velocity fields are [x,y,z,b,3] the same way.
+ *
+ * Arithmetic follows the reference operation for operation, in T (float for the FP32, FP16- and
+ * BF16-storage forms, double for the FP64 restatement).  Build with -ffp-contract=off so that no
+ * multiply-add is fused: the reference writes separate multiplies and adds.
+ *
+ * Additions that the reference kernel does not have, each stated where it is used:
+ *   - sources outside a non-periodic domain extent take the domain-face rules even inside an
+ *     allocated block (blocks need not tile the extent exactly); for extents that are block
+ *     multiples this is the reference's order (a neighbour block never lies outside the domain);
+ *   - periodic axes (the 64^3 Taylor-Green case) wrap through the neighbour table;
+ *   - bb_mode 1, halfway bounce-back on solid cells: the arriving population from a solid source
+ *     is the cell's own post-collision population in the opposite direction, the reference's
+ *     no-slip wall form (kernel_cumulant.jl:270-271, 340-341); bb_mode 0 is the reference's own
+ *     solid-cell reflection (kernel_cumulant.jl:405-417);
+ *   - a uniform body force enters through the reference's Guo-forcing slot (the wall-model
+ *     force, kernel_cumulant.jl:452-486 and 889-905), with the wall model itself off.
+ *
+ * Parity status of each function is documented in the header of oracle/oracle.py.
+ */
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------------------------------------
+// D3Q27 tables in the reference order: k = (cx+1) + 3 (cy+1) + 9 (cz+1)  (lattice.jl:16-19)
+// ---------------------------------------------------------------------------------------------
+struct Lattice {
+    int cx[27], cy[27], cz[27], opp[27], mir_y[27], mir_z[27], d2[27];
+    Lattice() {
+        int k = 0;
+        for (int z = -1; z <= 1; ++z)
+            for (int y = -1; y <= 1; ++y)
+                for (int x = -1; x <= 1; ++x) {
+                    cx[k] = x; cy[k] = y; cz[k] = z; d2[k] = x * x + y * y + z * z; ++k;
+                }
+        for (int i = 0; i < 27; ++i)
+            for (int j = 0; j < 27; ++j) {
+                if (cx[j] == -cx[i] && cy[j] == -cy[i] && cz[j] == -cz[i]) opp[i] = j;
+                if (cx[j] == cx[i] && cy[j] == -cy[i] && cz[j] == cz[i]) mir_y[i] = j;
+                if (cx[j] == cx[i] && cy[j] == cy[i] && cz[j] == -cz[i]) mir_z[i] = j;
+            }
+    }
+};
+const Lattice LAT;
+
+// weights as the reference builds them in the working precision (timestep.jl:2013)
+template <class T> T weight(int k) {
+    switch (LAT.d2[k]) {
+        case 0: return T(8) / T(27);
+        case 1: return T(2) / T(27);
+        case 2: return T(1) / T(54);
+        default: return T(1) / T(216);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// 16-bit storage conversions, round to nearest even (IEEE binary16 and bfloat16)
+// ---------------------------------------------------------------------------------------------
+inline uint16_t f32_to_f16(float f) {
+    uint32_t x; std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const uint32_t ex = (x >> 23) & 0xffu;
+    uint32_t mant = x & 0x7fffffu;
+    if (ex == 0xffu) return (uint16_t)(sign | 0x7c00u | (mant ? 0x200u : 0u));
+    const int e = (int)ex - 127 + 15;
+    if (e >= 31) return (uint16_t)(sign | 0x7c00u);
+    if (e <= 0) {
+        if (e < -10) return (uint16_t)sign;
+        mant |= 0x800000u;
+        const int shift = 14 - e;
+        uint32_t h = mant >> shift;
+        const uint32_t rem = mant & ((1u << shift) - 1u);
+        const uint32_t half = 1u << (shift - 1);
+        if (rem > half || (rem == half && (h & 1u))) ++h;
+        return (uint16_t)(sign | h);
+    }
+    uint32_t h = ((uint32_t)e << 10) | (mant >> 13);
+    const uint32_t rem = mant & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) ++h;
+    return (uint16_t)(sign | h);
+}
+
+inline float f16_to_f32(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    uint32_t ex = (h >> 10) & 0x1fu, mant = h & 0x3ffu, x;
+    if (ex == 0) {
+        if (mant == 0) x = sign;
+        else {
+            int e = -1;
+            do { ++e; mant <<= 1; } while (!(mant & 0x400u));
+            mant &= 0x3ffu;
+            x = sign | ((uint32_t)(127 - 15 - e) << 23) | (mant << 13);
+        }
+    } else if (ex == 0x1fu) {
+        x = sign | 0x7f800000u | (mant << 13);
+    } else {
+        x = sign | ((ex - 15 + 127) << 23) | (mant << 13);
+    }
+    float f; std::memcpy(&f, &x, 4);
+    return f;
+}
+
+inline uint16_t f32_to_bf16(float f) {
+    uint32_t u; std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+inline float bf16_to_f32(uint16_t h) {
+    const uint32_t u = (uint32_t)h << 16;
+    float f; std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// Storage policies: stored value <-> working value.  The 16-bit forms hold f - w_k.
+struct StF32 {
+    using st = float; using T = float;
+    static T ld(st v, T) { return v; }
+    static st sto(T f, T) { return f; }
+};
+struct StF64 {
+    using st = double; using T = double;
+    static T ld(st v, T) { return v; }
+    static st sto(T f, T) { return f; }
+};
+struct StF16S {
+    using st = uint16_t; using T = float;
+    static T ld(st v, T w) { return f16_to_f32(v) + w; }
+    static st sto(T f, T w) { return f32_to_f16(f - w); }
+};
+struct StBF16S {
+    using st = uint16_t; using T = float;
+    static T ld(st v, T w) { return bf16_to_f32(v) + w; }
+    static st sto(T f, T w) { return f32_to_bf16(f - w); }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Equilibria (physics_utils.jl:30-100) and the inlet noise hash (physics_utils.jl:5-16)
+// ---------------------------------------------------------------------------------------------
+template <class T>
+T feq2(T rho, T ux, T uy, T uz, T w, T cx, T cy, T cz) {
+    T cu = cx * ux + cy * uy + cz * uz;
+    T usq = ux * ux + uy * uy + uz * uz;
+    return rho * w * (T(1) + T(3) * cu + T(4.5) * cu * cu - T(1.5) * usq);
+}
+
+template <class T>
+T feq4(T rho, T ux, T uy, T uz, T w, T cx, T cy, T cz) {
+    T cu = cx * ux + cy * uy + cz * uz;
+    T usq = ux * ux + uy * uy + uz * uz;
+    T cu2 = cu * cu, cu3 = cu2 * cu, cu4 = cu2 * cu2, usq2 = usq * usq;
+    T val = T(1) + T(3) * cu + T(4.5) * cu2 - T(1.5) * usq;
+    T ho = (T(4.5) * cu3 - T(4.5) * cu * usq) + (T(3.375) * cu4 - T(6.75) * cu2 * usq + T(1.125) * usq2);
+    val += ho;
+    return rho * w * val;
+}
+
+template <class T>
+T feq_auto(T rho, T ux, T uy, T uz, T w, T cx, T cy, T cz, int compressible) {
+    return compressible == 1 ? feq4(rho, ux, uy, uz, w, cx, cy, cz) : feq2(rho, ux, uy, uz, w, cx, cy, cz);
+}
+
+inline uint32_t gpu_hash(int32_t x) {
+    uint32_t h = (uint32_t)x;
+    h = (h ^ (h >> 16)) * 0x85ebca6bu;
+    h = (h ^ (h >> 13)) * 0xc2b2ae35u;
+    return h ^ (h >> 16);
+}
+
+// Int32 arithmetic wraps in the reference (Julia Int32 * and +); unsigned arithmetic here.
+inline float gradient_noise(int32_t gx, int32_t gy, int32_t gz, int32_t seed) {
+    uint32_t c = (uint32_t)gx * 374761393u + (uint32_t)gy * 668265263u + (uint32_t)gz * 1274126177u + (uint32_t)seed;
+    uint32_t h = gpu_hash((int32_t)c);
+    return ((float)(h & 0xffffu) / 32768.0f) - 1.0f;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Chimera transforms (kernel_cumulant.jl:4-26) and the collision of one cell
+// (kernel_cumulant.jl:420-908, fluid branch)
+// ---------------------------------------------------------------------------------------------
+template <class T>
+inline void chimera_fwd(T& a, T& b, T& c, T u) {
+    T m0 = a + b + c;
+    T m1 = c - a;
+    T m2 = a + c;
+    a = m0;
+    b = m1 - u * m0;
+    c = m2 - T(2) * u * m1 + u * u * m0;
+}
+
+template <class T>
+inline void chimera_bwd(T& a, T& b, T& c, T u) {
+    T m0 = a;
+    T m1 = b + u * a;
+    T m2 = c + T(2) * u * b + u * u * a;
+    T fp = (m2 + m1) * T(0.5);
+    T fm = (m2 - m1) * T(0.5);
+    T f0 = m0 - m2;
+    a = fm; b = f0; c = fp;
+}
+
+}  // namespace
+
+// Parameters of one step.  All physics parameters are given in double and rounded to the working
+// precision T inside, as the reference receives them as Float32 kernel arguments.
+struct OrParams {
+    double tau;               // molecular relaxation time of the level (tau_molecular)
+    double omega_bulk, omega3, omega4;
+    double lambda;            // adaptive omega4 parameter
+    double c_wale, nu_sgs_bg, amr_sgs_scale;
+    double u_inlet, inlet_turb;
+    double body_force[3];
+    int32_t adaptive_omega4;
+    int32_t limiter;          // 0 none, 1 factored, 2 positivity
+    int32_t compressible;     // 0: O(u^2) equilibrium, 1: O(u^4)
+    int32_t ymin_wall_bc;     // 1: no-slip bounce-back at y_min
+    int32_t zwall_bc;         // 0: legacy mirror, 2: no-slip bounce-back
+    int32_t bb_mode;          // 0: reference solid reflection, 1: halfway bounce-back
+    int32_t time_step_seed;
+    int32_t pad;
+};
+
+struct OrGrid {
+    int32_t bs, nb;
+    const int32_t* bcoord;    // [3*nb]: block coordinates (bx, by, bz) in blocks, 0-based
+    const int32_t* nbr;       // [27*nb], dir-major: nbr[d*nb + b], -1 = none
+    int32_t n[3];             // domain extent in cells
+    int32_t periodic[3];
+    const uint8_t* solid;     // [nb*bs^3] or null
+};
+
+namespace {
+
+template <class T>
+struct Coll {
+    T tau, omega_bulk, omega3, omega4, lambda, c_wale, nu_sgs_bg, amr_scale, u_in, turb, F[3];
+    int adaptive, limiter, compressible;
+    explicit Coll(const OrParams& p)
+        : tau((T)p.tau), omega_bulk((T)p.omega_bulk), omega3((T)p.omega3), omega4((T)p.omega4),
+          lambda((T)p.lambda), c_wale((T)p.c_wale), nu_sgs_bg((T)p.nu_sgs_bg), amr_scale((T)p.amr_sgs_scale),
+          u_in((T)p.u_inlet), turb((T)p.inlet_turb), adaptive(p.adaptive_omega4), limiter(p.limiter),
+          compressible(p.compressible) {
+        F[0] = (T)p.body_force[0]; F[1] = (T)p.body_force[1]; F[2] = (T)p.body_force[2];
+    }
+};
+
+// WALE eddy viscosity from the velocity gradient g_ij = d u_i / d x_j (kernel_cumulant.jl:500-535)
+template <class T>
+T wale_nu(const T g[9], T c_wale) {
+    const T g11 = g[0], g12 = g[1], g13 = g[2], g21 = g[3], g22 = g[4], g23 = g[5], g31 = g[6], g32 = g[7], g33 = g[8];
+    T gsq11 = g11 * g11 + g12 * g21 + g13 * g31;
+    T gsq12 = g11 * g12 + g12 * g22 + g13 * g32;
+    T gsq13 = g11 * g13 + g12 * g23 + g13 * g33;
+    T gsq21 = g21 * g11 + g22 * g21 + g23 * g31;
+    T gsq22 = g21 * g12 + g22 * g22 + g23 * g32;
+    T gsq23 = g21 * g13 + g22 * g23 + g23 * g33;
+    T gsq31 = g31 * g11 + g32 * g21 + g33 * g31;
+    T gsq32 = g31 * g12 + g32 * g22 + g33 * g32;
+    T gsq33 = g31 * g13 + g32 * g23 + g33 * g33;
+    T tr_gsq = gsq11 + gsq22 + gsq33;
+    T tr_term = tr_gsq / T(3);
+    T Sd11 = gsq11 - tr_term, Sd22 = gsq22 - tr_term, Sd33 = gsq33 - tr_term;
+    T Sd12 = T(0.5) * (gsq12 + gsq21), Sd13 = T(0.5) * (gsq13 + gsq31), Sd23 = T(0.5) * (gsq23 + gsq32);
+    T S12 = T(0.5) * (g12 + g21), S13 = T(0.5) * (g13 + g31), S23 = T(0.5) * (g23 + g32);
+    T OP1 = Sd11 * Sd11 + Sd22 * Sd22 + Sd33 * Sd33 + T(2) * (Sd12 * Sd12 + Sd13 * Sd13 + Sd23 * Sd23);
+    T OP2 = g11 * g11 + g22 * g22 + g33 * g33 + T(2) * (S12 * S12 + S13 * S13 + S23 * S23);
+    T nu = T(0);
+    if (OP1 > T(1e-12f)) {
+        T OP1_32 = OP1 * std::sqrt(OP1);
+        T OP2_52 = OP2 * OP2 * std::sqrt(std::max(OP2, T(1e-12f)));
+        T denom = OP2_52 + OP1 * std::sqrt(std::sqrt(std::max(OP1, T(1e-12f))));
+        if (denom > T(1e-12f)) nu = (c_wale * c_wale) * OP1_32 / denom;
+    }
+    return nu;
+}
+
+// The fluid branch of the reference kernel for one cell: f holds the gathered populations and
+// (rho, jx, jy, jz) their sums in gather order; nu_eddy the WALE viscosity before scaling.
+// Writes the post-collision populations into out and the macroscopic state into rho_o, u_o.
+template <class T>
+void collide(const T fin[27], T rho, T jx, T jy, T jz, T nu_eddy, const Coll<T>& c, T out[27], T& rho_o, T u_o[3]) {
+    rho = std::max(rho, T(0.01f));
+    T inv_rho = T(1) / rho;
+    T ux = jx * inv_rho, uy = jy * inv_rho, uz = jz * inv_rho;
+    // (sponge_arr = 0 everywhere: kernel_cumulant.jl:428-446 inactive)
+    T Fx = c.F[0], Fy = c.F[1], Fz = c.F[2];
+    T ux_eq = ux + T(0.5) * Fx * inv_rho;
+    T uy_eq = uy + T(0.5) * Fy * inv_rho;
+    T uz_eq = uz + T(0.5) * Fz * inv_rho;
+    rho_o = rho; u_o[0] = ux; u_o[1] = uy; u_o[2] = uz;
+
+    nu_eddy = nu_eddy * c.amr_scale;
+    nu_eddy = std::max(nu_eddy, c.nu_sgs_bg * c.amr_scale);
+    T nu_mol = (c.tau - T(0.5)) / T(3);
+    T nu_eff = nu_mol + nu_eddy;
+    T tau_eff = T(0.5) + T(3) * nu_eff;
+    T omega1 = T(1) / std::max(tau_eff, T(0.500001f));
+
+    T omega_bulk = c.omega_bulk, omega3 = c.omega3, omega4 = c.omega4;
+    T omega4_diag = omega4;
+    if (c.adaptive == 1) {
+        T om1_m2 = omega1 - T(2);
+        T denom_adaptive = c.lambda * omega1 + T(1e-10f);
+        T A_val = -(om1_m2 * om1_m2) / denom_adaptive;
+        omega4_diag = std::min(std::max(A_val, T(0.01f)), T(2));
+    }
+
+    T A[27];
+    for (int k = 0; k < 27; ++k) A[k] = fin[k];
+    // x-pass, y-pass, z-pass (kernel_cumulant.jl:620-656); index = a + 3 b + 9 c
+    for (int iz = 0; iz < 3; ++iz)
+        for (int iy = 0; iy < 3; ++iy) {
+            int base = 3 * iy + 9 * iz;
+            chimera_fwd(A[base], A[base + 1], A[base + 2], ux_eq);
+        }
+    for (int iz = 0; iz < 3; ++iz)
+        for (int ma = 0; ma < 3; ++ma)
+            chimera_fwd(A[ma + 9 * iz], A[ma + 3 + 9 * iz], A[ma + 6 + 9 * iz], uy_eq);
+    for (int mb = 0; mb < 3; ++mb)
+        for (int ma = 0; ma < 3; ++ma)
+            chimera_fwd(A[ma + 3 * mb], A[ma + 3 * mb + 9], A[ma + 3 * mb + 18], uz_eq);
+
+    // central moments (kernel_cumulant.jl:661-676), 0-based index a + 3 b + 9 c
+    T k_000 = A[0];
+    T k_200 = A[2], k_020 = A[6], k_002 = A[18];
+    T k_110 = A[4], k_101 = A[10], k_011 = A[12];
+    T k_210 = A[5], k_120 = A[7], k_201 = A[11], k_021 = A[15], k_102 = A[19], k_012 = A[21];
+    T k_111 = A[13];
+    T k_220 = A[8], k_202 = A[20], k_022 = A[24];
+    T k_211 = A[14], k_121 = A[16], k_112 = A[22];
+    T inv_r = T(1) / std::max(k_000, T(0.01f));
+
+    T c200 = k_200 * inv_r, c020 = k_020 * inv_r, c002 = k_002 * inv_r;
+    T c110 = k_110 * inv_r, c101 = k_101 * inv_r, c011 = k_011 * inv_r;
+    T c210 = k_210 * inv_r, c120 = k_120 * inv_r, c201 = k_201 * inv_r;
+    T c021 = k_021 * inv_r, c102 = k_102 * inv_r, c012 = k_012 * inv_r;
+    T c111 = k_111 * inv_r;
+    T c220n = k_220 * inv_r, c202n = k_202 * inv_r, c022n = k_022 * inv_r;
+    T c211n = k_211 * inv_r, c121n = k_121 * inv_r, c112n = k_112 * inv_r;
+
+    T C_220 = c220n - c200 * c020 - T(2) * c110 * c110;
+    T C_202 = c202n - c200 * c002 - T(2) * c101 * c101;
+    T C_022 = c022n - c020 * c002 - T(2) * c011 * c011;
+    T C_211 = c211n - c200 * c011 - T(2) * c110 * c101;
+    T C_121 = c121n - c020 * c101 - T(2) * c110 * c011;
+    T C_112 = c112n - c002 * c110 - T(2) * c101 * c011;
+
+    // relaxation (kernel_cumulant.jl:705-754)
+    // cs2 = 0.333333333f0 and 3.0f0 * cs2 are Float32 in the reference: the product rounds to 1.0f
+    T D_trace = c200 + c020 + c002;
+    T D_trace_eq = T(3.0f * 0.333333333f);
+    T D_trace_s = (T(1) - omega_bulk) * D_trace + omega_bulk * D_trace_eq;
+    T D1 = c200 - c020, D2 = c200 - c002;
+    T D1_s = (T(1) - omega1) * D1, D2_s = (T(1) - omega1) * D2;
+    T c200_s = (D_trace_s + D1_s + D2_s) / T(3);
+    T c020_s = (D_trace_s - T(2) * D1_s + D2_s) / T(3);
+    T c002_s = (D_trace_s + D1_s - T(2) * D2_s) / T(3);
+    T c110_s = (T(1) - omega1) * c110;
+    T c101_s = (T(1) - omega1) * c101;
+    T c011_s = (T(1) - omega1) * c011;
+    if (c.limiter == 1) {
+        T b110 = std::sqrt(std::max(c200_s * c020_s, T(0)));
+        T b101 = std::sqrt(std::max(c200_s * c002_s, T(0)));
+        T b011 = std::sqrt(std::max(c020_s * c002_s, T(0)));
+        c110_s = std::min(std::max(c110_s, -b110), b110);
+        c101_s = std::min(std::max(c101_s, -b101), b101);
+        c011_s = std::min(std::max(c011_s, -b011), b011);
+    }
+    T c210_s = (T(1) - omega3) * c210, c120_s = (T(1) - omega3) * c120;
+    T c201_s = (T(1) - omega3) * c201, c021_s = (T(1) - omega3) * c021;
+    T c102_s = (T(1) - omega3) * c102, c012_s = (T(1) - omega3) * c012;
+    T c111_s = (T(1) - omega3) * c111;
+    T C_220_s = (T(1) - omega4_diag) * C_220;
+    T C_202_s = (T(1) - omega4_diag) * C_202;
+    T C_022_s = (T(1) - omega4_diag) * C_022;
+    T C_211_s = (T(1) - omega4) * C_211;
+    T C_121_s = (T(1) - omega4) * C_121;
+    T C_112_s = (T(1) - omega4) * C_112;
+
+    // rebuild (kernel_cumulant.jl:759-781)
+    T c220_s = C_220_s + c200_s * c020_s + T(2) * c110_s * c110_s;
+    T c202_s = C_202_s + c200_s * c002_s + T(2) * c101_s * c101_s;
+    T c022_s = C_022_s + c020_s * c002_s + T(2) * c011_s * c011_s;
+    T c211_s = C_211_s + c200_s * c011_s + T(2) * c110_s * c101_s;
+    T c121_s = C_121_s + c020_s * c101_s + T(2) * c110_s * c011_s;
+    T c112_s = C_112_s + c002_s * c110_s + T(2) * c101_s * c011_s;
+    T c221_s = c200_s * c021_s + c020_s * c201_s +
+               T(4) * c110_s * c111_s + T(2) * c101_s * c120_s + T(2) * c011_s * c210_s;
+    T c212_s = c200_s * c012_s + c002_s * c210_s +
+               T(2) * c110_s * c102_s + T(4) * c101_s * c111_s + T(2) * c011_s * c201_s;
+    T c122_s = c020_s * c102_s + c002_s * c120_s +
+               T(2) * c110_s * c012_s + T(4) * c011_s * c111_s + T(2) * c101_s * c021_s;
+    T c222_s = c200_s * C_022_s + c020_s * C_202_s + c002_s * C_220_s +
+               T(4) * (c110_s * C_112_s + c101_s * C_121_s + c011_s * C_211_s);
+    c222_s += T(2) * (c210_s * c012_s + c201_s * c021_s + c120_s * c102_s) + T(4) * c111_s * c111_s;
+    c222_s += c200_s * c020_s * c002_s +
+              T(2) * (c200_s * c011_s * c011_s + c020_s * c101_s * c101_s + c002_s * c110_s * c110_s) +
+              T(8) * c110_s * c101_s * c011_s;
+
+    // write back (kernel_cumulant.jl:786-804)
+    T r = k_000;
+    A[0] = r;
+    A[1] = T(0); A[3] = T(0); A[9] = T(0);
+    A[2] = r * c200_s; A[6] = r * c020_s; A[18] = r * c002_s;
+    A[4] = r * c110_s; A[10] = r * c101_s; A[12] = r * c011_s;
+    A[5] = r * c210_s; A[7] = r * c120_s; A[11] = r * c201_s;
+    A[15] = r * c021_s; A[19] = r * c102_s; A[21] = r * c012_s;
+    A[13] = r * c111_s;
+    A[8] = r * c220_s; A[20] = r * c202_s; A[24] = r * c022_s;
+    A[14] = r * c211_s; A[16] = r * c121_s; A[22] = r * c112_s;
+    A[17] = r * c221_s; A[23] = r * c212_s; A[25] = r * c122_s;
+    A[26] = r * c222_s;
+
+    // backward z, y, x (kernel_cumulant.jl:810-844)
+    for (int mb = 0; mb < 3; ++mb)
+        for (int ma = 0; ma < 3; ++ma)
+            chimera_bwd(A[ma + 3 * mb], A[ma + 3 * mb + 9], A[ma + 3 * mb + 18], uz_eq);
+    for (int iz = 0; iz < 3; ++iz)
+        for (int ma = 0; ma < 3; ++ma)
+            chimera_bwd(A[ma + 9 * iz], A[ma + 3 + 9 * iz], A[ma + 6 + 9 * iz], uy_eq);
+    for (int iz = 0; iz < 3; ++iz)
+        for (int iy = 0; iy < 3; ++iy) {
+            int base = 3 * iy + 9 * iz;
+            chimera_bwd(A[base], A[base + 1], A[base + 2], ux_eq);
+        }
+
+    // positivity limiter (kernel_cumulant.jl:849-884)
+    if (c.limiter == 2) {
+        T fmin = A[0];
+        for (int k = 1; k < 27; ++k) fmin = std::min(fmin, A[k]);
+        if (fmin < T(0)) {
+            T alpha = T(1);
+            for (int k = 0; k < 27; ++k) {
+                T feq = feq_auto(rho, ux_eq, uy_eq, uz_eq, weight<T>(k), (T)LAT.cx[k], (T)LAT.cy[k], (T)LAT.cz[k], c.compressible);
+                T fneq = A[k] - feq;
+                if (fneq < T(0) && feq > T(1e-15f)) {
+                    T ak = feq / (feq - fneq + T(1e-15f));
+                    alpha = std::min(alpha, ak);
+                }
+            }
+            alpha = std::max(alpha, T(0));
+            for (int k = 0; k < 27; ++k) {
+                T feq = feq_auto(rho, ux_eq, uy_eq, uz_eq, weight<T>(k), (T)LAT.cx[k], (T)LAT.cy[k], (T)LAT.cz[k], c.compressible);
+                A[k] = feq + alpha * (A[k] - feq);
+            }
+        }
+    }
+
+    // Guo forcing (kernel_cumulant.jl:889-905)
+    for (int k = 0; k < 27; ++k) {
+        T cxf = (T)LAT.cx[k], cyf = (T)LAT.cy[k], czf = (T)LAT.cz[k];
+        T wk = weight<T>(k);
+        T cu = cxf * ux_eq + cyf * uy_eq + czf * uz_eq;
+        T force_term = wk * T(3) * ((cxf - ux + T(3) * cu * cxf) * Fx +
+                                    (cyf - uy + T(3) * cu * cyf) * Fy +
+                                    (czf - uz + T(3) * cu * czf) * Fz);
+        out[k] = A[k] + (T(1) - T(0.5) * omega1) * force_term;
+    }
+}
+
+// Grid helpers
+struct Geo {
+    const OrGrid& g;
+    int bs; int64_t bs3, ncell;
+    explicit Geo(const OrGrid& gr) : g(gr), bs(gr.bs), bs3((int64_t)gr.bs * gr.bs * gr.bs), ncell((int64_t)gr.nb * gr.bs * gr.bs * gr.bs) {}
+    bool outside(int axis, int gc) const { return !g.periodic[axis] && (gc < 0 || gc >= g.n[axis]); }
+    // cell index of local (lx,ly,lz) relative to block b, possibly in a neighbour block; -1 if none
+    int64_t locate(int b, int lx, int ly, int lz) const {
+        int ox = lx < 0 ? -1 : (lx >= bs ? 1 : 0);
+        int oy = ly < 0 ? -1 : (ly >= bs ? 1 : 0);
+        int oz = lz < 0 ? -1 : (lz >= bs ? 1 : 0);
+        int nb = b;
+        if (ox | oy | oz) {
+            int d = (ox + 1) + 3 * (oy + 1) + 9 * (oz + 1);
+            nb = g.nbr[(int64_t)d * g.nb + b];
+            if (nb < 0) return -1;
+        }
+        int x = lx - ox * bs, y = ly - oy * bs, z = lz - oz * bs;
+        return (int64_t)nb * bs3 + ((int64_t)z * bs + y) * bs + x;
+    }
+};
+
+template <class P>
+void step_impl(const OrGrid& g, const OrParams& prm, const typename P::st* fin, typename P::st* fout,
+               const typename P::T* vel_in, typename P::T* vel_out, typename P::T* rho_out, double* force) {
+    using T = typename P::T;
+    const Geo geo(g);
+    const int bs = g.bs;
+    const int64_t ncell = geo.ncell;
+    const Coll<T> cp(prm);
+    T w[27];
+    for (int k = 0; k < 27; ++k) w[k] = weight<T>(k);
+    std::vector<double> fpart((size_t)g.nb * 3, 0.0);
+
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < g.nb; ++b) {
+        double fb[3] = {0.0, 0.0, 0.0};
+        const int bx = g.bcoord[3 * b], by = g.bcoord[3 * b + 1], bz = g.bcoord[3 * b + 2];
+        for (int z = 0; z < bs; ++z)
+            for (int y = 0; y < bs; ++y)
+                for (int x = 0; x < bs; ++x) {
+                    const int gx = bx * bs + x, gy = by * bs + y, gz = bz * bs + z;
+                    if (gx >= g.n[0] || gy >= g.n[1] || gz >= g.n[2]) continue;  // beyond the extent
+                    const int64_t own = (int64_t)b * geo.bs3 + ((int64_t)z * bs + y) * bs + x;
+                    const bool is_obs = g.solid && g.solid[own];
+                    if (is_obs && prm.bb_mode == 1) continue;
+                    auto own_f = [&](int k) { return P::ld(fin[(int64_t)k * ncell + own], w[k]); };
+
+                    T rho = T(0), jx = T(0), jy = T(0), jz = T(0);
+                    T fs[27];
+                    for (int k = 0; k < 27; ++k) {
+                        const int cx = LAT.cx[k], cy = LAT.cy[k], cz = LAT.cz[k];
+                        const int sgx = gx - cx, sgy = gy - cy, sgz = gz - cz;
+                        const bool xl = geo.outside(0, sgx) && sgx < 0, xh = geo.outside(0, sgx) && sgx >= 0;
+                        const bool yl = geo.outside(1, sgy) && sgy < 0, yh = geo.outside(1, sgy) && sgy >= 0;
+                        const bool zl = geo.outside(2, sgz) && sgz < 0, zh = geo.outside(2, sgz) && sgz >= 0;
+                        T val;
+                        if (xl || xh || yl || yh || zl || zh) {
+                            // domain-face rules in the reference's order (kernel_cumulant.jl:233-345)
+                            if (xl) {
+                                T noise = cp.turb > T(0)
+                                    ? (T)gradient_noise(gy + 1, gz + 1, prm.time_step_seed, 1234) * cp.turb * cp.u_in
+                                    : T(0);
+                                T u_inst = cp.u_in + noise;
+                                val = feq_auto(T(1), u_inst, T(0), T(0), w[k], (T)cx, (T)cy, (T)cz, cp.compressible);
+                            } else if (xh) {
+                                val = own_f(k);
+                            } else if (prm.zwall_bc != 0 && (zl || zh)) {
+                                val = own_f(LAT.opp[k]);   // zwall_bc 2: halfway bounce-back
+                            } else if (yl && prm.ymin_wall_bc == 1) {
+                                val = own_f(LAT.opp[k]);
+                            } else if (yl || yh) {
+                                val = own_f(LAT.mir_y[k]);
+                            } else {
+                                val = own_f(LAT.mir_z[k]);
+                            }
+                        } else {
+                            const int64_t src = geo.locate(b, x - cx, y - cy, z - cz);
+                            if (src < 0) {
+                                val = w[k];   // no neighbour on level 1: rest weight (kernel_cumulant.jl:389-390)
+                            } else if (g.solid && g.solid[src] && prm.bb_mode == 1) {
+                                val = own_f(LAT.opp[k]);   // halfway bounce-back: own population toward the wall
+                            } else {
+                                val = P::ld(fin[(int64_t)k * ncell + src], w[k]);
+                            }
+                        }
+                        fs[k] = val;
+                        rho += val;
+                        jx += val * (T)cx;
+                        jy += val * (T)cy;
+                        jz += val * (T)cz;
+                    }
+
+                    if (is_obs) {
+                        // reference solid cell: full-way reflection (kernel_cumulant.jl:405-417)
+                        if (vel_out) { vel_out[own] = T(0); vel_out[ncell + own] = T(0); vel_out[2 * ncell + own] = T(0); }
+                        if (rho_out) rho_out[own] = T(1);
+                        for (int k = 0; k < 27; ++k) fout[(int64_t)k * ncell + own] = P::sto(fs[LAT.opp[k]], w[k]);
+                        continue;
+                    }
+
+                    T nu_eddy = T(0);
+                    if (cp.c_wale != T(0) && vel_in) {
+                        // central differences of the previous step's velocity (physics_utils.jl:1981-2035, 2217-2233)
+                        T u0[3] = {vel_in[own], vel_in[ncell + own], vel_in[2 * ncell + own]};
+                        T un[6][3];
+                        const int dd[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+                        for (int q = 0; q < 6; ++q) {
+                            const int ngx = gx + dd[q][0], ngy = gy + dd[q][1], ngz = gz + dd[q][2];
+                            int64_t n = -1;
+                            if (!geo.outside(0, ngx) && !geo.outside(1, ngy) && !geo.outside(2, ngz))
+                                n = geo.locate(b, x + dd[q][0], y + dd[q][1], z + dd[q][2]);
+                            const bool use_own = n < 0 || (g.solid && g.solid[n]);
+                            for (int a = 0; a < 3; ++a) un[q][a] = use_own ? u0[a] : vel_in[a * ncell + n];
+                        }
+                        T gr[9];
+                        for (int a = 0; a < 3; ++a) {
+                            gr[3 * a + 0] = T(0.5) * (un[0][a] - un[1][a]);
+                            gr[3 * a + 1] = T(0.5) * (un[2][a] - un[3][a]);
+                            gr[3 * a + 2] = T(0.5) * (un[4][a] - un[5][a]);
+                        }
+                        nu_eddy = wale_nu(gr, cp.c_wale);
+                    }
+
+                    T out[27], rho_o, u_o[3];
+                    collide(fs, rho, jx, jy, jz, nu_eddy, cp, out, rho_o, u_o);
+                    if (vel_out) { vel_out[own] = u_o[0]; vel_out[ncell + own] = u_o[1]; vel_out[2 * ncell + own] = u_o[2]; }
+                    if (rho_out) rho_out[own] = rho_o;
+                    for (int k = 0; k < 27; ++k) fout[(int64_t)k * ncell + own] = P::sto(out[k], w[k]);
+                    // momentum exchange of the links this collision feeds: f_i*(x) leaves toward a solid
+                    // cell x + c_i and returns as f_opp(i)(x, t+1); F += (f_i* + f_opp(i)) c_i
+                    // (src/octree_q19/src/stream.jl:248-256), shifted by w_i (the w terms cancel over
+                    // a closed surface).  Halfway: f_opp(i)(x, t+1) = f_i*(x).  Reference reflection:
+                    // the solid cell returns f_i*(x) of the previous step.
+                    if (force && g.solid) {
+                        for (int i = 0; i < 27; ++i) {
+                            const int dgx = gx + LAT.cx[i], dgy = gy + LAT.cy[i], dgz = gz + LAT.cz[i];
+                            if (geo.outside(0, dgx) || geo.outside(1, dgy) || geo.outside(2, dgz)) continue;
+                            const int64_t dst = geo.locate(b, x + LAT.cx[i], y + LAT.cy[i], z + LAT.cz[i]);
+                            if (dst < 0 || !g.solid[dst]) continue;
+                            const double fi = (double)P::ld(P::sto(out[i], w[i]), w[i]) - (double)w[i];
+                            const double m = prm.bb_mode == 1 ? 2.0 * fi : fi + ((double)own_f(i) - (double)w[i]);
+                            fb[0] += m * LAT.cx[i]; fb[1] += m * LAT.cy[i]; fb[2] += m * LAT.cz[i];
+                        }
+                    }
+                }
+        fpart[3 * (size_t)b] = fb[0]; fpart[3 * (size_t)b + 1] = fb[1]; fpart[3 * (size_t)b + 2] = fb[2];
+    }
+    if (force) {
+        double F[3] = {0.0, 0.0, 0.0};
+        for (int b = 0; b < g.nb; ++b) for (int a = 0; a < 3; ++a) F[a] += fpart[3 * (size_t)b + a];
+        force[0] = F[0]; force[1] = F[1]; force[2] = F[2];
+    }
+}
+
+// init_eq! (main.jl:546-572) for per-cell (rho, u) fields
+template <class P>
+void init_impl(const OrGrid& g, const typename P::T* rho, const typename P::T* u, typename P::st* f) {
+    using T = typename P::T;
+    const Geo geo(g);
+    const int64_t ncell = geo.ncell;
+    const T cs2 = T(1) / T(3);
+    const T cs4 = cs2 * cs2;
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < ncell; ++c) {
+        T ux = u[c], uy = u[ncell + c], uz = u[2 * ncell + c], r = rho[c];
+        T usq = ux * ux + uy * uy + uz * uz;
+        T inv_cs2 = T(1) / cs2, inv_2cs4 = T(0.5) / cs4, inv_2cs2 = T(0.5) / cs2;
+        for (int k = 0; k < 27; ++k) {
+            T cu = (T)LAT.cx[k] * ux + (T)LAT.cy[k] * uy + (T)LAT.cz[k] * uz;
+            T feq = r * weight<T>(k) * (T(1) + inv_cs2 * cu + inv_2cs4 * cu * cu - inv_2cs2 * usq);
+            f[(int64_t)k * ncell + c] = P::sto(feq, weight<T>(k));
+        }
+    }
+}
+
+// FP64 mass and momentum over fluid cells inside the extent, from stored populations:
+// rho = 1 + sum (f_k - w_k), j = sum c_k (f_k - w_k)   (state.jl:333-355 sums in Float64)
+template <class P>
+void moments_impl(const OrGrid& g, const typename P::st* f, double* out4, double* rho_c, double* u_c) {
+    using T = typename P::T;
+    const Geo geo(g);
+    const int bs = g.bs;
+    double w[27];
+    for (int k = 0; k < 27; ++k) w[k] = (double)weight<T>(k);
+    std::vector<double> part((size_t)g.nb * 4, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < g.nb; ++b) {
+        double acc[4] = {0, 0, 0, 0};
+        for (int z = 0; z < bs; ++z) for (int y = 0; y < bs; ++y) for (int x = 0; x < bs; ++x) {
+            const int gx = g.bcoord[3 * b] * bs + x, gy = g.bcoord[3 * b + 1] * bs + y, gz = g.bcoord[3 * b + 2] * bs + z;
+            const int64_t c = (int64_t)b * geo.bs3 + ((int64_t)z * bs + y) * bs + x;
+            const bool valid = gx < g.n[0] && gy < g.n[1] && gz < g.n[2] && !(g.solid && g.solid[c]);
+            double dr = 0, jx = 0, jy = 0, jz = 0;
+            for (int k = 0; k < 27; ++k) {
+                const double s = (double)P::ld(f[(int64_t)k * geo.ncell + c], (T)w[k]) - w[k];
+                dr += s; jx += s * LAT.cx[k]; jy += s * LAT.cy[k]; jz += s * LAT.cz[k];
+            }
+            if (rho_c) rho_c[c] = valid ? 1.0 + dr : 0.0;
+            if (u_c) {
+                const double r = 1.0 + dr;
+                u_c[c] = valid ? jx / r : 0.0; u_c[geo.ncell + c] = valid ? jy / r : 0.0; u_c[2 * geo.ncell + c] = valid ? jz / r : 0.0;
+            }
+            if (valid) { acc[0] += 1.0 + dr; acc[1] += jx; acc[2] += jy; acc[3] += jz; }
+        }
+        for (int a = 0; a < 4; ++a) part[4 * (size_t)b + a] = acc[a];
+    }
+    double s[4] = {0, 0, 0, 0};
+    for (int b = 0; b < g.nb; ++b) for (int a = 0; a < 4; ++a) s[a] += part[4 * (size_t)b + a];
+    for (int a = 0; a < 4; ++a) out4[a] = s[a];
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_abi(void) { return 1; }
+
+void oracle_lattice(int32_t* cx, int32_t* cy, int32_t* cz, int32_t* opp, int32_t* mir_y, int32_t* mir_z, float* w32, double* w64) {
+    for (int k = 0; k < 27; ++k) {
+        cx[k] = LAT.cx[k]; cy[k] = LAT.cy[k]; cz[k] = LAT.cz[k];
+        opp[k] = LAT.opp[k]; mir_y[k] = LAT.mir_y[k]; mir_z[k] = LAT.mir_z[k];
+        w32[k] = weight<float>(k); w64[k] = weight<double>(k);
+    }
+}
+
+void oracle_f32_to_f16(const float* in, uint16_t* out, int64_t n) { for (int64_t i = 0; i < n; ++i) out[i] = f32_to_f16(in[i]); }
+void oracle_f16_to_f32(const uint16_t* in, float* out, int64_t n) { for (int64_t i = 0; i < n; ++i) out[i] = f16_to_f32(in[i]); }
+void oracle_f32_to_bf16(const float* in, uint16_t* out, int64_t n) { for (int64_t i = 0; i < n; ++i) out[i] = f32_to_bf16(in[i]); }
+void oracle_bf16_to_f32(const uint16_t* in, float* out, int64_t n) { for (int64_t i = 0; i < n; ++i) out[i] = bf16_to_f32(in[i]); }
+
+float oracle_gradient_noise(int32_t gx, int32_t gy, int32_t gz, int32_t seed) { return gradient_noise(gx, gy, gz, seed); }
+uint32_t oracle_gpu_hash(int32_t x) { return gpu_hash(x); }
+
+double oracle_feq_f64(double rho, double ux, double uy, double uz, int k, int compressible) {
+    return feq_auto(rho, ux, uy, uz, weight<double>(k), (double)LAT.cx[k], (double)LAT.cy[k], (double)LAT.cz[k], compressible);
+}
+float oracle_feq_f32(float rho, float ux, float uy, float uz, int k, int compressible) {
+    return feq_auto(rho, ux, uy, uz, weight<float>(k), (float)LAT.cx[k], (float)LAT.cy[k], (float)LAT.cz[k], compressible);
+}
+
+// three values through the forward (dir=+1) or backward (dir=-1) chimera transform
+void oracle_chimera_f64(double* abc, double u, int dir) {
+    if (dir > 0) chimera_fwd(abc[0], abc[1], abc[2], u); else chimera_bwd(abc[0], abc[1], abc[2], u);
+}
+
+// the fluid-cell collision of one cell from 27 pre-collision populations (gather-order sums)
+void oracle_collide_f64(const double* f, const OrParams* p, double nu_eddy, double* out, double* rho_u) {
+    double rho = 0, jx = 0, jy = 0, jz = 0;
+    for (int k = 0; k < 27; ++k) { rho += f[k]; jx += f[k] * LAT.cx[k]; jy += f[k] * LAT.cy[k]; jz += f[k] * LAT.cz[k]; }
+    Coll<double> c(*p);
+    double r, u[3];
+    collide(f, rho, jx, jy, jz, nu_eddy, c, out, r, u);
+    rho_u[0] = r; rho_u[1] = u[0]; rho_u[2] = u[1]; rho_u[3] = u[2];
+}
+void oracle_collide_f32(const float* f, const OrParams* p, float nu_eddy, float* out, float* rho_u) {
+    float rho = 0, jx = 0, jy = 0, jz = 0;
+    for (int k = 0; k < 27; ++k) { rho += f[k]; jx += f[k] * (float)LAT.cx[k]; jy += f[k] * (float)LAT.cy[k]; jz += f[k] * (float)LAT.cz[k]; }
+    Coll<float> c(*p);
+    float r, u[3];
+    collide(f, rho, jx, jy, jz, nu_eddy, c, out, r, u);
+    rho_u[0] = r; rho_u[1] = u[0]; rho_u[2] = u[1]; rho_u[3] = u[2];
+}
+double oracle_wale_nu_f64(const double* g, double c_wale) { return wale_nu(g, c_wale); }
+
+// storage codes: 0 f32, 1 f64, 2 f16 shifted, 3 bf16 shifted
+int oracle_step(int storage, const OrGrid* g, const OrParams* p, const void* fin, void* fout,
+                const void* vel_in, void* vel_out, void* rho_out, double* force) {
+    switch (storage) {
+        case 0: step_impl<StF32>(*g, *p, (const float*)fin, (float*)fout, (const float*)vel_in, (float*)vel_out, (float*)rho_out, force); return 0;
+        case 1: step_impl<StF64>(*g, *p, (const double*)fin, (double*)fout, (const double*)vel_in, (double*)vel_out, (double*)rho_out, force); return 0;
+        case 2: step_impl<StF16S>(*g, *p, (const uint16_t*)fin, (uint16_t*)fout, (const float*)vel_in, (float*)vel_out, (float*)rho_out, force); return 0;
+        case 3: step_impl<StBF16S>(*g, *p, (const uint16_t*)fin, (uint16_t*)fout, (const float*)vel_in, (float*)vel_out, (float*)rho_out, force); return 0;
+        default: return -1;
+    }
+}
+
+// rho [ncell] and u [3*ncell] in the working precision of the storage (double for f64, float otherwise)
+int oracle_init_eq(int storage, const OrGrid* g, const void* rho, const void* u, void* f) {
+    switch (storage) {
+        case 0: init_impl<StF32>(*g, (const float*)rho, (const float*)u, (float*)f); return 0;
+        case 1: init_impl<StF64>(*g, (const double*)rho, (const double*)u, (double*)f); return 0;
+        case 2: init_impl<StF16S>(*g, (const float*)rho, (const float*)u, (uint16_t*)f); return 0;
+        case 3: init_impl<StBF16S>(*g, (const float*)rho, (const float*)u, (uint16_t*)f); return 0;
+        default: return -1;
+    }
+}
+
+// out4 = (mass, jx, jy, jz) in FP64; rho_c [ncell], u_c [3*ncell] optional per-cell FP64 moments
+int oracle_moments(int storage, const OrGrid* g, const void* f, double* out4, double* rho_c, double* u_c) {
+    switch (storage) {
+        case 0: moments_impl<StF32>(*g, (const float*)f, out4, rho_c, u_c); return 0;
+        case 1: moments_impl<StF64>(*g, (const double*)f, out4, rho_c, u_c); return 0;
+        case 2: moments_impl<StF16S>(*g, (const uint16_t*)f, out4, rho_c, u_c); return 0;
+        case 3: moments_impl<StBF16S>(*g, (const uint16_t*)f, out4, rho_c, u_c); return 0;
+        default: return -1;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,239 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboal_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+as the checker; the product (openaerolab_alpha_01_amd) never imports it.
+
+Parity status of each oracle function and what pins it (tests/test_oracle_*.py):
+
+  lattice tables        pinned: isotropy identities (sum w = 1, sum w c c = I/3, 4th-order isotropy),
+                        opp / mirror involutions, index formula of lattice.jl:16-19
+  f16 / bf16 rounding   pinned: exhaustive over all 2^16 codes (f16 -> f32 -> f16 round trip) and
+                        against numpy's float16 cast (round to nearest even) on random floats
+  equilibria            pinned: exact moments rho, rho u, rho (u u + I/3) (2nd order); 3rd/4th-order
+                        Hermite moments of the O(u^4) form
+  inlet noise hash      pinned: bijectivity of the murmur3 finaliser on a large sample, range [-1, 1);
+                        fmix32(0) = 0.  The exact hash values are parity unpinned (no reference vector).
+  chimera transforms    pinned: forward equals brute-force central moments sum f (c-u)^a (FP64);
+                        backward(forward(f)) = f
+  collision (one cell)  pinned: mass and momentum conservation (no force); rest state fixed point;
+                        omega1 = 1 with omega_bulk = omega_3 = omega_4 = 1 gives the cumulant
+                        equilibrium (2nd-order cumulants I/3, higher cumulants zero) checked in moment
+                        space by brute force; relaxation of the deviatoric 2nd-order cumulants by
+                        (1 - omega1) checked by brute-force cumulants
+  step (periodic)       pinned: shear-wave decay nu = (tau - 1/2)/3 (1 %), Taylor-Green KE decay
+                        2nd-order convergence, mass/momentum conservation, rest state
+  halfway bounce-back   pinned: Poiseuille channel, momentum exchange = body force x fluid volume,
+  + momentum exchange   velocity profile vs the parabola
+  domain-face rules     pinned: uniform inflow through inlet/outlet/free-slip stays uniform; mirror
+                        rules by symmetry of a mirrored run
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboal_oracle.so"
+
+STORAGE = {"f32": 0, "f64": 1, "f16s": 2, "bf16s": 3}
+STORE_DTYPE = {"f32": np.float32, "f64": np.float64, "f16s": np.uint16, "bf16s": np.uint16}
+WORK_DTYPE = {"f32": np.float32, "f64": np.float64, "f16s": np.float32, "bf16s": np.float32}
+
+
+class OrParams(C.Structure):
+    _fields_ = [
+        ("tau", C.c_double), ("omega_bulk", C.c_double), ("omega3", C.c_double), ("omega4", C.c_double),
+        ("lambda_", C.c_double), ("c_wale", C.c_double), ("nu_sgs_bg", C.c_double), ("amr_sgs_scale", C.c_double),
+        ("u_inlet", C.c_double), ("inlet_turb", C.c_double), ("body_force", C.c_double * 3),
+        ("adaptive_omega4", C.c_int32), ("limiter", C.c_int32), ("compressible", C.c_int32),
+        ("ymin_wall_bc", C.c_int32), ("zwall_bc", C.c_int32), ("bb_mode", C.c_int32), ("time_step_seed", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+
+class OrGrid(C.Structure):
+    _fields_ = [
+        ("bs", C.c_int32), ("nb", C.c_int32), ("bcoord", C.POINTER(C.c_int32)), ("nbr", C.POINTER(C.c_int32)),
+        ("n", C.c_int32 * 3), ("periodic", C.c_int32 * 3), ("solid", C.POINTER(C.c_uint8)),
+    ]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB.exists():
+        subprocess.run(["make", "-C", str(HERE), "liboal_oracle.so"], check=True, stdout=subprocess.DEVNULL)
+    lib = C.CDLL(str(LIB))
+    vp = C.c_void_p
+    lib.oracle_step.argtypes = [C.c_int, C.POINTER(OrGrid), C.POINTER(OrParams), vp, vp, vp, vp, vp, C.POINTER(C.c_double)]
+    lib.oracle_step.restype = C.c_int
+    lib.oracle_init_eq.argtypes = [C.c_int, C.POINTER(OrGrid), vp, vp, vp]
+    lib.oracle_moments.argtypes = [C.c_int, C.POINTER(OrGrid), vp, C.POINTER(C.c_double), vp, vp]
+    lib.oracle_lattice.argtypes = [vp] * 8
+    lib.oracle_gradient_noise.argtypes = [C.c_int32] * 4
+    lib.oracle_gradient_noise.restype = C.c_float
+    lib.oracle_gpu_hash.argtypes = [C.c_int32]
+    lib.oracle_gpu_hash.restype = C.c_uint32
+    lib.oracle_feq_f64.argtypes = [C.c_double] * 4 + [C.c_int, C.c_int]
+    lib.oracle_feq_f64.restype = C.c_double
+    lib.oracle_feq_f32.argtypes = [C.c_float] * 4 + [C.c_int, C.c_int]
+    lib.oracle_feq_f32.restype = C.c_float
+    lib.oracle_chimera_f64.argtypes = [vp, C.c_double, C.c_int]
+    lib.oracle_collide_f64.argtypes = [vp, C.POINTER(OrParams), C.c_double, vp, vp]
+    lib.oracle_collide_f32.argtypes = [vp, C.POINTER(OrParams), C.c_float, vp, vp]
+    lib.oracle_wale_nu_f64.argtypes = [vp, C.c_double]
+    lib.oracle_wale_nu_f64.restype = C.c_double
+    for nm in ("oracle_f32_to_f16", "oracle_f16_to_f32", "oracle_f32_to_bf16", "oracle_bf16_to_f32"):
+        getattr(lib, nm).argtypes = [vp, vp, C.c_int64]
+    _lib = lib
+    return lib
+
+
+def f32(v: float) -> float:
+    """A parameter as the library receives it (Float32 kernel arguments of the reference)."""
+    return float(np.float32(v))
+
+
+def make_params(tau, *, omega_bulk=1.0, omega_3=1.0, omega_4=1.0, lambda_=1.0 / 6.0, adaptive_omega4=1, limiter=1,
+                compressible=0, c_wale=0.0, nu_sgs_background=0.0, amr_sgs_scale=1.0, u_inlet=0.0,
+                inlet_turbulence=0.0, body_force=(0.0, 0.0, 0.0), ymin_wall_bc=0, zwall_bc=0, bb_mode=1,
+                time_step=0) -> OrParams:
+    p = OrParams()
+    p.tau, p.omega_bulk, p.omega3, p.omega4 = f32(tau), f32(omega_bulk), f32(omega_3), f32(omega_4)
+    p.lambda_, p.c_wale, p.nu_sgs_bg, p.amr_sgs_scale = f32(lambda_), f32(c_wale), f32(nu_sgs_background), f32(amr_sgs_scale)
+    p.u_inlet, p.inlet_turb = f32(u_inlet), f32(inlet_turbulence)
+    p.body_force[:] = [f32(v) for v in body_force]
+    p.adaptive_omega4, p.limiter, p.compressible = adaptive_omega4, limiter, compressible
+    p.ymin_wall_bc, p.zwall_bc, p.bb_mode, p.time_step_seed = ymin_wall_bc, zwall_bc, bb_mode, time_step
+    return p
+
+
+class Grid:
+    """The oracle's view of a block grid (any object with n, periodic, coords, neighbor, solid, bs)."""
+
+    def __init__(self, g):
+        self.src = g
+        self.bs = int(g.bs)
+        self.nb = int(g.coords.shape[0])
+        self.ncell = self.nb * self.bs ** 3
+        self.bcoord = np.ascontiguousarray(np.asarray(g.coords, dtype=np.int32) - 1)
+        # dir-major 0-based table: nbr[d*nb + b]
+        self.nbr = np.ascontiguousarray(np.asarray(g.neighbor, dtype=np.int32).T - 1)
+        self.solid = None if g.solid is None else np.ascontiguousarray(g.solid, dtype=np.uint8)
+        s = OrGrid()
+        s.bs, s.nb = self.bs, self.nb
+        s.bcoord = self.bcoord.ctypes.data_as(C.POINTER(C.c_int32))
+        s.nbr = self.nbr.ctypes.data_as(C.POINTER(C.c_int32))
+        s.n[:] = [int(v) for v in g.n]
+        s.periodic[:] = [1 if p else 0 for p in g.periodic]
+        s.solid = None if self.solid is None else self.solid.ctypes.data_as(C.POINTER(C.c_uint8))
+        self.c = s
+
+
+def init_eq(storage: str, grid: Grid, rho: np.ndarray, u: np.ndarray) -> np.ndarray:
+    lib = load()
+    wt = WORK_DTYPE[storage]
+    r = np.ascontiguousarray(rho, dtype=wt)
+    v = np.ascontiguousarray(u, dtype=wt)
+    f = np.zeros((27, grid.ncell), dtype=STORE_DTYPE[storage])
+    lib.oracle_init_eq(STORAGE[storage], C.byref(grid.c), r.ctypes.data, v.ctypes.data, f.ctypes.data)
+    return f
+
+
+def step(storage: str, grid: Grid, params: OrParams, fin: np.ndarray, vel_in: np.ndarray | None = None,
+         want_macro: bool = False, want_force: bool = False, out: np.ndarray | None = None):
+    """One step; returns (fout, force or None, (rho, vel) or None).  fout starts as a copy of fin so
+    that cells the step does not write (solid, beyond the extent) keep their values; a caller-provided
+    `out` is used as is (the timing loop of the CPU baseline alternates two buffers)."""
+    lib = load()
+    fout = fin.copy() if out is None else out
+    wt = WORK_DTYPE[storage]
+    rho = np.zeros(grid.ncell, dtype=wt) if want_macro else None
+    vel = np.zeros((3, grid.ncell), dtype=wt) if want_macro else None
+    vin = None if vel_in is None else np.ascontiguousarray(vel_in, dtype=wt)
+    F = (C.c_double * 3)()
+    rc = lib.oracle_step(STORAGE[storage], C.byref(grid.c), C.byref(params), fin.ctypes.data, fout.ctypes.data,
+                         None if vin is None else vin.ctypes.data, None if vel is None else vel.ctypes.data,
+                         None if rho is None else rho.ctypes.data, F if want_force else None)
+    assert rc == 0
+    return fout, (np.array(F[:]) if want_force else None), ((rho, vel) if want_macro else None)
+
+
+def moments(storage: str, grid: Grid, f: np.ndarray):
+    """(mass, jx, jy, jz) in FP64 and per-cell rho [ncell], u [3, ncell] (FP64)."""
+    lib = load()
+    out = (C.c_double * 4)()
+    rho = np.zeros(grid.ncell)
+    u = np.zeros((3, grid.ncell))
+    lib.oracle_moments(STORAGE[storage], C.byref(grid.c), f.ctypes.data, out, rho.ctypes.data, u.ctypes.data)
+    return np.array(out[:]), rho, u
+
+
+def to_unshifted(storage: str, f: np.ndarray) -> np.ndarray:
+    """Stored populations -> unshifted float64 [27, ncell]."""
+    if storage in ("f32", "f64"):
+        return f.astype(np.float64)
+    lib = load()
+    flat = np.ascontiguousarray(f).ravel()
+    out = np.empty(flat.size, dtype=np.float32)
+    fn = lib.oracle_f16_to_f32 if storage == "f16s" else lib.oracle_bf16_to_f32
+    fn(flat.ctypes.data, out.ctypes.data, flat.size)
+    w = lattice()[6]
+    return (out.reshape(f.shape) + w.astype(np.float32)[:, None]).astype(np.float64)
+
+
+def from_unshifted(storage: str, f: np.ndarray) -> np.ndarray:
+    """Unshifted populations [27, ncell] -> the storage form (rounded to nearest even)."""
+    if storage == "f32":
+        return np.ascontiguousarray(f, dtype=np.float32)
+    if storage == "f64":
+        return np.ascontiguousarray(f, dtype=np.float64)
+    lib = load()
+    w = lattice()[6]
+    s = np.ascontiguousarray((f.astype(np.float32) - w.astype(np.float32)[:, None]).astype(np.float32))
+    out = np.empty(s.shape, dtype=np.uint16)
+    fn = lib.oracle_f32_to_f16 if storage == "f16s" else lib.oracle_f32_to_bf16
+    fn(s.ctypes.data, out.ctypes.data, s.size)
+    return out
+
+
+def lattice():
+    """(cx, cy, cz, opp, mir_y, mir_z, w32, w64) of the oracle."""
+    lib = load()
+    a = [np.zeros(27, dtype=np.int32) for _ in range(6)]
+    w32 = np.zeros(27, dtype=np.float32)
+    w64 = np.zeros(27, dtype=np.float64)
+    lib.oracle_lattice(*[x.ctypes.data for x in a], w32.ctypes.data, w64.ctypes.data)
+    return (*a, w32, w64)
+
+
+def collide_cell_f64(f: np.ndarray, params: OrParams, nu_eddy: float = 0.0):
+    lib = load()
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(27)
+    ru = np.zeros(4)
+    lib.oracle_collide_f64(f.ctypes.data, C.byref(params), nu_eddy, out.ctypes.data, ru.ctypes.data)
+    return out, ru
+
+
+def collide_cell_f32(f: np.ndarray, params: OrParams, nu_eddy: float = 0.0):
+    lib = load()
+    f = np.ascontiguousarray(f, dtype=np.float32)
+    out = np.zeros(27, dtype=np.float32)
+    ru = np.zeros(4, dtype=np.float32)
+    lib.oracle_collide_f32(f.ctypes.data, C.byref(params), nu_eddy, out.ctypes.data, ru.ctypes.data)
+    return out, ru
+
+
+def chimera_f64(abc, u: float, direction: int) -> np.ndarray:
+    lib = load()
+    a = np.ascontiguousarray(abc, dtype=np.float64).copy()
+    lib.oracle_chimera_f64(a.ctypes.data, u, direction)
+    return a

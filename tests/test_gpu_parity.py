@@ -1,0 +1,255 @@
+"""Parity of liboal (HIP, gfx950) against the CPU oracle, through the C ABI.
+
+Both sides start from the same init_eq! state (same FP64 fields, rounded to the working precision the
+same way) and run the same steps.  The oracle and the kernels evaluate the reference arithmetic in the
+same operation order without fused multiply-adds, so the populations must agree bit for bit in every
+storage form; FP64 reductions (force, mass) differ only in summation order: relative 1e-12.
+"""
+import numpy as np
+import pytest
+
+from openaerolab_alpha_01_amd import solver
+from openaerolab_alpha_01_amd.geometry import sphere_mask, uniform_grid
+
+pytestmark = pytest.mark.gpu
+
+NP = {"f32": np.float32, "f64": np.float64, "f16s": np.float32, "bf16s": np.float32}
+
+
+def fields(grid, seed=0, u0=(0.0, 0.0, 0.0), amp=0.02):
+    """Seeded smooth-plus-random initial rho and u (FP64) in block layout."""
+    rng = np.random.default_rng(seed)
+    x = grid.cell_coords().astype(np.float64) + 0.5
+    n = np.asarray(grid.n, dtype=np.float64)
+    ph = 2 * np.pi * x / n
+    u = np.stack([u0[0] + amp * np.sin(ph[:, 1]) * np.cos(ph[:, 2]),
+                  u0[1] + 0.7 * amp * np.cos(ph[:, 0]) * np.sin(ph[:, 2]),
+                  u0[2] + 0.5 * amp * np.sin(ph[:, 0] + ph[:, 1])])
+    u += 1e-3 * rng.uniform(-1, 1, u.shape)
+    rho = 1.0 + 0.01 * np.sin(ph[:, 0]) * np.cos(ph[:, 1]) + 1e-4 * rng.uniform(-1, 1, x.shape[0])
+    ins = grid.inside()
+    rho[~ins] = 1.0
+    u[:, ~ins] = 0.0
+    return rho, u
+
+
+def oracle_run(oracle, grid, storage, steps, okw, rho, u, force_steps=()):
+    G = oracle.Grid(grid)
+    f = oracle.init_eq(storage, G, rho, u)
+    vel = u.astype(NP[storage]) if okw.get("c_wale", 0) else None
+    forces = []
+    for t in range(steps):
+        p = oracle.make_params(time_step=t, **okw)
+        f, F, mac = oracle.step(storage, G, p, f, vel_in=vel, want_macro=vel is not None, want_force=t in force_steps)
+        if vel is not None:
+            vel = mac[1]
+        if t in force_steps:
+            forces.append(F)
+    return f, forces, G
+
+
+def gpu_plan(grid, storage, okw, stream="ab"):
+    params = solver.StepParams(omega_bulk=okw.get("omega_bulk", 1.0), omega_3=okw.get("omega_3", 1.0),
+                               omega_4=okw.get("omega_4", 1.0), lambda_=okw.get("lambda_", 1 / 6),
+                               adaptive_omega4=okw.get("adaptive_omega4", 1), limiter=okw.get("limiter", 1),
+                               compressible=okw.get("compressible", 0), c_wale=okw.get("c_wale", 0.0),
+                               nu_sgs_background=okw.get("nu_sgs_background", 0.0),
+                               inlet_turbulence=okw.get("inlet_turbulence", 0.0))
+    dom = solver.Domain(extent=grid.n, tau=okw["tau"], periodic=grid.periodic, u_inlet=okw.get("u_inlet", 0.0),
+                        body_force=tuple(okw.get("body_force", (0.0, 0.0, 0.0))),
+                        ymin_wall_bc=okw.get("ymin_wall_bc", 0), zwall_bc=okw.get("zwall_bc", 0))
+    bb = {1: "halfway", 0: "reference"}[okw.get("bb_mode", 1)]
+    return solver.Plan(solver.Level(grid, [dom]), storage=storage, stream=stream, bb_mode=bb, params=params)
+
+
+def gpu_run(grid, storage, steps, okw, rho, u, stream="ab", force_steps=()):
+    plan = gpu_plan(grid, storage, okw, stream)
+    plan.init_equilibrium(rho, u)
+    forces = []
+    for t in range(steps):
+        plan.level_step(0, t, force=t in force_steps)
+        if t in force_steps:
+            forces.append(plan.force()[0].copy())
+    return plan, forces
+
+
+def compare(oracle, plan, storage, f_or, grid):
+    """bit-exact comparison of all stored cells (populations in the working precision)"""
+    if storage == "f32":
+        got = plan.download_f(dtype=np.float32)
+        want = f_or
+    else:
+        got = plan.download_f(dtype=np.float64)
+        want = oracle.to_unshifted(storage, f_or)
+    bad = np.argwhere(got != want)
+    assert bad.size == 0, (
+        f"{len(bad)} mismatches, first {bad[:3].tolist()}, got {got[tuple(bad[0])]!r} want {want[tuple(bad[0])]!r}, "
+        f"max abs diff {np.abs(got.astype(np.float64) - want).max():.3e}")
+
+
+CASES = {
+    "periodic": dict(n=(32, 32, 32), periodic=(True, True, True), solid=None,
+                     okw=dict(tau=0.52)),
+    "channel_sphere": dict(n=(48, 32, 32), periodic=(False, False, False), solid=((16, 16, 16), 6.0),
+                           okw=dict(tau=0.53, u_inlet=0.05)),
+    "partial_blocks": dict(n=(40, 24, 20), periodic=(False, False, False), solid=((14, 12, 9), 4.0),
+                           okw=dict(tau=0.56, u_inlet=0.04)),
+}
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64", "f16s", "bf16s"])
+@pytest.mark.parametrize("case", list(CASES))
+def test_step_parity(oracle, storage, case):
+    cfg = CASES[case]
+    grid = uniform_grid(cfg["n"], periodic=cfg["periodic"])
+    if cfg["solid"]:
+        grid.solid = sphere_mask(grid, *cfg["solid"])
+    okw = dict(cfg["okw"])
+    rho, u = fields(grid, seed=1, u0=(okw.get("u_inlet", 0.0), 0, 0))
+    steps = 6
+    fst = (2, 5) if grid.solid is not None else ()
+    f_or, F_or, _ = oracle_run(oracle, grid, storage, steps, okw, rho, u, force_steps=fst)
+    plan, F_gpu = gpu_run(grid, storage, steps, okw, rho, u, force_steps=fst)
+    compare(oracle, plan, storage, f_or, grid)
+    for a, b in zip(F_gpu, F_or):
+        np.testing.assert_allclose(a, b, rtol=1e-11, atol=1e-14)
+    plan.close()
+
+
+VARIANTS = {
+    "reference_reflection": dict(bb_mode=0),
+    "no_limiter_fixed_omega4": dict(limiter=0, adaptive_omega4=0, omega_bulk=1.2, omega_3=0.9, omega_4=1.1),
+    "positivity_limiter": dict(limiter=2, tau=0.5005, omega_3=1.9, omega_4=1.9),
+    "compressible_inlet": dict(compressible=1, u_inlet=0.08),
+    "inlet_noise": dict(inlet_turbulence=0.01),
+    "walls": dict(ymin_wall_bc=1, zwall_bc=2),
+    "wale": dict(c_wale=0.2, nu_sgs_background=1e-4),
+    "body_force": dict(body_force=(1e-5, -2e-6, 3e-6)),
+}
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+@pytest.mark.parametrize("storage", ["f32", "f16s"])
+def test_variant_parity(oracle, variant, storage):
+    grid = uniform_grid((48, 32, 32))
+    grid.solid = sphere_mask(grid, (18, 15, 17), 5.5)
+    okw = dict(tau=0.53, u_inlet=0.05)
+    okw.update(VARIANTS[variant])
+    rho, u = fields(grid, seed=3, u0=(0.05, 0, 0), amp=0.03)
+    steps = 5
+    fst = (4,)
+    f_or, F_or, _ = oracle_run(oracle, grid, storage, steps, okw, rho, u, force_steps=fst)
+    plan, F_gpu = gpu_run(grid, storage, steps, okw, rho, u, force_steps=fst)
+    compare(oracle, plan, storage, f_or, grid)
+    np.testing.assert_allclose(F_gpu[0], F_or[0], rtol=1e-11, atol=1e-14)
+    plan.close()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16s", "f64"])
+def test_aa_pattern_equals_ab(storage):
+    """The in-place AA pattern reproduces the pull/AB step bit for bit (after every even step),
+    boundary rules, halfway bounce-back and momentum exchange included."""
+    grid = uniform_grid((48, 32, 40))
+    grid.solid = sphere_mask(grid, (20, 16, 19), 6.0)
+    okw = dict(tau=0.52, u_inlet=0.05, ymin_wall_bc=1, inlet_turbulence=0.01)
+    rho, u = fields(grid, seed=5, u0=(0.05, 0, 0))
+    ab = gpu_plan(grid, storage, okw, "ab")
+    aa = gpu_plan(grid, storage, okw, "aa")
+    ab.init_equilibrium(rho, u)
+    aa.init_equilibrium(rho, u)
+    for t in range(8):
+        want_f = t in (3, 6, 7)
+        ab.level_step(0, t, force=want_f)
+        aa.level_step(0, t, force=want_f)
+        if want_f:
+            np.testing.assert_allclose(aa.force(), ab.force(), rtol=1e-11, atol=1e-14)
+        if t % 2 == 1:
+            dt = np.float32 if storage == "f32" else np.float64
+            assert np.array_equal(aa.download_f(dtype=dt), ab.download_f(dtype=dt)), f"step {t}"
+            np.testing.assert_allclose(aa.mass(), ab.mass(), rtol=1e-13)
+    ab.close()
+    aa.close()
+
+
+def test_mass_macro_guard(oracle):
+    grid = uniform_grid((32, 32, 48))
+    grid.solid = sphere_mask(grid, (16, 16, 20), 5.0)
+    okw = dict(tau=0.55, u_inlet=0.03)
+    rho, u = fields(grid, seed=7, u0=(0.03, 0, 0))
+    f_or, _, G = oracle_run(oracle, grid, "f32", 3, okw, rho, u)
+    plan, _ = gpu_run(grid, "f32", 3, okw, rho, u)
+    m_or, r_or, u_or = oracle.moments("f32", G, f_or)
+    np.testing.assert_allclose(plan.mass()[0], m_or, rtol=1e-12, atol=1e-12)
+    r_g, u_g = plan.macro()
+    fl = (grid.solid == 0)
+    np.testing.assert_allclose(r_g[fl], r_or[fl], rtol=1e-14)
+    np.testing.assert_allclose(u_g[:, fl], u_or[:, fl], rtol=1e-12, atol=1e-16)
+    assert plan.guard(0.2, 0.5) == 0
+    # a NaN population uploaded from the host trips the guard
+    f = plan.download_f(dtype=np.float64)
+    f[5, 1000] = np.nan
+    f[7, 2000] = 5.0
+    plan.upload_f(f)
+    from openaerolab_alpha_01_amd import _lib
+    with pytest.raises(_lib.OalError) as e:
+        plan.guard(0.2, 0.5)
+    assert e.value.code == _lib.OAL_ERR_NONFINITE
+    plan.close()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f64", "f16s", "bf16s"])
+def test_upload_download_roundtrip(oracle, storage):
+    grid = uniform_grid((32, 16, 16))
+    rho, u = fields(grid, seed=11)
+    G = oracle.Grid(grid)
+    f_or = oracle.init_eq(storage, G, rho, u)
+    want = oracle.to_unshifted(storage, f_or)
+    for stream in ("ab", "aa"):
+        plan = gpu_plan(grid, storage, dict(tau=0.6), stream)
+        plan.upload_f(want)
+        got = plan.download_f(dtype=np.float64)
+        assert np.array_equal(got, want)
+        plan.init_equilibrium(rho, u)
+        assert np.array_equal(plan.download_f(dtype=np.float64), want)
+        plan.close()
+
+
+def test_many_domains_in_one_level(oracle):
+    """Independent boxes in one block list and one launch equal their separate runs."""
+    from openaerolab_alpha_01_amd.geometry import BlockGrid
+    specs = [((48, 32, 32), 0.05, 0.53, (18, 16, 16), 5.0), ((40, 24, 20), 0.03, 0.56, (15, 12, 10), 4.0),
+             ((32, 48, 16), 0.07, 0.52, (12, 20, 8), 3.5)]
+    grids, doms, coords, nbrs, solids, bdom = [], [], [], [], [], []
+    off = 0
+    for i, (n, uin, tau, c, r) in enumerate(specs):
+        g = uniform_grid(n)
+        g.solid = sphere_mask(g, c, r)
+        grids.append(g)
+        doms.append(solver.Domain(extent=n, tau=tau, u_inlet=uin))
+        coords.append(g.coords)
+        nb = g.neighbor.copy()
+        nb[nb > 0] += off
+        nbrs.append(nb)
+        solids.append(g.solid)
+        bdom.append(np.full(g.n_blocks, i, dtype=np.int32))
+        off += g.n_blocks
+    big = BlockGrid(n=(0, 0, 0), periodic=(False,) * 3, coords=np.concatenate(coords), neighbor=np.concatenate(nbrs),
+                    solid=np.concatenate(solids))
+    params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
+    plan = solver.Plan(solver.Level(big, doms, block_domain=np.concatenate(bdom)), storage="f16s", params=params)
+    rhos, us = zip(*[fields(g, seed=20 + i, u0=(specs[i][1], 0, 0)) for i, g in enumerate(grids)])
+    plan.init_equilibrium(np.concatenate(rhos), np.concatenate(us, axis=1))
+    steps = 4
+    for t in range(steps):
+        plan.level_step(0, t, force=t == steps - 1)
+    got = plan.download_f(dtype=np.float64)
+    F = plan.force()
+    o = 0
+    for i, g in enumerate(grids):
+        okw = dict(tau=specs[i][2], u_inlet=specs[i][1])
+        f_or, F_or, _ = oracle_run(oracle, g, "f16s", steps, okw, rhos[i], us[i], force_steps=(steps - 1,))
+        want = oracle.to_unshifted("f16s", f_or)
+        assert np.array_equal(got[:, o:o + g.n_cells], want), f"domain {i}"
+        np.testing.assert_allclose(F[i], F_or[0], rtol=1e-11, atol=1e-14)
+        o += g.n_cells
+    plan.close()

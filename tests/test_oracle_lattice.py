@@ -1,0 +1,126 @@
+"""Pins of the oracle's lattice tables, 16-bit storage conversions, equilibria and inlet hash.
+
+The checks come from the mathematics of the D3Q27 set and IEEE rounding, not from the oracle's own
+formulas: isotropy moments, involutions, brute-force moment sums, exhaustive conversion round trips.
+"""
+import numpy as np
+import pytest
+
+from openaerolab_alpha_01_amd.geometry import lattice_d3q27
+
+
+def test_lattice_order_and_tables(oracle):
+    cx, cy, cz, opp, my, mz, w32, w64 = oracle.lattice()
+    # reference order: for z in -1:1, y in -1:1, x in -1:1 (lattice.jl:16-19)
+    k = 0
+    for z in (-1, 0, 1):
+        for y in (-1, 0, 1):
+            for x in (-1, 0, 1):
+                assert (cx[k], cy[k], cz[k]) == (x, y, z)
+                k += 1
+    c = np.stack([cx, cy, cz], 1)
+    assert np.array_equal(c[opp], -c)
+    assert np.array_equal(opp[opp], np.arange(27))
+    assert np.array_equal(c[my], c * [1, -1, 1])
+    assert np.array_equal(c[mz], c * [1, 1, -1])
+    assert np.array_equal(my[my], np.arange(27)) and np.array_equal(mz[mz], np.arange(27))
+    # the host-side table of the package agrees
+    hc, hw, hopp, hmy, hmz = lattice_d3q27()
+    assert np.array_equal(hc, c) and np.array_equal(hopp, opp) and np.array_equal(hmy, my) and np.array_equal(hmz, mz)
+    np.testing.assert_allclose(hw, w64, rtol=0, atol=1e-16)
+
+
+def test_lattice_isotropy(oracle):
+    cx, cy, cz, _, _, _, w32, w64 = oracle.lattice()
+    c = np.stack([cx, cy, cz], 1).astype(float)
+    assert abs(w64.sum() - 1) < 1e-15
+    assert np.allclose(w64 @ c, 0)
+    assert np.allclose(np.einsum("k,ka,kb->ab", w64, c, c), np.eye(3) / 3, atol=1e-15)
+    assert np.allclose(np.einsum("k,ka,kb,kc->abc", w64, c, c, c), 0, atol=1e-15)
+    d = np.eye(3)
+    iso4 = (np.einsum("ab,cd->abcd", d, d) + np.einsum("ac,bd->abcd", d, d) + np.einsum("ad,bc->abcd", d, d)) / 9
+    assert np.allclose(np.einsum("k,ka,kb,kc,kd->abcd", w64, c, c, c, c), iso4, atol=1e-15)
+    # float weights are the correctly rounded float quotients (8f0/27f0 ... of timestep.jl:2013)
+    assert np.array_equal(w32, w64.astype(np.float32))
+
+
+def test_f16_conversion_exhaustive(oracle):
+    lib = oracle.load()
+    codes = np.arange(65536, dtype=np.uint16)
+    f = np.empty(65536, dtype=np.float32)
+    lib.oracle_f16_to_f32(codes.ctypes.data, f.ctypes.data, 65536)
+    ref = codes.view(np.float16).astype(np.float32)
+    fin = np.isfinite(ref)
+    assert np.array_equal(f[fin], ref[fin])
+    assert np.all(np.isnan(f[np.isnan(ref)])) and np.array_equal(f[np.isinf(ref)], ref[np.isinf(ref)])
+    back = np.empty(65536, dtype=np.uint16)
+    lib.oracle_f32_to_f16(f.ctypes.data, back.ctypes.data, 65536)
+    assert np.array_equal(back[fin], codes[fin])
+
+
+@pytest.mark.parametrize("scale", [1e-8, 1e-5, 1e-3, 1e-1, 10.0, 1e5])
+def test_f16_rounding_matches_numpy(oracle, scale):
+    lib = oracle.load()
+    rng = np.random.default_rng(7)
+    x = (rng.standard_normal(200000) * scale).astype(np.float32)
+    # exact ties: halfway values between two float16 neighbours
+    h = rng.integers(0, 0x7bff, 1000).astype(np.uint16)
+    lo = h.view(np.float16).astype(np.float32)
+    hi = (h + 1).view(np.float16).astype(np.float32)
+    x = np.concatenate([x, ((lo.astype(np.float64) + hi) / 2).astype(np.float32)])
+    out = np.empty(x.size, dtype=np.uint16)
+    lib.oracle_f32_to_f16(x.ctypes.data, out.ctypes.data, x.size)
+    assert np.array_equal(out, x.astype(np.float16).view(np.uint16))
+
+
+def test_bf16_rounding(oracle):
+    lib = oracle.load()
+    rng = np.random.default_rng(8)
+    x = (rng.standard_normal(100000) * 1e-2).astype(np.float32)
+    out = np.empty(x.size, dtype=np.uint16)
+    lib.oracle_f32_to_bf16(x.ctypes.data, out.ctypes.data, x.size)
+    back = np.empty(x.size, dtype=np.float32)
+    lib.oracle_bf16_to_f32(out.ctypes.data, back.ctypes.data, x.size)
+    # nearest: no other bf16 value is closer; ties resolved to an even mantissa
+    bits = x.view(np.uint32)
+    down = (bits & 0xFFFF0000).view(np.float32)
+    up = ((bits & 0xFFFF0000) + 0x10000).view(np.float32)
+    err = np.abs(back.astype(np.float64) - x)
+    best = np.minimum(np.abs(down.astype(np.float64) - x), np.abs(up.astype(np.float64) - x))
+    assert np.all(err <= best)
+    tie = (bits & 0xFFFF) == 0x8000
+    assert np.all((out[tie] & 1) == 0)
+
+
+@pytest.mark.parametrize("u", [(0.0, 0.0, 0.0), (0.05, 0.0, 0.0), (0.03, -0.02, 0.04), (0.1, 0.07, -0.05)])
+def test_equilibrium_moments(oracle, u):
+    lib = oracle.load()
+    cx, cy, cz, *_ = oracle.lattice()
+    c = np.stack([cx, cy, cz], 1).astype(float)
+    rho = 1.013
+    feq = np.array([lib.oracle_feq_f64(rho, *u, k, 0) for k in range(27)])
+    u = np.array(u)
+    assert abs(feq.sum() - rho) < 1e-14
+    np.testing.assert_allclose(feq @ c, rho * u, atol=1e-15)
+    np.testing.assert_allclose(np.einsum("k,ka,kb->ab", feq, c, c), rho * (np.outer(u, u) + np.eye(3) / 3), atol=1e-15)
+    # the O(u^4) form keeps the conserved moments and adds the 3rd-order Hermite moment rho u u u
+    f4 = np.array([lib.oracle_feq_f64(rho, *u, k, 1) for k in range(27)])
+    assert abs(f4.sum() - rho) < 1e-14
+    np.testing.assert_allclose(f4 @ c, rho * u, atol=1e-15)
+    Q = np.einsum("k,ka,kb,kc->abc", f4, c, c, c)
+    d = np.eye(3)
+    iso = (np.einsum("a,bc->abc", u, d) + np.einsum("b,ac->abc", u, d) + np.einsum("c,ab->abc", u, d)) / 3
+    # off-diagonal third moments Q_xyz are exactly rho u_x u_y u_z on D3Q27
+    assert abs(Q[0, 1, 2] - rho * u[0] * u[1] * u[2]) < 1e-15
+    assert abs(Q[0, 0, 1] - rho * (u[0] * u[0] * u[1] + iso[0, 0, 1])) < 1e-15
+
+
+def test_inlet_hash_properties(oracle):
+    lib = oracle.load()
+    assert lib.oracle_gpu_hash(0) == 0          # murmur3 fmix32 fixes zero
+    xs = np.arange(-300000, 300000, 7, dtype=np.int64)
+    hs = np.array([lib.oracle_gpu_hash(int(x)) for x in xs[:20000]], dtype=np.uint64)
+    assert np.unique(hs).size == hs.size        # fmix32 is a bijection of the 32-bit integers
+    vals = np.array([lib.oracle_gradient_noise(i, 2 * i + 1, 17, 1234) for i in range(5000)])
+    assert vals.min() >= -1.0 and vals.max() < 1.0
+    assert abs(vals.mean()) < 0.05               # uniform on [-1, 1)
