@@ -45,11 +45,15 @@ def bytes_per_lup(storage: str) -> int:
     return 2 * 27 * {"f32": 4, "f64": 8, "f16s": 2, "bf16s": 2}[storage]
 
 
-def gpu_leg(grid, storage: str, stream: str, steps: int, warmup: int):
+def gpu_leg(grid, storage: str, stream: str, steps: int, warmup: int, autotune: bool = True):
     from openaerolab_alpha_01_amd import solver
     params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
     plan = solver.uniform_plan(grid, tau_of(), storage=storage, stream=stream, params=params, u_inlet=U_IN)
     plan.init_equilibrium(None, None, 1.0, (U_IN, 0.0, 0.0))
+    tuning = None
+    if autotune:
+        best, table = plan.autotune(6)
+        tuning = {"pipe_planes": best, "ms_per_step_by_pipe_planes": {str(k): round(v, 4) for k, v in table.items()}}
     if warmup:
         plan.time_steps(warmup)
     plan.sync()
@@ -61,33 +65,17 @@ def gpu_leg(grid, storage: str, stream: str, steps: int, warmup: int):
     F = plan.force()[0]
     plan.guard(0.2, 0.5)
     plan.close()
-    return ms_total, ms_kernel, wall, F
+    return ms_total, ms_kernel, wall, F, tuning
 
 
-def copy_bandwidth_gbs() -> float | None:
-    """Device-to-device copy of 4 GiB (read + write), the measured HBM ceiling of this box."""
-    try:
-        import torch
-        if not torch.cuda.is_available():
-            return None
-        n = 1 << 30
-        a = torch.empty(n, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        reps = 10
-        for _ in range(reps):
-            b.copy_(a)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
-        del a, b
-        torch.cuda.empty_cache()
-        return 2 * 4 * n / (ms * 1e-3) / 1e9
-    except Exception:
-        return None
+def copy_bandwidth_gbs() -> float:
+    """liboal's streaming 16-byte copy of 4 GiB (read + write counted): the measured HBM ceiling."""
+    import ctypes as C
+    from openaerolab_alpha_01_amd import _lib
+    lib = _lib.load()
+    v = C.c_double()
+    _lib.check(lib.oal_copy_bandwidth(0, 4 << 30, 10, C.byref(v)), "oal_copy_bandwidth")
+    return v.value
 
 
 def cpu_baseline(grid, budget_s: float = 20.0):
@@ -102,7 +90,7 @@ def cpu_baseline(grid, budget_s: float = 20.0):
     f = O.init_eq("f32", G, rho, u)
     g = f.copy()
     del rho, u
-    p = O.make_params(tau_of(), u_inlet=U_IN)
+    p = O.make_params(tau_of(), u_inlet=U_IN, arith=O.LITERAL)
     done, t_used = 0, 0.0
     while done == 0 or (t_used < budget_s and done < 20):
         t0 = time.perf_counter()
@@ -122,6 +110,8 @@ def main(argv=None) -> int:
     ap.add_argument("--stream", default="ab", choices=["ab", "aa"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--no-copy-bw", action="store_true", help="skip the copy-bandwidth reference measurement")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -134,7 +124,7 @@ def main(argv=None) -> int:
     grid = workload_grid()
     n_cells = N ** 3
     n_fluid = int(n_cells - grid.solid.sum())
-    ms_total, ms_kernel, wall, F = gpu_leg(grid, "f32", args.stream, args.steps, args.warmup)
+    ms_total, ms_kernel, wall, F, tun = gpu_leg(grid, "f32", args.stream, args.steps, args.warmup, not args.no_autotune)
     mlups = n_cells * args.steps / (ms_total * 1e-3) / 1e6
     achieved = bytes_per_lup("f32") * n_fluid / (ms_kernel * 1e-3) / 1e9
     area = np.pi * (D / 2) ** 2
@@ -166,19 +156,21 @@ def main(argv=None) -> int:
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_lup": bytes_per_lup("f32"), "kernel_ms": round(ms_kernel, 5)},
         "cd_transient": round(cd, 5),
+        "tuning": tun,
     }
     if not args.no_fp16:
-        ms_t16, ms_k16, _, F16 = gpu_leg(grid, "f16s", args.stream, args.steps, args.warmup)
+        ms_t16, ms_k16, _, F16, tun16 = gpu_leg(grid, "f16s", args.stream, args.steps, args.warmup, not args.no_autotune)
         m16 = n_cells * args.steps / (ms_t16 * 1e-3) / 1e6
         a16 = bytes_per_lup("f16s") * n_fluid / (ms_k16 * 1e-3) / 1e9
         out["fp16_shifted"] = {"value": round(m16, 1), "unit": "MLUPS", "ms_per_step": round(ms_t16 / args.steps, 4),
-                               "speedup_vs_f32": round(m16 / mlups, 3),
+                               "speedup_vs_f32": round(m16 / mlups, 3), "tuning": tun16,
                                "roofline": {"achieved": round(a16, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                             "frac": round(a16 / HBM_PEAK_GBS, 4),
                                             "bytes_per_lup": bytes_per_lup("f16s")}}
-    bw = copy_bandwidth_gbs()
-    if bw is not None:
+    if not args.no_copy_bw:
+        bw = copy_bandwidth_gbs()
         out["roofline"]["measured_copy_gbs"] = round(bw, 1)
+        out["roofline"]["frac_of_measured_copy"] = round(achieved / bw, 4)
     if not args.no_cpu_baseline:
         v, cores, done, secs = cpu_baseline(grid)
         out["cpu_baseline"] = {"value": round(v, 2), "unit": "MLUPS", "cores": cores, "kind": "port",

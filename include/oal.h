@@ -182,8 +182,29 @@ int oal_guard(OalPlan* plan, int level, float rho_min, float u_max, int64_t* bad
 int oal_sync(OalPlan* plan);
 
 /* Times n steps of one level on the plan's stream with HIP events: *ms_total is the wall time of
- * the n launches, *ms_kernel the mean duration of one step kernel. */
+ * the n launches, *ms_kernel the mean duration of one step (all its launches). */
 int oal_time_steps(OalPlan* plan, int level, int n, int flags, double* ms_total, double* ms_kernel);
+
+/* Replaces: nothing.  Bandwidth of a streaming 16-byte device copy of `bytes` (read + write counted),
+ * best of `reps` back-to-back launches: the measured HBM ceiling beside the 8 TB/s datasheet peak. */
+int oal_copy_bandwidth(int device, int64_t bytes, int reps, double* gbs);
+
+/* Launch shape of the step kernels.  pipe_planes: z-planes of a 16^3 block one workgroup of the
+ * interior kernel walks with its next plane's loads in flight (1 = one plane per workgroup, no
+ * pipelining; 2, 4, 8 or 16). */
+typedef struct {
+    int32_t pipe_planes;
+    int32_t reserved[7];
+} OalTuning;
+
+/* Replaces: nothing (the reference picks its launch shape statically, src/collision/timestep.jl:1479). */
+int oal_get_tuning(const OalPlan* plan, OalTuning* tuning);
+int oal_set_tuning(OalPlan* plan, const OalTuning* tuning);
+
+/* The project's autotuner: times every candidate launch shape on `level` for n_steps steps each,
+ * keeps the fastest and restores the populations the plan held before; *best receives the choice
+ * and ms_per_step[5] (may be NULL) the mean step time of pipe_planes 1, 2, 4, 8, 16. */
+int oal_autotune(OalPlan* plan, int level, int n_steps, OalTuning* best, double* ms_per_step);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,7 @@ EXPORTS = [
     "oal_abi_version", "oal_strerror", "oal_last_error", "oal_plan_create", "oal_plan_destroy",
     "oal_level_cells", "oal_upload_f", "oal_download_f", "oal_init_equilibrium", "oal_level_step",
     "oal_step", "oal_force", "oal_mass", "oal_download_macro", "oal_guard", "oal_sync", "oal_time_steps",
+    "oal_get_tuning", "oal_set_tuning", "oal_autotune", "oal_copy_bandwidth",
 ]
 
 
@@ -62,6 +63,10 @@ class OalPlanDesc(C.Structure):
     ]
 
 
+class OalTuning(C.Structure):
+    _fields_ = [("pipe_planes", C.c_int32), ("reserved", C.c_int32 * 7)]
+
+
 class OalError(RuntimeError):
     def __init__(self, code: int, where: str, msg: str):
         super().__init__(f"{where}: {msg} (status {code})")
@@ -72,11 +77,11 @@ _lib = None
 
 
 def load(path: str | os.PathLike | None = None) -> C.CDLL:
-    """Load liboal.so once; raises if it has not been built."""
+    """Load liboal.so once (OAL_LIBRARY overrides its path); raises if it has not been built."""
     global _lib
     if _lib is not None:
         return _lib
-    p = Path(path) if path else LIB_PATH
+    p = Path(path) if path else Path(os.environ.get("OAL_LIBRARY", LIB_PATH))
     if not p.exists():
         raise RuntimeError(f"liboal.so not built ({p}); run `python -m openaerolab_alpha_01_amd.build`")
     lib = C.CDLL(str(p))
@@ -99,6 +104,10 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "oal_guard": ([vp, C.c_int, C.c_float, C.c_float, i64p], C.c_int),
         "oal_sync": ([vp], C.c_int),
         "oal_time_steps": ([vp, C.c_int, C.c_int, C.c_int, dp, dp], C.c_int),
+        "oal_get_tuning": ([vp, C.POINTER(OalTuning)], C.c_int),
+        "oal_set_tuning": ([vp, C.POINTER(OalTuning)], C.c_int),
+        "oal_autotune": ([vp, C.c_int, C.c_int, C.POINTER(OalTuning), dp], C.c_int),
+        "oal_copy_bandwidth": ([C.c_int, C.c_int64, C.c_int, dp], C.c_int),
     }
     for name, (argt, rest) in sig.items():
         fn = getattr(lib, name)

@@ -1,10 +1,10 @@
 // Device-side building blocks of liboal: D3Q27 constants, population storage forms and the
 // cumulant collision of one cell.  gfx950 only.
 //
-// The collision arithmetic follows src/collision/kernel_cumulant.jl:420-908 operation for operation
-// (the library is compiled with -ffp-contract=off, so every multiply and add rounds on its own as
-// in the reference), which lets the FP32 path match a CPU restatement of the same arithmetic bit
-// for bit.
+// The collision is the reference's (src/collision/kernel_cumulant.jl:420-908) with fused
+// multiply-adds written out explicitly; the library is compiled with -ffp-contract=off so that no
+// other operation is fused, which lets every storage form match a CPU restatement of the same
+// arithmetic bit for bit.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -51,7 +51,10 @@ template <class T> __device__ __forceinline__ T mn(T a, T b) { return b < a ? b 
 struct StF32 {
     using st = float;
     using T = float;
+    using raw = float;
     static constexpr int code = 0;
+    __device__ static __forceinline__ raw ld_raw(const st* p) { return *p; }
+    __device__ static __forceinline__ T from_raw(raw r, T) { return r; }
     __device__ static __forceinline__ T ld(const st* p, T) { return *p; }
     __device__ static __forceinline__ void sto(st* p, T f, T) { *p = f; }
     __device__ static __forceinline__ T rt(T f, T) { return f; }
@@ -59,7 +62,10 @@ struct StF32 {
 struct StF64 {
     using st = double;
     using T = double;
+    using raw = double;
     static constexpr int code = 1;
+    __device__ static __forceinline__ raw ld_raw(const st* p) { return *p; }
+    __device__ static __forceinline__ T from_raw(raw r, T) { return r; }
     __device__ static __forceinline__ T ld(const st* p, T) { return *p; }
     __device__ static __forceinline__ void sto(st* p, T f, T) { *p = f; }
     __device__ static __forceinline__ T rt(T f, T) { return f; }
@@ -67,7 +73,10 @@ struct StF64 {
 struct StF16S {
     using st = _Float16;
     using T = float;
+    using raw = uint32_t;   // the 16 stored bits, zero-extended
     static constexpr int code = 2;
+    __device__ static __forceinline__ raw ld_raw(const st* p) { return (uint32_t)*(const uint16_t*)p; }
+    __device__ static __forceinline__ T from_raw(raw r, T w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)r) + w; }
     __device__ static __forceinline__ T ld(const st* p, T w) { return (float)(*p) + w; }
     __device__ static __forceinline__ void sto(st* p, T f, T w) { *p = (_Float16)(f - w); }
     __device__ static __forceinline__ T rt(T f, T w) { return (float)(_Float16)(f - w) + w; }
@@ -82,7 +91,10 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
 struct StBF16S {
     using st = uint16_t;
     using T = float;
+    using raw = uint32_t;
     static constexpr int code = 3;
+    __device__ static __forceinline__ raw ld_raw(const st* p) { return (uint32_t)*p; }
+    __device__ static __forceinline__ T from_raw(raw r, T w) { return __uint_as_float(r << 16) + w; }
     __device__ static __forceinline__ T ld(const st* p, T w) { return __uint_as_float((uint32_t)(*p) << 16) + w; }
     __device__ static __forceinline__ void sto(st* p, T f, T w) { *p = f32_to_bf16_rne(f - w); }
     __device__ static __forceinline__ T rt(T f, T w) { return __uint_as_float((uint32_t)f32_to_bf16_rne(f - w) << 16) + w; }
@@ -174,165 +186,244 @@ __device__ __forceinline__ void chim_b(T& a, T& b, T& c, T u) {
     c = fp;
 }
 
-// The fluid-cell update after the gather: f holds the 27 arriving populations, (rho, jx, jy, jz)
-// their sums in direction order.  On return f holds the post-collision populations and rho, u the
-// macroscopic state the reference writes to rho_out / vel_out.
-template <class T>
-__device__ __forceinline__ void collide_cell(T (&f)[27], T rho, T jx, T jy, T jz, T nu_eddy, T tau, T amr_scale,
-                                             T Fx, T Fy, T Fz, bool forced, const CollParams& p,
-                                             T& rho_o, T& ux_o, T& uy_o, T& uz_o) {
-    rho = mx(rho, T(0.01f));
-    const T inv_rho = T(1) / rho;
-    const T ux = jx * inv_rho, uy = jy * inv_rho, uz = jz * inv_rho;
-    const T ux_eq = ux + T(0.5) * Fx * inv_rho;
-    const T uy_eq = uy + T(0.5) * Fy * inv_rho;
-    const T uz_eq = uz + T(0.5) * Fz * inv_rho;
-    rho_o = rho; ux_o = ux; uy_o = uy; uz_o = uz;
+template <class T> __device__ __forceinline__ T fmx(T a, T b, T c);
+template <> __device__ __forceinline__ float fmx<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <> __device__ __forceinline__ double fmx<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// omega1 and the diagonal 4th-order rate of a cell (kernel_cumulant.jl:592-610); computed per cell
+// only with WALE, otherwise once per domain on the host with the same operations
+template <class T>
+__host__ __device__ inline void cell_rates(T nu_eddy, T tau, T amr_scale, T nu_bg, int adaptive, T lambda, T omega4,
+                                           T& omega1, T& omega4_diag) {
     nu_eddy = nu_eddy * amr_scale;
-    nu_eddy = mx(nu_eddy, T(p.nu_sgs_bg) * amr_scale);
+    const T bg = nu_bg * amr_scale;
+    nu_eddy = nu_eddy < bg ? bg : nu_eddy;
     const T nu_mol = (tau - T(0.5)) / T(3);
     const T nu_eff = nu_mol + nu_eddy;
     const T tau_eff = T(0.5) + T(3) * nu_eff;
-    const T omega1 = T(1) / mx(tau_eff, T(0.500001f));
-    const T omega_bulk = T(p.omega_bulk), omega3 = T(p.omega_3), omega4 = T(p.omega_4);
-    T omega4_diag = omega4;
-    if (p.adaptive_omega4 == 1) {
+    omega1 = T(1) / (tau_eff < T(0.500001f) ? T(0.500001f) : tau_eff);
+    omega4_diag = omega4;
+    if (adaptive == 1) {
         const T om1_m2 = omega1 - T(2);
-        const T den = T(p.lambda) * omega1 + T(1e-10f);
+        const T den = lambda * omega1 + T(1e-10f);
         const T a_val = -(om1_m2 * om1_m2) / den;
-        omega4_diag = mn(mx(a_val, T(0.01f)), T(2));
+        const T lo = a_val < T(0.01f) ? T(0.01f) : a_val;
+        omega4_diag = T(2) < lo ? T(2) : lo;
     }
+}
 
-    // forward transform: x on (0,1,2)+3iy+9iz, y on ma+(0,3,6)+9iz, z on ma+3mb+(0,9,18)
+// Per-cell constants of the collision
+template <class T>
+struct CellRates {
+    T omega1, omega4_diag;
+    T Fx, Fy, Fz;
+    bool forced;
+};
+
+// The fluid-cell update after the gather (kernel_cumulant.jl:420-908): f holds the 27 arriving
+// populations in direction order; on return the post-collision ones, with rho and u the macroscopic
+// state the reference writes to rho_out / vel_out.
+//
+// Arithmetic: the reference's cumulant collision with fused multiply-adds, and with rho and j read
+// off the raw partial sums of the transform passes: the x pass turns each row (f-, f0, f+) into
+// (m0, d = f+ - f-, s = f- + f+); rho and j are sums of those over rows; the x, y and z passes then
+// shift to central moments about u (k1 = d - u m0, k2 = s - 2 u d + u^2 m0).  The total of the last
+// pass equals rho bit for bit, so 1/max(k000, 0.01) is the 1/max(rho, 0.01) already formed.
+template <class T>
+__device__ __forceinline__ void collide_cell(T (&f)[27], const CellRates<T>& q, const CollParams& p,
+                                             T& rho_o, T& ux_o, T& uy_o, T& uz_o) {
+    T A[27];
 #pragma unroll
-    for (int r = 0; r < 9; ++r) chim_f(f[3 * r], f[3 * r + 1], f[3 * r + 2], ux_eq);
+    for (int r = 0; r < 9; ++r) {
+        const T s = f[3 * r] + f[3 * r + 2];
+        const T d = f[3 * r + 2] - f[3 * r];
+        A[3 * r] = s + f[3 * r + 1];
+        A[3 * r + 1] = d;
+        A[3 * r + 2] = s;
+    }
+    T y0[3], yd0[3], y1[3];
+#pragma unroll
+    for (int iz = 0; iz < 3; ++iz) {
+        const T a = A[9 * iz], b = A[3 + 9 * iz], c = A[6 + 9 * iz];
+        const T ys = a + c;
+        yd0[iz] = c - a;
+        y0[iz] = ys + b;
+        y1[iz] = (A[1 + 9 * iz] + A[7 + 9 * iz]) + A[4 + 9 * iz];
+    }
+    const T rho = (y0[0] + y0[2]) + y0[1];
+    const T jz = y0[2] - y0[0];
+    const T jy = (yd0[0] + yd0[2]) + yd0[1];
+    const T jx = (y1[0] + y1[2]) + y1[1];
+
+    const T rc = mx(rho, T(0.01f));
+    const T inv = T(1) / rc;
+    const T ux = jx * inv, uy = jy * inv, uz = jz * inv;
+    T uxe = ux, uye = uy, uze = uz;
+    if (q.forced) {
+        uxe = fmx(T(0.5) * q.Fx, inv, ux);
+        uye = fmx(T(0.5) * q.Fy, inv, uy);
+        uze = fmx(T(0.5) * q.Fz, inv, uz);
+    }
+    rho_o = rc; ux_o = ux; uy_o = uy; uz_o = uz;
+
+    {
+        const T u = uxe, uu = u * u, t = T(-2) * u;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            const T m0 = A[3 * r], d = A[3 * r + 1], s = A[3 * r + 2];
+            A[3 * r + 1] = fmx(-u, m0, d);
+            A[3 * r + 2] = fmx(uu, m0, fmx(t, d, s));
+        }
+    }
+    auto pass = [&](int i0, int i1, int i2, T u) {
+        const T uu = u * u, t = T(-2) * u;
+        const T a = A[i0], b = A[i1], c = A[i2];
+        const T ys = a + c, yd = c - a, ym0 = ys + b;
+        A[i0] = ym0;
+        A[i1] = fmx(-u, ym0, yd);
+        A[i2] = fmx(uu, ym0, fmx(t, yd, ys));
+    };
 #pragma unroll
     for (int iz = 0; iz < 3; ++iz)
 #pragma unroll
-        for (int ma = 0; ma < 3; ++ma) chim_f(f[ma + 9 * iz], f[ma + 3 + 9 * iz], f[ma + 6 + 9 * iz], uy_eq);
+        for (int ma = 0; ma < 3; ++ma) pass(ma + 9 * iz, ma + 3 + 9 * iz, ma + 6 + 9 * iz, uye);
 #pragma unroll
-    for (int r = 0; r < 9; ++r) chim_f(f[r], f[r + 9], f[r + 18], uz_eq);
+    for (int r = 0; r < 9; ++r) pass(r, r + 9, r + 18, uze);
 
-    // normalised central moments, index a + 3b + 9c for order (a,b,c)
-    const T k000 = f[0];
-    const T inv_r = T(1) / mx(k000, T(0.01f));
-    const T c200 = f[2] * inv_r, c020 = f[6] * inv_r, c002 = f[18] * inv_r;
-    const T c110 = f[4] * inv_r, c101 = f[10] * inv_r, c011 = f[12] * inv_r;
-    const T c210 = f[5] * inv_r, c120 = f[7] * inv_r, c201 = f[11] * inv_r;
-    const T c021 = f[15] * inv_r, c102 = f[19] * inv_r, c012 = f[21] * inv_r;
-    const T c111 = f[13] * inv_r;
-    const T c220n = f[8] * inv_r, c202n = f[20] * inv_r, c022n = f[24] * inv_r;
-    const T c211n = f[14] * inv_r, c121n = f[16] * inv_r, c112n = f[22] * inv_r;
+    // normalised central moments (index a + 3b + 9c for order (a,b,c)) and 4th-order cumulants
+    const T c200 = A[2] * inv, c020 = A[6] * inv, c002 = A[18] * inv;
+    const T c110 = A[4] * inv, c101 = A[10] * inv, c011 = A[12] * inv;
+    const T c210 = A[5] * inv, c120 = A[7] * inv, c201 = A[11] * inv;
+    const T c021 = A[15] * inv, c102 = A[19] * inv, c012 = A[21] * inv;
+    const T c111 = A[13] * inv;
+    const T c220n = A[8] * inv, c202n = A[20] * inv, c022n = A[24] * inv;
+    const T c211n = A[14] * inv, c121n = A[16] * inv, c112n = A[22] * inv;
+    const T x110 = T(2) * c110, x101 = T(2) * c101, x011 = T(2) * c011;
+    const T C220 = fmx(-x110, c110, fmx(-c200, c020, c220n));
+    const T C202 = fmx(-x101, c101, fmx(-c200, c002, c202n));
+    const T C022 = fmx(-x011, c011, fmx(-c020, c002, c022n));
+    const T C211 = fmx(-x110, c101, fmx(-c200, c011, c211n));
+    const T C121 = fmx(-x110, c011, fmx(-c020, c101, c121n));
+    const T C112 = fmx(-x101, c011, fmx(-c002, c110, c112n));
 
-    const T C220 = c220n - c200 * c020 - T(2) * c110 * c110;
-    const T C202 = c202n - c200 * c002 - T(2) * c101 * c101;
-    const T C022 = c022n - c020 * c002 - T(2) * c011 * c011;
-    const T C211 = c211n - c200 * c011 - T(2) * c110 * c101;
-    const T C121 = c121n - c020 * c101 - T(2) * c110 * c011;
-    const T C112 = c112n - c002 * c110 - T(2) * c101 * c011;
-
-    // second order: trace with omega_bulk, deviators with omega1 (kernel_cumulant.jl:708-723)
-    const T tr = c200 + c020 + c002;
-    const T tr_eq = T(3.0f * 0.333333333f);   // Float32 in the reference: rounds to 1
-    const T tr_s = (T(1) - omega_bulk) * tr + omega_bulk * tr_eq;
-    const T d1 = c200 - c020, d2 = c200 - c002;
-    const T d1s = (T(1) - omega1) * d1, d2s = (T(1) - omega1) * d2;
-    const T s200 = (tr_s + d1s + d2s) / T(3);
-    const T s020 = (tr_s - T(2) * d1s + d2s) / T(3);
-    const T s002 = (tr_s + d1s - T(2) * d2s) / T(3);
-    T s110 = (T(1) - omega1) * c110;
-    T s101 = (T(1) - omega1) * c101;
-    T s011 = (T(1) - omega1) * c011;
-    if (p.limiter == 1) {   // realizability clamp (kernel_cumulant.jl:726-735)
-        const T b110 = sqrt(mx(s200 * s020, T(0)));
-        const T b101 = sqrt(mx(s200 * s002, T(0)));
-        const T b011 = sqrt(mx(s020 * s002, T(0)));
-        s110 = mn(mx(s110, -b110), b110);
-        s101 = mn(mx(s101, -b101), b101);
-        s011 = mn(mx(s011, -b011), b011);
+    // relaxation (kernel_cumulant.jl:705-754): trace toward 1 at omega_bulk, deviators at omega1,
+    // realizability clamp, 3rd order at omega3, 4th-order cumulants at omega4 / omega4_diag
+    const T third = T(1) / T(3);
+    const T ob = T(p.omega_bulk);
+    const T tr = (c200 + c020) + c002;
+    const T tr_s = fmx(T(1) - ob, tr, ob);
+    const T g1 = T(1) - q.omega1;
+    const T d1s = g1 * (c200 - c020), d2s = g1 * (c200 - c002);
+    const T t3 = tr_s * third;
+    const T s200 = fmx(d1s + d2s, third, t3);
+    const T s020 = fmx(fmx(T(-2), d1s, d2s), third, t3);
+    const T s002 = fmx(fmx(T(-2), d2s, d1s), third, t3);
+    T s110 = g1 * c110, s101 = g1 * c101, s011 = g1 * c011;
+    if (p.limiter == 1) {
+        const T q110 = mx(s200 * s020, T(0)), q101 = mx(s200 * s002, T(0)), q011 = mx(s020 * s002, T(0));
+        if (s110 * s110 > q110) s110 = copysign(sqrt(q110), s110);
+        if (s101 * s101 > q101) s101 = copysign(sqrt(q101), s101);
+        if (s011 * s011 > q011) s011 = copysign(sqrt(q011), s011);
     }
-    const T s210 = (T(1) - omega3) * c210, s120 = (T(1) - omega3) * c120;
-    const T s201 = (T(1) - omega3) * c201, s021 = (T(1) - omega3) * c021;
-    const T s102 = (T(1) - omega3) * c102, s012 = (T(1) - omega3) * c012;
-    const T s111 = (T(1) - omega3) * c111;
-    const T S220 = (T(1) - omega4_diag) * C220;
-    const T S202 = (T(1) - omega4_diag) * C202;
-    const T S022 = (T(1) - omega4_diag) * C022;
-    const T S211 = (T(1) - omega4) * C211;
-    const T S121 = (T(1) - omega4) * C121;
-    const T S112 = (T(1) - omega4) * C112;
+    const T g3 = T(1) - T(p.omega_3), g4 = T(1) - T(p.omega_4), g4d = T(1) - q.omega4_diag;
+    const T s210 = g3 * c210, s120 = g3 * c120, s201 = g3 * c201, s021 = g3 * c021;
+    const T s102 = g3 * c102, s012 = g3 * c012, s111 = g3 * c111;
+    const T S220 = g4d * C220, S202 = g4d * C202, S022 = g4d * C022;
+    const T S211 = g4 * C211, S121 = g4 * C121, S112 = g4 * C112;
 
     // central moments from the relaxed cumulants; 5th and 6th order cumulants vanish
-    const T s220 = S220 + s200 * s020 + T(2) * s110 * s110;
-    const T s202 = S202 + s200 * s002 + T(2) * s101 * s101;
-    const T s022 = S022 + s020 * s002 + T(2) * s011 * s011;
-    const T s211 = S211 + s200 * s011 + T(2) * s110 * s101;
-    const T s121 = S121 + s020 * s101 + T(2) * s110 * s011;
-    const T s112 = S112 + s002 * s110 + T(2) * s101 * s011;
-    const T s221 = s200 * s021 + s020 * s201 + T(4) * s110 * s111 + T(2) * s101 * s120 + T(2) * s011 * s210;
-    const T s212 = s200 * s012 + s002 * s210 + T(2) * s110 * s102 + T(4) * s101 * s111 + T(2) * s011 * s201;
-    const T s122 = s020 * s102 + s002 * s120 + T(2) * s110 * s012 + T(4) * s011 * s111 + T(2) * s101 * s021;
-    T s222 = s200 * S022 + s020 * S202 + s002 * S220 + T(4) * (s110 * S112 + s101 * S121 + s011 * S211);
-    s222 += T(2) * (s210 * s012 + s201 * s021 + s120 * s102) + T(4) * s111 * s111;
-    s222 += s200 * s020 * s002 + T(2) * (s200 * s011 * s011 + s020 * s101 * s101 + s002 * s110 * s110) +
-            T(8) * s110 * s101 * s011;
+    const T y110 = T(2) * s110, y101 = T(2) * s101, y011 = T(2) * s011;
+    const T s220 = fmx(y110, s110, fmx(s200, s020, S220));
+    const T s202 = fmx(y101, s101, fmx(s200, s002, S202));
+    const T s022 = fmx(y011, s011, fmx(s020, s002, S022));
+    const T s211 = fmx(y110, s101, fmx(s200, s011, S211));
+    const T s121 = fmx(y110, s011, fmx(s020, s101, S121));
+    const T s112 = fmx(y101, s011, fmx(s002, s110, S112));
+    const T s221 = fmx(y011, s210, fmx(y101, s120, fmx(T(2) * y110, s111, fmx(s020, s201, s200 * s021))));
+    const T s212 = fmx(y011, s201, fmx(T(2) * y101, s111, fmx(y110, s102, fmx(s002, s210, s200 * s012))));
+    const T s122 = fmx(y101, s021, fmx(T(2) * y011, s111, fmx(y110, s012, fmx(s002, s120, s020 * s102))));
+    T e = s200 * S022;
+    e = fmx(s020, S202, e);
+    e = fmx(s002, S220, e);
+    e = fmx(T(2) * y110, S112, e);
+    e = fmx(T(2) * y101, S121, e);
+    e = fmx(T(2) * y011, S211, e);
+    e = fmx(T(2) * s210, s012, e);
+    e = fmx(T(2) * s201, s021, e);
+    e = fmx(T(2) * s120, s102, e);
+    e = fmx(T(4) * s111, s111, e);
+    e = fmx(s200 * s020, s002, e);
+    e = fmx(s200 * y011, s011, e);
+    e = fmx(s020 * y101, s101, e);
+    e = fmx(s002 * y110, s110, e);
+    e = fmx((T(4) * y110) * s101, s011, e);
 
-    const T r = k000;
-    f[0] = r;
-    f[1] = T(0); f[3] = T(0); f[9] = T(0);
-    f[2] = r * s200; f[6] = r * s020; f[18] = r * s002;
-    f[4] = r * s110; f[10] = r * s101; f[12] = r * s011;
-    f[5] = r * s210; f[7] = r * s120; f[11] = r * s201;
-    f[15] = r * s021; f[19] = r * s102; f[21] = r * s012;
-    f[13] = r * s111;
-    f[8] = r * s220; f[20] = r * s202; f[24] = r * s022;
-    f[14] = r * s211; f[16] = r * s121; f[22] = r * s112;
-    f[17] = r * s221; f[23] = r * s212; f[25] = r * s122;
-    f[26] = r * s222;
+    const T r = rho;
+    A[0] = r;
+    A[1] = T(0); A[3] = T(0); A[9] = T(0);
+    A[2] = r * s200; A[6] = r * s020; A[18] = r * s002;
+    A[4] = r * s110; A[10] = r * s101; A[12] = r * s011;
+    A[5] = r * s210; A[7] = r * s120; A[11] = r * s201;
+    A[15] = r * s021; A[19] = r * s102; A[21] = r * s012;
+    A[13] = r * s111;
+    A[8] = r * s220; A[20] = r * s202; A[24] = r * s022;
+    A[14] = r * s211; A[16] = r * s121; A[22] = r * s112;
+    A[17] = r * s221; A[23] = r * s212; A[25] = r * s122;
+    A[26] = r * e;
 
-    // backward transform z, y, x
+    // backward passes z, y, x: (k0, k1, k2) -> (f-, f0, f+) about u
+    auto back = [&](int i0, int i1, int i2, T u) {
+        const T uu = u * u, t = T(2) * u;
+        const T a = A[i0], b = A[i1], c = A[i2];
+        const T m1 = fmx(u, a, b);
+        const T m2 = fmx(uu, a, fmx(t, b, c));
+        const T h = T(0.5) * m1;
+        A[i0] = fmx(T(0.5), m2, -h);
+        A[i1] = a - m2;
+        A[i2] = fmx(T(0.5), m2, h);
+    };
 #pragma unroll
-    for (int q = 0; q < 9; ++q) chim_b(f[q], f[q + 9], f[q + 18], uz_eq);
+    for (int r2 = 0; r2 < 9; ++r2) back(r2, r2 + 9, r2 + 18, uze);
 #pragma unroll
     for (int iz = 0; iz < 3; ++iz)
 #pragma unroll
-        for (int ma = 0; ma < 3; ++ma) chim_b(f[ma + 9 * iz], f[ma + 3 + 9 * iz], f[ma + 6 + 9 * iz], uy_eq);
+        for (int ma = 0; ma < 3; ++ma) back(ma + 9 * iz, ma + 3 + 9 * iz, ma + 6 + 9 * iz, uye);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) chim_b(f[3 * q], f[3 * q + 1], f[3 * q + 2], ux_eq);
+    for (int r2 = 0; r2 < 9; ++r2) back(3 * r2, 3 * r2 + 1, 3 * r2 + 2, uxe);
 
     if (p.limiter == 2) {   // positivity (kernel_cumulant.jl:849-884)
-        T fmin = f[0];
+        T fmin = A[0];
 #pragma unroll
-        for (int k = 1; k < 27; ++k) fmin = mn(fmin, f[k]);
+        for (int k = 1; k < 27; ++k) fmin = mn(fmin, A[k]);
         if (fmin < T(0)) {
             T alpha = T(1);
 #pragma unroll
             for (int k = 0; k < 27; ++k) {
-                const T fe = feq_auto(rho, ux_eq, uy_eq, uz_eq, wgt<T>(k), T(dcx(k)), T(dcy(k)), T(dcz(k)), p.compressible);
-                const T fn = f[k] - fe;
+                const T fe = feq_auto(rc, uxe, uye, uze, wgt<T>(k), T(dcx(k)), T(dcy(k)), T(dcz(k)), p.compressible);
+                const T fn = A[k] - fe;
                 if (fn < T(0) && fe > T(1e-15f)) alpha = mn(alpha, fe / (fe - fn + T(1e-15f)));
             }
             alpha = mx(alpha, T(0));
 #pragma unroll
             for (int k = 0; k < 27; ++k) {
-                const T fe = feq_auto(rho, ux_eq, uy_eq, uz_eq, wgt<T>(k), T(dcx(k)), T(dcy(k)), T(dcz(k)), p.compressible);
-                f[k] = fe + alpha * (f[k] - fe);
+                const T fe = feq_auto(rc, uxe, uye, uze, wgt<T>(k), T(dcx(k)), T(dcy(k)), T(dcz(k)), p.compressible);
+                A[k] = fe + alpha * (A[k] - fe);
             }
         }
     }
-
-    if (forced) {   // Guo term of the force density (kernel_cumulant.jl:889-905)
-        const T g = T(1) - T(0.5) * omega1;
+    if (q.forced) {   // Guo term of the force density (kernel_cumulant.jl:889-905)
+        const T g = T(1) - T(0.5) * q.omega1;
 #pragma unroll
         for (int k = 0; k < 27; ++k) {
             const T cxf = T(dcx(k)), cyf = T(dcy(k)), czf = T(dcz(k));
-            const T cu = cxf * ux_eq + cyf * uy_eq + czf * uz_eq;
-            const T ft = wgt<T>(k) * T(3) * ((cxf - ux + T(3) * cu * cxf) * Fx + (cyf - uy + T(3) * cu * cyf) * Fy +
-                                             (czf - uz + T(3) * cu * czf) * Fz);
-            f[k] = f[k] + g * ft;
+            const T cu = cxf * uxe + cyf * uye + czf * uze;
+            const T ft = wgt<T>(k) * T(3) * ((cxf - ux + T(3) * cu * cxf) * q.Fx + (cyf - uy + T(3) * cu * cyf) * q.Fy +
+                                             (czf - uz + T(3) * cu * czf) * q.Fz);
+            A[k] = A[k] + g * ft;
         }
     }
+#pragma unroll
+    for (int k = 0; k < 27; ++k) f[k] = A[k];
 }
 
 }  // namespace oal

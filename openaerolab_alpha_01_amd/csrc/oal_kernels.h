@@ -29,13 +29,16 @@ struct DevDomain {
     float tau, u_in, amr_scale, force_scale;
     float F[3];
     int32_t ymin_wall, zwall, forced;
+    float omega1, omega4d;          // collision rates without WALE, FP32 forms
+    double omega1_d, omega4d_d;     // and FP64 forms
+    int32_t nbx, nby;               // blocks per row / per layer of the domain's lexicographic grid
 };
 
 // arguments of the fused stream-collide kernels
 struct StepArgs {
     const void* fin;             // populations read this step
     void* fout;                  // populations written (== fin for the AA pattern)
-    const int32_t* nbr;          // [27 * nb] dir-major, 0-based, -1 = none
+    const int32_t* nbr_bm;       // [32 * nb] block-major neighbour rows, 0-based, -1 = none
     const int32_t* borg;         // [3 * nb] block origin in cells
     const int32_t* bdom;         // [nb] domain index
     const uint32_t* bflags;      // [nb]
@@ -45,12 +48,22 @@ struct StepArgs {
     void* vel_out;               // [3 * ncell] velocity written (WALE, macro output)
     void* rho_out;               // [ncell] density written (macro output) or null
     double* fpart;               // [n_items * 4] per-workgroup force partials or null
+    const int32_t* blist;        // blocks of this launch (item / PLANES -> block), null = all in order
+    const int32_t* blist_regular;    // [n_regular] interior blocks whose neighbours sit at lexicographic offsets
+    const int32_t* blist_interior;   // [n_interior] other blocks with all flags clear
+    const int32_t* blist_boundary_regular;  // [n_boundary_regular] boundary blocks with lexicographic neighbours
+    const int32_t* blist_boundary;   // [n_boundary] the other blocks
+    uint32_t n_regular, n_interior, n_boundary_regular, n_boundary;
+    const int32_t* lslot;        // [nb] link-mask slot of a block with solid cells near, -1 = none
+    const uint32_t* lmask;       // [n_lslots * BS3] per cell: bit k source of direction k solid, bit 27 cell solid
+    const int32_t* oslot;        // [nb] AA pattern: outlet side-buffer slot of the block, -1 = none
+    void* obuf;                  // [n_oslots * 9 * BS2] AA pattern: outlet-face populations with cx = -1
     uint32_t ncell;              // cells per direction (stride between directions)
     uint32_t nb;                 // blocks
     uint32_t n_items;            // workgroups launched (nb * PLANES)
     int32_t time_step;
     int32_t parity;              // AA pattern: 1 odd step (neighbour slots), 0 even step (own slots)
-    int32_t pad;
+    int32_t pipe_planes;         // planes per workgroup of the pipelined interior kernel (1 = off; divides 16)
     CollParams cp;
 };
 
@@ -63,7 +76,8 @@ struct ReduceArgs {
 };
 
 // launchers (oal_kernels.hip); all asynchronous on `stream`
-hipError_t launch_step(int storage, int stream_mode, bool wale, const StepArgs& a, hipStream_t stream);
+hipError_t launch_step(int storage, int stream_mode, bool wale, const StepArgs& a, hipStream_t stream, hipStream_t side);
+hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t stream);
 hipError_t launch_init_eq(int storage, void* f, uint32_t ncell, const double* rho, const double* u, double rho0,
                           double ux0, double uy0, double uz0, int permute, void* vel, hipStream_t stream);
 hipError_t launch_convert_in(int storage, void* f, const void* src, int src_dtype, uint32_t ncell, int permute,
@@ -72,6 +86,7 @@ hipError_t launch_convert_out(int storage, const void* f, void* dst, int dst_dty
                               hipStream_t stream);
 hipError_t launch_macro(int storage, const void* f, double* rho, double* u, uint32_t ncell, int permute, void* vel,
                         hipStream_t stream);
+hipError_t launch_aa_outlet_prep(int storage, const StepArgs& a, hipStream_t stream);
 hipError_t launch_mass(int storage, const void* f, const uint8_t* solid, const int32_t* borg, const int32_t* bdom,
                        const DevDomain* doms, uint32_t nb, int permute, double* part, hipStream_t stream);
 hipError_t launch_reduce(const ReduceArgs& r, hipStream_t stream);

@@ -60,6 +60,12 @@ struct Level {
     DevBuf<unsigned char> fA, fB;    // populations (AB: read A write B then swap; AA: A only)
     DevBuf<unsigned char> velA, velB;  // WALE velocity, working precision
     DevBuf<double> fpart, mpart, red_out;
+    DevBuf<int32_t> oslot;             // AA: outlet side-buffer slot per block
+    DevBuf<int32_t> blist_rg, blist_in, blist_br, blist_bd;  // block lists per kernel kind
+    uint32_t n_regular = 0, n_interior = 0, n_boundary_regular = 0, n_boundary = 0;
+    DevBuf<int32_t> lslot;             // link-mask slot per block
+    DevBuf<uint32_t> lmask;            // per-cell solid-link masks of blocks near solid cells
+    DevBuf<unsigned char> obuf;        // AA: outlet-face populations with cx = -1
     DevBuf<unsigned long long> guard;
     bool has_solid = false;
     bool have_f = false;
@@ -68,7 +74,8 @@ struct Level {
     void release() {
         nbr.release(); borg.release(); bdom.release(); dom_item0.release(); dom_block0.release();
         bflags.release(); solid.release(); doms.release(); fA.release(); fB.release(); velA.release(); velB.release();
-        fpart.release(); mpart.release(); red_out.release(); guard.release();
+        fpart.release(); mpart.release(); red_out.release(); guard.release(); oslot.release(); obuf.release(); blist_rg.release(); blist_in.release(); blist_br.release(); blist_bd.release();
+        lslot.release(); lmask.release();
     }
 };
 
@@ -76,13 +83,16 @@ struct Level {
 
 struct OalPlan {
     int device = 0;
+    int pipe_planes = 4;   // planes per workgroup of the pipelined interior kernel
     int storage = OAL_F32;
     int stream_mode = OAL_STREAM_AB;
     int bb_mode = OAL_BB_HALFWAY;
     bool wale = false;
     CollParams cp{};
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;             // boundary launches, overlapping the interior ones
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     std::vector<Level> levels;
 };
 
@@ -105,13 +115,15 @@ int upload(DevBuf<T>& d, const std::vector<T>& h) {
 
 // Host tables of one level, built and validated before any device call
 struct HostLevel {
-    std::vector<int32_t> borg, bdom, nbr;
+    std::vector<int32_t> borg, bdom, nbr, nbr_bm, oslot, blist_rg, blist_in, blist_br, blist_bd, lslot;
+    std::vector<uint32_t> lmask;
+    int32_t n_oslots = 0;
     std::vector<uint32_t> flags;
     std::vector<uint8_t> solid;
 };
 
 // Host-side construction of one level's tables from the reference-convention descriptor.
-int prepare_level(const OalLevelDesc& d, Level& L, HostLevel& H) {
+int prepare_level(const OalLevelDesc& d, const OalCollision& coll, Level& L, HostLevel& H) {
     if (d.n_blocks <= 0) return fail(OAL_ERR_ARG, "level with no blocks");
     if (!d.map_x || !d.map_y || !d.map_z || !d.neighbor_table) return fail(OAL_ERR_ARG, "level block arrays missing");
     if (d.n_domains <= 0 || !d.domains) return fail(OAL_ERR_ARG, "level has no domain descriptors");
@@ -168,6 +180,10 @@ int prepare_level(const OalLevelDesc& d, Level& L, HostLevel& H) {
         t.ymin_wall = s.ymin_wall_bc;
         t.zwall = s.zwall_bc;
         t.forced = (s.body_force[0] != 0.0f || s.body_force[1] != 0.0f || s.body_force[2] != 0.0f) ? 1 : 0;
+        cell_rates<float>(0.0f, s.tau, s.amr_sgs_scale, coll.nu_sgs_background, coll.adaptive_omega4, coll.lambda,
+                          coll.omega_4, t.omega1, t.omega4d);
+        cell_rates<double>(0.0, (double)s.tau, (double)s.amr_sgs_scale, (double)coll.nu_sgs_background,
+                           coll.adaptive_omega4, (double)coll.lambda, (double)coll.omega_4, t.omega1_d, t.omega4d_d);
     }
     // first block / first item of every domain (blocks sorted by domain)
     L.dom_block0_h.assign(d.n_domains + 1, 0);
@@ -219,10 +235,80 @@ int prepare_level(const OalLevelDesc& d, Level& L, HostLevel& H) {
         }
         flags[b] = fl;
     }
+    // block lists: interior blocks (all flags clear) whose 26 neighbours sit at the lexicographic offsets
+    // of their domain's block grid take the arithmetic-address kernel; other interior blocks the
+    // table kernel; the rest the boundary kernel
+    for (auto& dm : L.doms_h) {
+        dm.nbx = (dm.ext[0] + BS - 1) / BS;
+        dm.nby = (dm.ext[1] + BS - 1) / BS;
+    }
+    // (a boundary block qualifies when no neighbour inside the extent is absent and every present one is
+    // at its lexicographic offset: its out-of-extent sources take the face rules and are never read)
+    for (uint32_t b = 0; b < nb; ++b) {
+        const DevDomain& dm = L.doms_h[bdom[b]];
+        bool regular = !(flags[b] & BF_MISSING);
+        for (int k = 0; k < 27 && regular; ++k) {
+            const int ox = k % 3 - 1, oy = (k / 3) % 3 - 1, oz = k / 9 - 1;
+            const int64_t want = (int64_t)b + ox + (int64_t)oy * dm.nbx + (int64_t)oz * dm.nbx * dm.nby;
+            const int32_t have = k == 13 ? (int32_t)b : nbr[(size_t)k * nb + b];
+            if (have < 0) continue;   // absent: outside the extent (BF_MISSING is clear)
+            regular = have == want;
+        }
+        if (flags[b] == 0) (regular ? H.blist_rg : H.blist_in).push_back((int32_t)b);
+        else (regular ? H.blist_br : H.blist_bd).push_back((int32_t)b);
+    }
+    L.n_regular = (uint32_t)H.blist_rg.size();
+    L.n_interior = (uint32_t)H.blist_in.size();
+    L.n_boundary_regular = (uint32_t)H.blist_br.size();
+    L.n_boundary = (uint32_t)H.blist_bd.size();
+    // block-major copy of the neighbour table: one 32-entry row per block (scalar loads)
+    H.nbr_bm.assign((size_t)nb * 32, -1);
+    for (uint32_t b = 0; b < nb; ++b)
+        for (int k = 0; k < 27; ++k) H.nbr_bm[(size_t)b * 32 + k] = k == 13 ? (int32_t)b : nbr[(size_t)k * nb + b];
+    // blocks holding an outlet face (x = extent - 1 of a non-periodic x axis): AA side-buffer slots
+    H.oslot.assign(nb, -1);
+    for (uint32_t b = 0; b < nb; ++b) {
+        const DevDomain& dm = L.doms_h[bdom[b]];
+        const int o = borg[3 * b];
+        if (!dm.per[0] && o <= dm.ext[0] - 1 && dm.ext[0] - 1 < o + BS) H.oslot[b] = H.n_oslots++;
+    }
     L.has_solid = d.obstacle != nullptr;
     if (d.obstacle) {
         H.solid.resize(L.ncell);
         for (size_t i = 0; i < L.ncell; ++i) H.solid[i] = d.obstacle[i] ? 1 : 0;
+    }
+    // solid-link masks of the blocks flagged solid: bit k when the source x - c_k of a cell is a solid
+    // cell inside the extent, bit 27 when the cell itself is solid
+    H.lslot.assign(nb, -1);
+    int32_t nls = 0;
+    for (uint32_t b = 0; b < nb; ++b)
+        if (flags[b] & BF_SOLID) H.lslot[b] = nls++;
+    H.lmask.assign((size_t)nls * BS3, 0u);
+    for (uint32_t b = 0; b < nb; ++b) {
+        if (H.lslot[b] < 0) continue;
+        const DevDomain& dm = L.doms_h[bdom[b]];
+        uint32_t* m = H.lmask.data() + (size_t)H.lslot[b] * BS3;
+        for (int z = 0; z < BS; ++z)
+            for (int y = 0; y < BS; ++y)
+                for (int x = 0; x < BS; ++x) {
+                    uint32_t bits = H.solid[(size_t)b * BS3 + (z * BS + y) * BS + x] ? (1u << 27) : 0u;
+                    for (int k = 0; k < 27; ++k) {
+                        const int cx = k % 3 - 1, cy = (k / 3) % 3 - 1, cz = k / 9 - 1;
+                        const int g[3] = {borg[3 * b] + x - cx, borg[3 * b + 1] + y - cy, borg[3 * b + 2] + z - cz};
+                        bool in = true;
+                        for (int a = 0; a < 3; ++a)
+                            if (!dm.per[a] && (g[a] < 0 || g[a] >= dm.ext[a])) in = false;
+                        if (!in) continue;
+                        const int sx = x - cx, sy = y - cy, sz = z - cz;
+                        const int ox = sx < 0 ? -1 : (sx >= BS ? 1 : 0), oy = sy < 0 ? -1 : (sy >= BS ? 1 : 0),
+                                  oz = sz < 0 ? -1 : (sz >= BS ? 1 : 0);
+                        const int32_t nbk = (ox | oy | oz) ? nbr[(size_t)((ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)) * nb + b] : (int32_t)b;
+                        if (nbk < 0) continue;
+                        const size_t src = (size_t)nbk * BS3 + ((sz - BS * oz) * BS + (sy - BS * oy)) * BS + (sx - BS * ox);
+                        if (H.solid[src]) bits |= 1u << k;
+                    }
+                    m[(z * BS + y) * BS + x] = bits;
+                }
     }
     return OAL_OK;
 }
@@ -231,11 +317,20 @@ int prepare_level(const OalLevelDesc& d, Level& L, HostLevel& H) {
 int upload_level(OalPlan* p, Level& L, const HostLevel& H) {
     const uint32_t nb = L.nb;
     int rc;
-    if ((rc = upload(L.nbr, H.nbr)) || (rc = upload(L.borg, H.borg)) || (rc = upload(L.bdom, H.bdom)) ||
+    if ((rc = upload(L.nbr, H.nbr_bm)) || (rc = upload(L.borg, H.borg)) || (rc = upload(L.bdom, H.bdom)) ||
         (rc = upload(L.bflags, H.flags)) || (rc = upload(L.doms, L.doms_h)) || (rc = upload(L.dom_item0, L.dom_item0_h)) ||
         (rc = upload(L.dom_block0, L.dom_block0_h)))
         return rc;
     if (!H.solid.empty() && (rc = upload(L.solid, H.solid))) return rc;
+    if ((rc = upload(L.blist_rg, H.blist_rg)) || (rc = upload(L.blist_in, H.blist_in)) ||
+        (rc = upload(L.blist_br, H.blist_br)) || (rc = upload(L.blist_bd, H.blist_bd)) ||
+        (rc = upload(L.lslot, H.lslot)) || (rc = upload(L.lmask, H.lmask)))
+        return rc;
+    if (p->stream_mode == OAL_STREAM_AA) {
+        if ((rc = upload(L.oslot, H.oslot))) return rc;
+        if (H.n_oslots && L.obuf.alloc((size_t)H.n_oslots * 9 * BS2 * storage_bytes(p->storage)) != hipSuccess)
+            return fail(OAL_ERR_NOMEM, "outlet buffer allocation failed");
+    }
     const size_t fbytes = (size_t)L.ncell * 27 * storage_bytes(p->storage);
     if (L.fA.alloc(fbytes) != hipSuccess) return fail(OAL_ERR_NOMEM, "population buffer allocation failed");
     if (p->stream_mode == OAL_STREAM_AB && L.fB.alloc(fbytes) != hipSuccess)
@@ -250,6 +345,8 @@ int upload_level(OalPlan* p, Level& L, const HostLevel& H) {
     if (L.fpart.alloc((size_t)L.n_items * 4) != hipSuccess || L.mpart.alloc((size_t)nb * 4) != hipSuccess ||
         L.red_out.alloc((size_t)L.n_domains * 4) != hipSuccess || L.guard.alloc(1) != hipSuccess)
         return fail(OAL_ERR_NOMEM, "reduction buffer allocation failed");
+    // force partials start at zero: the pipelined interior kernel (no solid links) never writes them
+    OAL_HIP(hipMemset(L.fpart.p, 0, (size_t)L.n_items * 4 * sizeof(double)));
     return OAL_OK;
 }
 
@@ -257,7 +354,7 @@ StepArgs make_args(OalPlan* p, Level& L, int32_t time_step, bool want_force) {
     StepArgs a{};
     a.fin = L.fA.p;
     a.fout = p->stream_mode == OAL_STREAM_AB ? (void*)L.fB.p : (void*)L.fA.p;
-    a.nbr = L.nbr.p;
+    a.nbr_bm = L.nbr.p;
     a.borg = L.borg.p;
     a.bdom = L.bdom.p;
     a.bflags = L.bflags.p;
@@ -267,11 +364,25 @@ StepArgs make_args(OalPlan* p, Level& L, int32_t time_step, bool want_force) {
     a.vel_out = p->wale ? L.velB.p : nullptr;
     a.rho_out = nullptr;
     a.fpart = want_force ? L.fpart.p : nullptr;
+    a.oslot = L.oslot.p;
+    a.blist = nullptr;
+    a.blist_regular = L.blist_rg.p;
+    a.blist_interior = L.blist_in.p;
+    a.blist_boundary_regular = L.blist_br.p;
+    a.blist_boundary = L.blist_bd.p;
+    a.n_regular = L.n_regular;
+    a.n_interior = L.n_interior;
+    a.n_boundary_regular = L.n_boundary_regular;
+    a.n_boundary = L.n_boundary;
+    a.lslot = L.lslot.p;
+    a.lmask = L.lmask.p;
+    a.obuf = L.obuf.p;
     a.ncell = L.ncell;
     a.nb = L.nb;
     a.n_items = L.n_items;
     a.time_step = time_step;
     a.parity = p->stream_mode == OAL_STREAM_AA ? (L.aa_phase == 0 ? 1 : 0) : 0;
+    a.pipe_planes = p->pipe_planes;
     a.cp = p->cp;
     return a;
 }
@@ -282,7 +393,17 @@ int level_step(OalPlan* p, int level, int32_t time_step, int flags, hipStream_t 
     const bool want_force = (flags & OAL_STEP_FORCE) != 0;
     if (want_force && !L.has_solid) return fail(OAL_ERR_STATE, "force requested on a level without obstacle cells");
     StepArgs a = make_args(p, L, time_step, want_force);
-    OAL_HIP(launch_step(p->storage, p->stream_mode, p->wale, a, s));
+    const bool split = L.n_regular + L.n_interior > 0 && L.n_boundary_regular + L.n_boundary > 0;
+    if (split) {
+        // fork: the side stream starts after everything queued on s, then s waits for the side stream
+        OAL_HIP(hipEventRecord(p->fork, s));
+        OAL_HIP(hipStreamWaitEvent(p->side, p->fork, 0));
+    }
+    OAL_HIP(launch_step(p->storage, p->stream_mode, p->wale, a, s, split ? p->side : nullptr));
+    if (split) {
+        OAL_HIP(hipEventRecord(p->join, p->side));
+        OAL_HIP(hipStreamWaitEvent(s, p->join, 0));
+    }
     if (p->stream_mode == OAL_STREAM_AB) std::swap(L.fA, L.fB);
     else L.aa_phase ^= 1;
     if (p->wale) std::swap(L.velA, L.velB);
@@ -297,6 +418,13 @@ int level_step(OalPlan* p, int level, int32_t time_step, int flags, hipStream_t 
 }
 
 int permute_of(const OalPlan* p) { return p->stream_mode == OAL_STREAM_AA ? 1 : 0; }
+
+// AA pattern: the outlet side buffer mirrors the freshly set state (the first step is an odd one)
+hipError_t aa_prep(OalPlan* p, Level& L) {
+    if (p->stream_mode != OAL_STREAM_AA || !L.obuf.p) return hipSuccess;
+    StepArgs a = make_args(p, L, 0, false);
+    return launch_aa_outlet_prep(p->storage, a, p->stream);
+}
 
 int require_plain_state(const OalPlan* p, const Level& L) {
     if (!L.have_f) return fail(OAL_ERR_STATE, "no populations on this level");
@@ -374,13 +502,16 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
     p->levels.resize(desc->n_levels);
     std::vector<HostLevel> host(desc->n_levels);
     for (int l = 0; l < desc->n_levels; ++l) {
-        const int rc = prepare_level(desc->levels[l], p->levels[l], host[l]);
+        const int rc = prepare_level(desc->levels[l], desc->collision, p->levels[l], host[l]);
         if (rc) return bail(rc);
     }
     hipError_t e = hipSetDevice(p->device);
     if (e != hipSuccess) return bail(fail(OAL_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)));
     if ((e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipEventCreate(&p->ev0)) != hipSuccess || (e = hipEventCreate(&p->ev1)) != hipSuccess)
+        (e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&p->ev0)) != hipSuccess || (e = hipEventCreate(&p->ev1)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&p->join, hipEventDisableTiming)) != hipSuccess)
         return bail(fail(OAL_ERR_HIP, std::string("stream/event creation: ") + hipGetErrorString(e)));
     for (int l = 0; l < desc->n_levels; ++l) {
         const int rc = upload_level(p, p->levels[l], host[l]);
@@ -397,8 +528,12 @@ int oal_plan_destroy(OalPlan* p) {
         (void)hipStreamSynchronize(p->stream);
     }
     for (auto& L : p->levels) L.release();
+    if (p->side) (void)hipStreamSynchronize(p->side);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
     if (p->ev1) (void)hipEventDestroy(p->ev1);
+    if (p->fork) (void)hipEventDestroy(p->fork);
+    if (p->join) (void)hipEventDestroy(p->join);
+    if (p->side) (void)hipStreamDestroy(p->side);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return OAL_OK;
@@ -422,6 +557,7 @@ int oal_upload_f(OalPlan* p, int level, const void* host_f, int dt) {
     hipError_t e = hipMemcpyAsync(stage, host_f, bytes, hipMemcpyHostToDevice, p->stream);
     if (e == hipSuccess) e = launch_convert_in(p->storage, L.fA.p, stage, dt, L.ncell, permute_of(p), p->stream);
     if (e == hipSuccess && L.fB.p) e = hipMemcpyAsync(L.fB.p, L.fA.p, L.fA.n, hipMemcpyDeviceToDevice, p->stream);
+    if (e == hipSuccess) e = aa_prep(p, L);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
     (void)hipFree(stage);
     if (e != hipSuccess) return fail(OAL_ERR_HIP, std::string("oal_upload_f: ") + hipGetErrorString(e));
@@ -468,6 +604,7 @@ int oal_init_equilibrium(OalPlan* p, int level, const double* rho, const double*
         e = launch_init_eq(p->storage, L.fA.p, L.ncell, drho, du, rho0, ux0, uy0, uz0, permute_of(p),
                            p->wale ? (void*)L.velA.p : nullptr, p->stream);
     if (e == hipSuccess && L.fB.p) e = hipMemcpyAsync(L.fB.p, L.fA.p, L.fA.n, hipMemcpyDeviceToDevice, p->stream);
+    if (e == hipSuccess) e = aa_prep(p, L);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
     if (drho) (void)hipFree(drho);
     if (du) (void)hipFree(du);
@@ -579,6 +716,96 @@ int oal_time_steps(OalPlan* p, int level, int n, int flags, double* ms_total, do
     if (ms_total) *ms_total = ms;
     if (ms_kernel) *ms_kernel = (double)ms / n;
     return OAL_OK;
+}
+
+int oal_copy_bandwidth(int device, int64_t bytes, int reps, double* gbs) {
+    if (bytes < (1 << 20) || reps <= 0 || !gbs) return fail(OAL_ERR_ARG, "bytes >= 1 MiB, reps > 0 and an output are needed");
+    OAL_HIP(hipSetDevice(device));
+    const size_t nb = (size_t)bytes & ~(size_t)15;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, nb);
+    if (e == hipSuccess) e = hipMalloc(&b, nb);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, nb, s);
+    if (e == hipSuccess) e = launch_copy16(a, b, nb, s);   // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, s);
+    for (int i = 0; i < reps && e == hipSuccess; ++i) e = launch_copy16(a, b, nb, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) return fail(OAL_ERR_HIP, std::string("oal_copy_bandwidth: ") + hipGetErrorString(e));
+    *gbs = 2.0 * (double)nb * reps / (ms * 1e-3) / 1e9;
+    return OAL_OK;
+}
+
+int oal_get_tuning(const OalPlan* p, OalTuning* t) {
+    if (!p || !t) return fail(OAL_ERR_ARG, "NULL argument");
+    *t = OalTuning{};
+    t->pipe_planes = p->pipe_planes;
+    return OAL_OK;
+}
+
+int oal_set_tuning(OalPlan* p, const OalTuning* t) {
+    if (!p || !t) return fail(OAL_ERR_ARG, "NULL argument");
+    const int np = t->pipe_planes;
+    if (np != 1 && np != 2 && np != 4 && np != 8 && np != 16) return fail(OAL_ERR_ARG, "pipe_planes must be 1, 2, 4, 8 or 16");
+    p->pipe_planes = np;
+    return OAL_OK;
+}
+
+int oal_autotune(OalPlan* p, int level, int n_steps, OalTuning* best, double* ms_per_step) {
+    if (int rc = check_level(p, level)) return rc;
+    if (n_steps <= 0 || (n_steps & 1)) return fail(OAL_ERR_ARG, "n_steps must be positive and even");
+    Level& L = p->levels[level];
+    if (!L.have_f) return fail(OAL_ERR_STATE, "no populations on this level");
+    OAL_HIP(hipSetDevice(p->device));
+    // save the state: the candidate runs advance it
+    void* save = nullptr;
+    OAL_HIP(hipMalloc(&save, L.fA.n));
+    hipError_t e = hipMemcpyAsync(save, L.fA.p, L.fA.n, hipMemcpyDeviceToDevice, p->stream);
+    void* vsave = nullptr;
+    if (e == hipSuccess && p->wale) {
+        e = hipMalloc(&vsave, L.velA.n);
+        if (e == hipSuccess) e = hipMemcpyAsync(vsave, L.velA.p, L.velA.n, hipMemcpyDeviceToDevice, p->stream);
+    }
+    const int aa_phase = L.aa_phase;
+    const int cand[5] = {1, 2, 4, 8, 16};
+    int bestc = p->pipe_planes;
+    double bestms = 1e300;
+    int rc = OAL_OK;
+    if (e != hipSuccess) rc = fail(OAL_ERR_HIP, std::string("oal_autotune: ") + hipGetErrorString(e));
+    for (int i = 0; i < 5 && rc == OAL_OK; ++i) {
+        p->pipe_planes = cand[i];
+        double tot = 0.0, ker = 0.0;
+        rc = oal_time_steps(p, level, 2, 0, &tot, &ker);   // warm-up of this shape
+        if (rc == OAL_OK) rc = oal_time_steps(p, level, n_steps, 0, &tot, &ker);
+        if (rc != OAL_OK) break;
+        if (ms_per_step) ms_per_step[i] = ker;
+        if (ker < bestms) { bestms = ker; bestc = cand[i]; }
+    }
+    p->pipe_planes = bestc;
+    // restore
+    if (e == hipSuccess) e = hipMemcpyAsync(L.fA.p, save, L.fA.n, hipMemcpyDeviceToDevice, p->stream);
+    if (e == hipSuccess && L.fB.p) e = hipMemcpyAsync(L.fB.p, save, L.fA.n, hipMemcpyDeviceToDevice, p->stream);
+    if (e == hipSuccess && vsave) e = hipMemcpyAsync(L.velA.p, vsave, L.velA.n, hipMemcpyDeviceToDevice, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    (void)hipFree(save);
+    if (vsave) (void)hipFree(vsave);
+    L.aa_phase = aa_phase;
+    L.force_valid = false;
+    if (rc == OAL_OK && e != hipSuccess) rc = fail(OAL_ERR_HIP, std::string("oal_autotune: ") + hipGetErrorString(e));
+    if (rc == OAL_OK && best) { *best = OalTuning{}; best->pipe_planes = bestc; }
+    return rc;
 }
 
 }  // extern "C"

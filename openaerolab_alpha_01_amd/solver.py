@@ -216,6 +216,26 @@ class Plan:
                                       C.byref(ker)), "oal_time_steps")
         return tot.value, ker.value
 
+    # -- launch shape ------------------------------------------------------------------------------
+    @property
+    def pipe_planes(self) -> int:
+        t = _lib.OalTuning()
+        check(self.lib.oal_get_tuning(self.h, C.byref(t)), "oal_get_tuning")
+        return t.pipe_planes
+
+    @pipe_planes.setter
+    def pipe_planes(self, v: int) -> None:
+        t = _lib.OalTuning()
+        t.pipe_planes = int(v)
+        check(self.lib.oal_set_tuning(self.h, C.byref(t)), "oal_set_tuning")
+
+    def autotune(self, n_steps: int = 10, level: int = 0) -> tuple[int, dict]:
+        """Time every launch shape, keep the fastest; the populations are restored afterwards."""
+        best = _lib.OalTuning()
+        ms = (C.c_double * 5)()
+        check(self.lib.oal_autotune(self.h, level, n_steps, C.byref(best), ms), "oal_autotune")
+        return best.pipe_planes, dict(zip((1, 2, 4, 8, 16), list(ms)))
+
     # -- results -----------------------------------------------------------------------------------
     def force(self, level: int = 0) -> np.ndarray:
         nd = len(self.levels[level].domains)

@@ -245,7 +245,7 @@ struct OrParams {
     int32_t zwall_bc;         // 0: legacy mirror, 2: no-slip bounce-back
     int32_t bb_mode;          // 0: reference solid reflection, 1: halfway bounce-back
     int32_t time_step_seed;
-    int32_t pad;
+    int32_t arith;            // 0: literal reference arithmetic, 1: the library's fused arithmetic
 };
 
 struct OrGrid {
@@ -486,6 +486,216 @@ void collide(const T fin[27], T rho, T jx, T jy, T jz, T nu_eddy, const Coll<T>&
     }
 }
 
+// omega1 and the diagonal 4th-order rate of a cell (kernel_cumulant.jl:592-610), shared by both
+// arithmetic forms
+template <class T>
+void rates(T nu_eddy, const Coll<T>& c, T& omega1, T& omega4_diag) {
+    nu_eddy = nu_eddy * c.amr_scale;
+    nu_eddy = std::max(nu_eddy, c.nu_sgs_bg * c.amr_scale);
+    T nu_mol = (c.tau - T(0.5)) / T(3);
+    T nu_eff = nu_mol + nu_eddy;
+    T tau_eff = T(0.5) + T(3) * nu_eff;
+    omega1 = T(1) / std::max(tau_eff, T(0.500001f));
+    omega4_diag = c.omega4;
+    if (c.adaptive == 1) {
+        T om1_m2 = omega1 - T(2);
+        T denom_adaptive = c.lambda * omega1 + T(1e-10f);
+        T A_val = -(om1_m2 * om1_m2) / denom_adaptive;
+        omega4_diag = std::min(std::max(A_val, T(0.01f)), T(2));
+    }
+}
+
+template <class T> inline T fma_(T a, T b, T c) { return std::fma(a, b, c); }
+
+// The library's arithmetic of the fluid-cell update: the same cumulant collision with fused
+// multiply-adds and with rho and j taken from the raw partial sums of the transform passes (one
+// division fewer, no separate accumulation).  Every operation below is the one liboal executes, in
+// the same order; the result differs from the literal form by rounding only.
+template <class T>
+void collide_fast(T f[27], T nu_eddy, const Coll<T>& c, T& rho_o, T u_o[3]) {
+    T A[27];
+    // x-pass raw sums per row r = iy + 3 iz: (f-, f0, f+) -> (m0, d = f+ - f-, s = f- + f+)
+    for (int r = 0; r < 9; ++r) {
+        const T s = f[3 * r] + f[3 * r + 2];
+        const T d = f[3 * r + 2] - f[3 * r];
+        A[3 * r] = s + f[3 * r + 1];
+        A[3 * r + 1] = d;
+        A[3 * r + 2] = s;
+    }
+    // raw y-sums of the zeroth and first x-moment columns give rho and j
+    T y0[3], yd0[3], y1[3];
+    for (int iz = 0; iz < 3; ++iz) {
+        const T a = A[9 * iz], b = A[3 + 9 * iz], cc = A[6 + 9 * iz];
+        const T ys = a + cc;
+        yd0[iz] = cc - a;
+        y0[iz] = ys + b;
+        const T a1 = A[1 + 9 * iz], b1 = A[4 + 9 * iz], c1 = A[7 + 9 * iz];
+        y1[iz] = (a1 + c1) + b1;
+    }
+    const T rho = (y0[0] + y0[2]) + y0[1];
+    const T jz = y0[2] - y0[0];
+    const T jy = (yd0[0] + yd0[2]) + yd0[1];
+    const T jx = (y1[0] + y1[2]) + y1[1];
+
+    const T rc = std::max(rho, T(0.01f));
+    const T inv = T(1) / rc;
+    const T ux = jx * inv, uy = jy * inv, uz = jz * inv;
+    T uxe = ux, uye = uy, uze = uz;
+    const bool forced = c.F[0] != T(0) || c.F[1] != T(0) || c.F[2] != T(0);
+    if (forced) {
+        uxe = fma_(T(0.5) * c.F[0], inv, ux);
+        uye = fma_(T(0.5) * c.F[1], inv, uy);
+        uze = fma_(T(0.5) * c.F[2], inv, uz);
+    }
+    rho_o = rc; u_o[0] = ux; u_o[1] = uy; u_o[2] = uz;
+    T omega1, omega4_diag;
+    rates(nu_eddy, c, omega1, omega4_diag);
+
+    // x shift to central moments
+    {
+        const T u = uxe, uu = u * u, t = T(-2) * u;
+        for (int r = 0; r < 9; ++r) {
+            const T m0 = A[3 * r], d = A[3 * r + 1], s = A[3 * r + 2];
+            A[3 * r + 1] = fma_(-u, m0, d);
+            A[3 * r + 2] = fma_(uu, m0, fma_(t, d, s));
+        }
+    }
+    // y and z passes
+    auto pass = [&](int i0, int i1, int i2, T u) {
+        const T uu = u * u, t = T(-2) * u;
+        const T a = A[i0], b = A[i1], cc = A[i2];
+        const T ys = a + cc, yd = cc - a, ym0 = ys + b;
+        A[i0] = ym0;
+        A[i1] = fma_(-u, ym0, yd);
+        A[i2] = fma_(uu, ym0, fma_(t, yd, ys));
+    };
+    for (int iz = 0; iz < 3; ++iz)
+        for (int ma = 0; ma < 3; ++ma) pass(ma + 9 * iz, ma + 3 + 9 * iz, ma + 6 + 9 * iz, uye);
+    for (int q = 0; q < 9; ++q) pass(q, q + 9, q + 18, uze);
+
+    // A[0] equals rho bit for bit (same sums in the same order): 1 / max(k000, 0.01) = inv
+    const T c200 = A[2] * inv, c020 = A[6] * inv, c002 = A[18] * inv;
+    const T c110 = A[4] * inv, c101 = A[10] * inv, c011 = A[12] * inv;
+    const T c210 = A[5] * inv, c120 = A[7] * inv, c201 = A[11] * inv;
+    const T c021 = A[15] * inv, c102 = A[19] * inv, c012 = A[21] * inv;
+    const T c111 = A[13] * inv;
+    const T c220n = A[8] * inv, c202n = A[20] * inv, c022n = A[24] * inv;
+    const T c211n = A[14] * inv, c121n = A[16] * inv, c112n = A[22] * inv;
+    const T x110 = T(2) * c110, x101 = T(2) * c101, x011 = T(2) * c011;
+    const T C220 = fma_(-x110, c110, fma_(-c200, c020, c220n));
+    const T C202 = fma_(-x101, c101, fma_(-c200, c002, c202n));
+    const T C022 = fma_(-x011, c011, fma_(-c020, c002, c022n));
+    const T C211 = fma_(-x110, c101, fma_(-c200, c011, c211n));
+    const T C121 = fma_(-x110, c011, fma_(-c020, c101, c121n));
+    const T C112 = fma_(-x101, c011, fma_(-c002, c110, c112n));
+
+    const T third = T(1) / T(3);
+    const T tr = (c200 + c020) + c002;
+    const T tr_s = fma_(T(1) - c.omega_bulk, tr, c.omega_bulk);
+    const T g1 = T(1) - omega1;
+    const T d1s = g1 * (c200 - c020), d2s = g1 * (c200 - c002);
+    const T t3 = tr_s * third;
+    const T s200 = fma_(d1s + d2s, third, t3);
+    const T s020 = fma_(fma_(T(-2), d1s, d2s), third, t3);
+    const T s002 = fma_(fma_(T(-2), d2s, d1s), third, t3);
+    T s110 = g1 * c110, s101 = g1 * c101, s011 = g1 * c011;
+    if (c.limiter == 1) {
+        const T q110 = std::max(s200 * s020, T(0)), q101 = std::max(s200 * s002, T(0)), q011 = std::max(s020 * s002, T(0));
+        if (s110 * s110 > q110) s110 = std::copysign(std::sqrt(q110), s110);
+        if (s101 * s101 > q101) s101 = std::copysign(std::sqrt(q101), s101);
+        if (s011 * s011 > q011) s011 = std::copysign(std::sqrt(q011), s011);
+    }
+    const T g3 = T(1) - c.omega3, g4 = T(1) - c.omega4, g4d = T(1) - omega4_diag;
+    const T s210 = g3 * c210, s120 = g3 * c120, s201 = g3 * c201, s021 = g3 * c021;
+    const T s102 = g3 * c102, s012 = g3 * c012, s111 = g3 * c111;
+    const T S220 = g4d * C220, S202 = g4d * C202, S022 = g4d * C022;
+    const T S211 = g4 * C211, S121 = g4 * C121, S112 = g4 * C112;
+
+    const T y110 = T(2) * s110, y101 = T(2) * s101, y011 = T(2) * s011;
+    const T s220 = fma_(y110, s110, fma_(s200, s020, S220));
+    const T s202 = fma_(y101, s101, fma_(s200, s002, S202));
+    const T s022 = fma_(y011, s011, fma_(s020, s002, S022));
+    const T s211 = fma_(y110, s101, fma_(s200, s011, S211));
+    const T s121 = fma_(y110, s011, fma_(s020, s101, S121));
+    const T s112 = fma_(y101, s011, fma_(s002, s110, S112));
+    const T s221 = fma_(y011, s210, fma_(y101, s120, fma_(T(2) * y110, s111, fma_(s020, s201, s200 * s021))));
+    const T s212 = fma_(y011, s201, fma_(T(2) * y101, s111, fma_(y110, s102, fma_(s002, s210, s200 * s012))));
+    const T s122 = fma_(y101, s021, fma_(T(2) * y011, s111, fma_(y110, s012, fma_(s002, s120, s020 * s102))));
+    T e = s200 * S022;
+    e = fma_(s020, S202, e);
+    e = fma_(s002, S220, e);
+    e = fma_(T(2) * y110, S112, e);
+    e = fma_(T(2) * y101, S121, e);
+    e = fma_(T(2) * y011, S211, e);
+    e = fma_(T(2) * s210, s012, e);
+    e = fma_(T(2) * s201, s021, e);
+    e = fma_(T(2) * s120, s102, e);
+    e = fma_(T(4) * s111, s111, e);
+    e = fma_(s200 * s020, s002, e);
+    e = fma_(s200 * y011, s011, e);
+    e = fma_(s020 * y101, s101, e);
+    e = fma_(s002 * y110, s110, e);
+    e = fma_((T(4) * y110) * s101, s011, e);
+    const T s222 = e;
+
+    const T r = rho;
+    A[0] = r;
+    A[1] = T(0); A[3] = T(0); A[9] = T(0);
+    A[2] = r * s200; A[6] = r * s020; A[18] = r * s002;
+    A[4] = r * s110; A[10] = r * s101; A[12] = r * s011;
+    A[5] = r * s210; A[7] = r * s120; A[11] = r * s201;
+    A[15] = r * s021; A[19] = r * s102; A[21] = r * s012;
+    A[13] = r * s111;
+    A[8] = r * s220; A[20] = r * s202; A[24] = r * s022;
+    A[14] = r * s211; A[16] = r * s121; A[22] = r * s112;
+    A[17] = r * s221; A[23] = r * s212; A[25] = r * s122;
+    A[26] = r * s222;
+
+    auto back = [&](int i0, int i1, int i2, T u) {
+        const T uu = u * u, t = T(2) * u;
+        const T a = A[i0], b = A[i1], cc = A[i2];
+        const T m1 = fma_(u, a, b);
+        const T m2 = fma_(uu, a, fma_(t, b, cc));
+        const T h = T(0.5) * m1;
+        A[i0] = fma_(T(0.5), m2, -h);
+        A[i1] = a - m2;
+        A[i2] = fma_(T(0.5), m2, h);
+    };
+    for (int q = 0; q < 9; ++q) back(q, q + 9, q + 18, uze);
+    for (int iz = 0; iz < 3; ++iz)
+        for (int ma = 0; ma < 3; ++ma) back(ma + 9 * iz, ma + 3 + 9 * iz, ma + 6 + 9 * iz, uye);
+    for (int q = 0; q < 9; ++q) back(3 * q, 3 * q + 1, 3 * q + 2, uxe);
+
+    if (c.limiter == 2) {   // positivity (kernel_cumulant.jl:849-884), as in the literal form
+        T fmin = A[0];
+        for (int k = 1; k < 27; ++k) fmin = std::min(fmin, A[k]);
+        if (fmin < T(0)) {
+            T alpha = T(1);
+            for (int k = 0; k < 27; ++k) {
+                T feq = feq_auto(rc, uxe, uye, uze, weight<T>(k), (T)LAT.cx[k], (T)LAT.cy[k], (T)LAT.cz[k], c.compressible);
+                T fneq = A[k] - feq;
+                if (fneq < T(0) && feq > T(1e-15f)) alpha = std::min(alpha, feq / (feq - fneq + T(1e-15f)));
+            }
+            alpha = std::max(alpha, T(0));
+            for (int k = 0; k < 27; ++k) {
+                T feq = feq_auto(rc, uxe, uye, uze, weight<T>(k), (T)LAT.cx[k], (T)LAT.cy[k], (T)LAT.cz[k], c.compressible);
+                A[k] = feq + alpha * (A[k] - feq);
+            }
+        }
+    }
+    if (forced) {   // Guo term (kernel_cumulant.jl:889-905)
+        const T g = T(1) - T(0.5) * omega1;
+        for (int k = 0; k < 27; ++k) {
+            const T cxf = (T)LAT.cx[k], cyf = (T)LAT.cy[k], czf = (T)LAT.cz[k];
+            const T cu = cxf * uxe + cyf * uye + czf * uze;
+            const T ft = weight<T>(k) * T(3) * ((cxf - ux + T(3) * cu * cxf) * c.F[0] + (cyf - uy + T(3) * cu * cyf) * c.F[1] +
+                                                (czf - uz + T(3) * cu * czf) * c.F[2]);
+            A[k] = A[k] + g * ft;
+        }
+    }
+    for (int k = 0; k < 27; ++k) f[k] = A[k];
+}
+
 // Grid helpers
 struct Geo {
     const OrGrid& g;
@@ -611,7 +821,12 @@ void step_impl(const OrGrid& g, const OrParams& prm, const typename P::st* fin, 
                     }
 
                     T out[27], rho_o, u_o[3];
-                    collide(fs, rho, jx, jy, jz, nu_eddy, cp, out, rho_o, u_o);
+                    if (prm.arith == 1) {
+                        for (int k = 0; k < 27; ++k) out[k] = fs[k];
+                        collide_fast(out, nu_eddy, cp, rho_o, u_o);
+                    } else {
+                        collide(fs, rho, jx, jy, jz, nu_eddy, cp, out, rho_o, u_o);
+                    }
                     if (vel_out) { vel_out[own] = u_o[0]; vel_out[ncell + own] = u_o[1]; vel_out[2 * ncell + own] = u_o[2]; }
                     if (rho_out) rho_out[own] = rho_o;
                     for (int k = 0; k < 27; ++k) fout[(int64_t)k * ncell + own] = P::sto(out[k], w[k]);
@@ -750,6 +965,21 @@ void oracle_collide_f32(const float* f, const OrParams* p, float nu_eddy, float*
     rho_u[0] = r; rho_u[1] = u[0]; rho_u[2] = u[1]; rho_u[3] = u[2];
 }
 double oracle_wale_nu_f64(const double* g, double c_wale) { return wale_nu(g, c_wale); }
+
+void oracle_collide_fast_f64(const double* f, const OrParams* p, double nu_eddy, double* out, double* rho_u) {
+    Coll<double> c(*p);
+    double r, u[3];
+    for (int k = 0; k < 27; ++k) out[k] = f[k];
+    collide_fast(out, nu_eddy, c, r, u);
+    rho_u[0] = r; rho_u[1] = u[0]; rho_u[2] = u[1]; rho_u[3] = u[2];
+}
+void oracle_collide_fast_f32(const float* f, const OrParams* p, float nu_eddy, float* out, float* rho_u) {
+    Coll<float> c(*p);
+    float r, u[3];
+    for (int k = 0; k < 27; ++k) out[k] = f[k];
+    collide_fast(out, nu_eddy, c, r, u);
+    rho_u[0] = r; rho_u[1] = u[0]; rho_u[2] = u[1]; rho_u[3] = u[2];
+}
 
 // storage codes: 0 f32, 1 f64, 2 f16 shifted, 3 bf16 shifted
 int oracle_step(int storage, const OrGrid* g, const OrParams* p, const void* fin, void* fout,

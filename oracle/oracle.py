@@ -39,6 +39,7 @@ HERE = Path(__file__).resolve().parent
 LIB = HERE / "liboal_oracle.so"
 
 STORAGE = {"f32": 0, "f64": 1, "f16s": 2, "bf16s": 3}
+LITERAL, FAST = 0, 1
 STORE_DTYPE = {"f32": np.float32, "f64": np.float64, "f16s": np.uint16, "bf16s": np.uint16}
 WORK_DTYPE = {"f32": np.float32, "f64": np.float64, "f16s": np.float32, "bf16s": np.float32}
 
@@ -50,7 +51,7 @@ class OrParams(C.Structure):
         ("u_inlet", C.c_double), ("inlet_turb", C.c_double), ("body_force", C.c_double * 3),
         ("adaptive_omega4", C.c_int32), ("limiter", C.c_int32), ("compressible", C.c_int32),
         ("ymin_wall_bc", C.c_int32), ("zwall_bc", C.c_int32), ("bb_mode", C.c_int32), ("time_step_seed", C.c_int32),
-        ("pad", C.c_int32),
+        ("arith", C.c_int32),
     ]
 
 
@@ -88,6 +89,8 @@ def load() -> C.CDLL:
     lib.oracle_chimera_f64.argtypes = [vp, C.c_double, C.c_int]
     lib.oracle_collide_f64.argtypes = [vp, C.POINTER(OrParams), C.c_double, vp, vp]
     lib.oracle_collide_f32.argtypes = [vp, C.POINTER(OrParams), C.c_float, vp, vp]
+    lib.oracle_collide_fast_f64.argtypes = [vp, C.POINTER(OrParams), C.c_double, vp, vp]
+    lib.oracle_collide_fast_f32.argtypes = [vp, C.POINTER(OrParams), C.c_float, vp, vp]
     lib.oracle_wale_nu_f64.argtypes = [vp, C.c_double]
     lib.oracle_wale_nu_f64.restype = C.c_double
     for nm in ("oracle_f32_to_f16", "oracle_f16_to_f32", "oracle_f32_to_bf16", "oracle_bf16_to_f32"):
@@ -104,7 +107,10 @@ def f32(v: float) -> float:
 def make_params(tau, *, omega_bulk=1.0, omega_3=1.0, omega_4=1.0, lambda_=1.0 / 6.0, adaptive_omega4=1, limiter=1,
                 compressible=0, c_wale=0.0, nu_sgs_background=0.0, amr_sgs_scale=1.0, u_inlet=0.0,
                 inlet_turbulence=0.0, body_force=(0.0, 0.0, 0.0), ymin_wall_bc=0, zwall_bc=0, bb_mode=1,
-                time_step=0) -> OrParams:
+                time_step=0, arith=LITERAL) -> OrParams:
+    """Step parameters; arith LITERAL restates the reference arithmetic operation for operation, FAST the
+    library's fused arithmetic (same mathematics, fused multiply-adds, rho and j from the transform's
+    partial sums)."""
     p = OrParams()
     p.tau, p.omega_bulk, p.omega3, p.omega4 = f32(tau), f32(omega_bulk), f32(omega_3), f32(omega_4)
     p.lambda_, p.c_wale, p.nu_sgs_bg, p.amr_sgs_scale = f32(lambda_), f32(c_wale), f32(nu_sgs_background), f32(amr_sgs_scale)
@@ -112,6 +118,7 @@ def make_params(tau, *, omega_bulk=1.0, omega_3=1.0, omega_4=1.0, lambda_=1.0 / 
     p.body_force[:] = [f32(v) for v in body_force]
     p.adaptive_omega4, p.limiter, p.compressible = adaptive_omega4, limiter, compressible
     p.ymin_wall_bc, p.zwall_bc, p.bb_mode, p.time_step_seed = ymin_wall_bc, zwall_bc, bb_mode, time_step
+    p.arith = arith
     return p
 
 
@@ -215,11 +222,13 @@ def lattice():
 
 
 def collide_cell_f64(f: np.ndarray, params: OrParams, nu_eddy: float = 0.0):
+    """One fluid-cell collision in FP64, in the arithmetic form params.arith selects."""
     lib = load()
     f = np.ascontiguousarray(f, dtype=np.float64)
     out = np.zeros(27)
     ru = np.zeros(4)
-    lib.oracle_collide_f64(f.ctypes.data, C.byref(params), nu_eddy, out.ctypes.data, ru.ctypes.data)
+    fn = lib.oracle_collide_fast_f64 if params.arith == FAST else lib.oracle_collide_f64
+    fn(f.ctypes.data, C.byref(params), nu_eddy, out.ctypes.data, ru.ctypes.data)
     return out, ru
 
 
@@ -228,7 +237,8 @@ def collide_cell_f32(f: np.ndarray, params: OrParams, nu_eddy: float = 0.0):
     f = np.ascontiguousarray(f, dtype=np.float32)
     out = np.zeros(27, dtype=np.float32)
     ru = np.zeros(4, dtype=np.float32)
-    lib.oracle_collide_f32(f.ctypes.data, C.byref(params), nu_eddy, out.ctypes.data, ru.ctypes.data)
+    fn = lib.oracle_collide_fast_f32 if params.arith == FAST else lib.oracle_collide_f32
+    fn(f.ctypes.data, C.byref(params), nu_eddy, out.ctypes.data, ru.ctypes.data)
     return out, ru
 
 
