@@ -45,15 +45,21 @@ def bytes_per_lup(storage: str) -> int:
     return 2 * 27 * {"f32": 4, "f64": 8, "f16s": 2, "bf16s": 2}[storage]
 
 
-def gpu_leg(grid, storage: str, stream: str, steps: int, warmup: int, autotune: bool = True):
+def gpu_leg(grid, storage: str, stream: str, steps: int, warmup: int, autotune: bool = True, fixed=None):
     from openaerolab_alpha_01_amd import solver
     params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
     plan = solver.uniform_plan(grid, tau_of(), storage=storage, stream=stream, params=params, u_inlet=U_IN)
     plan.init_equilibrium(None, None, 1.0, (U_IN, 0.0, 0.0))
+    info = plan.level_info()
     tuning = None
     if autotune:
         best, table = plan.autotune(6)
-        tuning = {"pipe_planes": best, "ms_per_step_by_pipe_planes": {str(k): round(v, 4) for k, v in table.items()}}
+        tuning = dict(best, ms_per_step={f"pipe{pp}_overlap{ov}_face{fk}": round(v, 4)
+                                         for (pp, ov, fk), v in table.items()})
+    else:
+        if fixed:
+            plan.set_tuning(**fixed)
+        tuning = plan.tuning
     if warmup:
         plan.time_steps(warmup)
     plan.sync()
@@ -65,7 +71,7 @@ def gpu_leg(grid, storage: str, stream: str, steps: int, warmup: int, autotune: 
     F = plan.force()[0]
     plan.guard(0.2, 0.5)
     plan.close()
-    return ms_total, ms_kernel, wall, F, tuning
+    return ms_total, ms_kernel, wall, F, tuning, info
 
 
 def copy_bandwidth_gbs() -> float:
@@ -112,11 +118,22 @@ def main(argv=None) -> int:
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-copy-bw", action="store_true", help="skip the copy-bandwidth reference measurement")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--pipe-planes", type=int, default=None, choices=[1, 2, 4, 8, 16],
+                    help="fixed planes per workgroup of the bulk kernel (implies --no-autotune)")
+    ap.add_argument("--overlap", type=int, default=None, choices=[0, 1],
+                    help="fixed: boundary launches on a second stream (1) or in order (0) (implies --no-autotune)")
+    ap.add_argument("--face-kernel", type=int, default=None, choices=[0, 1],
+                    help="fixed: face cells by a separate kernel (1) or by the bulk kernel's rules (0) "
+                         "(implies --no-autotune)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU halo exchange is a design, not an implementation)", file=sys.stderr)
         return 2
+    fixed = {k: v for k, v in (("pipe_planes", args.pipe_planes), ("overlap", args.overlap),
+                               ("face_kernel", args.face_kernel)) if v is not None}
+    if fixed:
+        args.no_autotune = True
     if args.steps % 2 or args.warmup % 2:
         args.steps += args.steps % 2
         args.warmup += args.warmup % 2
@@ -124,9 +141,16 @@ def main(argv=None) -> int:
     grid = workload_grid()
     n_cells = N ** 3
     n_fluid = int(n_cells - grid.solid.sum())
-    ms_total, ms_kernel, wall, F, tun = gpu_leg(grid, "f32", args.stream, args.steps, args.warmup, not args.no_autotune)
+    ms_total, ms_kernel, wall, F, tun, info = gpu_leg(grid, "f32", args.stream, args.steps, args.warmup,
+                                                      not args.no_autotune, fixed)
     mlups = n_cells * args.steps / (ms_total * 1e-3) / 1e6
-    achieved = bytes_per_lup("f32") * n_fluid / (ms_kernel * 1e-3) / 1e9
+    ms_step = ms_total / args.steps
+    pp = tun["pipe_planes"]
+    # dominant kernel: the bulk launch over the regular blocks (every cell fluid) less the face cells,
+    # which the face-cell kernel steps
+    bulk_cells = (info["n_regular"] + info["n_interior"]) * 16 ** 3 - info["n_face_cells"]
+    achieved = bytes_per_lup("f32") * bulk_cells / (ms_kernel * 1e-3) / 1e9
+    step_gbs = bytes_per_lup("f32") * n_fluid / (ms_step * 1e-3) / 1e9
     area = np.pi * (D / 2) ** 2
     cd = float(F[0] / (0.5 * U_IN ** 2 * area))
 
@@ -154,19 +178,30 @@ def main(argv=None) -> int:
                    "cells": n_cells, "fluid_cells": n_fluid, "stream": args.stream, "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "bytes_per_lup": bytes_per_lup("f32"), "kernel_ms": round(ms_kernel, 5)},
+                     "kernel": (f"k_step_pipe<StF32, {pp}>" if args.stream == "ab" and pp > 1
+                                else "k_step<StF32, regular>"),
+                     "kernel_ms": round(ms_kernel, 5), "cells_per_launch": bulk_cells,
+                     "bytes_per_lup": bytes_per_lup("f32"),
+                     "step_ms": round(ms_step, 5), "step_achieved_gbs": round(step_gbs, 1),
+                     "step_frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "blocks": {k: info[k] for k in ("n_blocks", "n_regular", "n_interior", "n_boundary_regular", "n_boundary",
+                                        "n_face_cells")},
         "cd_transient": round(cd, 5),
         "tuning": tun,
     }
     if not args.no_fp16:
-        ms_t16, ms_k16, _, F16, tun16 = gpu_leg(grid, "f16s", args.stream, args.steps, args.warmup, not args.no_autotune)
+        ms_t16, ms_k16, _, F16, tun16, _ = gpu_leg(grid, "f16s", args.stream, args.steps, args.warmup,
+                                                   not args.no_autotune, fixed)
         m16 = n_cells * args.steps / (ms_t16 * 1e-3) / 1e6
-        a16 = bytes_per_lup("f16s") * n_fluid / (ms_k16 * 1e-3) / 1e9
+        a16 = bytes_per_lup("f16s") * bulk_cells / (ms_k16 * 1e-3) / 1e9
+        s16 = bytes_per_lup("f16s") * n_fluid / (ms_t16 / args.steps * 1e-3) / 1e9
         out["fp16_shifted"] = {"value": round(m16, 1), "unit": "MLUPS", "ms_per_step": round(ms_t16 / args.steps, 4),
                                "speedup_vs_f32": round(m16 / mlups, 3), "tuning": tun16,
                                "roofline": {"achieved": round(a16, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                            "frac": round(a16 / HBM_PEAK_GBS, 4),
-                                            "bytes_per_lup": bytes_per_lup("f16s")}}
+                                            "frac": round(a16 / HBM_PEAK_GBS, 4), "kernel_ms": round(ms_k16, 5),
+                                            "bytes_per_lup": bytes_per_lup("f16s"),
+                                            "step_achieved_gbs": round(s16, 1),
+                                            "step_frac": round(s16 / HBM_PEAK_GBS, 4)}}
     if not args.no_copy_bw:
         bw = copy_bandwidth_gbs()
         out["roofline"]["measured_copy_gbs"] = round(bw, 1)

@@ -132,6 +132,25 @@ int oal_plan_destroy(OalPlan* plan);
 /* Number of cells (n_blocks * bs^3) of a level: the length of one direction of f. */
 int oal_level_cells(const OalPlan* plan, int level, int64_t* n_cells);
 
+/* Replaces: nothing (introspection).  How a level's blocks split over the step kernels: blocks on
+ * a regular lexicographic grid with no boundary rule or solid link (n_regular, one arithmetic-address
+ * launch; in the AB pattern also whole blocks on a domain face with no solid near), other blocks
+ * with no boundary rule (n_interior), and the boundary blocks of both kinds, launched on a second
+ * stream that overlaps the bulk launches. */
+typedef struct OalLevelInfo {
+    int64_t n_cells;         /* n_blocks * bs^3 */
+    int64_t n_fluid_cells;   /* inside the domain extent and not solid */
+    int32_t n_blocks;
+    int32_t n_regular;
+    int32_t n_interior;
+    int32_t n_boundary_regular;
+    int32_t n_boundary;
+    int32_t n_face_cells;    /* AB pattern: cells of regular blocks on a domain face, stepped by the
+                                face-cell kernel on the second stream */
+    int32_t reserved[2];
+} OalLevelInfo;
+int oal_level_info(const OalPlan* plan, int level, OalLevelInfo* info);
+
 /* Populations in/out in the reference layout f[x,y,z,b,k], unshifted; host_dtype OAL_F32 or
  * OAL_F64.  The library converts to its storage (and from it).  Replaces the copyto! of
  * BlockLevel.f / f_temp (src/geometry/blocks.jl:604-690, src/solver/solver_control.jl:830-851). */
@@ -181,29 +200,42 @@ int oal_guard(OalPlan* plan, int level, float rho_min, float u_max, int64_t* bad
 /* Waits for all work of the plan. */
 int oal_sync(OalPlan* plan);
 
-/* Times n steps of one level on the plan's stream with HIP events: *ms_total is the wall time of
- * the n launches, *ms_kernel the mean duration of one step (all its launches). */
+/* Times n steps of one level on the plan's stream with HIP events: *ms_total is the time of the n
+ * steps (all their launches), *ms_kernel the mean duration of the bulk launches of one step (the
+ * regular and interior block lists, events on the plan's stream around them; the mean step time
+ * when the level has no such blocks). */
 int oal_time_steps(OalPlan* plan, int level, int n, int flags, double* ms_total, double* ms_kernel);
 
 /* Replaces: nothing.  Bandwidth of a streaming 16-byte device copy of `bytes` (read + write counted),
- * best of `reps` back-to-back launches: the measured HBM ceiling beside the 8 TB/s datasheet peak. */
+ * mean over `reps` back-to-back launches after one warm-up: the measured HBM ceiling beside the
+ * 8 TB/s datasheet peak. */
 int oal_copy_bandwidth(int device, int64_t bytes, int reps, double* gbs);
 
 /* Launch shape of the step kernels.  pipe_planes: z-planes of a 16^3 block one workgroup of the
  * interior kernel walks with its next plane's loads in flight (1 = one plane per workgroup, no
- * pipelining; 2, 4, 8 or 16). */
+ * pipelining; 2, 4, 8 or 16).  overlap: 1 = boundary and face launches on a second stream,
+ * concurrent with the bulk launch; 0 = every launch in order on the plan's stream.  face_kernel
+ * (pull/AB pattern, blocks on a domain face): 0 = the pipelined kernel applies the domain-face rules
+ * itself; 1 = it skips the cells on the faces and a separate face-cell kernel steps them. */
 typedef struct {
     int32_t pipe_planes;
-    int32_t reserved[7];
+    int32_t overlap;
+    int32_t face_kernel;
+    int32_t reserved[5];
 } OalTuning;
+
+/* number of launch shapes the autotuner times: pipe_planes {1, 2, 4, 8, 16} x overlap {1, 0} x
+ * face_kernel {0, 1} */
+#define OAL_TUNE_SHAPES 20
 
 /* Replaces: nothing (the reference picks its launch shape statically, src/collision/timestep.jl:1479). */
 int oal_get_tuning(const OalPlan* plan, OalTuning* tuning);
 int oal_set_tuning(OalPlan* plan, const OalTuning* tuning);
 
 /* The project's autotuner: times every candidate launch shape on `level` for n_steps steps each,
- * keeps the fastest and restores the populations the plan held before; *best receives the choice
- * and ms_per_step[5] (may be NULL) the mean step time of pipe_planes 1, 2, 4, 8, 16. */
+ * keeps the fastest and restores the populations the plan held before; *best receives the choice.
+ * ms_per_step[OAL_TUNE_SHAPES] (may be NULL) receives the mean step time of shape i: pipe_planes
+ * {1, 2, 4, 8, 16}[i % 5], overlap 1 for (i / 5) even, face_kernel 1 for i >= 10. */
 int oal_autotune(OalPlan* plan, int level, int n_steps, OalTuning* best, double* ms_per_step);
 
 #ifdef __cplusplus

@@ -24,7 +24,7 @@ STORAGE_CODES = {"f32": OAL_F32, "f64": OAL_F64, "f16s": OAL_F16S, "bf16s": OAL_
 # every entry point declared in include/oal.h
 EXPORTS = [
     "oal_abi_version", "oal_strerror", "oal_last_error", "oal_plan_create", "oal_plan_destroy",
-    "oal_level_cells", "oal_upload_f", "oal_download_f", "oal_init_equilibrium", "oal_level_step",
+    "oal_level_cells", "oal_level_info", "oal_upload_f", "oal_download_f", "oal_init_equilibrium", "oal_level_step",
     "oal_step", "oal_force", "oal_mass", "oal_download_macro", "oal_guard", "oal_sync", "oal_time_steps",
     "oal_get_tuning", "oal_set_tuning", "oal_autotune", "oal_copy_bandwidth",
 ]
@@ -63,8 +63,21 @@ class OalPlanDesc(C.Structure):
     ]
 
 
+class OalLevelInfo(C.Structure):
+    _fields_ = [
+        ("n_cells", C.c_int64), ("n_fluid_cells", C.c_int64), ("n_blocks", C.c_int32), ("n_regular", C.c_int32),
+        ("n_interior", C.c_int32), ("n_boundary_regular", C.c_int32), ("n_boundary", C.c_int32),
+        ("n_face_cells", C.c_int32), ("reserved", C.c_int32 * 2),
+    ]
+
+
 class OalTuning(C.Structure):
-    _fields_ = [("pipe_planes", C.c_int32), ("reserved", C.c_int32 * 7)]
+    _fields_ = [("pipe_planes", C.c_int32), ("overlap", C.c_int32), ("face_kernel", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
+
+
+OAL_TUNE_SHAPES = 20
+TUNE_KEYS = [((1, 2, 4, 8, 16)[i % 5], int((i // 5) % 2 == 0), int(i >= 10)) for i in range(OAL_TUNE_SHAPES)]
 
 
 class OalError(RuntimeError):
@@ -93,6 +106,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "oal_plan_create": ([C.POINTER(OalPlanDesc), C.POINTER(vp)], C.c_int),
         "oal_plan_destroy": ([vp], C.c_int),
         "oal_level_cells": ([vp, C.c_int, i64p], C.c_int),
+        "oal_level_info": ([vp, C.c_int, C.POINTER(OalLevelInfo)], C.c_int),
         "oal_upload_f": ([vp, C.c_int, vp, C.c_int], C.c_int),
         "oal_download_f": ([vp, C.c_int, vp, C.c_int], C.c_int),
         "oal_init_equilibrium": ([vp, C.c_int, dp, dp, C.c_double, dp], C.c_int),

@@ -8,33 +8,55 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "liboal.so"
-SOURCES = [CSRC / "oal_kernels.hip", CSRC / "oal_plan.hip"]
+# the step kernels of each storage form sit in their own translation unit and compile in parallel
+SOURCES = [CSRC / "oal_step_f32.hip", CSRC / "oal_step_f16s.hip", CSRC / "oal_step_bf16s.hip",
+           CSRC / "oal_step_f64.hip", CSRC / "oal_kernels.hip", CSRC / "oal_plan.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -ffp-contract=off: every multiply and add rounds separately, as in the reference arithmetic.
 # -packed-fp32-ops off: no v_pk_*_f32 instructions are generated.
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", str(ROOT / "include"),
 ]
 
 
-def build_library(verbose: bool = False, extra: list[str] | None = None) -> Path:
-    """Compile liboal.so for gfx950 (always recompiles)."""
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, *HIP_FLAGS, *(extra or []), "-o", str(tmp), *map(str, SOURCES)]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True, cwd=str(CSRC))
+def build_library(verbose: bool = False, extra: list[str] | None = None, jobs: int | None = None) -> Path:
+    """Compile liboal.so for gfx950 (always recompiles): one object per source, in parallel, then link."""
+    jobs = jobs or min(len(SOURCES), max(1, len(os.sched_getaffinity(0))))
+    with tempfile.TemporaryDirectory(prefix="oal_build_") as tmpdir:
+        objs = [Path(tmpdir) / (src.stem + ".o") for src in SOURCES]
+        cmds = [[HIPCC, *HIP_FLAGS, *(extra or []), "-c", "-o", str(o), str(src)] for src, o in zip(SOURCES, objs)]
+
+        def run(cmd):
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            return subprocess.run(cmd, cwd=str(CSRC), capture_output=True, text=True)
+
+        with ThreadPoolExecutor(jobs) as pool:
+            results = list(pool.map(run, cmds))
+        for r in results:
+            if r.stdout or r.stderr:
+                sys.stderr.write(r.stdout + r.stderr)
+        bad = [c for c, r in zip(cmds, results) if r.returncode != 0]
+        if bad:
+            raise subprocess.CalledProcessError(1, bad[0])
+        tmp = LIB.with_suffix(".so.tmp")
+        link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+        if verbose:
+            print(" ".join(link), flush=True)
+        subprocess.run(link, check=True, cwd=str(CSRC))
     os.replace(tmp, LIB)
     return LIB
 

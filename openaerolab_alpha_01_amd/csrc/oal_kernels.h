@@ -56,6 +56,11 @@ struct StepArgs {
     uint32_t n_regular, n_interior, n_boundary_regular, n_boundary;
     const int32_t* lslot;        // [nb] link-mask slot of a block with solid cells near, -1 = none
     const uint32_t* lmask;       // [n_lslots * BS3] per cell: bit k source of direction k solid, bit 27 cell solid
+    const uint32_t* face_cells;  // [n_face] AB pattern: cells of regular-list blocks with a source outside the extent
+    uint32_t n_face;
+    int32_t regular_face;        // AB pattern: the regular list holds blocks on a domain face
+    int32_t face_kernel;         // 1: the pipelined kernel skips face cells and the face-cell kernel steps
+                                 // them; 0: it applies the face rules itself
     const int32_t* oslot;        // [nb] AA pattern: outlet side-buffer slot of the block, -1 = none
     void* obuf;                  // [n_oslots * 9 * BS2] AA pattern: outlet-face populations with cx = -1
     uint32_t ncell;              // cells per direction (stride between directions)
@@ -76,7 +81,19 @@ struct ReduceArgs {
 };
 
 // launchers (oal_kernels.hip); all asynchronous on `stream`
-hipError_t launch_step(int storage, int stream_mode, bool wale, const StepArgs& a, hipStream_t stream, hipStream_t side);
+// When bulk0/bulk1 are given they are recorded on `stream` around the bulk launches (regular and
+// interior block lists), the kernels that carry most of the cells.
+hipError_t launch_step(int storage, int stream_mode, bool wale, const StepArgs& a, hipStream_t stream, hipStream_t side,
+                       hipEvent_t bulk0 = nullptr, hipEvent_t bulk1 = nullptr);
+// per-storage step launchers (oal_step_<storage>.hip)
+hipError_t launch_step_f32(int stream_mode, bool wale, const StepArgs& a, hipStream_t stream, hipStream_t side,
+                           hipEvent_t bulk0, hipEvent_t bulk1);
+hipError_t launch_step_f64(int stream_mode, bool wale, const StepArgs& a, hipStream_t stream, hipStream_t side,
+                           hipEvent_t bulk0, hipEvent_t bulk1);
+hipError_t launch_step_f16s(int stream_mode, bool wale, const StepArgs& a, hipStream_t stream, hipStream_t side,
+                            hipEvent_t bulk0, hipEvent_t bulk1);
+hipError_t launch_step_bf16s(int stream_mode, bool wale, const StepArgs& a, hipStream_t stream, hipStream_t side,
+                             hipEvent_t bulk0, hipEvent_t bulk1);
 hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t stream);
 hipError_t launch_init_eq(int storage, void* f, uint32_t ncell, const double* rho, const double* u, double rho0,
                           double ux0, double uy0, double uz0, int permute, void* vel, hipStream_t stream);

@@ -63,18 +63,22 @@ struct Level {
     DevBuf<int32_t> oslot;             // AA: outlet side-buffer slot per block
     DevBuf<int32_t> blist_rg, blist_in, blist_br, blist_bd;  // block lists per kernel kind
     uint32_t n_regular = 0, n_interior = 0, n_boundary_regular = 0, n_boundary = 0;
+    DevBuf<uint32_t> face_cells;       // AB: face cells of the regular-list blocks
+    uint32_t n_face = 0;
+    bool regular_face = false;         // AB: the regular list holds blocks on a domain face
     DevBuf<int32_t> lslot;             // link-mask slot per block
     DevBuf<uint32_t> lmask;            // per-cell solid-link masks of blocks near solid cells
     DevBuf<unsigned char> obuf;        // AA: outlet-face populations with cx = -1
     DevBuf<unsigned long long> guard;
     bool has_solid = false;
+    int64_t n_fluid = 0;
     bool have_f = false;
     bool force_valid = false;
     int aa_phase = 0;                // AA: 0 = populations in stored form (next step odd), 1 = after an odd step
     void release() {
         nbr.release(); borg.release(); bdom.release(); dom_item0.release(); dom_block0.release();
         bflags.release(); solid.release(); doms.release(); fA.release(); fB.release(); velA.release(); velB.release();
-        fpart.release(); mpart.release(); red_out.release(); guard.release(); oslot.release(); obuf.release(); blist_rg.release(); blist_in.release(); blist_br.release(); blist_bd.release();
+        fpart.release(); mpart.release(); red_out.release(); guard.release(); oslot.release(); obuf.release(); blist_rg.release(); blist_in.release(); blist_br.release(); blist_bd.release(); face_cells.release();
         lslot.release(); lmask.release();
     }
 };
@@ -84,6 +88,8 @@ struct Level {
 struct OalPlan {
     int device = 0;
     int pipe_planes = 4;   // planes per workgroup of the pipelined interior kernel
+    bool overlap = true;   // boundary and face launches on the side stream
+    bool face_kernel = false;   // face cells by the face-cell kernel instead of rules in the pipelined one
     int storage = OAL_F32;
     int stream_mode = OAL_STREAM_AB;
     int bb_mode = OAL_BB_HALFWAY;
@@ -93,6 +99,7 @@ struct OalPlan {
     hipStream_t side = nullptr;             // boundary launches, overlapping the interior ones
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    std::vector<hipEvent_t> bulk_ev;        // pairs around the bulk launches of timed steps
     std::vector<Level> levels;
 };
 
@@ -117,13 +124,14 @@ int upload(DevBuf<T>& d, const std::vector<T>& h) {
 struct HostLevel {
     std::vector<int32_t> borg, bdom, nbr, nbr_bm, oslot, blist_rg, blist_in, blist_br, blist_bd, lslot;
     std::vector<uint32_t> lmask;
+    std::vector<uint32_t> face_cells;
     int32_t n_oslots = 0;
     std::vector<uint32_t> flags;
     std::vector<uint8_t> solid;
 };
 
 // Host-side construction of one level's tables from the reference-convention descriptor.
-int prepare_level(const OalLevelDesc& d, const OalCollision& coll, Level& L, HostLevel& H) {
+int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, Level& L, HostLevel& H) {
     if (d.n_blocks <= 0) return fail(OAL_ERR_ARG, "level with no blocks");
     if (!d.map_x || !d.map_y || !d.map_z || !d.neighbor_table) return fail(OAL_ERR_ARG, "level block arrays missing");
     if (d.n_domains <= 0 || !d.domains) return fail(OAL_ERR_ARG, "level has no domain descriptors");
@@ -254,9 +262,32 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, Level& L, Hos
             if (have < 0) continue;   // absent: outside the extent (BF_MISSING is clear)
             regular = have == want;
         }
-        if (flags[b] == 0) (regular ? H.blist_rg : H.blist_in).push_back((int32_t)b);
-        else (regular ? H.blist_br : H.blist_bd).push_back((int32_t)b);
+        if (flags[b] == 0) {
+            (regular ? H.blist_rg : H.blist_in).push_back((int32_t)b);
+        } else if (ab && regular && flags[b] == BF_EDGE) {
+            // AB pattern: a whole block on a domain face, no solid near, joins the regular list; the
+            // cells on its faces (sources outside the extent) go to the face-cell list
+            H.blist_rg.push_back((int32_t)b);
+            L.regular_face = true;
+            const int o[3] = {borg[3 * b], borg[3 * b + 1], borg[3 * b + 2]};
+            bool lo[3], hi[3];
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = !dm.per[a] && o[a] == 0;
+                hi[a] = !dm.per[a] && o[a] + BS == dm.ext[a];
+            }
+            for (int z = 0; z < BS; ++z)
+                for (int y = 0; y < BS; ++y)
+                    for (int x = 0; x < BS; ++x) {
+                        const int l[3] = {x, y, z};
+                        bool face = false;
+                        for (int a = 0; a < 3; ++a) face = face || (lo[a] && l[a] == 0) || (hi[a] && l[a] == BS - 1);
+                        if (face) H.face_cells.push_back(b * BS3 + (uint32_t)(x + BS * (y + BS * z)));
+                    }
+        } else {
+            (regular ? H.blist_br : H.blist_bd).push_back((int32_t)b);
+        }
     }
+    L.n_face = (uint32_t)H.face_cells.size();
     L.n_regular = (uint32_t)H.blist_rg.size();
     L.n_interior = (uint32_t)H.blist_in.size();
     L.n_boundary_regular = (uint32_t)H.blist_br.size();
@@ -276,6 +307,19 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, Level& L, Hos
     if (d.obstacle) {
         H.solid.resize(L.ncell);
         for (size_t i = 0; i < L.ncell; ++i) H.solid[i] = d.obstacle[i] ? 1 : 0;
+    }
+    // fluid cells: inside the extent of their domain and not solid
+    L.n_fluid = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+        const DevDomain& dm = L.doms_h[bdom[b]];
+        for (int z = 0; z < BS; ++z)
+            for (int y = 0; y < BS; ++y)
+                for (int x = 0; x < BS; ++x) {
+                    if (borg[3 * b] + x >= dm.ext[0] || borg[3 * b + 1] + y >= dm.ext[1] || borg[3 * b + 2] + z >= dm.ext[2])
+                        continue;
+                    const size_t c = (size_t)b * BS3 + x + BS * (y + BS * z);
+                    if (!d.obstacle || !d.obstacle[c]) ++L.n_fluid;
+                }
     }
     // solid-link masks of the blocks flagged solid: bit k when the source x - c_k of a cell is a solid
     // cell inside the extent, bit 27 when the cell itself is solid
@@ -324,7 +368,7 @@ int upload_level(OalPlan* p, Level& L, const HostLevel& H) {
     if (!H.solid.empty() && (rc = upload(L.solid, H.solid))) return rc;
     if ((rc = upload(L.blist_rg, H.blist_rg)) || (rc = upload(L.blist_in, H.blist_in)) ||
         (rc = upload(L.blist_br, H.blist_br)) || (rc = upload(L.blist_bd, H.blist_bd)) ||
-        (rc = upload(L.lslot, H.lslot)) || (rc = upload(L.lmask, H.lmask)))
+        (rc = upload(L.lslot, H.lslot)) || (rc = upload(L.lmask, H.lmask)) || (rc = upload(L.face_cells, H.face_cells)))
         return rc;
     if (p->stream_mode == OAL_STREAM_AA) {
         if ((rc = upload(L.oslot, H.oslot))) return rc;
@@ -376,6 +420,12 @@ StepArgs make_args(OalPlan* p, Level& L, int32_t time_step, bool want_force) {
     a.n_boundary = L.n_boundary;
     a.lslot = L.lslot.p;
     a.lmask = L.lmask.p;
+    a.face_cells = L.face_cells.p;
+    a.n_face = L.n_face;
+    a.regular_face = L.regular_face ? 1 : 0;
+    // the in-kernel face rules address by 32-bit byte offsets: levels of 4 GiB or more take the face-cell kernel
+    const bool small = (uint64_t)L.ncell * 27u * (uint64_t)storage_bytes(p->storage) <= 0xffffffffull;
+    a.face_kernel = (p->face_kernel || !small) ? 1 : 0;
     a.obuf = L.obuf.p;
     a.ncell = L.ncell;
     a.nb = L.nb;
@@ -387,19 +437,20 @@ StepArgs make_args(OalPlan* p, Level& L, int32_t time_step, bool want_force) {
     return a;
 }
 
-int level_step(OalPlan* p, int level, int32_t time_step, int flags, hipStream_t s) {
+int level_step(OalPlan* p, int level, int32_t time_step, int flags, hipStream_t s, hipEvent_t bulk0 = nullptr,
+               hipEvent_t bulk1 = nullptr) {
     Level& L = p->levels[level];
     if (!L.have_f) return fail(OAL_ERR_STATE, "no populations: call oal_upload_f or oal_init_equilibrium first");
     const bool want_force = (flags & OAL_STEP_FORCE) != 0;
     if (want_force && !L.has_solid) return fail(OAL_ERR_STATE, "force requested on a level without obstacle cells");
     StepArgs a = make_args(p, L, time_step, want_force);
-    const bool split = L.n_regular + L.n_interior > 0 && L.n_boundary_regular + L.n_boundary > 0;
+    const bool split = p->overlap && L.n_regular + L.n_interior > 0 && L.n_boundary_regular + L.n_boundary + L.n_face > 0;
     if (split) {
         // fork: the side stream starts after everything queued on s, then s waits for the side stream
         OAL_HIP(hipEventRecord(p->fork, s));
         OAL_HIP(hipStreamWaitEvent(p->side, p->fork, 0));
     }
-    OAL_HIP(launch_step(p->storage, p->stream_mode, p->wale, a, s, split ? p->side : nullptr));
+    OAL_HIP(launch_step(p->storage, p->stream_mode, p->wale, a, s, split ? p->side : nullptr, bulk0, bulk1));
     if (split) {
         OAL_HIP(hipEventRecord(p->join, p->side));
         OAL_HIP(hipStreamWaitEvent(s, p->join, 0));
@@ -502,7 +553,8 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
     p->levels.resize(desc->n_levels);
     std::vector<HostLevel> host(desc->n_levels);
     for (int l = 0; l < desc->n_levels; ++l) {
-        const int rc = prepare_level(desc->levels[l], desc->collision, p->levels[l], host[l]);
+        const int rc = prepare_level(desc->levels[l], desc->collision, desc->stream_mode == OAL_STREAM_AB, p->levels[l],
+                                     host[l]);
         if (rc) return bail(rc);
     }
     hipError_t e = hipSetDevice(p->device);
@@ -533,6 +585,7 @@ int oal_plan_destroy(OalPlan* p) {
     if (p->ev1) (void)hipEventDestroy(p->ev1);
     if (p->fork) (void)hipEventDestroy(p->fork);
     if (p->join) (void)hipEventDestroy(p->join);
+    for (hipEvent_t e : p->bulk_ev) (void)hipEventDestroy(e);
     if (p->side) (void)hipStreamDestroy(p->side);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
@@ -543,6 +596,22 @@ int oal_level_cells(const OalPlan* p, int level, int64_t* n) {
     if (int rc = check_level(p, level)) return rc;
     if (!n) return fail(OAL_ERR_ARG, "NULL output");
     *n = p->levels[level].ncell;
+    return OAL_OK;
+}
+
+int oal_level_info(const OalPlan* p, int level, OalLevelInfo* info) {
+    if (int rc = check_level(p, level)) return rc;
+    if (!info) return fail(OAL_ERR_ARG, "NULL output");
+    const Level& L = p->levels[level];
+    *info = OalLevelInfo{};
+    info->n_cells = L.ncell;
+    info->n_fluid_cells = L.n_fluid;
+    info->n_blocks = (int32_t)L.nb;
+    info->n_regular = (int32_t)L.n_regular;
+    info->n_interior = (int32_t)L.n_interior;
+    info->n_boundary_regular = (int32_t)L.n_boundary_regular;
+    info->n_boundary = (int32_t)L.n_boundary;
+    info->n_face_cells = (int32_t)L.n_face;
     return OAL_OK;
 }
 
@@ -706,15 +775,32 @@ int oal_time_steps(OalPlan* p, int level, int n, int flags, double* ms_total, do
     if (n <= 0) return fail(OAL_ERR_ARG, "n must be positive");
     OAL_HIP(hipSetDevice(p->device));
     OAL_HIP(hipStreamSynchronize(p->stream));
+    const Level& L = p->levels[level];
+    const bool bulk = L.n_regular + L.n_interior > 0;
+    if (bulk) {
+        while (p->bulk_ev.size() < 2 * (size_t)n) {
+            hipEvent_t e = nullptr;
+            OAL_HIP(hipEventCreate(&e));
+            p->bulk_ev.push_back(e);
+        }
+    }
     OAL_HIP(hipEventRecord(p->ev0, p->stream));
-    for (int i = 0; i < n; ++i)
-        if (int rc = level_step(p, level, i, flags, p->stream)) return rc;
+    for (int i = 0; i < n; ++i) {
+        hipEvent_t b0 = bulk ? p->bulk_ev[2 * i] : nullptr, b1 = bulk ? p->bulk_ev[2 * i + 1] : nullptr;
+        if (int rc = level_step(p, level, i, flags, p->stream, b0, b1)) return rc;
+    }
     OAL_HIP(hipEventRecord(p->ev1, p->stream));
     OAL_HIP(hipEventSynchronize(p->ev1));
     float ms = 0.0f;
     OAL_HIP(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+    double kern = 0.0;
+    for (int i = 0; bulk && i < n; ++i) {
+        float d = 0.0f;
+        OAL_HIP(hipEventElapsedTime(&d, p->bulk_ev[2 * i], p->bulk_ev[2 * i + 1]));
+        kern += d;
+    }
     if (ms_total) *ms_total = ms;
-    if (ms_kernel) *ms_kernel = (double)ms / n;
+    if (ms_kernel) *ms_kernel = bulk ? kern / n : (double)ms / n;
     return OAL_OK;
 }
 
@@ -752,6 +838,8 @@ int oal_get_tuning(const OalPlan* p, OalTuning* t) {
     if (!p || !t) return fail(OAL_ERR_ARG, "NULL argument");
     *t = OalTuning{};
     t->pipe_planes = p->pipe_planes;
+    t->overlap = p->overlap ? 1 : 0;
+    t->face_kernel = p->face_kernel ? 1 : 0;
     return OAL_OK;
 }
 
@@ -759,7 +847,11 @@ int oal_set_tuning(OalPlan* p, const OalTuning* t) {
     if (!p || !t) return fail(OAL_ERR_ARG, "NULL argument");
     const int np = t->pipe_planes;
     if (np != 1 && np != 2 && np != 4 && np != 8 && np != 16) return fail(OAL_ERR_ARG, "pipe_planes must be 1, 2, 4, 8 or 16");
+    if (t->overlap != 0 && t->overlap != 1) return fail(OAL_ERR_ARG, "overlap must be 0 or 1");
+    if (t->face_kernel != 0 && t->face_kernel != 1) return fail(OAL_ERR_ARG, "face_kernel must be 0 or 1");
     p->pipe_planes = np;
+    p->overlap = t->overlap != 0;
+    p->face_kernel = t->face_kernel != 0;
     return OAL_OK;
 }
 
@@ -781,19 +873,25 @@ int oal_autotune(OalPlan* p, int level, int n_steps, OalTuning* best, double* ms
     const int aa_phase = L.aa_phase;
     const int cand[5] = {1, 2, 4, 8, 16};
     int bestc = p->pipe_planes;
+    bool besto = p->overlap, bestf = p->face_kernel;
     double bestms = 1e300;
     int rc = OAL_OK;
     if (e != hipSuccess) rc = fail(OAL_ERR_HIP, std::string("oal_autotune: ") + hipGetErrorString(e));
-    for (int i = 0; i < 5 && rc == OAL_OK; ++i) {
-        p->pipe_planes = cand[i];
+    for (int i = 0; i < OAL_TUNE_SHAPES && rc == OAL_OK; ++i) {
+        p->pipe_planes = cand[i % 5];
+        p->overlap = (i / 5) % 2 == 0;
+        p->face_kernel = i >= 10;
         double tot = 0.0, ker = 0.0;
         rc = oal_time_steps(p, level, 2, 0, &tot, &ker);   // warm-up of this shape
         if (rc == OAL_OK) rc = oal_time_steps(p, level, n_steps, 0, &tot, &ker);
         if (rc != OAL_OK) break;
-        if (ms_per_step) ms_per_step[i] = ker;
-        if (ker < bestms) { bestms = ker; bestc = cand[i]; }
+        const double ms = tot / n_steps;   // whole steps, all launches
+        if (ms_per_step) ms_per_step[i] = ms;
+        if (ms < bestms) { bestms = ms; bestc = cand[i % 5]; besto = p->overlap; bestf = p->face_kernel; }
     }
     p->pipe_planes = bestc;
+    p->overlap = besto;
+    p->face_kernel = bestf;
     // restore
     if (e == hipSuccess) e = hipMemcpyAsync(L.fA.p, save, L.fA.n, hipMemcpyDeviceToDevice, p->stream);
     if (e == hipSuccess && L.fB.p) e = hipMemcpyAsync(L.fB.p, save, L.fA.n, hipMemcpyDeviceToDevice, p->stream);
@@ -804,7 +902,12 @@ int oal_autotune(OalPlan* p, int level, int n_steps, OalTuning* best, double* ms
     L.aa_phase = aa_phase;
     L.force_valid = false;
     if (rc == OAL_OK && e != hipSuccess) rc = fail(OAL_ERR_HIP, std::string("oal_autotune: ") + hipGetErrorString(e));
-    if (rc == OAL_OK && best) { *best = OalTuning{}; best->pipe_planes = bestc; }
+    if (rc == OAL_OK && best) {
+        *best = OalTuning{};
+        best->pipe_planes = bestc;
+        best->overlap = besto ? 1 : 0;
+        best->face_kernel = bestf ? 1 : 0;
+    }
     return rc;
 }
 

@@ -1,0 +1,792 @@
+// Device code of the fused stream-collide step (templates), included by the per-storage translation
+// units oal_step_<storage>.hip so that the four storage forms compile in parallel.
+//
+// Work decomposition of the step kernels: one work-item per cell, one workgroup per 16 x 16 plane of
+// a 16^3 block (256 lanes = 4 wavefronts of 16 x 4 cells).  Populations are SoA per direction with
+// every block's 4096 cells contiguous (the reference's f[x,y,z,b,k] layout), so the 64 lanes of a
+// wavefront touch 4 consecutive 64-byte rows of every direction array.  A workgroup's block, plane,
+// domain and neighbour-block indices are uniform, which keeps them in scalar registers.
+//
+// Reference: src/collision/kernel_cumulant.jl:29-912 (stream_collide_cumulant!).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "oal_kernels.h"
+
+namespace oal {
+
+// Workgroups are dispatched round-robin over the 8 XCDs of the MI355X.  Remap the workgroup index so
+// that each XCD receives a contiguous range of items: neighbouring planes and blocks, which share
+// halo cache lines, then meet in the same L2.
+__device__ __forceinline__ uint32_t xcd_item(uint32_t i, uint32_t n) {
+    const uint32_t q = n >> 3, r = n & 7u, x = i & 7u, j = i >> 3;
+    return x < r ? x * (q + 1u) + j : r * (q + 1u) + (x - r) * q + j;
+}
+
+// how the population arriving in one direction is obtained
+enum : int {
+    SRC_REG = 0,     // pulled from the source cell
+    SRC_INLET = 1,   // equilibrium at (1, u_inlet)                       kernel_cumulant.jl:235-239
+    SRC_OWN = 2,     // the cell's own population, same direction (outlet) kernel_cumulant.jl:240-245
+    SRC_OPP = 3,     // own opposite population: no-slip wall            kernel_cumulant.jl:270-271, 340-341
+    SRC_MIRY = 4,    // own y-mirrored population: free slip             kernel_cumulant.jl:342-343
+    SRC_MIRZ = 5,    // own z-mirrored population: free slip             kernel_cumulant.jl:344-345
+    SRC_REST = 6,    // rest weight (no neighbour block on level 1)      kernel_cumulant.jl:389-390
+    SRC_SOLID = 7    // halfway bounce-back from a solid source cell
+};
+
+enum : int { PAT_AB = 0, PAT_AA_EVEN = 1, PAT_AA_ODD = 2 };
+
+// Uniform (per-workgroup) and per-lane state of one cell
+struct CellCtx {
+    uint32_t b, flags, own;
+    uint32_t lmask;   // bit k: the source of direction k is solid; bit 27: the cell is solid
+    int lx, ly, z, gx, gy, gz;
+    int ext0, ext1, ext2, per0, per1, per2;
+    int zwall, ymin_wall;
+};
+
+// The source of direction K for this cell: classification and, for SRC_REG / SRC_SOLID, the cell index.
+// GEN = false is the interior path (block flags all clear): every source is a regular cell of this
+// block or of one of its 26 neighbours, which all exist.
+template <int K, bool GEN, bool REG = false>
+__device__ __forceinline__ int classify(const StepArgs& a, const CellCtx& c, uint32_t& src) {
+    constexpr int cx = dcx(K), cy = dcy(K), cz = dcz(K);
+    if constexpr (GEN) {
+        if (c.flags & BF_EDGE) {
+            const int sgx = c.gx - cx, sgy = c.gy - cy, sgz = c.gz - cz;
+            const bool xl = cx > 0 && !c.per0 && sgx < 0;
+            const bool xh = cx < 0 && !c.per0 && sgx >= c.ext0;
+            const bool yl = cy > 0 && !c.per1 && sgy < 0;
+            const bool yh = cy < 0 && !c.per1 && sgy >= c.ext1;
+            const bool zl = cz > 0 && !c.per2 && sgz < 0;
+            const bool zh = cz < 0 && !c.per2 && sgz >= c.ext2;
+            if (xl) return SRC_INLET;
+            if (xh) return SRC_OWN;
+            if (c.zwall != 0 && (zl || zh)) return SRC_OPP;
+            if (yl && c.ymin_wall == 1) return SRC_OPP;
+            if (yl || yh) return SRC_MIRY;
+            if (zl || zh) return SRC_MIRZ;
+        }
+    }
+    if constexpr (REG) {
+        // regular addressing: the caller forms the address; only the solid test remains
+        if constexpr (GEN) {
+            if ((c.flags & BF_SOLID) && a.cp.bb_mode == 1 && ((c.lmask >> K) & 1u)) return SRC_SOLID;
+        }
+        return SRC_REG;
+    }
+    const int sx = c.lx - cx, sy = c.ly - cy, sz = c.z - cz;
+    int32_t nbk = (int32_t)c.b;
+    if constexpr (cx != 0 || cy != 0 || cz != 0) {
+        // candidate neighbour blocks are uniform (scalar) values; the lane picks one by its crossing
+        const int oz = cz == 0 ? 0 : (sz < 0 ? -1 : (sz >= BS ? 1 : 0));
+        const int dz = 9 * (oz + 1);
+        const int32_t* nt = a.nbr_bm + (size_t)c.b * 32;
+        int32_t n_c = oz == 0 ? (int32_t)c.b : nt[4 + dz];
+        if constexpr (!GEN) n_c = max(n_c, 0);
+        int32_t pick = n_c;
+        if constexpr (cx != 0 && cy != 0) {
+            int32_t n_x = nt[(1 - cx) + 3 + dz], n_y = nt[1 + 3 * (1 - cy) + dz], n_xy = nt[(1 - cx) + 3 * (1 - cy) + dz];
+            if constexpr (!GEN) { n_x = max(n_x, 0); n_y = max(n_y, 0); n_xy = max(n_xy, 0); }
+            const bool crx = (unsigned)sx >= (unsigned)BS, cry = (unsigned)sy >= (unsigned)BS;
+            pick = crx ? (cry ? n_xy : n_x) : (cry ? n_y : n_c);
+        } else if constexpr (cx != 0) {
+            int32_t n_x = nt[(1 - cx) + 3 + dz];
+            if constexpr (!GEN) n_x = max(n_x, 0);
+            pick = ((unsigned)sx >= (unsigned)BS) ? n_x : n_c;
+        } else if constexpr (cy != 0) {
+            int32_t n_y = nt[1 + 3 * (1 - cy) + dz];
+            if constexpr (!GEN) n_y = max(n_y, 0);
+            pick = ((unsigned)sy >= (unsigned)BS) ? n_y : n_c;
+        }
+        nbk = pick;
+    }
+    if constexpr (GEN) {
+        if (nbk < 0) return SRC_REST;   // absent block inside the extent (BF_MISSING); never an address
+    }
+    src = (uint32_t)nbk * BS3 + ((uint32_t)(sz & (BS - 1)) << 8) + ((uint32_t)(sy & (BS - 1)) << 4) + (uint32_t)(sx & (BS - 1));
+    if constexpr (GEN) {
+        if ((c.flags & BF_SOLID) && a.cp.bb_mode == 1 && ((c.lmask >> K) & 1u)) return SRC_SOLID;
+    }
+    return SRC_REG;
+}
+
+// Regular interior blocks: the block's neighbours sit at lexicographic offsets (b +- 1, +- nbx,
+// +- nbx*nby), so a source cell's index is arithmetic.  Each lane forms, once, the byte offsets of its
+// nine in-plane source classes (cx, cy) - own cell shifted by -c in x and y, into the neighbour block
+// where it crosses - and every direction's access adds a uniform z part to its scalar base pointer.
+template <class SP>
+struct RegAddr {
+    uint32_t v[3][3];          // [cx+1][cy+1]: byte offset of cell (x - cx, y - cy, zref)
+    int64_t plane_cells;       // cells between consecutive block layers (nbx * nby * BS3)
+    int zref;                  // the plane the offsets were formed for (other planes: uniform shift)
+    __device__ __forceinline__ void init(const CellCtx& c, int nbx, int nby) {
+        plane_cells = (int64_t)nbx * nby * BS3;
+        zref = c.z;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int cx = i - 1, cy = j - 1;
+                const int sx = c.lx - cx, sy = c.ly - cy;
+                const int dx = sx < 0 ? -1 : (sx >= BS ? 1 : 0);
+                const int dy = sy < 0 ? -1 : (sy >= BS ? 1 : 0);
+                const int32_t idx = (int32_t)c.own + dx * BS3 + dy * nbx * BS3 + (sx - BS * dx - c.lx) + BS * (sy - BS * dy - c.ly);
+                v[i][j] = (uint32_t)idx * (uint32_t)sizeof(typename SP::st);
+            }
+    }
+    // uniform base of direction-slot `slot` for plane z of the block and a source shifted by -cz in z
+    __device__ __forceinline__ const char* base(const void* f, uint32_t n, int slot, int z, int cz) const {
+        const int sz = z - cz;
+        const int dz = sz < 0 ? -1 : (sz >= BS ? 1 : 0);
+        const int64_t cells = (int64_t)slot * n + dz * plane_cells + (int64_t)(sz - BS * dz - zref) * BS2;
+        return (const char*)f + cells * (int64_t)sizeof(typename SP::st);
+    }
+};
+
+// Faces of non-periodic axes that a block of the regular list touches.  Such blocks are whole (not
+// partial), so a face is where the block's origin is 0 or its far side is the extent.  Cells on a face
+// have sources outside the extent: the pipelined kernel applies their rules itself (FACE_RULES) or
+// skips these cells (FACE_SKIP), the one-plane regular kernel skips them; the face-cell kernel steps
+// the skipped cells.
+struct FaceBits {
+    bool xlo, xhi, ylo, yhi, zlo, zhi;
+};
+
+__device__ __forceinline__ FaceBits face_bits(const StepArgs& a, uint32_t b, const DevDomain* dm) {
+    const int ox = a.borg[3 * b], oy = a.borg[3 * b + 1], oz = a.borg[3 * b + 2];
+    FaceBits f;
+    f.xlo = !dm->per[0] && ox == 0;
+    f.xhi = !dm->per[0] && ox + BS == dm->ext[0];
+    f.ylo = !dm->per[1] && oy == 0;
+    f.yhi = !dm->per[1] && oy + BS == dm->ext[1];
+    f.zlo = !dm->per[2] && oz == 0;
+    f.zhi = !dm->per[2] && oz + BS == dm->ext[2];
+    return f;
+}
+
+__device__ __forceinline__ bool face_lane(const FaceBits& f, int lx, int ly) {
+    return (f.xlo && lx == 0) || (f.xhi && lx == BS - 1) || (f.ylo && ly == 0) || (f.yhi && ly == BS - 1);
+}
+
+__device__ __forceinline__ bool face_plane(const FaceBits& f, int z) {
+    return (f.zlo && z == 0) || (f.zhi && z == BS - 1);
+}
+
+// Population J of this cell's own post-collision state of the previous step: slot (x, J) after an
+// AB step, slot (x, opp J) after an AA even step.
+template <class SP, int PAT, int J>
+__device__ __forceinline__ typename SP::T own_prev(const StepArgs& a, const CellCtx& c) {
+    constexpr int slot = PAT == PAT_AB ? J : dopp(J);
+    return SP::ld((const typename SP::st*)a.fin + (size_t)slot * a.ncell + c.own, wgt<typename SP::T>(J));
+}
+
+template <int J>
+using dir_c = std::integral_constant<int, J>;
+
+// The value of a boundary rule for direction K.  `own(dir_c<j>)` yields population j of the cell's
+// own post-collision state of the step the rule refers to.
+template <int K, class T, class OWN>
+__device__ __forceinline__ T boundary_value(int kind, const StepArgs& a, const CellCtx& c, float u_in, int32_t step,
+                                            OWN&& own) {
+    switch (kind) {
+        case SRC_INLET: {
+            const float turb = a.cp.inlet_turb;
+            const float noise = turb > 0.0f ? inlet_noise(c.gy + 1, c.gz + 1, step, 1234) * turb * u_in : 0.0f;
+            const T u_inst = T(u_in) + T(noise);
+            return feq_auto(T(1), u_inst, T(0), T(0), wgt<T>(K), T(dcx(K)), T(dcy(K)), T(dcz(K)), a.cp.compressible);
+        }
+        case SRC_OWN: return own(dir_c<K>{});
+        case SRC_OPP:
+        case SRC_SOLID: return own(dir_c<dopp(K)>{});
+        case SRC_MIRY: return own(dir_c<dmir_y(K)>{});
+        case SRC_MIRZ: return own(dir_c<dmir_z(K)>{});
+        default: return wgt<T>(K);
+    }
+}
+
+// workgroup sum of 3 doubles into part[item*4 + 0..2]; every lane of the workgroup must call it
+__device__ __forceinline__ void wg_sum3(double v0, double v1, double v2, double* part, uint32_t item) {
+    __shared__ double red[3][WG / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v0 += __shfl_xor(v0, off, 64);
+        v1 += __shfl_xor(v1, off, 64);
+        v2 += __shfl_xor(v2, off, 64);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { red[0][wave] = v0; red[1][wave] = v1; red[2][wave] = v2; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int w = 0; w < WG / 64; ++w) { s0 += red[0][w]; s1 += red[1][w]; s2 += red[2][w]; }
+        part[(size_t)item * 4 + 0] = s0;
+        part[(size_t)item * 4 + 1] = s1;
+        part[(size_t)item * 4 + 2] = s2;
+    }
+}
+
+// WALE: central differences of the previous step's velocity over the six face neighbours; a solid or
+// absent neighbour contributes the cell's own velocity (physics_utils.jl:1981-2035, 2217-2233)
+template <class T>
+__device__ __forceinline__ T wale_of(const StepArgs& a, const CellCtx& c) {
+    const T* v = (const T*)a.vel_in;
+    const uint32_t n = a.ncell;
+    T un[6][3];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int ddx = q == 0 ? 1 : (q == 1 ? -1 : 0);
+        const int ddy = q == 2 ? 1 : (q == 3 ? -1 : 0);
+        const int ddz = q == 4 ? 1 : (q == 5 ? -1 : 0);
+        bool ok = true;
+        if (c.flags & BF_EDGE) {
+            const int ngx = c.gx + ddx, ngy = c.gy + ddy, ngz = c.gz + ddz;
+            if (!c.per0 && (ngx < 0 || ngx >= c.ext0)) ok = false;
+            if (!c.per1 && (ngy < 0 || ngy >= c.ext1)) ok = false;
+            if (!c.per2 && (ngz < 0 || ngz >= c.ext2)) ok = false;
+        }
+        const int sx = c.lx + ddx, sy = c.ly + ddy, sz = c.z + ddz;
+        const int ox = sx < 0 ? -1 : (sx >= BS ? 1 : 0);
+        const int oy = sy < 0 ? -1 : (sy >= BS ? 1 : 0);
+        const int oz = sz < 0 ? -1 : (sz >= BS ? 1 : 0);
+        int32_t nbk = (int32_t)c.b;
+        if (ok && (ox | oy | oz)) nbk = a.nbr_bm[(size_t)c.b * 32 + (ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)];
+        if (nbk < 0) ok = false;
+        // the face neighbour x + e is the source of the direction -e: its solid bit in the link mask
+        constexpr int kq[6] = {12, 14, 10, 16, 4, 22};
+        uint32_t idx = c.own;
+        if (ok) {
+            idx = (uint32_t)nbk * BS3 + ((uint32_t)(sz & (BS - 1)) << 8) + ((uint32_t)(sy & (BS - 1)) << 4) + (uint32_t)(sx & (BS - 1));
+            if ((c.lmask >> kq[q]) & 1u) idx = c.own;
+        }
+        un[q][0] = v[idx]; un[q][1] = v[n + idx]; un[q][2] = v[2 * (size_t)n + idx];
+    }
+    const T h = T(0.5);
+    return wale_nu(h * (un[0][0] - un[1][0]), h * (un[2][0] - un[3][0]), h * (un[4][0] - un[5][0]),
+                   h * (un[0][1] - un[1][1]), h * (un[2][1] - un[3][1]), h * (un[4][1] - un[5][1]),
+                   h * (un[0][2] - un[1][2]), h * (un[2][2] - un[3][2]), h * (un[4][2] - un[5][2]), T(a.cp.c_wale));
+}
+
+// The fused stream-collide step of one cell.  PAT_AB pulls from fin into fout (the reference's
+// scheme); PAT_AA_EVEN / PAT_AA_ODD are the two half-steps of the in-place AA pattern on one buffer:
+//   even: read f_k(x) from slot (x, k), write f*_k(x) to slot (x, opp k);
+//   odd:  read f_k(x) from slot (x - c_k, opp k), write f*_k(x) to slot (x + c_k, k).
+// Boundary rules of the odd step's writes: where x + c_k is not a cell that steps, the rule of the
+// opposite direction is evaluated from the fresh post-collision state and stored in (x, opp k), the
+// slot the following even step reads, so the even step needs no boundary logic.
+//
+// Momentum exchange (OAL_STEP_FORCE): summed over the links whose populations this step's collision
+// produced, i.e. the exchange of the streaming that follows it: 2 (f_i* - w_i) c_i for halfway
+// bounce-back, (f_i*(t) + f_i*(t-1) - 2 w_i) c_i for the reference's solid reflection (the solid cell
+// returns last step's population), i toward a solid cell (src/octree_q19/src/stream.jl:248-256).
+template <class SP, int PAT, bool WALE, bool GEN, bool REG = false>
+__device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, const DevDomain* dm, bool active,
+                                          bool solid_own, double& F0, double& F1, double& F2) {
+    using T = typename SP::T;
+    using st = typename SP::st;
+    if (!active) return;
+    const uint32_t b = c.b;
+    const float u_in = dm->u_in;
+    st* fo = (st*)a.fout;
+    const st* fi = (const st*)a.fin;
+    const uint32_t n = a.ncell;
+    RegAddr<SP> ra;
+    if constexpr (REG) ra.init(c, dm->nbx, dm->nby);
+
+    T f[27];
+    for_dirs([&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        constexpr T w = wgt<T>(K);
+        T val;
+        if constexpr (REG && !GEN && PAT == PAT_AA_EVEN) {
+            val = SP::ld((const st*)(ra.base(fi, n, K, c.z, 0) + ra.v[1][1]), w);
+        } else if constexpr (REG && !GEN) {
+            constexpr int slot = PAT == PAT_AB ? K : dopp(K);
+            val = SP::ld((const st*)(ra.base(fi, n, slot, c.z, dcz(K)) + ra.v[dcx(K) + 1][dcy(K) + 1]), w);
+        } else if constexpr (PAT == PAT_AA_EVEN) {
+            // inlet links are evaluated here in full precision, as the pull step does, rather than
+            // read back from a slot that went through the storage rounding
+            bool inlet = false;
+            if constexpr (GEN && dcx(K) > 0) inlet = (c.flags & BF_EDGE) && !c.per0 && c.gx == 0;
+            if (inlet)
+                val = boundary_value<K, T>(SRC_INLET, a, c, u_in, a.time_step, [&](auto) { return T(0); });
+            else
+                val = SP::ld(fi + (size_t)K * n + c.own, w);
+        } else {
+            uint32_t src = 0;
+            const int kind = classify<K, GEN, REG>(a, c, src);
+            if (kind == SRC_REG) {
+                constexpr int slot = PAT == PAT_AB ? K : dopp(K);
+                if constexpr (REG)
+                    val = SP::ld((const st*)(ra.base(fi, n, slot, c.z, dcz(K)) + ra.v[dcx(K) + 1][dcy(K) + 1]), w);
+                else
+                    val = SP::ld(fi + (size_t)slot * n + src, w);
+            } else if (PAT == PAT_AA_ODD && kind == SRC_OWN) {
+                // outlet: the own population of direction K (cx = -1) sits in slot (x, opp K), which
+                // the west neighbour overwrites during this step; the even step saved a copy
+                val = SP::ld((const st*)a.obuf + ((size_t)a.oslot[b] * 9 + K / 3) * BS2 + c.ly * BS + c.z, w);
+            } else {
+                val = boundary_value<K, T>(kind, a, c, u_in, a.time_step,
+                                           [&](auto jc) { return own_prev<SP, PAT, decltype(jc)::value>(a, c); });
+            }
+        }
+        f[K] = val;
+    });
+
+    T rho_o, ux_o, uy_o, uz_o;
+    if (GEN && solid_own) {
+        // reference solid cell: full-way reflection of the gathered populations (kernel_cumulant.jl:405-417)
+        T g[27];
+#pragma unroll
+        for (int k = 0; k < 27; ++k) g[k] = f[26 - k];
+#pragma unroll
+        for (int k = 0; k < 27; ++k) f[k] = g[k];
+        rho_o = T(1); ux_o = uy_o = uz_o = T(0);
+    } else {
+        CellRates<T> q;
+        if constexpr (WALE) {
+            const T nu_eddy = wale_of<T>(a, c);
+            cell_rates<T>(nu_eddy, T(dm->tau), T(dm->amr_scale), T(a.cp.nu_sgs_bg), a.cp.adaptive_omega4,
+                          T(a.cp.lambda), T(a.cp.omega_4), q.omega1, q.omega4_diag);
+        } else if constexpr (sizeof(T) == 8) {
+            q.omega1 = dm->omega1_d; q.omega4_diag = dm->omega4d_d;
+        } else {
+            q.omega1 = dm->omega1; q.omega4_diag = dm->omega4d;
+        }
+        q.forced = dm->forced != 0;
+        q.Fx = T(dm->F[0]); q.Fy = T(dm->F[1]); q.Fz = T(dm->F[2]);
+        collide_cell<T>(f, q, a.cp, rho_o, ux_o, uy_o, uz_o);
+    }
+    if (a.vel_out) {
+        T* vo = (T*)a.vel_out;
+        vo[c.own] = ux_o; vo[n + c.own] = uy_o; vo[2 * (size_t)n + c.own] = uz_o;
+    }
+    if (a.rho_out) ((T*)a.rho_out)[c.own] = rho_o;
+
+    const bool want_force = a.fpart != nullptr;
+    const bool link_force = GEN && want_force && (c.flags & BF_SOLID) && !solid_own;
+    const bool outlet_cell = GEN && PAT == PAT_AA_EVEN && (c.flags & BF_EDGE) && !c.per0 && c.gx == c.ext0 - 1;
+    for_dirs([&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        constexpr int J = dopp(K);   // the source of direction J is x + c_K, where f*_K goes
+        if constexpr (REG && PAT == PAT_AB) {
+            SP::sto((st*)(ra.base(fo, n, K, c.z, 0) + ra.v[1][1]), f[K], wgt<T>(K));
+        } else if constexpr (REG && PAT == PAT_AA_EVEN) {
+            SP::sto((st*)(ra.base(fo, n, J, c.z, 0) + ra.v[1][1]), f[K], wgt<T>(K));
+            if constexpr (GEN && dcx(K) < 0) {
+                if (outlet_cell) SP::sto((st*)a.obuf + ((size_t)a.oslot[b] * 9 + K / 3) * BS2 + c.ly * BS + c.z, f[K], wgt<T>(K));
+            }
+        } else if constexpr (REG && !GEN) {
+            SP::sto((st*)(ra.base(fo, n, K, c.z, -dcz(K)) + ra.v[1 - dcx(K)][1 - dcy(K)]), f[K], wgt<T>(K));
+        } else if constexpr (REG) {
+            uint32_t dst = 0;
+            const int kind = classify<J, GEN, REG>(a, c, dst);
+            if (kind == SRC_REG) {
+                SP::sto((st*)(ra.base(fo, n, K, c.z, -dcz(K)) + ra.v[1 - dcx(K)][1 - dcy(K)]), f[K], wgt<T>(K));
+            } else if (kind != SRC_INLET) {
+                const T v = boundary_value<J, T>(kind, a, c, u_in, a.time_step + 1,
+                                                 [&](auto jc) { return f[decltype(jc)::value]; });
+                SP::sto(fo + (size_t)J * n + c.own, v, wgt<T>(J));
+            }
+        } else if constexpr (PAT == PAT_AB) {
+            SP::sto(fo + (size_t)K * n + c.own, f[K], wgt<T>(K));
+        } else if constexpr (PAT == PAT_AA_EVEN) {
+            SP::sto(fo + (size_t)J * n + c.own, f[K], wgt<T>(K));
+            if constexpr (GEN && dcx(K) < 0) {
+                if (outlet_cell) SP::sto((st*)a.obuf + ((size_t)a.oslot[b] * 9 + K / 3) * BS2 + c.ly * BS + c.z, f[K], wgt<T>(K));
+            }
+        } else {
+            uint32_t dst = 0;
+            const int kind = classify<J, GEN>(a, c, dst);
+            if (kind == SRC_REG) {
+                SP::sto(fo + (size_t)K * n + dst, f[K], wgt<T>(K));
+            } else if (kind != SRC_INLET) {   // inlet links are evaluated by the even step itself
+                const T v = boundary_value<J, T>(kind, a, c, u_in, a.time_step + 1,
+                                                 [&](auto jc) { return f[decltype(jc)::value]; });
+                SP::sto(fo + (size_t)J * n + c.own, v, wgt<T>(J));
+            }
+        }
+        if constexpr (GEN) {
+            if (link_force) {
+                uint32_t dst = 0;
+                const int kind = classify<J, GEN, REG>(a, c, dst);
+                double m = 0.0;
+                if (kind == SRC_SOLID) {
+                    m = 2.0 * ((double)SP::rt(f[K], wgt<T>(K)) - (double)wgt<T>(K));
+                } else if (PAT == PAT_AB && a.cp.bb_mode == 0 && kind == SRC_REG && ((c.lmask >> J) & 1u)) {
+                    m = ((double)SP::rt(f[K], wgt<T>(K)) - (double)wgt<T>(K)) +
+                        ((double)own_prev<SP, PAT, K>(a, c) - (double)wgt<T>(K));
+                }
+                F0 += m * dcx(K); F1 += m * dcy(K); F2 += m * dcz(K);
+            }
+        }
+    });
+}
+
+// SEL: which blocks a launch covers.  SEL_ALL dispatches on the block flags inside one launch;
+// SEL_INTERIOR / SEL_BOUNDARY run the interior and the boundary block lists as separate launches, so
+// that the interior kernel is compiled for its own (smaller) register budget.
+enum : int { SEL_ALL = 0, SEL_INTERIOR = 1, SEL_BOUNDARY = 2, SEL_REGULAR = 3, SEL_BOUNDARY_REGULAR = 4 };
+
+#ifndef OAL_STEP_WAVES
+#define OAL_STEP_WAVES 3
+#endif
+template <class SP, int PAT, bool WALE, int SEL = SEL_ALL>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(OAL_STEP_WAVES))) k_step(const StepArgs a) {
+    const uint32_t litem = xcd_item(blockIdx.x, gridDim.x);
+    const uint32_t lb = litem / PLANES;
+    const uint32_t b = a.blist ? (uint32_t)a.blist[lb] : lb;
+    const uint32_t item = b * PLANES + litem % PLANES;
+    CellCtx c;
+    c.b = b;
+    c.z = (int)(item % PLANES);
+    c.lx = threadIdx.x & (BS - 1);
+    c.ly = threadIdx.x >> 4;
+    c.flags = a.bflags[b];
+    const DevDomain* dm = a.doms + a.bdom[b];
+    c.ext0 = dm->ext[0]; c.ext1 = dm->ext[1]; c.ext2 = dm->ext[2];
+    c.per0 = dm->per[0]; c.per1 = dm->per[1]; c.per2 = dm->per[2];
+    c.zwall = dm->zwall; c.ymin_wall = dm->ymin_wall;
+    c.gx = a.borg[3 * b] + c.lx;
+    c.gy = a.borg[3 * b + 1] + c.ly;
+    c.gz = a.borg[3 * b + 2] + c.z;
+    c.own = b * BS3 + (uint32_t)c.z * BS2 + (uint32_t)c.ly * BS + (uint32_t)c.lx;
+    double F0 = 0.0, F1 = 0.0, F2 = 0.0;
+    c.lmask = 0;
+    if constexpr (SEL == SEL_REGULAR) {
+        bool live = true;
+        if constexpr (PAT == PAT_AB) {
+            if (a.regular_face) {
+                const FaceBits fb = face_bits(a, b, dm);
+                live = !(face_lane(fb, c.lx, c.ly) || face_plane(fb, c.z));
+            }
+        }
+        step_cell<SP, PAT, WALE, false, true>(a, c, dm, live, false, F0, F1, F2);
+    } else if constexpr (SEL == SEL_INTERIOR) {
+        step_cell<SP, PAT, WALE, false>(a, c, dm, true, false, F0, F1, F2);
+    } else {
+        const bool inside = !(c.flags & BF_PARTIAL) || (c.gx < c.ext0 && c.gy < c.ext1 && c.gz < c.ext2);
+        if (c.flags & BF_SOLID) c.lmask = a.lmask[(size_t)a.lslot[b] * BS3 + (c.own - b * BS3)];
+        const bool solid_own = (c.lmask >> 27) & 1u;
+        const bool active = inside && !(solid_own && a.cp.bb_mode == 1);
+        if constexpr (SEL == SEL_BOUNDARY_REGULAR)
+            step_cell<SP, PAT, WALE, true, true>(a, c, dm, active, solid_own, F0, F1, F2);
+        else
+            step_cell<SP, PAT, WALE, true>(a, c, dm, active, solid_own, F0, F1, F2);
+    }
+    if (a.fpart) wg_sum3(F0 * dm->force_scale, F1 * dm->force_scale, F2 * dm->force_scale, a.fpart, item);
+}
+
+// Regular interior blocks, pull/AB pattern, software-pipelined over z: a workgroup walks `a.pipe_planes`
+// consecutive planes of one block and issues the loads of the next plane before the collision of the
+// current one, so every wave keeps a plane of loads in flight while it computes.
+//
+// Vector-memory loads and stores share one completion counter on gfx950, and with both pending the
+// compiler may only wait for all of them.  The schedule is therefore: wait for everything at the top of
+// an iteration (the current plane's loads and the previous plane's stores, both issued one collision
+// earlier), issue the next plane's loads, then collide and store the current plane from registers that
+// are already complete.  Prefetched values stay in their storage form until the collision uses them.
+template <class SP>
+__device__ __forceinline__ void reg_load_raw(const StepArgs& a, const RegAddr<SP>& ra, int z, typename SP::raw (&r)[27]) {
+    using st = typename SP::st;
+    for_dirs([&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        r[K] = SP::ld_raw((const st*)(ra.base(a.fin, a.ncell, K, z, dcz(K)) + ra.v[dcx(K) + 1][dcy(K) + 1]));
+    });
+}
+
+__device__ __forceinline__ void wait_all_vmem() {
+    __builtin_amdgcn_s_waitcnt(0x0f70);   // vmcnt(0), expcnt and lgkmcnt left open
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// inlet: the equilibrium at (1, u_inlet + noise) for the directions entering through x = 0
+struct InletRule {
+    bool on;          // this lane's cell lies on the inlet face
+    float u_in;
+    int gy, gz;       // the cell's global y and z (the noise position)
+};
+
+template <class SP>
+__device__ __forceinline__ void reg_collide_store(const StepArgs& a, const RegAddr<SP>& ra, const CellRates<typename SP::T>& q,
+                                                  int z, const typename SP::raw (&r)[27], const InletRule* inlet = nullptr) {
+    using T = typename SP::T;
+    using st = typename SP::st;
+    T f[27];
+    for_dirs([&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        f[K] = SP::from_raw(r[K], wgt<T>(K));
+    });
+    if (inlet && inlet->on) {
+        CellCtx c{};
+        c.gy = inlet->gy;
+        c.gz = inlet->gz;
+        for_dirs([&](auto kc) {
+            constexpr int K = decltype(kc)::value;
+            if constexpr (dcx(K) > 0)
+                f[K] = boundary_value<K, T>(SRC_INLET, a, c, inlet->u_in, a.time_step, [&](auto) { return T(0); });
+        });
+    }
+    T rho_o, ux_o, uy_o, uz_o;
+    collide_cell<T>(f, q, a.cp, rho_o, ux_o, uy_o, uz_o);
+    for_dirs([&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        SP::sto((st*)(ra.base(a.fout, a.ncell, K, z, 0) + ra.v[1][1]), f[K], wgt<T>(K));
+    });
+}
+
+// Domain-face rules inside the pipelined kernel (AB pattern).  A source outside the extent is replaced
+// by a population of the cell's own state (kernel_cumulant.jl:233-345, the order of classify): the
+// rules only change the address a lane reads, except at the inlet, whose equilibrium replaces the
+// loaded value after conversion.
+struct FaceRules {
+    bool xl, xh, yl, yh;          // per lane: the cell lies on the x-min / x-max / y-min / y-max face
+    bool zlo, zhi;                // the block touches the z-min / z-max face (uniform)
+    bool zwall, ymin_wall;        // no-slip variants of the z faces and of the y-min face (uniform)
+};
+
+// Gather with the domain-face rules.  Every read is addressed by its byte offset from the start of
+// fin, formed in 32-bit arithmetic modulo 2^32: the host uses this form only for levels whose
+// populations span less than 4 GiB, where every offset read is exact.  The rules pick, per lane, one
+// of a few candidate offsets that are all computed (uniform parts in scalar registers), so that the
+// choice is a select rather than a branch.
+template <class SP>
+__device__ __forceinline__ void reg_load_raw_face(const StepArgs& a, const RegAddr<SP>& ra, const FaceRules& fr, int z,
+                                                  typename SP::raw (&r)[27]) {
+    using st = typename SP::st;
+    const char* f0 = (const char*)a.fin;
+    const uint32_t nbytes = a.ncell * (uint32_t)sizeof(st);   // one direction slot
+    uint32_t zs[3];                                           // plane shift of sources with cz = -1, 0, 1
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int sz = z - (q - 1);
+        const int dz = sz < 0 ? -1 : (sz >= BS ? 1 : 0);
+        zs[q] = (uint32_t)((dz * ra.plane_cells + (int64_t)(sz - BS * dz - ra.zref) * BS2) * (int64_t)sizeof(st));
+    }
+    const bool zlo = fr.zlo && z == 0, zhi = fr.zhi && z == BS - 1;   // uniform
+    // uniform parts pass through readfirstlane, which the optimiser cannot see through: it must not
+    // fold them into per-lane sums and hoist 27 of those out of the plane loop
+    auto uni = [](uint32_t u) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)u); };
+    const uint32_t v11 = ra.v[1][1];
+    for_dirs([&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        constexpr int cx = dcx(K), cy = dcy(K), cz = dcz(K);
+        uint32_t o = ra.v[cx + 1][cy + 1] + uni((uint32_t)K * nbytes + zs[cz + 1]);
+        auto own = [&](uint32_t slot) { return v11 + uni(slot * nbytes + zs[1]); };
+        // lowest precedence first; each later rule overrides (kernel_cumulant.jl:233-345)
+        if constexpr (cz != 0) {
+            const bool zo = cz > 0 ? zlo : zhi;
+            const uint32_t o_z = own(fr.zwall ? dopp(K) : dmir_z(K));
+            o = zo ? o_z : o;                                               // z faces
+            if constexpr (cy != 0) {
+                const bool yo = cy > 0 ? fr.yl : fr.yh;
+                const uint32_t o_y = own(cy > 0 && fr.ymin_wall ? dopp(K) : dmir_y(K));
+                o = (yo && !(zo && fr.zwall)) ? o_y : o;                    // y faces, below no-slip z walls
+            }
+        } else if constexpr (cy != 0) {
+            const bool yo = cy > 0 ? fr.yl : fr.yh;
+            const uint32_t o_y = own(cy > 0 && fr.ymin_wall ? dopp(K) : dmir_y(K));
+            o = yo ? o_y : o;
+        }
+        if constexpr (cx != 0) {
+            const uint32_t o_x = own(K);
+            o = (cx > 0 ? fr.xl : fr.xh) ? o_x : o;    // outlet: own population; inlet: any valid address
+        }
+        r[K] = SP::ld_raw((const st*)(f0 + o));
+    });
+}
+
+// FACE: how blocks on a domain face are handled
+enum : int { FACE_NONE = 0, FACE_RULES = 1, FACE_SKIP = 2 };
+
+// Register budget: with one plane of populations in flight beside the collision of another the kernel
+// fits the 168 VGPRs of 3 waves per SIMD; the in-kernel face rules add per-lane address selects and
+// scalar offsets that only fit the 256 VGPRs of 2 waves.
+template <int FACE>
+constexpr int pipe_waves() { return FACE == FACE_RULES ? 2 : 3; }
+
+template <class SP, int NP, int FACE>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(pipe_waves<FACE>(), pipe_waves<FACE>())))
+k_step_pipe(const StepArgs a) {
+    using T = typename SP::T;
+    using st = typename SP::st;
+    // the loop bound stays a run-time value (equal to NP): with a constant bound the prefetch
+    // conditions fold away, the scheduler mixes the next plane's loads into the collision and the
+    // kernel spills
+    const int np = a.pipe_planes;
+    constexpr uint32_t groups = (uint32_t)(PLANES / NP);
+    const uint32_t litem = xcd_item(blockIdx.x, gridDim.x);
+    const uint32_t b = (uint32_t)a.blist[litem / groups];
+    const int z0 = (int)(litem % groups) * np;
+    CellCtx c;
+    c.b = b;
+    c.lx = threadIdx.x & (BS - 1);
+    c.ly = threadIdx.x >> 4;
+    c.z = z0;
+    c.own = b * BS3 + (uint32_t)z0 * BS2 + (uint32_t)c.ly * BS + (uint32_t)c.lx;
+    const DevDomain* dm = a.doms + a.bdom[b];
+    RegAddr<SP> ra;
+    ra.init(c, dm->nbx, dm->nby);   // offsets of plane z0
+    // every domain parameter is read before the first store: reads issued after stores to the
+    // population arrays could not be proven independent of them
+    CellRates<T> q;
+    if constexpr (sizeof(T) == 8) { q.omega1 = dm->omega1_d; q.omega4_diag = dm->omega4d_d; }
+    else { q.omega1 = dm->omega1; q.omega4_diag = dm->omega4d; }
+    q.forced = dm->forced != 0;
+    q.Fx = T(dm->F[0]); q.Fy = T(dm->F[1]); q.Fz = T(dm->F[2]);
+    // blocks on a domain face: FACE_RULES applies the face rules here, FACE_SKIP leaves the face cells
+    // to the face-cell kernel
+    FaceRules fr{};
+    InletRule in{};
+    bool lane_face = false;
+    FaceBits fskip{};
+    if constexpr (FACE == FACE_SKIP) {
+        fskip = face_bits(a, b, dm);
+        lane_face = face_lane(fskip, c.lx, c.ly);
+    }
+    auto live = [&](int z) { return FACE != FACE_SKIP || !(lane_face || face_plane(fskip, z)); };
+    if constexpr (FACE == FACE_RULES) {
+        const FaceBits fb = face_bits(a, b, dm);
+        fr.xl = fb.xlo && c.lx == 0;
+        fr.xh = fb.xhi && c.lx == BS - 1;
+        fr.yl = fb.ylo && c.ly == 0;
+        fr.yh = fb.yhi && c.ly == BS - 1;
+        fr.zlo = fb.zlo;
+        fr.zhi = fb.zhi;
+        fr.zwall = dm->zwall != 0;
+        fr.ymin_wall = dm->ymin_wall == 1;
+        in.on = fr.xl;
+        in.u_in = dm->u_in;
+        in.gy = a.borg[3 * b + 1] + c.ly;
+        in.gz = a.borg[3 * b + 2];      // + z per plane
+    }
+    auto load = [&](const RegAddr<SP>& r, int z, typename SP::raw (&dst)[27]) {
+        if constexpr (FACE == FACE_RULES) reg_load_raw_face<SP>(a, r, fr, z, dst);
+        else if (live(z)) reg_load_raw<SP>(a, r, z, dst);
+    };
+    auto collide = [&](const RegAddr<SP>& r, int z, const typename SP::raw (&src)[27]) {
+        if constexpr (FACE == FACE_RULES) {
+            InletRule iz = in;
+            iz.gz += z;
+            reg_collide_store<SP>(a, r, q, z, src, &iz);
+        } else if (live(z)) {
+            reg_collide_store<SP>(a, r, q, z, src);
+        }
+    };
+    // the per-lane offsets stay those of plane z0; later planes shift the uniform base
+    typename SP::raw fa[27], fb[27];
+    load(ra, z0, fa);
+    for (int z = z0; z < z0 + np; z += 2) {
+        wait_all_vmem();
+        if (z + 1 < z0 + np) load(ra, z + 1, fb);
+        collide(ra, z, fa);
+        if (z + 1 >= z0 + np) break;
+        wait_all_vmem();
+        if (z + 2 < z0 + np) load(ra, z + 2, fa);
+        collide(ra, z + 1, fb);
+    }
+}
+
+// Face cells of the regular-list blocks (AB pattern): one lane per listed cell, the general path with
+// the domain-face rules.  These blocks hold no solid cell and no solid link, so no force is summed.
+template <class SP, bool WALE>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(OAL_STEP_WAVES))) k_face(const StepArgs a) {
+    const uint32_t i = blockIdx.x * WG + threadIdx.x;
+    if (i >= a.n_face) return;
+    const uint32_t own = a.face_cells[i];
+    const uint32_t b = own / BS3, loc = own % BS3;
+    CellCtx c;
+    c.b = b;
+    c.own = own;
+    c.lx = (int)(loc & (BS - 1));
+    c.ly = (int)((loc >> 4) & (BS - 1));
+    c.z = (int)(loc >> 8);
+    c.flags = a.bflags[b];
+    const DevDomain* dm = a.doms + a.bdom[b];
+    c.ext0 = dm->ext[0]; c.ext1 = dm->ext[1]; c.ext2 = dm->ext[2];
+    c.per0 = dm->per[0]; c.per1 = dm->per[1]; c.per2 = dm->per[2];
+    c.zwall = dm->zwall; c.ymin_wall = dm->ymin_wall;
+    c.gx = a.borg[3 * b] + c.lx;
+    c.gy = a.borg[3 * b + 1] + c.ly;
+    c.gz = a.borg[3 * b + 2] + c.z;
+    c.lmask = 0;
+    double F0 = 0.0, F1 = 0.0, F2 = 0.0;
+    step_cell<SP, PAT_AB, WALE, true>(a, c, dm, true, false, F0, F1, F2);
+}
+
+template <class SP, int NP>
+static void launch_pipe(int face, dim3 grid, hipStream_t s, const StepArgs& r) {
+    if (face == FACE_RULES) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_RULES>), grid, dim3(WG), 0, s, r);
+    else if (face == FACE_SKIP) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_SKIP>), grid, dim3(WG), 0, s, r);
+    else hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_NONE>), grid, dim3(WG), 0, s, r);
+}
+
+// One step of a level: launches over disjoint block lists (regular, other interior, regular boundary,
+// boundary), each compiled for its own register budget, plus, in the AB pattern, the face cells that
+// the regular kernels skip.  The boundary and face launches go to `side` (when given) so that they
+// overlap the bandwidth-bound bulk launch; the caller joins the two streams.
+template <class SP, int PAT, bool WALE>
+static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t side, hipEvent_t bulk0, hipEvent_t bulk1) {
+    StepArgs r = a;
+    hipStream_t bs = side ? side : stream;
+    if (a.n_boundary_regular) {
+        r.blist = a.blist_boundary_regular;
+        hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), dim3(a.n_boundary_regular * PLANES), dim3(WG), 0,
+                           bs, r);
+    }
+    if (a.n_boundary) {
+        r.blist = a.blist_boundary;
+        hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY>), dim3(a.n_boundary * PLANES), dim3(WG), 0, bs, r);
+    }
+    // face cells: the pipelined kernel applies the face rules itself, the one-plane regular kernel
+    // leaves them to the face-cell kernel
+    constexpr bool pipe_capable = PAT == PAT_AB && !WALE && sizeof(typename SP::T) == 4;
+    if constexpr (PAT == PAT_AB) {
+        const bool piped = pipe_capable && a.pipe_planes > 1;
+        if (a.n_face && (!piped || a.face_kernel))
+            hipLaunchKernelGGL((k_face<SP, WALE>), dim3((a.n_face + WG - 1) / WG), dim3(WG), 0, bs, a);
+    }
+    if (bulk0) {
+        const hipError_t e = hipEventRecord(bulk0, stream);
+        if (e != hipSuccess) return e;
+    }
+    if (a.n_regular) {
+        r.blist = a.blist_regular;
+        const dim3 g(a.n_regular * (PLANES / (a.pipe_planes > 1 ? a.pipe_planes : 1)));
+        if constexpr (pipe_capable) {   // (no force on regular blocks)
+            const int fc = !a.regular_face ? FACE_NONE : (a.face_kernel ? FACE_SKIP : FACE_RULES);
+            switch (a.pipe_planes) {
+                case 2: launch_pipe<SP, 2>(fc, g, stream, r); break;
+                case 4: launch_pipe<SP, 4>(fc, g, stream, r); break;
+                case 8: launch_pipe<SP, 8>(fc, g, stream, r); break;
+                case 16: launch_pipe<SP, 16>(fc, g, stream, r); break;
+                default: hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_REGULAR>), g, dim3(WG), 0, stream, r); break;
+            }
+        } else {
+            hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_REGULAR>), dim3(a.n_regular * PLANES), dim3(WG), 0, stream, r);
+        }
+    }
+    if (a.n_interior) {
+        r.blist = a.blist_interior;
+        hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_INTERIOR>), dim3(a.n_interior * PLANES), dim3(WG), 0, stream, r);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !bulk1) return e;
+    return hipEventRecord(bulk1, stream);
+}
+
+template <class SP>
+static hipError_t launch_sp(int stream_mode, bool wale, const StepArgs& a, hipStream_t s, hipStream_t side, hipEvent_t e0,
+                            hipEvent_t e1) {
+    if (stream_mode == 0)
+        return wale ? launch_pat<SP, PAT_AB, true>(a, s, side, e0, e1) : launch_pat<SP, PAT_AB, false>(a, s, side, e0, e1);
+    if (a.parity == 0)
+        return wale ? launch_pat<SP, PAT_AA_EVEN, true>(a, s, side, e0, e1)
+                    : launch_pat<SP, PAT_AA_EVEN, false>(a, s, side, e0, e1);
+    return wale ? launch_pat<SP, PAT_AA_ODD, true>(a, s, side, e0, e1) : launch_pat<SP, PAT_AA_ODD, false>(a, s, side, e0, e1);
+}
+
+}  // namespace oal

@@ -210,7 +210,14 @@ class Plan:
     def sync(self) -> None:
         check(self.lib.oal_sync(self.h), "oal_sync")
 
+    def level_info(self, level: int = 0) -> dict:
+        """Cell counts of a level and how its blocks split over the step kernels."""
+        info = _lib.OalLevelInfo()
+        check(self.lib.oal_level_info(self.h, level, C.byref(info)), "oal_level_info")
+        return {name: getattr(info, name) for name, _ in _lib.OalLevelInfo._fields_ if name != "reserved"}
+
     def time_steps(self, n: int, level: int = 0, force: bool = False) -> tuple[float, float]:
+        """Times n steps with HIP events: (ms for all n steps, mean ms of one step's bulk launches)."""
         tot, ker = C.c_double(), C.c_double()
         check(self.lib.oal_time_steps(self.h, level, n, _lib.OAL_STEP_FORCE if force else 0, C.byref(tot),
                                       C.byref(ker)), "oal_time_steps")
@@ -218,23 +225,37 @@ class Plan:
 
     # -- launch shape ------------------------------------------------------------------------------
     @property
-    def pipe_planes(self) -> int:
+    def tuning(self) -> dict:
+        """Launch shape: pipe_planes (1, 2, 4, 8, 16), overlap (0 or 1), face_kernel (0 or 1)."""
         t = _lib.OalTuning()
         check(self.lib.oal_get_tuning(self.h, C.byref(t)), "oal_get_tuning")
-        return t.pipe_planes
+        return {"pipe_planes": t.pipe_planes, "overlap": t.overlap, "face_kernel": t.face_kernel}
+
+    def set_tuning(self, pipe_planes: int | None = None, overlap: int | None = None,
+                   face_kernel: int | None = None) -> None:
+        cur = self.tuning
+        t = _lib.OalTuning()
+        t.pipe_planes = int(cur["pipe_planes"] if pipe_planes is None else pipe_planes)
+        t.overlap = int(cur["overlap"] if overlap is None else overlap)
+        t.face_kernel = int(cur["face_kernel"] if face_kernel is None else face_kernel)
+        check(self.lib.oal_set_tuning(self.h, C.byref(t)), "oal_set_tuning")
+
+    @property
+    def pipe_planes(self) -> int:
+        return self.tuning["pipe_planes"]
 
     @pipe_planes.setter
     def pipe_planes(self, v: int) -> None:
-        t = _lib.OalTuning()
-        t.pipe_planes = int(v)
-        check(self.lib.oal_set_tuning(self.h, C.byref(t)), "oal_set_tuning")
+        self.set_tuning(pipe_planes=v)
 
-    def autotune(self, n_steps: int = 10, level: int = 0) -> tuple[int, dict]:
-        """Time every launch shape, keep the fastest; the populations are restored afterwards."""
+    def autotune(self, n_steps: int = 10, level: int = 0) -> tuple[dict, dict]:
+        """Time every launch shape, keep the fastest; the populations are restored afterwards.
+        Returns the choice and {(pipe_planes, overlap, face_kernel): ms per step}."""
         best = _lib.OalTuning()
-        ms = (C.c_double * 5)()
+        ms = (C.c_double * _lib.OAL_TUNE_SHAPES)()
         check(self.lib.oal_autotune(self.h, level, n_steps, C.byref(best), ms), "oal_autotune")
-        return best.pipe_planes, dict(zip((1, 2, 4, 8, 16), list(ms)))
+        choice = {"pipe_planes": best.pipe_planes, "overlap": best.overlap, "face_kernel": best.face_kernel}
+        return choice, dict(zip(_lib.TUNE_KEYS, list(ms)))
 
     # -- results -----------------------------------------------------------------------------------
     def force(self, level: int = 0) -> np.ndarray:

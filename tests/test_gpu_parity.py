@@ -66,8 +66,10 @@ def gpu_plan(grid, storage, okw, stream="ab"):
     return solver.Plan(solver.Level(grid, [dom]), storage=storage, stream=stream, bb_mode=bb, params=params)
 
 
-def gpu_run(grid, storage, steps, okw, rho, u, stream="ab", force_steps=()):
+def gpu_run(grid, storage, steps, okw, rho, u, stream="ab", force_steps=(), tuning=None):
     plan = gpu_plan(grid, storage, okw, stream)
+    if tuning:
+        plan.set_tuning(**tuning)
     plan.init_equilibrium(rho, u)
     forces = []
     for t in range(steps):
@@ -134,18 +136,21 @@ VARIANTS = {
 }
 
 
+@pytest.mark.parametrize("face_kernel", [0, 1])
 @pytest.mark.parametrize("variant", list(VARIANTS))
 @pytest.mark.parametrize("storage", ["f32", "f16s"])
-def test_variant_parity(oracle, variant, storage):
-    grid = uniform_grid((48, 32, 32))
-    grid.solid = sphere_mask(grid, (18, 15, 17), 5.5)
+def test_variant_parity(oracle, variant, storage, face_kernel):
+    # 4 x 2 x 2 blocks: the sphere's blocks in the middle, the inlet and outlet blocks on the regular
+    # path with its in-kernel face rules
+    grid = uniform_grid((64, 32, 32))
+    grid.solid = sphere_mask(grid, (31, 15, 17), 5.5)
     okw = dict(tau=0.53, u_inlet=0.05)
     okw.update(VARIANTS[variant])
     rho, u = fields(grid, seed=3, u0=(0.05, 0, 0), amp=0.03)
     steps = 5
     fst = (4,)
     f_or, F_or, _ = oracle_run(oracle, grid, storage, steps, okw, rho, u, force_steps=fst)
-    plan, F_gpu = gpu_run(grid, storage, steps, okw, rho, u, force_steps=fst)
+    plan, F_gpu = gpu_run(grid, storage, steps, okw, rho, u, force_steps=fst, tuning=dict(face_kernel=face_kernel))
     compare(oracle, plan, storage, f_or, grid)
     np.testing.assert_allclose(F_gpu[0], F_or[0], rtol=1e-11, atol=1e-14)
     plan.close()
@@ -293,8 +298,9 @@ def test_fp32_within_1e5_of_literal_reference(oracle, case):
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16s", "bf16s"])
+@pytest.mark.parametrize("face_kernel", [0, 1])
 @pytest.mark.parametrize("planes", [1, 2, 4, 8, 16])
-def test_interior_pipeline_shapes(oracle, storage, planes):
+def test_interior_pipeline_shapes(oracle, storage, planes, face_kernel):
     """Every launch shape of the pipelined interior kernel gives the same bits (regular interior
     blocks present: 5 x 3 x 3 blocks)."""
     grid = uniform_grid((80, 48, 48))
@@ -304,8 +310,9 @@ def test_interior_pipeline_shapes(oracle, storage, planes):
     steps = 3
     f_or, _, _ = oracle_run(oracle, grid, storage, steps, okw, rho, u)
     plan = gpu_plan(grid, storage, okw)
-    plan.pipe_planes = planes
-    assert plan.pipe_planes == planes
+    shape = {"pipe_planes": planes, "overlap": int(planes % 4 == 0), "face_kernel": face_kernel}
+    plan.set_tuning(**shape)
+    assert plan.tuning == shape
     plan.init_equilibrium(rho, u)
     for t in range(steps):
         plan.level_step(0, t)
@@ -320,7 +327,8 @@ def test_autotune_restores_state():
     plan.init_equilibrium(rho, u)
     before = plan.download_f(dtype=np.float32)
     best, ms = plan.autotune(4)
-    assert best in (1, 2, 4, 8, 16) and plan.pipe_planes == best
-    assert all(v > 0 for v in ms.values())
+    assert best["pipe_planes"] in (1, 2, 4, 8, 16) and best["overlap"] in (0, 1) and plan.tuning == best
+    assert len(ms) == 20 and all(v > 0 for v in ms.values())
+    assert ms[(best["pipe_planes"], best["overlap"], best["face_kernel"])] == min(ms.values())
     assert np.array_equal(plan.download_f(dtype=np.float32), before)
     plan.close()
