@@ -14,6 +14,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -84,6 +85,20 @@ def copy_bandwidth_gbs() -> float:
     return v.value
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (copy-calibrated), or None when the summary covers another kernel or launch shape."""
+    path = os.path.join(ROOT, "profiles", "step_f32_256_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    k = re.search(r"^\s*kernel = (.+)$", text, re.M)
+    b = re.search(r"^\s*hbm_bytes_per_launch = (\d+)$", text, re.M)
+    if not (k and b) or k.group(1).strip() != kernel:
+        return None
+    return int(b.group(1))
+
+
 def cpu_baseline(grid, budget_s: float = 20.0):
     """The CPU oracle (this project's C++ restatement of the reference arithmetic, OpenMP) on the same
     workload, FP32: as many whole 256^3 steps as fit in the budget (at least one)."""
@@ -146,19 +161,18 @@ def main(argv=None) -> int:
     mlups = n_cells * args.steps / (ms_total * 1e-3) / 1e6
     ms_step = ms_total / args.steps
     pp = tun["pipe_planes"]
-    # dominant kernel: the bulk launch over the regular blocks (every cell fluid) less the face cells,
-    # which the face-cell kernel steps
-    bulk_cells = (info["n_regular"] + info["n_interior"]) * 16 ** 3 - info["n_face_cells"]
+    # dominant kernel: the bulk launch over the regular blocks (every cell fluid); the face cells are
+    # stepped by it when it applies the face rules itself, else by the face-cell kernel
+    face_in_bulk = args.stream == "ab" and pp > 1 and not tun["face_kernel"]
+    bulk_cells = (info["n_regular"] + info["n_interior"]) * 16 ** 3 - (0 if face_in_bulk else info["n_face_cells"])
     achieved = bytes_per_lup("f32") * bulk_cells / (ms_kernel * 1e-3) / 1e9
     step_gbs = bytes_per_lup("f32") * n_fluid / (ms_step * 1e-3) / 1e9
     area = np.pi * (D / 2) ** 2
     cd = float(F[0] / (0.5 * U_IN ** 2 * area))
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_step_f32_256.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    kernel = (f"k_step_pipe<StF32, {pp}, {2 if tun['face_kernel'] else 1}>" if args.stream == "ab" and pp > 1
+              else "k_step<StF32, regular>")
+    traffic = measured_traffic(kernel)
 
     out = {
         "metric": "MLUPS (D3Q27 cumulant, FP32 populations, 256^3 sphere box)",
@@ -178,8 +192,7 @@ def main(argv=None) -> int:
                    "cells": n_cells, "fluid_cells": n_fluid, "stream": args.stream, "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": (f"k_step_pipe<StF32, {pp}>" if args.stream == "ab" and pp > 1
-                                else "k_step<StF32, regular>"),
+                     "kernel": kernel,
                      "kernel_ms": round(ms_kernel, 5), "cells_per_launch": bulk_cells,
                      "bytes_per_lup": bytes_per_lup("f32"),
                      "step_ms": round(ms_step, 5), "step_achieved_gbs": round(step_gbs, 1),
@@ -193,7 +206,9 @@ def main(argv=None) -> int:
         ms_t16, ms_k16, _, F16, tun16, _ = gpu_leg(grid, "f16s", args.stream, args.steps, args.warmup,
                                                    not args.no_autotune, fixed)
         m16 = n_cells * args.steps / (ms_t16 * 1e-3) / 1e6
-        a16 = bytes_per_lup("f16s") * bulk_cells / (ms_k16 * 1e-3) / 1e9
+        face16 = args.stream == "ab" and tun16["pipe_planes"] > 1 and not tun16["face_kernel"]
+        bulk16 = (info["n_regular"] + info["n_interior"]) * 16 ** 3 - (0 if face16 else info["n_face_cells"])
+        a16 = bytes_per_lup("f16s") * bulk16 / (ms_k16 * 1e-3) / 1e9
         s16 = bytes_per_lup("f16s") * n_fluid / (ms_t16 / args.steps * 1e-3) / 1e9
         out["fp16_shifted"] = {"value": round(m16, 1), "unit": "MLUPS", "ms_per_step": round(ms_t16 / args.steps, 4),
                                "speedup_vs_f32": round(m16 / mlups, 3), "tuning": tun16,

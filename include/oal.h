@@ -145,8 +145,9 @@ typedef struct OalLevelInfo {
     int32_t n_interior;
     int32_t n_boundary_regular;
     int32_t n_boundary;
-    int32_t n_face_cells;    /* AB pattern: cells of regular blocks on a domain face, stepped by the
-                                face-cell kernel on the second stream */
+    int32_t n_face_cells;    /* AB pattern: cells of regular blocks on a domain face; the bulk kernel
+                                applies their rules itself or leaves them to the face-cell kernel
+                                (OalTuning.face_kernel) */
     int32_t reserved[2];
 } OalLevelInfo;
 int oal_level_info(const OalPlan* plan, int level, OalLevelInfo* info);
@@ -173,8 +174,9 @@ enum {
  * time_step is the reference's time_step_seed (inlet noise). */
 int oal_level_step(OalPlan* plan, int level, int32_t time_step, int flags, void* stream);
 
-/* n coarse steps of every level with 2:1 sub-cycling and level transfers: execute_timestep_batch!
- * (src/solver/solver_control.jl:796-824) over recursive_step! (:55-422). */
+/* n coarse steps: execute_timestep_batch! (src/solver/solver_control.jl:796-824) over
+ * recursive_step! (:55-422).  Plans hold one level in this version (oal_plan_create rejects
+ * n_levels != 1), so a coarse step is one step of level 0 with time_step first_time_step + i. */
 int oal_step(OalPlan* plan, int n_steps, int32_t first_time_step, int flags, void* stream);
 
 /* The momentum-exchange force of the last step collected with OAL_STEP_FORCE, FP64, per domain of

@@ -522,7 +522,7 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
     if (desc->bb_mode != OAL_BB_REFERENCE && desc->bb_mode != OAL_BB_HALFWAY) return fail(OAL_ERR_ARG, "unknown bb_mode");
     if (desc->stream_mode == OAL_STREAM_AA && desc->bb_mode != OAL_BB_HALFWAY)
         return fail(OAL_ERR_UNSUPPORTED, "the AA pattern needs halfway bounce-back");
-    if (desc->n_levels != 1) return fail(OAL_ERR_UNSUPPORTED, "multi-level plans are built by oal_plan_create_octree");
+    if (desc->n_levels != 1) return fail(OAL_ERR_UNSUPPORTED, "plans of more than one level are not supported in this version");
     if (!desc->levels) return fail(OAL_ERR_ARG, "no level descriptors");
     const OalCollision& c = desc->collision;
     if (c.limiter < 0 || c.limiter > 2) return fail(OAL_ERR_ARG, "unknown limiter");
