@@ -123,6 +123,62 @@ def cpu_baseline(grid, budget_s: float = 20.0):
     return n * done / t_used / 1e6, threads, done, t_used
 
 
+def validation(steps: int, seed: int, chunk: int = 1000) -> dict:
+    """Accuracy of the storage forms on the same workload against an FP64 run of the same kernels (the
+    FP64 path equals the FP64 oracle bit for bit in the parity tests).
+
+    C2: FP32 populations from the uniform start.  C3: FP16 and BF16 shifted storage from a start whose
+    velocity carries a seeded perturbation, uniform in [-1e-3, 1e-3] per component (numpy PCG64,
+    `seed`), with the FP64 run started from the same field.  After `steps` steps (the last one
+    collecting the momentum exchange): Cd = F_x / (1/2 rho0 u_in^2 pi R^2) and
+    L2(u) = |u - u_64| / |u_64| over the fluid cells."""
+    from openaerolab_alpha_01_amd import solver
+    grid = workload_grid()
+    fluid = grid.solid == 0
+    n = grid.n_cells
+    area = np.pi * (D / 2) ** 2
+    params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
+    out = {"steps": steps, "seed": seed, "cd_reference_area": round(area, 4)}
+    for case, sd, forms in (("C2", None, ("f32",)), ("C3", seed, ("f16s", "bf16s"))):
+        u0 = np.zeros((3, n))
+        u0[0] = U_IN
+        if sd is not None:
+            u0 += np.random.default_rng(sd).uniform(-1e-3, 1e-3, size=(3, n))
+        rho0 = np.ones(n)
+        ref = None
+        res = {}
+        for st in ("f64",) + forms:
+            t0 = time.perf_counter()
+            plan = solver.uniform_plan(grid, tau_of(), storage=st, params=params, u_inlet=U_IN)
+            plan.init_equilibrium(rho0, u0)
+            t = 0
+            while t < steps - 1:
+                k = min(chunk, steps - 1 - t)
+                plan.step(k, t)
+                plan.sync()
+                t += k
+                print(f"[validate] {case} {st} step {t}/{steps} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
+                      flush=True)
+            plan.level_step(0, t, force=True)
+            F = plan.force()[0]
+            _, u = plan.macro()
+            plan.guard(0.2, 0.5)
+            plan.close()
+            uf = u[:, fluid]
+            del u
+            cd = float(F[0] / (0.5 * U_IN ** 2 * area))
+            entry = {"cd": round(cd, 6), "cl_y": round(float(F[1] / (0.5 * U_IN ** 2 * area)), 7),
+                     "cl_z": round(float(F[2] / (0.5 * U_IN ** 2 * area)), 7)}
+            if ref is None:
+                ref = (cd, uf)
+            else:
+                entry["cd_rel_err_vs_f64"] = float((cd - ref[0]) / ref[0])
+                entry["l2_u_vs_f64"] = float(np.linalg.norm(uf - ref[1]) / np.linalg.norm(ref[1]))
+            res[st] = entry
+        out[case] = res
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +196,10 @@ def main(argv=None) -> int:
     ap.add_argument("--face-kernel", type=int, default=None, choices=[0, 1],
                     help="fixed: face cells by a separate kernel (1) or by the bulk kernel's rules (0) "
                          "(implies --no-autotune)")
+    ap.add_argument("--validate", action="store_true",
+                    help="accuracy run instead of the benchmark: Cd and L2(u) of FP32 / FP16 / BF16 vs FP64")
+    ap.add_argument("--val-steps", type=int, default=20000)
+    ap.add_argument("--seed", type=int, default=1234)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -149,6 +209,9 @@ def main(argv=None) -> int:
                                ("face_kernel", args.face_kernel)) if v is not None}
     if fixed:
         args.no_autotune = True
+    if args.validate:
+        print(json.dumps({"validation": validation(args.val_steps, args.seed)}), flush=True)
+        return 0
     if args.steps % 2 or args.warmup % 2:
         args.steps += args.steps % 2
         args.warmup += args.warmup % 2

@@ -41,7 +41,8 @@ enum {
 enum {
     OAL_F32 = 0,     /* FP32 populations, unshifted (the reference's storage) */
     OAL_F64 = 1,     /* FP64 populations and arithmetic (validation runs only) */
-    OAL_F16S = 2,    /* FP16 storage of f - w_k, FP32 arithmetic */
+    OAL_F16S = 2,    /* FP16 storage of (f - w_k) * 2^14, FP32 arithmetic (the exact power-of-two
+                        factor keeps the shifted values of low-Mach flows in FP16's normal range) */
     OAL_BF16S = 3    /* BF16 storage of f - w_k, FP32 arithmetic */
 };
 
@@ -49,6 +50,17 @@ enum {
 enum {
     OAL_STREAM_AB = 0,   /* pull from buffer A into buffer B, then swap (the reference's scheme) */
     OAL_STREAM_AA = 1    /* in-place AA pattern, one buffer, even/odd kernels */
+};
+
+/* rounding of the populations a step stores in the 16-bit forms (OAL_F16S, OAL_BF16S).  Stochastic
+ * rounding draws its random bits from a hash of (cell, direction, time_step): results are
+ * reproducible, and the same for every launch shape and streaming pattern.  Deterministic rounding
+ * of populations that barely change from step to step repeats the same error every step, a steady
+ * mass and momentum source that biases steady flows (sphere drag: 2 % in FP16, 25 % in BF16 at
+ * Re = 200); stochastic rounding does not.  Uploads and initialisation round to nearest. */
+enum {
+    OAL_ROUND_STOCHASTIC = 0,
+    OAL_ROUND_NEAREST = 1
 };
 
 /* treatment of obstacle cells */
@@ -113,7 +125,7 @@ typedef struct {
     const OalLevelDesc* levels;
     OalCollision collision;
     int32_t device;            /* HIP device ordinal */
-    int32_t reserved;
+    int32_t rounding;          /* OAL_ROUND_STOCHASTIC (0) or OAL_ROUND_NEAREST: 16-bit forms only */
 } OalPlanDesc;
 
 typedef struct OalPlan OalPlan;
@@ -171,7 +183,8 @@ enum {
 
 /* One step of one level: perform_timestep_v2! (src/collision/timestep.jl:946-2000) with the
  * cumulant operator of get_collision_kernel (src/collision/physics_kernels.jl:50-74).
- * time_step is the reference's time_step_seed (inlet noise). */
+ * time_step is the reference's time_step_seed (inlet noise) and keys the stochastic rounding of
+ * the 16-bit forms: pass the step number, so that consecutive steps draw different bits. */
 int oal_level_step(OalPlan* plan, int level, int32_t time_step, int flags, void* stream);
 
 /* n coarse steps: execute_timestep_batch! (src/solver/solver_control.jl:796-824) over

@@ -17,6 +17,7 @@ OAL_F32, OAL_F64, OAL_F16S, OAL_BF16S = 0, 1, 2, 3
 OAL_STREAM_AB, OAL_STREAM_AA = 0, 1
 OAL_BB_REFERENCE, OAL_BB_HALFWAY = 0, 1
 OAL_LIMITER_NONE, OAL_LIMITER_FACTORED, OAL_LIMITER_POSITIVITY = 0, 1, 2
+OAL_ROUND_STOCHASTIC, OAL_ROUND_NEAREST = 0, 1
 OAL_STEP_FORCE = 1
 
 STORAGE_CODES = {"f32": OAL_F32, "f64": OAL_F64, "f16s": OAL_F16S, "bf16s": OAL_BF16S}
@@ -59,7 +60,7 @@ class OalPlanDesc(C.Structure):
     _fields_ = [
         ("abi_version", C.c_int32), ("storage", C.c_int32), ("block_size", C.c_int32), ("stream_mode", C.c_int32),
         ("bb_mode", C.c_int32), ("n_levels", C.c_int32), ("levels", C.POINTER(OalLevelDesc)),
-        ("collision", OalCollision), ("device", C.c_int32), ("reserved", C.c_int32),
+        ("collision", OalCollision), ("device", C.c_int32), ("rounding", C.c_int32),
     ]
 
 

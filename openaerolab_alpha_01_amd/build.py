@@ -32,8 +32,10 @@ HIP_FLAGS = [
 ]
 
 
-def build_library(verbose: bool = False, extra: list[str] | None = None, jobs: int | None = None) -> Path:
+def build_library(verbose: bool = False, extra: list[str] | None = None, jobs: int | None = None,
+                  out: Path | None = None) -> Path:
     """Compile liboal.so for gfx950 (always recompiles): one object per source, in parallel, then link."""
+    lib = Path(out).resolve() if out else LIB
     jobs = jobs or min(len(SOURCES), max(1, len(os.sched_getaffinity(0))))
     with tempfile.TemporaryDirectory(prefix="oal_build_") as tmpdir:
         objs = [Path(tmpdir) / (src.stem + ".o") for src in SOURCES]
@@ -52,13 +54,13 @@ def build_library(verbose: bool = False, extra: list[str] | None = None, jobs: i
         bad = [c for c, r in zip(cmds, results) if r.returncode != 0]
         if bad:
             raise subprocess.CalledProcessError(1, bad[0])
-        tmp = LIB.with_suffix(".so.tmp")
+        tmp = lib.with_suffix(".so.tmp")
         link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
         if verbose:
             print(" ".join(link), flush=True)
         subprocess.run(link, check=True, cwd=str(CSRC))
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 def build_oracle(verbose: bool = False) -> Path:

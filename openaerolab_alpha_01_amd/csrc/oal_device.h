@@ -44,42 +44,91 @@ template <class T> __device__ __forceinline__ T mx(T a, T b) { return a < b ? b 
 template <class T> __device__ __forceinline__ T mn(T a, T b) { return b < a ? b : a; }
 
 // ------------------------------------------------------------------------------------------------
-// Storage forms.  ld() returns the working-precision population, sto() stores one.  The 16-bit forms
-// hold f - w_k (shifted), which keeps the small deviations from rest that matter.  rt() is the value
-// a stored population reads back as.
+// Storage forms.  ld() returns the working-precision population, sto() stores one rounded to
+// nearest (initialisation, uploads), sto_r() stores one with the step's rounding: to nearest, or
+// stochastic with the random bits r (the 16-bit forms).  rt_r() is the value a population stored by
+// sto_r() reads back as.  The 16-bit forms hold f - w_k (shifted), which keeps the small deviations
+// from rest that matter; FP16 holds it times 2^14, an exact power-of-two factor that keeps the
+// shifted values of a low-Mach flow in FP16's normal range.
+//
+// Stochastic rounding: deterministic rounding of a population that barely changes from step to step
+// makes the same error every step, a steady source of mass and momentum; unbiased random rounding
+// does not.  The random bits come from a counter hash of (cell, direction, time step), so results are
+// reproducible and independent of the launch shape and of the streaming pattern.
 // ------------------------------------------------------------------------------------------------
+__host__ __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
+}
+// uniform per step
+__host__ __device__ __forceinline__ uint32_t round_key(int32_t time_step) {
+    return fmix32((uint32_t)time_step * 0x9e3779b9u + 0x632be5abu);
+}
+// per cell
+__host__ __device__ __forceinline__ uint32_t round_cell(uint32_t key, uint32_t cell) { return fmix32(cell ^ key); }
+// per population: 32 random bits, the high ones used
+__host__ __device__ __forceinline__ uint32_t round_bits(uint32_t hc, int k) {
+    return (hc ^ ((uint32_t)k * 0x9e3779b9u)) * 0x85ebca6bu;
+}
+
 struct StF32 {
     using st = float;
     using T = float;
     using raw = float;
     static constexpr int code = 0;
+    static constexpr bool dithered = false;
     __device__ static __forceinline__ raw ld_raw(const st* p) { return *p; }
     __device__ static __forceinline__ T from_raw(raw r, T) { return r; }
     __device__ static __forceinline__ T ld(const st* p, T) { return *p; }
     __device__ static __forceinline__ void sto(st* p, T f, T) { *p = f; }
-    __device__ static __forceinline__ T rt(T f, T) { return f; }
+    __device__ static __forceinline__ void sto_r(st* p, T f, T, uint32_t, bool) { *p = f; }
+    __device__ static __forceinline__ T rt_r(T f, T, uint32_t, bool) { return f; }
 };
 struct StF64 {
     using st = double;
     using T = double;
     using raw = double;
     static constexpr int code = 1;
+    static constexpr bool dithered = false;
     __device__ static __forceinline__ raw ld_raw(const st* p) { return *p; }
     __device__ static __forceinline__ T from_raw(raw r, T) { return r; }
     __device__ static __forceinline__ T ld(const st* p, T) { return *p; }
     __device__ static __forceinline__ void sto(st* p, T f, T) { *p = f; }
-    __device__ static __forceinline__ T rt(T f, T) { return f; }
+    __device__ static __forceinline__ void sto_r(st* p, T f, T, uint32_t, bool) { *p = f; }
+    __device__ static __forceinline__ T rt_r(T f, T, uint32_t, bool) { return f; }
 };
 struct StF16S {
     using st = _Float16;
     using T = float;
     using raw = uint32_t;   // the 16 stored bits, zero-extended
     static constexpr int code = 2;
+    static constexpr bool dithered = true;
+    static constexpr float scale = 16384.0f, inv_scale = 1.0f / 16384.0f;
     __device__ static __forceinline__ raw ld_raw(const st* p) { return (uint32_t)*(const uint16_t*)p; }
-    __device__ static __forceinline__ T from_raw(raw r, T w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)r) + w; }
-    __device__ static __forceinline__ T ld(const st* p, T w) { return (float)(*p) + w; }
-    __device__ static __forceinline__ void sto(st* p, T f, T w) { *p = (_Float16)(f - w); }
-    __device__ static __forceinline__ T rt(T f, T w) { return (float)(_Float16)(f - w) + w; }
+    __device__ static __forceinline__ T from_raw(raw r, T w) {
+        return (float)__builtin_bit_cast(_Float16, (uint16_t)r) * inv_scale + w;
+    }
+    __device__ static __forceinline__ T ld(const st* p, T w) { return (float)(*p) * inv_scale + w; }
+    // FP32 -> FP16: to nearest (hardware conversion), or stochastic in the normal range: the 13
+    // dropped mantissa bits plus 13 random bits, truncated (carries into the exponent are exact)
+    __device__ static __forceinline__ uint16_t enc(float x, uint32_t r, bool sr) {
+        const uint32_t u = __float_as_uint(x);
+        const uint32_t mag = u & 0x7fffffffu;
+        const uint32_t m = mag + (r >> 19);
+        if (sr && mag >= 0x38800000u && m < 0x47800000u)
+            return (uint16_t)(((u >> 16) & 0x8000u) | ((m >> 13) - (112u << 10)));
+        return __builtin_bit_cast(uint16_t, (_Float16)x);
+    }
+    __device__ static __forceinline__ float dec(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+    __device__ static __forceinline__ void sto(st* p, T f, T w) { *p = (_Float16)((f - w) * scale); }
+    __device__ static __forceinline__ void sto_r(st* p, T f, T w, uint32_t r, bool sr) {
+        *(uint16_t*)p = enc((f - w) * scale, r, sr);
+    }
+    __device__ static __forceinline__ T rt_r(T f, T w, uint32_t r, bool sr) { return dec(enc((f - w) * scale, r, sr)) * inv_scale + w; }
 };
 // bfloat16 round to nearest even, done on integers so that host and device agree bit for bit
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
@@ -93,11 +142,22 @@ struct StBF16S {
     using T = float;
     using raw = uint32_t;
     static constexpr int code = 3;
+    static constexpr bool dithered = true;
     __device__ static __forceinline__ raw ld_raw(const st* p) { return (uint32_t)*p; }
     __device__ static __forceinline__ T from_raw(raw r, T w) { return __uint_as_float(r << 16) + w; }
     __device__ static __forceinline__ T ld(const st* p, T w) { return __uint_as_float((uint32_t)(*p) << 16) + w; }
+    // FP32 -> BF16: to nearest even, or stochastic for finite values: the 16 dropped bits plus 16
+    // random bits, truncated (subnormals included: their encoding is linear too)
+    __device__ static __forceinline__ uint16_t enc(float x, uint32_t r, bool sr) {
+        const uint32_t u = __float_as_uint(x);
+        if (sr && (u & 0x7f800000u) != 0x7f800000u) return (uint16_t)((u + (r >> 16)) >> 16);
+        return f32_to_bf16_rne(x);
+    }
     __device__ static __forceinline__ void sto(st* p, T f, T w) { *p = f32_to_bf16_rne(f - w); }
-    __device__ static __forceinline__ T rt(T f, T w) { return __uint_as_float((uint32_t)f32_to_bf16_rne(f - w) << 16) + w; }
+    __device__ static __forceinline__ void sto_r(st* p, T f, T w, uint32_t r, bool sr) { *p = enc(f - w, r, sr); }
+    __device__ static __forceinline__ T rt_r(T f, T w, uint32_t r, bool sr) {
+        return __uint_as_float((uint32_t)enc(f - w, r, sr) << 16) + w;
+    }
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -132,7 +192,7 @@ struct CollParams {
     float omega_bulk, omega_3, omega_4, lambda;
     int32_t adaptive_omega4, limiter, compressible, bb_mode;
     float c_wale, nu_sgs_bg, inlet_turb;
-    int32_t pad;
+    int32_t dither;   // 16-bit storage forms: stochastic rounding of the step's stores
 };
 
 // WALE eddy viscosity of the gradient g_ij = d u_i / d x_j (kernel_cumulant.jl:500-535)

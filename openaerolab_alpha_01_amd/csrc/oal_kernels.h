@@ -32,6 +32,9 @@ struct DevDomain {
     float omega1, omega4d;          // collision rates without WALE, FP32 forms
     double omega1_d, omega4d_d;     // and FP64 forms
     int32_t nbx, nby;               // blocks per row / per layer of the domain's lexicographic grid
+    uint32_t cell0;                 // first cell of the domain: cells are numbered from it for the
+                                    // rounding hash, so a domain's results do not depend on its place
+    int32_t pad;
 };
 
 // arguments of the fused stream-collide kernels

@@ -200,6 +200,7 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, Leve
         for (uint32_t b = 0; b < nb; ++b) count[bdom[b]]++;
         for (int i = 0; i < d.n_domains; ++i) L.dom_block0_h[i + 1] = L.dom_block0_h[i] + count[i];
     }
+    for (int i = 0; i < d.n_domains; ++i) L.doms_h[i].cell0 = (uint32_t)L.dom_block0_h[i] * BS3;
     L.dom_item0_h.resize(d.n_domains + 1);
     for (int i = 0; i <= d.n_domains; ++i) L.dom_item0_h[i] = L.dom_block0_h[i] * PLANES;
 
@@ -526,6 +527,8 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
     if (!desc->levels) return fail(OAL_ERR_ARG, "no level descriptors");
     const OalCollision& c = desc->collision;
     if (c.limiter < 0 || c.limiter > 2) return fail(OAL_ERR_ARG, "unknown limiter");
+    if (desc->rounding != OAL_ROUND_STOCHASTIC && desc->rounding != OAL_ROUND_NEAREST)
+        return fail(OAL_ERR_ARG, "rounding must be OAL_ROUND_STOCHASTIC or OAL_ROUND_NEAREST");
 
     OalPlan* p = new (std::nothrow) OalPlan();
     if (!p) return fail(OAL_ERR_NOMEM, "host allocation failed");
@@ -545,6 +548,7 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
     p->cp.c_wale = c.c_wale;
     p->cp.nu_sgs_bg = c.nu_sgs_background;
     p->cp.inlet_turb = c.inlet_turbulence;
+    p->cp.dither = desc->rounding == OAL_ROUND_STOCHASTIC ? 1 : 0;
 
     auto bail = [&](int rc) {
         oal_plan_destroy(p);

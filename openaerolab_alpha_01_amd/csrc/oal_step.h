@@ -207,6 +207,19 @@ __device__ __forceinline__ T boundary_value(int kind, const StepArgs& a, const C
     }
 }
 
+// The population of the cell's own post-collision state that a boundary rule for direction K returns
+// (the rest weight, SRC_REST, does not depend on it)
+template <int K>
+__device__ __forceinline__ int rule_dir(int kind) {
+    switch (kind) {
+        case SRC_OPP:
+        case SRC_SOLID: return dopp(K);
+        case SRC_MIRY: return dmir_y(K);
+        case SRC_MIRZ: return dmir_z(K);
+        default: return K;
+    }
+}
+
 // workgroup sum of 3 doubles into part[item*4 + 0..2]; every lane of the workgroup must call it
 __device__ __forceinline__ void wg_sum3(double v0, double v1, double v2, double* part, uint32_t item) {
     __shared__ double red[3][WG / 64];
@@ -369,44 +382,52 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
     const bool want_force = a.fpart != nullptr;
     const bool link_force = GEN && want_force && (c.flags & BF_SOLID) && !solid_own;
     const bool outlet_cell = GEN && PAT == PAT_AA_EVEN && (c.flags & BF_EDGE) && !c.per0 && c.gx == c.ext0 - 1;
+    // rounding of the stored post-collision populations: population j of this cell takes the random
+    // bits of (cell, j, step) whichever slot it goes to, so every pattern stores the same values
+    const bool dith = SP::dithered && a.cp.dither;
+    const uint32_t hc = SP::dithered ? round_cell(round_key(a.time_step), c.own - dm->cell0) : 0u;
+    auto rb = [&](int j) { return round_bits(hc, j); };
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
         constexpr int J = dopp(K);   // the source of direction J is x + c_K, where f*_K goes
+        constexpr T wK = wgt<T>(K);
         if constexpr (REG && PAT == PAT_AB) {
-            SP::sto((st*)(ra.base(fo, n, K, c.z, 0) + ra.v[1][1]), f[K], wgt<T>(K));
+            SP::sto_r((st*)(ra.base(fo, n, K, c.z, 0) + ra.v[1][1]), f[K], wK, rb(K), dith);
         } else if constexpr (REG && PAT == PAT_AA_EVEN) {
-            SP::sto((st*)(ra.base(fo, n, J, c.z, 0) + ra.v[1][1]), f[K], wgt<T>(K));
+            SP::sto_r((st*)(ra.base(fo, n, J, c.z, 0) + ra.v[1][1]), f[K], wK, rb(K), dith);
             if constexpr (GEN && dcx(K) < 0) {
-                if (outlet_cell) SP::sto((st*)a.obuf + ((size_t)a.oslot[b] * 9 + K / 3) * BS2 + c.ly * BS + c.z, f[K], wgt<T>(K));
+                if (outlet_cell)
+                    SP::sto_r((st*)a.obuf + ((size_t)a.oslot[b] * 9 + K / 3) * BS2 + c.ly * BS + c.z, f[K], wK, rb(K), dith);
             }
         } else if constexpr (REG && !GEN) {
-            SP::sto((st*)(ra.base(fo, n, K, c.z, -dcz(K)) + ra.v[1 - dcx(K)][1 - dcy(K)]), f[K], wgt<T>(K));
+            SP::sto_r((st*)(ra.base(fo, n, K, c.z, -dcz(K)) + ra.v[1 - dcx(K)][1 - dcy(K)]), f[K], wK, rb(K), dith);
         } else if constexpr (REG) {
             uint32_t dst = 0;
             const int kind = classify<J, GEN, REG>(a, c, dst);
             if (kind == SRC_REG) {
-                SP::sto((st*)(ra.base(fo, n, K, c.z, -dcz(K)) + ra.v[1 - dcx(K)][1 - dcy(K)]), f[K], wgt<T>(K));
+                SP::sto_r((st*)(ra.base(fo, n, K, c.z, -dcz(K)) + ra.v[1 - dcx(K)][1 - dcy(K)]), f[K], wK, rb(K), dith);
             } else if (kind != SRC_INLET) {
                 const T v = boundary_value<J, T>(kind, a, c, u_in, a.time_step + 1,
                                                  [&](auto jc) { return f[decltype(jc)::value]; });
-                SP::sto(fo + (size_t)J * n + c.own, v, wgt<T>(J));
+                SP::sto_r(fo + (size_t)J * n + c.own, v, wgt<T>(J), rb(rule_dir<J>(kind)), dith);
             }
         } else if constexpr (PAT == PAT_AB) {
-            SP::sto(fo + (size_t)K * n + c.own, f[K], wgt<T>(K));
+            SP::sto_r(fo + (size_t)K * n + c.own, f[K], wK, rb(K), dith);
         } else if constexpr (PAT == PAT_AA_EVEN) {
-            SP::sto(fo + (size_t)J * n + c.own, f[K], wgt<T>(K));
+            SP::sto_r(fo + (size_t)J * n + c.own, f[K], wK, rb(K), dith);
             if constexpr (GEN && dcx(K) < 0) {
-                if (outlet_cell) SP::sto((st*)a.obuf + ((size_t)a.oslot[b] * 9 + K / 3) * BS2 + c.ly * BS + c.z, f[K], wgt<T>(K));
+                if (outlet_cell)
+                    SP::sto_r((st*)a.obuf + ((size_t)a.oslot[b] * 9 + K / 3) * BS2 + c.ly * BS + c.z, f[K], wK, rb(K), dith);
             }
         } else {
             uint32_t dst = 0;
             const int kind = classify<J, GEN>(a, c, dst);
             if (kind == SRC_REG) {
-                SP::sto(fo + (size_t)K * n + dst, f[K], wgt<T>(K));
+                SP::sto_r(fo + (size_t)K * n + dst, f[K], wK, rb(K), dith);
             } else if (kind != SRC_INLET) {   // inlet links are evaluated by the even step itself
                 const T v = boundary_value<J, T>(kind, a, c, u_in, a.time_step + 1,
                                                  [&](auto jc) { return f[decltype(jc)::value]; });
-                SP::sto(fo + (size_t)J * n + c.own, v, wgt<T>(J));
+                SP::sto_r(fo + (size_t)J * n + c.own, v, wgt<T>(J), rb(rule_dir<J>(kind)), dith);
             }
         }
         if constexpr (GEN) {
@@ -415,10 +436,10 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
                 const int kind = classify<J, GEN, REG>(a, c, dst);
                 double m = 0.0;
                 if (kind == SRC_SOLID) {
-                    m = 2.0 * ((double)SP::rt(f[K], wgt<T>(K)) - (double)wgt<T>(K));
+                    m = 2.0 * ((double)SP::rt_r(f[K], wK, rb(K), dith) - (double)wK);
                 } else if (PAT == PAT_AB && a.cp.bb_mode == 0 && kind == SRC_REG && ((c.lmask >> J) & 1u)) {
-                    m = ((double)SP::rt(f[K], wgt<T>(K)) - (double)wgt<T>(K)) +
-                        ((double)own_prev<SP, PAT, K>(a, c) - (double)wgt<T>(K));
+                    m = ((double)SP::rt_r(f[K], wK, rb(K), dith) - (double)wK) +
+                        ((double)own_prev<SP, PAT, K>(a, c) - (double)wK);
                 }
                 F0 += m * dcx(K); F1 += m * dcy(K); F2 += m * dcz(K);
             }
@@ -512,7 +533,8 @@ struct InletRule {
 
 template <class SP>
 __device__ __forceinline__ void reg_collide_store(const StepArgs& a, const RegAddr<SP>& ra, const CellRates<typename SP::T>& q,
-                                                  int z, const typename SP::raw (&r)[27], const InletRule* inlet = nullptr) {
+                                                  int z, uint32_t cell, const typename SP::raw (&r)[27],
+                                                  const InletRule* inlet = nullptr) {
     using T = typename SP::T;
     using st = typename SP::st;
     T f[27];
@@ -532,9 +554,11 @@ __device__ __forceinline__ void reg_collide_store(const StepArgs& a, const RegAd
     }
     T rho_o, ux_o, uy_o, uz_o;
     collide_cell<T>(f, q, a.cp, rho_o, ux_o, uy_o, uz_o);
+    const bool dith = SP::dithered && a.cp.dither;
+    const uint32_t hc = SP::dithered ? round_cell(round_key(a.time_step), cell) : 0u;
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
-        SP::sto((st*)(ra.base(a.fout, a.ncell, K, z, 0) + ra.v[1][1]), f[K], wgt<T>(K));
+        SP::sto_r((st*)(ra.base(a.fout, a.ncell, K, z, 0) + ra.v[1][1]), f[K], wgt<T>(K), round_bits(hc, K), dith);
     });
 }
 
@@ -667,13 +691,15 @@ k_step_pipe(const StepArgs a) {
         if constexpr (FACE == FACE_RULES) reg_load_raw_face<SP>(a, r, fr, z, dst);
         else if (live(z)) reg_load_raw<SP>(a, r, z, dst);
     };
+    const uint32_t cell0 = dm->cell0;   // read before the first store, as every domain parameter
     auto collide = [&](const RegAddr<SP>& r, int z, const typename SP::raw (&src)[27]) {
+        const uint32_t cell = c.own + (uint32_t)(z - z0) * BS2 - cell0;
         if constexpr (FACE == FACE_RULES) {
             InletRule iz = in;
             iz.gz += z;
-            reg_collide_store<SP>(a, r, q, z, src, &iz);
+            reg_collide_store<SP>(a, r, q, z, cell, src, &iz);
         } else if (live(z)) {
-            reg_collide_store<SP>(a, r, q, z, src);
+            reg_collide_store<SP>(a, r, q, z, cell, src);
         }
     };
     // the per-lane offsets stay those of plane z0; later planes shift the uniform base

@@ -100,7 +100,8 @@ class Plan:
     """A solver run on the GPU: the device state of every level and the step entry points."""
 
     def __init__(self, levels: list[Level] | Level, storage: str = "f32", stream: str = "ab",
-                 bb_mode: str = "halfway", params: StepParams | None = None, device: int = 0):
+                 bb_mode: str = "halfway", params: StepParams | None = None, device: int = 0,
+                 rounding: str = "stochastic"):
         self.lib = load()
         if isinstance(levels, Level):
             levels = [levels]
@@ -144,6 +145,7 @@ class Plan:
         desc.levels = C.cast(descs, C.POINTER(_lib.OalLevelDesc))
         desc.collision = self.params.to_c()
         desc.device = device
+        desc.rounding = {"stochastic": _lib.OAL_ROUND_STOCHASTIC, "nearest": _lib.OAL_ROUND_NEAREST}[rounding]
         self._keep.append(descs)
         h = C.c_void_p()
         check(self.lib.oal_plan_create(C.byref(desc), C.byref(h)), "oal_plan_create")
@@ -302,8 +304,8 @@ def execute_timestep_batch(plan: Plan, t_start: int, batch: int, collect_force_l
 
 def uniform_plan(grid: BlockGrid, tau: float, storage: str = "f32", stream: str = "ab", bb_mode: str = "halfway",
                  params: StepParams | None = None, u_inlet: float = 0.0, body_force=(0.0, 0.0, 0.0),
-                 ymin_wall_bc: int = 0, zwall_bc: int = 0) -> Plan:
+                 ymin_wall_bc: int = 0, zwall_bc: int = 0, rounding: str = "stochastic") -> Plan:
     """A single-level plan over one uniform domain covering the grid's extent."""
     dom = Domain(extent=grid.n, tau=tau, periodic=grid.periodic, u_inlet=u_inlet, body_force=tuple(body_force),
                  ymin_wall_bc=ymin_wall_bc, zwall_bc=zwall_bc)
-    return Plan(Level(grid, [dom]), storage=storage, stream=stream, bb_mode=bb_mode, params=params)
+    return Plan(Level(grid, [dom]), storage=storage, stream=stream, bb_mode=bb_mode, params=params, rounding=rounding)

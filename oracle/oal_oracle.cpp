@@ -140,26 +140,64 @@ inline float bf16_to_f32(uint16_t h) {
     return f;
 }
 
-// Storage policies: stored value <-> working value.  The 16-bit forms hold f - w_k.
+// ---------------------------------------------------------------------------------------------
+// Stochastic rounding of the 16-bit storage forms (the library's storage rounding; the reference
+// stores FP32 and has none).  A step stores population j of cell c with 32 random bits from a
+// counter hash of (c, j, time step); the stored code keeps the high bits of the value plus random
+// bits below them, i.e. it rounds up with the probability of the dropped fraction.
+// ---------------------------------------------------------------------------------------------
+inline uint32_t fmix32(uint32_t h) {   // murmur3 finaliser
+    h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+    return h;
+}
+inline uint32_t round_bits(int64_t cell, int k, int32_t time_step) {
+    const uint32_t key = fmix32((uint32_t)time_step * 0x9e3779b9u + 0x632be5abu);
+    const uint32_t hc = fmix32((uint32_t)cell ^ key);
+    return (hc ^ ((uint32_t)k * 0x9e3779b9u)) * 0x85ebca6bu;
+}
+// FP32 -> FP16 with the 13 dropped mantissa bits plus 13 random bits, truncated, for FP16 normal
+// results; nearest even elsewhere
+inline uint16_t f32_to_f16_sr(float f, uint32_t r) {
+    uint32_t x; std::memcpy(&x, &f, 4);
+    const uint32_t mag = x & 0x7fffffffu;
+    const uint32_t m = mag + (r >> 19);
+    if (mag >= 0x38800000u && m < 0x47800000u) return (uint16_t)(((x >> 16) & 0x8000u) | ((m >> 13) - (112u << 10)));
+    return f32_to_f16(f);
+}
+// FP32 -> BF16 with the 16 dropped bits plus 16 random bits, truncated, for finite values
+inline uint16_t f32_to_bf16_sr(float f, uint32_t r) {
+    uint32_t u; std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return f32_to_bf16(f);
+    return (uint16_t)((u + (r >> 16)) >> 16);
+}
+
+// Storage policies: stored value <-> working value.  The 16-bit forms hold f - w_k, FP16 scaled by
+// 2^14 (exact: it only moves the exponent range).  sto() rounds to nearest; sto_r() applies the
+// step's rounding (stochastic with the bits r when sr is set).
 struct StF32 {
     using st = float; using T = float;
     static T ld(st v, T) { return v; }
     static st sto(T f, T) { return f; }
+    static st sto_r(T f, T, uint32_t, bool) { return f; }
 };
 struct StF64 {
     using st = double; using T = double;
     static T ld(st v, T) { return v; }
     static st sto(T f, T) { return f; }
+    static st sto_r(T f, T, uint32_t, bool) { return f; }
 };
 struct StF16S {
     using st = uint16_t; using T = float;
-    static T ld(st v, T w) { return f16_to_f32(v) + w; }
-    static st sto(T f, T w) { return f32_to_f16(f - w); }
+    static constexpr float scale = 16384.0f;
+    static T ld(st v, T w) { return f16_to_f32(v) * (1.0f / scale) + w; }
+    static st sto(T f, T w) { return f32_to_f16((f - w) * scale); }
+    static st sto_r(T f, T w, uint32_t r, bool sr) { return sr ? f32_to_f16_sr((f - w) * scale, r) : sto(f, w); }
 };
 struct StBF16S {
     using st = uint16_t; using T = float;
     static T ld(st v, T w) { return bf16_to_f32(v) + w; }
     static st sto(T f, T w) { return f32_to_bf16(f - w); }
+    static st sto_r(T f, T w, uint32_t r, bool sr) { return sr ? f32_to_bf16_sr(f - w, r) : sto(f, w); }
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -246,6 +284,7 @@ struct OrParams {
     int32_t bb_mode;          // 0: reference solid reflection, 1: halfway bounce-back
     int32_t time_step_seed;
     int32_t arith;            // 0: literal reference arithmetic, 1: the library's fused arithmetic
+    int32_t rounding;         // 16-bit storage: 0 stochastic (the library's default), 1 to nearest
 };
 
 struct OrGrid {
@@ -729,6 +768,7 @@ void step_impl(const OrGrid& g, const OrParams& prm, const typename P::st* fin, 
     T w[27];
     for (int k = 0; k < 27; ++k) w[k] = weight<T>(k);
     std::vector<double> fpart((size_t)g.nb * 3, 0.0);
+    const bool sr = prm.rounding == 0;   // stochastic rounding of the stored populations (16-bit forms)
 
 #pragma omp parallel for schedule(static)
     for (int b = 0; b < g.nb; ++b) {
@@ -793,7 +833,9 @@ void step_impl(const OrGrid& g, const OrParams& prm, const typename P::st* fin, 
                         // reference solid cell: full-way reflection (kernel_cumulant.jl:405-417)
                         if (vel_out) { vel_out[own] = T(0); vel_out[ncell + own] = T(0); vel_out[2 * ncell + own] = T(0); }
                         if (rho_out) rho_out[own] = T(1);
-                        for (int k = 0; k < 27; ++k) fout[(int64_t)k * ncell + own] = P::sto(fs[LAT.opp[k]], w[k]);
+                        for (int k = 0; k < 27; ++k)
+                            fout[(int64_t)k * ncell + own] =
+                                P::sto_r(fs[LAT.opp[k]], w[k], round_bits(own, k, prm.time_step_seed), sr);
                         continue;
                     }
 
@@ -829,7 +871,8 @@ void step_impl(const OrGrid& g, const OrParams& prm, const typename P::st* fin, 
                     }
                     if (vel_out) { vel_out[own] = u_o[0]; vel_out[ncell + own] = u_o[1]; vel_out[2 * ncell + own] = u_o[2]; }
                     if (rho_out) rho_out[own] = rho_o;
-                    for (int k = 0; k < 27; ++k) fout[(int64_t)k * ncell + own] = P::sto(out[k], w[k]);
+                    for (int k = 0; k < 27; ++k)
+                        fout[(int64_t)k * ncell + own] = P::sto_r(out[k], w[k], round_bits(own, k, prm.time_step_seed), sr);
                     // momentum exchange of the links this collision feeds: f_i*(x) leaves toward a solid
                     // cell x + c_i and returns as f_opp(i)(x, t+1); F += (f_i* + f_opp(i)) c_i
                     // (src/octree_q19/src/stream.jl:248-256), shifted by w_i (the w terms cancel over
@@ -841,7 +884,9 @@ void step_impl(const OrGrid& g, const OrParams& prm, const typename P::st* fin, 
                             if (geo.outside(0, dgx) || geo.outside(1, dgy) || geo.outside(2, dgz)) continue;
                             const int64_t dst = geo.locate(b, x + LAT.cx[i], y + LAT.cy[i], z + LAT.cz[i]);
                             if (dst < 0 || !g.solid[dst]) continue;
-                            const double fi = (double)P::ld(P::sto(out[i], w[i]), w[i]) - (double)w[i];
+                            const double fi =
+                                (double)P::ld(P::sto_r(out[i], w[i], round_bits(own, i, prm.time_step_seed), sr), w[i]) -
+                                (double)w[i];
                             const double m = prm.bb_mode == 1 ? 2.0 * fi : fi + ((double)own_f(i) - (double)w[i]);
                             fb[0] += m * LAT.cx[i]; fb[1] += m * LAT.cy[i]; fb[2] += m * LAT.cz[i];
                         }
@@ -931,6 +976,16 @@ void oracle_f32_to_f16(const float* in, uint16_t* out, int64_t n) { for (int64_t
 void oracle_f16_to_f32(const uint16_t* in, float* out, int64_t n) { for (int64_t i = 0; i < n; ++i) out[i] = f16_to_f32(in[i]); }
 void oracle_f32_to_bf16(const float* in, uint16_t* out, int64_t n) { for (int64_t i = 0; i < n; ++i) out[i] = f32_to_bf16(in[i]); }
 void oracle_bf16_to_f32(const uint16_t* in, float* out, int64_t n) { for (int64_t i = 0; i < n; ++i) out[i] = bf16_to_f32(in[i]); }
+// stochastic-rounding conversions and the random bits of the step's stores (test access)
+void oracle_f32_to_f16_sr(const float* in, const uint32_t* r, uint16_t* out, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) out[i] = f32_to_f16_sr(in[i], r[i]);
+}
+void oracle_f32_to_bf16_sr(const float* in, const uint32_t* r, uint16_t* out, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) out[i] = f32_to_bf16_sr(in[i], r[i]);
+}
+void oracle_round_bits(const int64_t* cell, const int32_t* k, int32_t time_step, uint32_t* out, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) out[i] = round_bits(cell[i], k[i], time_step);
+}
 
 float oracle_gradient_noise(int32_t gx, int32_t gy, int32_t gz, int32_t seed) { return gradient_noise(gx, gy, gz, seed); }
 uint32_t oracle_gpu_hash(int32_t x) { return gpu_hash(x); }

@@ -40,6 +40,8 @@ LIB = HERE / "liboal_oracle.so"
 
 STORAGE = {"f32": 0, "f64": 1, "f16s": 2, "bf16s": 3}
 LITERAL, FAST = 0, 1
+STOCHASTIC, NEAREST = 0, 1
+F16_SCALE = np.float32(16384.0)   # FP16 storage holds (f - w) * 2^14
 STORE_DTYPE = {"f32": np.float32, "f64": np.float64, "f16s": np.uint16, "bf16s": np.uint16}
 WORK_DTYPE = {"f32": np.float32, "f64": np.float64, "f16s": np.float32, "bf16s": np.float32}
 
@@ -51,7 +53,7 @@ class OrParams(C.Structure):
         ("u_inlet", C.c_double), ("inlet_turb", C.c_double), ("body_force", C.c_double * 3),
         ("adaptive_omega4", C.c_int32), ("limiter", C.c_int32), ("compressible", C.c_int32),
         ("ymin_wall_bc", C.c_int32), ("zwall_bc", C.c_int32), ("bb_mode", C.c_int32), ("time_step_seed", C.c_int32),
-        ("arith", C.c_int32),
+        ("arith", C.c_int32), ("rounding", C.c_int32),
     ]
 
 
@@ -95,6 +97,9 @@ def load() -> C.CDLL:
     lib.oracle_wale_nu_f64.restype = C.c_double
     for nm in ("oracle_f32_to_f16", "oracle_f16_to_f32", "oracle_f32_to_bf16", "oracle_bf16_to_f32"):
         getattr(lib, nm).argtypes = [vp, vp, C.c_int64]
+    for nm in ("oracle_f32_to_f16_sr", "oracle_f32_to_bf16_sr"):
+        getattr(lib, nm).argtypes = [vp, vp, vp, C.c_int64]
+    lib.oracle_round_bits.argtypes = [vp, vp, C.c_int32, vp, C.c_int64]
     _lib = lib
     return lib
 
@@ -107,10 +112,11 @@ def f32(v: float) -> float:
 def make_params(tau, *, omega_bulk=1.0, omega_3=1.0, omega_4=1.0, lambda_=1.0 / 6.0, adaptive_omega4=1, limiter=1,
                 compressible=0, c_wale=0.0, nu_sgs_background=0.0, amr_sgs_scale=1.0, u_inlet=0.0,
                 inlet_turbulence=0.0, body_force=(0.0, 0.0, 0.0), ymin_wall_bc=0, zwall_bc=0, bb_mode=1,
-                time_step=0, arith=LITERAL) -> OrParams:
+                time_step=0, arith=LITERAL, rounding=STOCHASTIC) -> OrParams:
     """Step parameters; arith LITERAL restates the reference arithmetic operation for operation, FAST the
     library's fused arithmetic (same mathematics, fused multiply-adds, rho and j from the transform's
-    partial sums)."""
+    partial sums).  rounding: how the 16-bit forms store a step's populations (STOCHASTIC, the
+    library's default, or NEAREST)."""
     p = OrParams()
     p.tau, p.omega_bulk, p.omega3, p.omega4 = f32(tau), f32(omega_bulk), f32(omega_3), f32(omega_4)
     p.lambda_, p.c_wale, p.nu_sgs_bg, p.amr_sgs_scale = f32(lambda_), f32(c_wale), f32(nu_sgs_background), f32(amr_sgs_scale)
@@ -119,6 +125,7 @@ def make_params(tau, *, omega_bulk=1.0, omega_3=1.0, omega_4=1.0, lambda_=1.0 / 
     p.adaptive_omega4, p.limiter, p.compressible = adaptive_omega4, limiter, compressible
     p.ymin_wall_bc, p.zwall_bc, p.bb_mode, p.time_step_seed = ymin_wall_bc, zwall_bc, bb_mode, time_step
     p.arith = arith
+    p.rounding = rounding
     return p
 
 
@@ -192,6 +199,8 @@ def to_unshifted(storage: str, f: np.ndarray) -> np.ndarray:
     out = np.empty(flat.size, dtype=np.float32)
     fn = lib.oracle_f16_to_f32 if storage == "f16s" else lib.oracle_bf16_to_f32
     fn(flat.ctypes.data, out.ctypes.data, flat.size)
+    if storage == "f16s":
+        out *= np.float32(1.0) / F16_SCALE
     w = lattice()[6]
     return (out.reshape(f.shape) + w.astype(np.float32)[:, None]).astype(np.float64)
 
@@ -205,9 +214,38 @@ def from_unshifted(storage: str, f: np.ndarray) -> np.ndarray:
     lib = load()
     w = lattice()[6]
     s = np.ascontiguousarray((f.astype(np.float32) - w.astype(np.float32)[:, None]).astype(np.float32))
+    if storage == "f16s":
+        s *= F16_SCALE
     out = np.empty(s.shape, dtype=np.uint16)
     fn = lib.oracle_f32_to_f16 if storage == "f16s" else lib.oracle_f32_to_bf16
     fn(s.ctypes.data, out.ctypes.data, s.size)
+    return out
+
+
+def f32_to_f16_sr(x: np.ndarray, r: np.ndarray) -> np.ndarray:
+    """FP32 -> FP16 codes, stochastic rounding with the 32-bit random words r."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    r = np.ascontiguousarray(r, dtype=np.uint32)
+    out = np.empty(x.shape, dtype=np.uint16)
+    load().oracle_f32_to_f16_sr(x.ctypes.data, r.ctypes.data, out.ctypes.data, x.size)
+    return out
+
+
+def f32_to_bf16_sr(x: np.ndarray, r: np.ndarray) -> np.ndarray:
+    """FP32 -> BF16 codes, stochastic rounding with the 32-bit random words r."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    r = np.ascontiguousarray(r, dtype=np.uint32)
+    out = np.empty(x.shape, dtype=np.uint16)
+    load().oracle_f32_to_bf16_sr(x.ctypes.data, r.ctypes.data, out.ctypes.data, x.size)
+    return out
+
+
+def round_bits(cell: np.ndarray, k: np.ndarray, time_step: int) -> np.ndarray:
+    """The random words of the stores of population k of cell `cell` at a time step."""
+    cell = np.ascontiguousarray(cell, dtype=np.int64)
+    k = np.ascontiguousarray(k, dtype=np.int32)
+    out = np.empty(cell.shape, dtype=np.uint32)
+    load().oracle_round_bits(cell.ctypes.data, k.ctypes.data, int(time_step), out.ctypes.data, cell.size)
     return out
 
 

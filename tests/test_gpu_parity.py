@@ -63,7 +63,9 @@ def gpu_plan(grid, storage, okw, stream="ab"):
                         body_force=tuple(okw.get("body_force", (0.0, 0.0, 0.0))),
                         ymin_wall_bc=okw.get("ymin_wall_bc", 0), zwall_bc=okw.get("zwall_bc", 0))
     bb = {1: "halfway", 0: "reference"}[okw.get("bb_mode", 1)]
-    return solver.Plan(solver.Level(grid, [dom]), storage=storage, stream=stream, bb_mode=bb, params=params)
+    rounding = {0: "stochastic", 1: "nearest"}[okw.get("rounding", 0)]
+    return solver.Plan(solver.Level(grid, [dom]), storage=storage, stream=stream, bb_mode=bb, params=params,
+                       rounding=rounding)
 
 
 def gpu_run(grid, storage, steps, okw, rho, u, stream="ab", force_steps=(), tuning=None):
@@ -133,12 +135,13 @@ VARIANTS = {
     "walls": dict(ymin_wall_bc=1, zwall_bc=2),
     "wale": dict(c_wale=0.2, nu_sgs_background=1e-4),
     "body_force": dict(body_force=(1e-5, -2e-6, 3e-6)),
+    "nearest_rounding": dict(rounding=1),
 }
 
 
 @pytest.mark.parametrize("face_kernel", [0, 1])
 @pytest.mark.parametrize("variant", list(VARIANTS))
-@pytest.mark.parametrize("storage", ["f32", "f16s"])
+@pytest.mark.parametrize("storage", ["f32", "f16s", "bf16s"])
 def test_variant_parity(oracle, variant, storage, face_kernel):
     # 4 x 2 x 2 blocks: the sphere's blocks in the middle, the inlet and outlet blocks on the regular
     # path with its in-kernel face rules
@@ -156,7 +159,7 @@ def test_variant_parity(oracle, variant, storage, face_kernel):
     plan.close()
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16s", "f64"])
+@pytest.mark.parametrize("storage", ["f32", "f16s", "bf16s", "f64"])
 def test_aa_pattern_equals_ab(storage):
     """The in-place AA pattern reproduces the pull/AB step bit for bit (after every even step),
     boundary rules, halfway bounce-back and momentum exchange included."""

@@ -124,3 +124,60 @@ def test_inlet_hash_properties(oracle):
     vals = np.array([lib.oracle_gradient_noise(i, 2 * i + 1, 17, 1234) for i in range(5000)])
     assert vals.min() >= -1.0 and vals.max() < 1.0
     assert abs(vals.mean()) < 0.05               # uniform on [-1, 1)
+
+
+def _decode(oracle, storage, codes):
+    lib = oracle.load()
+    codes = np.ascontiguousarray(codes, dtype=np.uint16)
+    out = np.empty(codes.shape, dtype=np.float32)
+    fn = lib.oracle_f16_to_f32 if storage == "f16" else lib.oracle_bf16_to_f32
+    fn(codes.ctypes.data, out.ctypes.data, codes.size)
+    return out.astype(np.float64)
+
+
+def test_f16_stochastic_rounding_is_exact_in_mean(oracle):
+    """Over all 2^13 values of the random bits, the stochastically rounded FP16 code averages to the
+    FP32 input exactly, and every result is one of the two neighbouring FP16 values."""
+    rng = np.random.default_rng(11)
+    sign = np.where(rng.random(40) < 0.5, -1.0, 1.0)
+    x = (sign * rng.uniform(1, 2, 40) * 2.0 ** rng.integers(-14, 15, 40)).astype(np.float32)   # FP16 normal range
+    x = np.concatenate([x, np.float32([2.0 ** -14, -(2.0 ** -14), 1.0, 65000.0])])
+    d = np.arange(8192, dtype=np.uint32) << 19
+    # below the normal range the conversion rounds to nearest, whatever the random bits
+    tiny = np.float32([3e-7, -1.7630312e-07, 5e-8, 0.0])
+    for v in tiny:
+        assert set(oracle.f32_to_f16_sr(np.full(64, v, np.float32), d[::128]).tolist()) == {int(np.float16(v).view(np.uint16))}
+    for v in x:
+        codes = oracle.f32_to_f16_sr(np.full(8192, v, dtype=np.float32), d)
+        vals = _decode(oracle, "f16", codes)
+        assert vals.mean() == float(v)
+        # only the two FP16 neighbours of v occur, less than one FP16 spacing (2^-10 relative) apart
+        assert len(set(vals.tolist())) <= 2 and np.abs(vals - float(v)).max() <= abs(float(v)) * 2.0 ** -10
+
+
+def test_bf16_stochastic_rounding_is_exact_in_mean(oracle):
+    rng = np.random.default_rng(12)
+    x = (rng.uniform(-1, 1, 40) * 2.0 ** rng.integers(-30, 30, 40)).astype(np.float32)
+    d = np.arange(65536, dtype=np.uint32) << 16
+    for v in x:
+        codes = oracle.f32_to_bf16_sr(np.full(65536, v, dtype=np.float32), d)
+        vals = _decode(oracle, "bf16", codes)
+        assert vals.mean() == float(v)
+        assert len(set(vals.tolist())) <= 2
+
+
+def test_stochastic_rounding_bits_are_uniform(oracle):
+    """The random words' top 13 bits (the FP16 dither) are uniform over cells, over directions and over
+    time steps: chi-square within 5 sigma of its mean."""
+    n = 1 << 20
+    cell = np.arange(n, dtype=np.int64)
+    for k, t in ((0, 0), (13, 1), (26, 12345)):
+        r = oracle.round_bits(cell, np.full(n, k, dtype=np.int32), t) >> 19
+        h = np.bincount(r.astype(np.int64), minlength=8192)
+        chi2 = ((h - n / 8192) ** 2 / (n / 8192)).sum()
+        assert abs(chi2 - 8191) < 5 * np.sqrt(2 * 8191), chi2
+    # different directions of one cell, and one direction at consecutive steps, are uncorrelated
+    a = (oracle.round_bits(cell, np.zeros(n, np.int32), 7) >> 19).astype(np.float64)
+    b = (oracle.round_bits(cell, np.ones(n, np.int32), 7) >> 19).astype(np.float64)
+    c = (oracle.round_bits(cell, np.zeros(n, np.int32), 8) >> 19).astype(np.float64)
+    assert abs(np.corrcoef(a, b)[0, 1]) < 5e-3 and abs(np.corrcoef(a, c)[0, 1]) < 5e-3
