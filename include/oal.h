@@ -115,6 +115,13 @@ typedef struct {
     const uint8_t* obstacle;         /* [bs^3 * n_blocks] reference layout, nonzero = solid; NULL = none */
 } OalLevelDesc;
 
+/* A refined plan (n_levels > 1, levels coarsest first) is a block octree: level l + 1 has twice the
+ * extent of level l, and a level-l block with children is refined into all 8 of them.  Requirements:
+ * one domain per level (its tau in that level's lattice units; the acoustic rule is
+ * tau_{l+1} - 1/2 = 2 (tau_l - 1/2), src/octree_q19/src/config.jl:328-335), extents multiples of 16,
+ * the AB pattern, no WALE, and a margin of one coarser block around every refined region.  The
+ * library derives the neighbour tables of refined plans from the block coordinates, and adds a layer of
+ * ghost blocks around every finer level. */
 typedef struct {
     int32_t abi_version;       /* OAL_ABI_VERSION */
     int32_t storage;           /* OAL_F32 / OAL_F64 / OAL_F16S / OAL_BF16S */
@@ -141,7 +148,8 @@ const char* oal_last_error(void);
 int oal_plan_create(const OalPlanDesc* desc, OalPlan** plan);
 int oal_plan_destroy(OalPlan* plan);
 
-/* Number of cells (n_blocks * bs^3) of a level: the length of one direction of f. */
+/* Number of cells (n_blocks * bs^3, the caller's blocks) of a level: the length of one direction of
+ * the host arrays of f, rho and u. */
 int oal_level_cells(const OalPlan* plan, int level, int64_t* n_cells);
 
 /* Replaces: nothing (introspection).  How a level's blocks split over the step kernels: blocks on
@@ -160,7 +168,10 @@ typedef struct OalLevelInfo {
     int32_t n_face_cells;    /* AB pattern: cells of regular blocks on a domain face; the bulk kernel
                                 applies their rules itself or leaves them to the face-cell kernel
                                 (OalTuning.face_kernel) */
-    int32_t reserved[2];
+    int32_t n_ghost_blocks;  /* refined plans: blocks the library adds around the level (filled from
+                                the coarser level, not stepped; not part of n_blocks or n_cells) */
+    int32_t n_covered_blocks;/* refined plans: blocks of n_blocks whose children exist (written by
+                                restriction, not stepped) */
 } OalLevelInfo;
 int oal_level_info(const OalPlan* plan, int level, OalLevelInfo* info);
 
@@ -188,8 +199,14 @@ enum {
 int oal_level_step(OalPlan* plan, int level, int32_t time_step, int flags, void* stream);
 
 /* n coarse steps: execute_timestep_batch! (src/solver/solver_control.jl:796-824) over
- * recursive_step! (:55-422).  Plans hold one level in this version (oal_plan_create rejects
- * n_levels != 1), so a coarse step is one step of level 0 with time_step first_time_step + i. */
+ * recursive_step! (:55-422).  One level: n steps of level 0 with time_step first_time_step + i.
+ * Refined plans (n_levels > 1): each coarse step steps level 0 once and, recursively, every finer
+ * level twice per step of its parent (acoustic sub-cycling), with the octree's transfers
+ * (src/octree_q19/src/coupling.jl:134-268): restriction into the covered cells after each fine
+ * sub-step, ghost fill at the half and the full coarse time.  Level l's sub-step k of coarse step t
+ * uses time_step 2^l t + k.  OAL_STEP_FORCE collects the force on the last sub-step of the last coarse
+ * step of every level holding obstacle cells.  The first call after populations were set runs the
+ * initial restriction and fill. */
 int oal_step(OalPlan* plan, int n_steps, int32_t first_time_step, int flags, void* stream);
 
 /* The momentum-exchange force of the last step collected with OAL_STEP_FORCE, FP64, per domain of

@@ -19,7 +19,9 @@ enum : uint32_t {
     BF_EDGE = 1u,      // a source of some cell may lie outside the domain extent (domain-face rules)
     BF_SOLID = 2u,     // the block or its one-cell halo holds a solid cell
     BF_MISSING = 4u,   // some neighbour block is absent inside the extent (rest-weight fallback)
-    BF_PARTIAL = 8u    // some cells of the block lie beyond the extent
+    BF_PARTIAL = 8u,   // some cells of the block lie beyond the extent
+    BF_INACTIVE = 16u  // refined level: a covered block (written by restriction) or a ghost block (written by
+                       // the fill from the coarser level); never stepped
 };
 
 // one domain of a level on the device
@@ -98,20 +100,34 @@ hipError_t launch_step_f16s(int stream_mode, bool wale, const StepArgs& a, hipSt
 hipError_t launch_step_bf16s(int stream_mode, bool wale, const StepArgs& a, hipStream_t stream, hipStream_t side,
                              hipEvent_t bulk0, hipEvent_t bulk1);
 hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t stream);
-hipError_t launch_init_eq(int storage, void* f, uint32_t ncell, const double* rho, const double* u, double rho0,
-                          double ux0, double uy0, double uz0, int permute, void* vel, hipStream_t stream);
-hipError_t launch_convert_in(int storage, void* f, const void* src, int src_dtype, uint32_t ncell, int permute,
+// Host arrays of init / conversion / macro output cover the first nhost cells of a level (the caller's
+// blocks); the device arrays have ncell cells per direction.
+hipError_t launch_init_eq(int storage, void* f, uint32_t ncell, uint32_t nhost, const double* rho, const double* u,
+                          double rho0, double ux0, double uy0, double uz0, int permute, void* vel, hipStream_t stream);
+hipError_t launch_convert_in(int storage, void* f, const void* src, int src_dtype, uint32_t ncell, uint32_t nhost,
+                             int permute, hipStream_t stream);
+hipError_t launch_convert_out(int storage, const void* f, void* dst, int dst_dtype, uint32_t ncell, uint32_t nhost,
+                              int permute, hipStream_t stream);
+hipError_t launch_macro(int storage, const void* f, double* rho, double* u, uint32_t ncell, uint32_t nhost, int permute,
+                        void* vel, hipStream_t stream);
+// Level transfers of a refined plan.  gather: the working-precision populations of each listed cell
+// (bit 31 set = not blended in time) of fnew, or of (1 - w1) fold + w1 fnew when fold is given:
+// out[27 * i + k].  rebuild: for each target cell, the weighted sum of the gathered populations over its
+// 64 stencil nodes (slot[n * ntarget + i], -1 = unused; weight w[...]), its second-order
+// non-equilibrium moments set to scale * keep times their value, written to f.  copy_cells: the listed
+// cells' populations from src to dst.
+hipError_t launch_xfer_gather(int storage, const void* fnew, const void* fold, const uint32_t* cells, uint32_t n,
+                               uint32_t ncell, double w1, void* out, hipStream_t stream);
+hipError_t launch_xfer_rebuild(int storage, void* f, uint32_t ncell, const uint32_t* tcells, uint32_t n, const int32_t* slot,
+                               const float* w, const void* m, double scale, double keep, hipStream_t stream);
+hipError_t launch_copy_cells(int storage, void* dst, const void* src, const uint32_t* cells, uint32_t n, uint32_t ncell,
                              hipStream_t stream);
-hipError_t launch_convert_out(int storage, const void* f, void* dst, int dst_dtype, uint32_t ncell, int permute,
-                              hipStream_t stream);
-hipError_t launch_macro(int storage, const void* f, double* rho, double* u, uint32_t ncell, int permute, void* vel,
-                        hipStream_t stream);
 hipError_t launch_aa_outlet_prep(int storage, const StepArgs& a, hipStream_t stream);
 hipError_t launch_mass(int storage, const void* f, const uint8_t* solid, const int32_t* borg, const int32_t* bdom,
-                       const DevDomain* doms, uint32_t nb, int permute, double* part, hipStream_t stream);
+                       const uint32_t* bflags, const DevDomain* doms, uint32_t nb, int permute, double* part, hipStream_t stream);
 hipError_t launch_reduce(const ReduceArgs& r, hipStream_t stream);
 hipError_t launch_guard(int storage, const void* f, const uint8_t* solid, const int32_t* borg, const int32_t* bdom,
-                        const DevDomain* doms, uint32_t nb, float rho_min, float u_max,
+                        const uint32_t* bflags, const DevDomain* doms, uint32_t nb, float rho_min, float u_max,
                         unsigned long long* count, hipStream_t stream);
 
 }  // namespace oal

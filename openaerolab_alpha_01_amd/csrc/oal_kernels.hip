@@ -37,16 +37,18 @@ hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t s
 // Initialisation: init_eq! (src/main.jl:546-572), per-cell or uniform (rho, u)
 // ------------------------------------------------------------------------------------------------
 template <class SP>
-__global__ void __launch_bounds__(WG) k_init_eq(typename SP::st* f, uint32_t ncell, const double* rho, const double* u,
-                                                double rho0, double ux0, double uy0, double uz0, int permute,
-                                                typename SP::T* vel) {
+__global__ void __launch_bounds__(WG) k_init_eq(typename SP::st* f, uint32_t ncell, uint32_t nhost, const double* rho,
+                                                const double* u, double rho0, double ux0, double uy0, double uz0,
+                                                int permute, typename SP::T* vel) {
     using T = typename SP::T;
     const uint32_t cidx = blockIdx.x * WG + threadIdx.x;
     if (cidx >= ncell) return;
-    const T r = rho ? T(rho[cidx]) : T(rho0);
-    const T ux = u ? T(u[cidx]) : T(ux0);
-    const T uy = u ? T(u[ncell + cidx]) : T(uy0);
-    const T uz = u ? T(u[2 * (size_t)ncell + cidx]) : T(uz0);
+    // host fields cover the first nhost cells; the rest (ghost blocks of a refined level) take (rho0, u0)
+    const bool h = cidx < nhost;
+    const T r = rho && h ? T(rho[cidx]) : T(rho0);
+    const T ux = u && h ? T(u[cidx]) : T(ux0);
+    const T uy = u && h ? T(u[nhost + cidx]) : T(uy0);
+    const T uz = u && h ? T(u[2 * (size_t)nhost + cidx]) : T(uz0);
     const T cs2 = T(1) / T(3);
     const T cs4 = cs2 * cs2;
     const T usq = ux * ux + uy * uy + uz * uz;
@@ -61,20 +63,20 @@ __global__ void __launch_bounds__(WG) k_init_eq(typename SP::st* f, uint32_t nce
 }
 
 template <class SP>
-static hipError_t init_sp(void* f, uint32_t ncell, const double* rho, const double* u, double rho0, double ux0,
-                          double uy0, double uz0, int permute, void* vel, hipStream_t s) {
-    hipLaunchKernelGGL((k_init_eq<SP>), dim3((ncell + WG - 1) / WG), dim3(WG), 0, s, (typename SP::st*)f, ncell, rho, u,
-                       rho0, ux0, uy0, uz0, permute, (typename SP::T*)vel);
+static hipError_t init_sp(void* f, uint32_t ncell, uint32_t nhost, const double* rho, const double* u, double rho0,
+                          double ux0, double uy0, double uz0, int permute, void* vel, hipStream_t s) {
+    hipLaunchKernelGGL((k_init_eq<SP>), dim3((ncell + WG - 1) / WG), dim3(WG), 0, s, (typename SP::st*)f, ncell, nhost, rho,
+                       u, rho0, ux0, uy0, uz0, permute, (typename SP::T*)vel);
     return hipGetLastError();
 }
 
-hipError_t launch_init_eq(int storage, void* f, uint32_t ncell, const double* rho, const double* u, double rho0,
-                          double ux0, double uy0, double uz0, int permute, void* vel, hipStream_t s) {
+hipError_t launch_init_eq(int storage, void* f, uint32_t ncell, uint32_t nhost, const double* rho, const double* u,
+                          double rho0, double ux0, double uy0, double uz0, int permute, void* vel, hipStream_t s) {
     switch (storage) {
-        case 0: return init_sp<StF32>(f, ncell, rho, u, rho0, ux0, uy0, uz0, permute, vel, s);
-        case 1: return init_sp<StF64>(f, ncell, rho, u, rho0, ux0, uy0, uz0, permute, vel, s);
-        case 2: return init_sp<StF16S>(f, ncell, rho, u, rho0, ux0, uy0, uz0, permute, vel, s);
-        case 3: return init_sp<StBF16S>(f, ncell, rho, u, rho0, ux0, uy0, uz0, permute, vel, s);
+        case 0: return init_sp<StF32>(f, ncell, nhost, rho, u, rho0, ux0, uy0, uz0, permute, vel, s);
+        case 1: return init_sp<StF64>(f, ncell, nhost, rho, u, rho0, ux0, uy0, uz0, permute, vel, s);
+        case 2: return init_sp<StF16S>(f, ncell, nhost, rho, u, rho0, ux0, uy0, uz0, permute, vel, s);
+        case 3: return init_sp<StBF16S>(f, ncell, nhost, rho, u, rho0, ux0, uy0, uz0, permute, vel, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -83,53 +85,57 @@ hipError_t launch_init_eq(int storage, void* f, uint32_t ncell, const double* rh
 // Layout conversion between host populations (unshifted f32/f64) and the storage form
 // ------------------------------------------------------------------------------------------------
 template <class SP, class H>
-__global__ void __launch_bounds__(WG) k_convert_in(typename SP::st* f, const H* src, uint32_t ncell, int permute) {
+__global__ void __launch_bounds__(WG) k_convert_in(typename SP::st* f, const H* src, uint32_t ncell, uint32_t nhost,
+                                                   int permute) {
     using T = typename SP::T;
     const uint32_t cidx = blockIdx.x * WG + threadIdx.x;
     const int k = blockIdx.y;
-    if (cidx >= ncell) return;
-    SP::sto(f + (size_t)(permute ? 26 - k : k) * ncell + cidx, T(src[(size_t)k * ncell + cidx]), wgt<T>(k));
+    if (cidx >= nhost) return;
+    SP::sto(f + (size_t)(permute ? 26 - k : k) * ncell + cidx, T(src[(size_t)k * nhost + cidx]), wgt<T>(k));
 }
 
 template <class SP, class H>
-__global__ void __launch_bounds__(WG) k_convert_out(const typename SP::st* f, H* dst, uint32_t ncell, int permute) {
+__global__ void __launch_bounds__(WG) k_convert_out(const typename SP::st* f, H* dst, uint32_t ncell, uint32_t nhost,
+                                                    int permute) {
     using T = typename SP::T;
     const uint32_t cidx = blockIdx.x * WG + threadIdx.x;
     const int k = blockIdx.y;
-    if (cidx >= ncell) return;
-    dst[(size_t)k * ncell + cidx] = H(SP::ld(f + (size_t)(permute ? 26 - k : k) * ncell + cidx, wgt<T>(k)));
+    if (cidx >= nhost) return;
+    dst[(size_t)k * nhost + cidx] = H(SP::ld(f + (size_t)(permute ? 26 - k : k) * ncell + cidx, wgt<T>(k)));
 }
 
 template <class SP>
-static hipError_t conv_in_sp(void* f, const void* src, int dt, uint32_t ncell, int permute, hipStream_t s) {
-    dim3 grid((ncell + WG - 1) / WG, 27);
-    if (dt == 0) hipLaunchKernelGGL((k_convert_in<SP, float>), grid, dim3(WG), 0, s, (typename SP::st*)f, (const float*)src, ncell, permute);
-    else hipLaunchKernelGGL((k_convert_in<SP, double>), grid, dim3(WG), 0, s, (typename SP::st*)f, (const double*)src, ncell, permute);
+static hipError_t conv_in_sp(void* f, const void* src, int dt, uint32_t ncell, uint32_t nhost, int permute, hipStream_t s) {
+    dim3 grid((nhost + WG - 1) / WG, 27);
+    if (dt == 0) hipLaunchKernelGGL((k_convert_in<SP, float>), grid, dim3(WG), 0, s, (typename SP::st*)f, (const float*)src, ncell, nhost, permute);
+    else hipLaunchKernelGGL((k_convert_in<SP, double>), grid, dim3(WG), 0, s, (typename SP::st*)f, (const double*)src, ncell, nhost, permute);
     return hipGetLastError();
 }
 template <class SP>
-static hipError_t conv_out_sp(const void* f, void* dst, int dt, uint32_t ncell, int permute, hipStream_t s) {
-    dim3 grid((ncell + WG - 1) / WG, 27);
-    if (dt == 0) hipLaunchKernelGGL((k_convert_out<SP, float>), grid, dim3(WG), 0, s, (const typename SP::st*)f, (float*)dst, ncell, permute);
-    else hipLaunchKernelGGL((k_convert_out<SP, double>), grid, dim3(WG), 0, s, (const typename SP::st*)f, (double*)dst, ncell, permute);
+static hipError_t conv_out_sp(const void* f, void* dst, int dt, uint32_t ncell, uint32_t nhost, int permute, hipStream_t s) {
+    dim3 grid((nhost + WG - 1) / WG, 27);
+    if (dt == 0) hipLaunchKernelGGL((k_convert_out<SP, float>), grid, dim3(WG), 0, s, (const typename SP::st*)f, (float*)dst, ncell, nhost, permute);
+    else hipLaunchKernelGGL((k_convert_out<SP, double>), grid, dim3(WG), 0, s, (const typename SP::st*)f, (double*)dst, ncell, nhost, permute);
     return hipGetLastError();
 }
 
-hipError_t launch_convert_in(int storage, void* f, const void* src, int dt, uint32_t ncell, int permute, hipStream_t s) {
+hipError_t launch_convert_in(int storage, void* f, const void* src, int dt, uint32_t ncell, uint32_t nhost, int permute,
+                             hipStream_t s) {
     switch (storage) {
-        case 0: return conv_in_sp<StF32>(f, src, dt, ncell, permute, s);
-        case 1: return conv_in_sp<StF64>(f, src, dt, ncell, permute, s);
-        case 2: return conv_in_sp<StF16S>(f, src, dt, ncell, permute, s);
-        case 3: return conv_in_sp<StBF16S>(f, src, dt, ncell, permute, s);
+        case 0: return conv_in_sp<StF32>(f, src, dt, ncell, nhost, permute, s);
+        case 1: return conv_in_sp<StF64>(f, src, dt, ncell, nhost, permute, s);
+        case 2: return conv_in_sp<StF16S>(f, src, dt, ncell, nhost, permute, s);
+        case 3: return conv_in_sp<StBF16S>(f, src, dt, ncell, nhost, permute, s);
         default: return hipErrorInvalidValue;
     }
 }
-hipError_t launch_convert_out(int storage, const void* f, void* dst, int dt, uint32_t ncell, int permute, hipStream_t s) {
+hipError_t launch_convert_out(int storage, const void* f, void* dst, int dt, uint32_t ncell, uint32_t nhost, int permute,
+                             hipStream_t s) {
     switch (storage) {
-        case 0: return conv_out_sp<StF32>(f, dst, dt, ncell, permute, s);
-        case 1: return conv_out_sp<StF64>(f, dst, dt, ncell, permute, s);
-        case 2: return conv_out_sp<StF16S>(f, dst, dt, ncell, permute, s);
-        case 3: return conv_out_sp<StBF16S>(f, dst, dt, ncell, permute, s);
+        case 0: return conv_out_sp<StF32>(f, dst, dt, ncell, nhost, permute, s);
+        case 1: return conv_out_sp<StF64>(f, dst, dt, ncell, nhost, permute, s);
+        case 2: return conv_out_sp<StF16S>(f, dst, dt, ncell, nhost, permute, s);
+        case 3: return conv_out_sp<StBF16S>(f, dst, dt, ncell, nhost, permute, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -139,10 +145,10 @@ hipError_t launch_convert_out(int storage, const void* f, void* dst, int dt, uin
 // ------------------------------------------------------------------------------------------------
 template <class SP>
 __global__ void __launch_bounds__(WG) k_macro(const typename SP::st* f, double* rho, double* u, uint32_t ncell,
-                                              int permute, typename SP::T* vel) {
+                                              uint32_t nhost, int permute, typename SP::T* vel) {
     using T = typename SP::T;
     const uint32_t cidx = blockIdx.x * WG + threadIdx.x;
-    if (cidx >= ncell) return;
+    if (cidx >= nhost) return;
     double dr = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
 #pragma unroll
     for (int k = 0; k < 27; ++k) {
@@ -151,24 +157,25 @@ __global__ void __launch_bounds__(WG) k_macro(const typename SP::st* f, double* 
     }
     const double r = 1.0 + dr;
     if (rho) rho[cidx] = r;
-    if (u) { u[cidx] = jx / r; u[ncell + cidx] = jy / r; u[2 * (size_t)ncell + cidx] = jz / r; }
+    if (u) { u[cidx] = jx / r; u[nhost + cidx] = jy / r; u[2 * (size_t)nhost + cidx] = jz / r; }
     if (vel) { vel[cidx] = T(jx / r); vel[ncell + cidx] = T(jy / r); vel[2 * (size_t)ncell + cidx] = T(jz / r); }
 }
 
 template <class SP>
-static hipError_t macro_sp(const void* f, double* rho, double* u, uint32_t ncell, int permute, void* vel, hipStream_t s) {
-    hipLaunchKernelGGL((k_macro<SP>), dim3((ncell + WG - 1) / WG), dim3(WG), 0, s, (const typename SP::st*)f, rho, u, ncell,
-                       permute, (typename SP::T*)vel);
+static hipError_t macro_sp(const void* f, double* rho, double* u, uint32_t ncell, uint32_t nhost, int permute, void* vel,
+                          hipStream_t s) {
+    hipLaunchKernelGGL((k_macro<SP>), dim3((nhost + WG - 1) / WG), dim3(WG), 0, s, (const typename SP::st*)f, rho, u, ncell,
+                       nhost, permute, (typename SP::T*)vel);
     return hipGetLastError();
 }
 
-hipError_t launch_macro(int storage, const void* f, double* rho, double* u, uint32_t ncell, int permute, void* vel,
-                        hipStream_t s) {
+hipError_t launch_macro(int storage, const void* f, double* rho, double* u, uint32_t ncell, uint32_t nhost, int permute,
+                        void* vel, hipStream_t s) {
     switch (storage) {
-        case 0: return macro_sp<StF32>(f, rho, u, ncell, permute, vel, s);
-        case 1: return macro_sp<StF64>(f, rho, u, ncell, permute, vel, s);
-        case 2: return macro_sp<StF16S>(f, rho, u, ncell, permute, vel, s);
-        case 3: return macro_sp<StBF16S>(f, rho, u, ncell, permute, vel, s);
+        case 0: return macro_sp<StF32>(f, rho, u, ncell, nhost, permute, vel, s);
+        case 1: return macro_sp<StF64>(f, rho, u, ncell, nhost, permute, vel, s);
+        case 2: return macro_sp<StF16S>(f, rho, u, ncell, nhost, permute, vel, s);
+        case 3: return macro_sp<StBF16S>(f, rho, u, ncell, nhost, permute, vel, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -210,10 +217,14 @@ hipError_t launch_aa_outlet_prep(int storage, const StepArgs& a, hipStream_t s) 
 // ------------------------------------------------------------------------------------------------
 template <class SP>
 __global__ void __launch_bounds__(WG) k_mass(const typename SP::st* f, const uint8_t* solid, const int32_t* borg,
-                                             const int32_t* bdom, const DevDomain* doms, uint32_t nb, int permute,
-                                             double* part) {
+                                             const int32_t* bdom, const uint32_t* bflags, const DevDomain* doms,
+                                             uint32_t nb, int permute, double* part) {
     using T = typename SP::T;
     const uint32_t b = blockIdx.x;
+    if (bflags[b] & BF_INACTIVE) {   // covered or ghost block of a refined level: counted on another level
+        if (threadIdx.x < 4) part[(size_t)b * 4 + threadIdx.x] = 0.0;
+        return;
+    }
     const DevDomain* dm = doms + bdom[b];
     const uint32_t ncell = nb * BS3;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -248,12 +259,12 @@ __global__ void __launch_bounds__(WG) k_mass(const typename SP::st* f, const uin
 }
 
 hipError_t launch_mass(int storage, const void* f, const uint8_t* solid, const int32_t* borg, const int32_t* bdom,
-                       const DevDomain* doms, uint32_t nb, int permute, double* part, hipStream_t s) {
+                       const uint32_t* bflags, const DevDomain* doms, uint32_t nb, int permute, double* part, hipStream_t s) {
     switch (storage) {
-        case 0: hipLaunchKernelGGL((k_mass<StF32>), dim3(nb), dim3(WG), 0, s, (const float*)f, solid, borg, bdom, doms, nb, permute, part); break;
-        case 1: hipLaunchKernelGGL((k_mass<StF64>), dim3(nb), dim3(WG), 0, s, (const double*)f, solid, borg, bdom, doms, nb, permute, part); break;
-        case 2: hipLaunchKernelGGL((k_mass<StF16S>), dim3(nb), dim3(WG), 0, s, (const _Float16*)f, solid, borg, bdom, doms, nb, permute, part); break;
-        case 3: hipLaunchKernelGGL((k_mass<StBF16S>), dim3(nb), dim3(WG), 0, s, (const uint16_t*)f, solid, borg, bdom, doms, nb, permute, part); break;
+        case 0: hipLaunchKernelGGL((k_mass<StF32>), dim3(nb), dim3(WG), 0, s, (const float*)f, solid, borg, bdom, bflags, doms, nb, permute, part); break;
+        case 1: hipLaunchKernelGGL((k_mass<StF64>), dim3(nb), dim3(WG), 0, s, (const double*)f, solid, borg, bdom, bflags, doms, nb, permute, part); break;
+        case 2: hipLaunchKernelGGL((k_mass<StF16S>), dim3(nb), dim3(WG), 0, s, (const _Float16*)f, solid, borg, bdom, bflags, doms, nb, permute, part); break;
+        case 3: hipLaunchKernelGGL((k_mass<StBF16S>), dim3(nb), dim3(WG), 0, s, (const uint16_t*)f, solid, borg, bdom, bflags, doms, nb, permute, part); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -290,10 +301,11 @@ hipError_t launch_reduce(const ReduceArgs& r, hipStream_t s) {
 // ------------------------------------------------------------------------------------------------
 template <class SP>
 __global__ void __launch_bounds__(WG) k_guard(const typename SP::st* f, const uint8_t* solid, const int32_t* borg,
-                                              const int32_t* bdom, const DevDomain* doms, uint32_t nb, float rho_min,
-                                              float u_max, unsigned long long* count) {
+                                              const int32_t* bdom, const uint32_t* bflags, const DevDomain* doms,
+                                              uint32_t nb, float rho_min, float u_max, unsigned long long* count) {
     using T = typename SP::T;
     const uint32_t b = blockIdx.x;
+    if (bflags[b] & BF_INACTIVE) return;
     const DevDomain* dm = doms + bdom[b];
     const uint32_t ncell = nb * BS3;
     unsigned bad = 0;
@@ -318,16 +330,185 @@ __global__ void __launch_bounds__(WG) k_guard(const typename SP::st* f, const ui
 }
 
 hipError_t launch_guard(int storage, const void* f, const uint8_t* solid, const int32_t* borg, const int32_t* bdom,
-                        const DevDomain* doms, uint32_t nb, float rho_min, float u_max, unsigned long long* count,
-                        hipStream_t s) {
+                        const uint32_t* bflags, const DevDomain* doms, uint32_t nb, float rho_min, float u_max,
+                        unsigned long long* count, hipStream_t s) {
     switch (storage) {
-        case 0: hipLaunchKernelGGL((k_guard<StF32>), dim3(nb), dim3(WG), 0, s, (const float*)f, solid, borg, bdom, doms, nb, rho_min, u_max, count); break;
-        case 1: hipLaunchKernelGGL((k_guard<StF64>), dim3(nb), dim3(WG), 0, s, (const double*)f, solid, borg, bdom, doms, nb, rho_min, u_max, count); break;
-        case 2: hipLaunchKernelGGL((k_guard<StF16S>), dim3(nb), dim3(WG), 0, s, (const _Float16*)f, solid, borg, bdom, doms, nb, rho_min, u_max, count); break;
-        case 3: hipLaunchKernelGGL((k_guard<StBF16S>), dim3(nb), dim3(WG), 0, s, (const uint16_t*)f, solid, borg, bdom, doms, nb, rho_min, u_max, count); break;
+        case 0: hipLaunchKernelGGL((k_guard<StF32>), dim3(nb), dim3(WG), 0, s, (const float*)f, solid, borg, bdom, bflags, doms, nb, rho_min, u_max, count); break;
+        case 1: hipLaunchKernelGGL((k_guard<StF64>), dim3(nb), dim3(WG), 0, s, (const double*)f, solid, borg, bdom, bflags, doms, nb, rho_min, u_max, count); break;
+        case 2: hipLaunchKernelGGL((k_guard<StF16S>), dim3(nb), dim3(WG), 0, s, (const _Float16*)f, solid, borg, bdom, bflags, doms, nb, rho_min, u_max, count); break;
+        case 3: hipLaunchKernelGGL((k_guard<StBF16S>), dim3(nb), dim3(WG), 0, s, (const uint16_t*)f, solid, borg, bdom, bflags, doms, nb, rho_min, u_max, count); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Level transfers of a refined plan (the role of the octree's coupling, src/octree_q19/src/coupling.jl:
+// 86-233): a written cell receives the stencil-weighted sum of its donors' post-collision populations,
+// whose second-order non-equilibrium moments are then set to those of the receiving level: the donors'
+// Pi_neq times `scale` (1/(1 - omega) of the sending level and the ratio of the levels' relaxation
+// times) times `keep` (1 - omega of the receiving level), through the second-order Hermite term.  All
+// other moments of the interpolated state are kept: the cumulant collision's higher-order state
+// crosses the interface unchanged (a rebuild from rho, u and Pi_neq alone, as the octree's D3Q19
+// regularised collision allows, would impose the polynomial equilibrium's higher moments and make a
+// uniform flow change at the interfaces).  A cell whose corrected state has a population below 1e-6
+// of its weight takes the equilibrium of its rho and u (kernel.jl:94-100).  Every operation is written
+// out in a fixed order: the CPU oracle restates the same sequence and must agree bit for bit.
+// ------------------------------------------------------------------------------------------------
+template <class SP>
+__global__ void __launch_bounds__(WG) k_xfer_gather(const typename SP::st* fnew, const typename SP::st* fold,
+                                                    const uint32_t* cells, uint32_t n, uint32_t ncell, typename SP::T w1,
+                                                    typename SP::T* out) {
+    using T = typename SP::T;
+    const uint32_t i = blockIdx.x * WG + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t e = cells[i];
+    const uint32_t c = e & 0x7fffffffu;
+    const bool blend = fold && !(e >> 31);
+    const T w0 = T(1) - w1;
+    for_dirs([&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        T v = SP::ld(fnew + (size_t)K * ncell + c, wgt<T>(K));
+        if (blend) v = SP::ld(fold + (size_t)K * ncell + c, wgt<T>(K)) * w0 + v * w1;
+        out[(size_t)i * 27 + K] = v;
+    });
+}
+
+template <class SP>
+__global__ void __launch_bounds__(WG) k_xfer_rebuild(typename SP::st* f, uint32_t ncell, const uint32_t* tcells, uint32_t n,
+                                                     const int32_t* slot, const float* wt, const typename SP::T* m,
+                                                     typename SP::T scale, typename SP::T keep) {
+    using T = typename SP::T;
+    const uint32_t i = blockIdx.x * WG + threadIdx.x;
+    if (i >= n) return;
+    T acc[27];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) acc[k] = T(0);
+    for (int nn = 0; nn < 64; ++nn) {
+        const int32_t s = slot[(size_t)nn * n + i];
+        if (s < 0) continue;
+        const T wn = T(wt[(size_t)nn * n + i]);
+#pragma unroll
+        for (int k = 0; k < 27; ++k) acc[k] = acc[k] + wn * m[(size_t)s * 27 + k];
+    }
+    // moments of the interpolated state
+    T rho = T(0), jx = T(0), jy = T(0), jz = T(0), pxx = T(0), pyy = T(0), pzz = T(0), pxy = T(0), pxz = T(0), pyz = T(0);
+    for_dirs([&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        constexpr int cx = dcx(K), cy = dcy(K), cz = dcz(K);
+        const T v = acc[K];
+        rho = rho + v;
+        if constexpr (cx > 0) jx = jx + v;
+        if constexpr (cx < 0) jx = jx - v;
+        if constexpr (cy > 0) jy = jy + v;
+        if constexpr (cy < 0) jy = jy - v;
+        if constexpr (cz > 0) jz = jz + v;
+        if constexpr (cz < 0) jz = jz - v;
+        if constexpr (cx != 0) pxx = pxx + v;
+        if constexpr (cy != 0) pyy = pyy + v;
+        if constexpr (cz != 0) pzz = pzz + v;
+        if constexpr (cx * cy > 0) pxy = pxy + v;
+        if constexpr (cx * cy < 0) pxy = pxy - v;
+        if constexpr (cx * cz > 0) pxz = pxz + v;
+        if constexpr (cx * cz < 0) pxz = pxz - v;
+        if constexpr (cy * cz > 0) pyz = pyz + v;
+        if constexpr (cy * cz < 0) pyz = pyz - v;
+    });
+    const T ux = jx / rho, uy = jy / rho, uz = jz / rho;
+    const T third = T(1.0 / 3.0);
+    const T rx = rho * ux, ry = rho * uy;
+    const T qxx = (pxx - rho * third) - rx * ux, qyy = (pyy - rho * third) - ry * uy, qzz = (pzz - rho * third) - (rho * uz) * uz;
+    const T qxy = pxy - rx * uy, qxz = pxz - rx * uz, qyz = pyz - ry * uz;
+    // change of the second-order non-equilibrium moments
+    const T sk = scale * keep;
+    const T dxx = sk * qxx - qxx, dyy = sk * qyy - qyy, dzz = sk * qzz - qzz;
+    const T dxy = sk * qxy - qxy, dxz = sk * qxz - qxz, dyz = sk * qyz - qyz;
+    const T tr = (dxx + dyy) + dzz;
+    const T usq = (ux * ux + uy * uy) + uz * uz;
+    T out[27], eq[27];
+    bool bad = false;
+    for_dirs([&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        const T cx = T(dcx(K)), cy = T(dcy(K)), cz = T(dcz(K));
+        constexpr T w = wgt<T>(K);
+        const T cac = ((cx * cx * dxx + cy * cy * dyy) + cz * cz * dzz) + T(2) * ((cx * cy * dxy + cx * cz * dxz) + cy * cz * dyz);
+        out[K] = acc[K] + (w * T(4.5)) * (cac - tr * third);
+        const T cu = (cx * ux + cy * uy) + cz * uz;
+        eq[K] = (w * rho) * (((T(1) + T(3) * cu) + T(4.5) * cu * cu) - T(1.5) * usq);
+        bad = bad || out[K] < T(1e-6) * w;
+    });
+    const uint32_t c = tcells[i];
+    for_dirs([&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        SP::sto(f + (size_t)K * ncell + c, bad ? eq[K] : out[K], wgt<T>(K));
+    });
+}
+
+template <class SP>
+__global__ void __launch_bounds__(WG) k_copy_cells(typename SP::st* dst, const typename SP::st* src, const uint32_t* cells,
+                                                   uint32_t n, uint32_t ncell) {
+    const uint32_t i = blockIdx.x * WG + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = cells[i];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) dst[(size_t)k * ncell + c] = src[(size_t)k * ncell + c];
+}
+
+template <class SP>
+static hipError_t xfer_sp(const void* fnew, const void* fold, const uint32_t* cells, uint32_t n, uint32_t ncell, double w1,
+                          void* out, hipStream_t s) {
+    using T = typename SP::T;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL((k_xfer_gather<SP>), dim3((n + WG - 1) / WG), dim3(WG), 0, s, (const typename SP::st*)fnew,
+                       (const typename SP::st*)fold, cells, n, ncell, T(w1), (T*)out);
+    return hipGetLastError();
+}
+template <class SP>
+static hipError_t rebuild_sp(void* f, uint32_t ncell, const uint32_t* tcells, uint32_t n, const int32_t* slot, const float* w,
+                             const void* m, double scale, double keep, hipStream_t s) {
+    using T = typename SP::T;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL((k_xfer_rebuild<SP>), dim3((n + WG - 1) / WG), dim3(WG), 0, s, (typename SP::st*)f, ncell, tcells, n,
+                       slot, w, (const T*)m, T(scale), T(keep));
+    return hipGetLastError();
+}
+template <class SP>
+static hipError_t copy_cells_sp(void* dst, const void* src, const uint32_t* cells, uint32_t n, uint32_t ncell, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL((k_copy_cells<SP>), dim3((n + WG - 1) / WG), dim3(WG), 0, s, (typename SP::st*)dst,
+                       (const typename SP::st*)src, cells, n, ncell);
+    return hipGetLastError();
+}
+
+hipError_t launch_xfer_gather(int storage, const void* fnew, const void* fold, const uint32_t* cells, uint32_t n,
+                               uint32_t ncell, double w1, void* out, hipStream_t s) {
+    switch (storage) {
+        case 0: return xfer_sp<StF32>(fnew, fold, cells, n, ncell, w1, out, s);
+        case 1: return xfer_sp<StF64>(fnew, fold, cells, n, ncell, w1, out, s);
+        case 2: return xfer_sp<StF16S>(fnew, fold, cells, n, ncell, w1, out, s);
+        case 3: return xfer_sp<StBF16S>(fnew, fold, cells, n, ncell, w1, out, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+hipError_t launch_xfer_rebuild(int storage, void* f, uint32_t ncell, const uint32_t* tcells, uint32_t n, const int32_t* slot,
+                               const float* w, const void* m, double scale, double keep, hipStream_t s) {
+    switch (storage) {
+        case 0: return rebuild_sp<StF32>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
+        case 1: return rebuild_sp<StF64>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
+        case 2: return rebuild_sp<StF16S>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
+        case 3: return rebuild_sp<StBF16S>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+hipError_t launch_copy_cells(int storage, void* dst, const void* src, const uint32_t* cells, uint32_t n, uint32_t ncell,
+                             hipStream_t s) {
+    switch (storage) {
+        case 0: return copy_cells_sp<StF32>(dst, src, cells, n, ncell, s);
+        case 1: return copy_cells_sp<StF64>(dst, src, cells, n, ncell, s);
+        case 2: return copy_cells_sp<StF16S>(dst, src, cells, n, ncell, s);
+        case 3: return copy_cells_sp<StBF16S>(dst, src, cells, n, ncell, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace oal

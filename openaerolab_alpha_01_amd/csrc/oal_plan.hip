@@ -8,6 +8,8 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <array>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/oal.h"
@@ -70,6 +72,24 @@ struct Level {
     DevBuf<uint32_t> lmask;            // per-cell solid-link masks of blocks near solid cells
     DevBuf<unsigned char> obuf;        // AA: outlet-face populations with cx = -1
     DevBuf<unsigned long long> guard;
+    // refined plans: the caller's blocks come first (ncell_host cells), the level's ghost blocks after them
+    uint32_t ncell_host = 0;
+    uint32_t n_caller_blocks = 0, n_ghost_blocks = 0, n_covered_blocks = 0;
+    // fill of this level's ghost cells from the coarser level: moments of g_src cells (coarser level),
+    // stencils (g_slot, g_w: [64][n_gdst]) onto g_dst cells (this level)
+    DevBuf<uint32_t> g_src, g_dst;
+    DevBuf<int32_t> g_slot;
+    DevBuf<float> g_w;
+    DevBuf<unsigned char> g_m;
+    uint32_t n_gsrc = 0, n_gdst = 0;
+    double fill_scale = 0.0, fill_keep = 0.0;
+    // restriction of the finer level into this level's covered cells
+    DevBuf<uint32_t> r_src, r_dst;
+    DevBuf<int32_t> r_slot;
+    DevBuf<float> r_w;
+    DevBuf<unsigned char> r_m;
+    uint32_t n_rsrc = 0, n_rdst = 0;
+    double restrict_scale = 0.0, restrict_keep = 0.0;
     bool has_solid = false;
     int64_t n_fluid = 0;
     bool have_f = false;
@@ -80,6 +100,8 @@ struct Level {
         bflags.release(); solid.release(); doms.release(); fA.release(); fB.release(); velA.release(); velB.release();
         fpart.release(); mpart.release(); red_out.release(); guard.release(); oslot.release(); obuf.release(); blist_rg.release(); blist_in.release(); blist_br.release(); blist_bd.release(); face_cells.release();
         lslot.release(); lmask.release();
+        g_src.release(); g_dst.release(); g_slot.release(); g_w.release(); g_m.release();
+        r_src.release(); r_dst.release(); r_slot.release(); r_w.release(); r_m.release();
     }
 };
 
@@ -100,6 +122,8 @@ struct OalPlan {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     std::vector<hipEvent_t> bulk_ev;        // pairs around the bulk launches of timed steps
+    bool multilevel = false;                // a refined plan: levels coupled by ghost fill and restriction
+    bool coupled = false;                   // the initial restriction and fill have run since the last upload
     std::vector<Level> levels;
 };
 
@@ -131,7 +155,8 @@ struct HostLevel {
 };
 
 // Host-side construction of one level's tables from the reference-convention descriptor.
-int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, Level& L, HostLevel& H) {
+int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, Level& L, HostLevel& H,
+                  const std::vector<uint8_t>* inactive = nullptr) {
     if (d.n_blocks <= 0) return fail(OAL_ERR_ARG, "level with no blocks");
     if (!d.map_x || !d.map_y || !d.map_z || !d.neighbor_table) return fail(OAL_ERR_ARG, "level block arrays missing");
     if (d.n_domains <= 0 || !d.domains) return fail(OAL_ERR_ARG, "level has no domain descriptors");
@@ -139,6 +164,8 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, Leve
     if (ncell64 * 27ull >= (1ull << 32)) return fail(OAL_ERR_UNSUPPORTED, "level too large for 32-bit cell indexing (n_blocks*4096*27 >= 2^32)");
     L.nb = (uint32_t)d.n_blocks;
     L.ncell = (uint32_t)ncell64;
+    L.ncell_host = L.ncell;
+    L.n_caller_blocks = L.nb;
     L.n_items = L.nb * PLANES;
     L.n_domains = d.n_domains;
     const uint32_t nb = L.nb;
@@ -223,6 +250,7 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, Leve
             if (o >= dm.ext[a]) return fail(OAL_ERR_ARG, "a block lies entirely outside its domain extent");
         }
         if (block_solid[b]) fl |= BF_SOLID;
+        if (inactive && (*inactive)[b]) fl |= BF_INACTIVE;
         for (int k = 0; k < 27; ++k) {
             if (k == 13) continue;
             const int ox = k % 3 - 1, oy = (k / 3) % 3 - 1, oz = k / 9 - 1;
@@ -255,6 +283,7 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, Leve
     // at its lexicographic offset: its out-of-extent sources take the face rules and are never read)
     for (uint32_t b = 0; b < nb; ++b) {
         const DevDomain& dm = L.doms_h[bdom[b]];
+        if (flags[b] & BF_INACTIVE) continue;   // never stepped
         bool regular = !(flags[b] & BF_MISSING);
         for (int k = 0; k < 27 && regular; ++k) {
             const int ox = k % 3 - 1, oy = (k / 3) % 3 - 1, oz = k / 9 - 1;
@@ -313,6 +342,7 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, Leve
     L.n_fluid = 0;
     for (uint32_t b = 0; b < nb; ++b) {
         const DevDomain& dm = L.doms_h[bdom[b]];
+        if (inactive && (*inactive)[b]) continue;
         for (int z = 0; z < BS; ++z)
             for (int y = 0; y < BS; ++y)
                 for (int x = 0; x < BS; ++x) {
@@ -354,6 +384,351 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, Leve
                     }
                     m[(z * BS + y) * BS + x] = bits;
                 }
+    }
+    return OAL_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Refined plans: a block octree of 16^3 blocks.  Level l + 1 refines level l by 2 in every direction;
+// a refined block is replaced by its 8 children.  From the callers' block lists the library derives
+//   covered blocks  level-l blocks whose children exist: not stepped, written by restriction;
+//   ghost blocks    level-(l+1) blocks next to the level's blocks that the level does not hold: not
+//                   stepped, their cells next to stepped cells written by the fill from level l;
+//   transfer lists  the cells each fill / restriction writes, the donor cells it reads and the 64-node
+//                   stencils (cubic Lagrange where complete, else trilinear over the admissible donors;
+//                   restriction: cubic at the coarse centre, else the children averaged), as the octree
+//                   generator builds them (src/octree_q19/src/generator.jl:692-780, octree.jl:82-86).
+// ------------------------------------------------------------------------------------------------
+enum : uint8_t { OCT_STEPPED = 0, OCT_COVERED = 1, OCT_GHOST = 2 };
+
+struct OctLevel {
+    int ext[3] = {0, 0, 0}, per[3] = {0, 0, 0};
+    int nbx = 0, nby = 0, nbz = 0;            // blocks per axis of the level's extent
+    int n_caller = 0;
+    std::vector<int32_t> mx, my, mz;          // 1-based block coordinates: the caller's blocks, then ghosts
+    std::vector<uint8_t> status;
+    std::vector<uint8_t> solid;               // per cell
+    std::vector<int32_t> nbr;                 // column-major nb x 27, 1-based, 0 = none
+    std::unordered_map<int64_t, int32_t> index;
+    int64_t key(int bx, int by, int bz) const { return ((int64_t)bz * nby + by) * nbx + bx; }
+    static int wrap(int v, int n) { return ((v % n) + n) % n; }
+    // block at 0-based block coordinates (periodic axes wrap), -1 = none
+    int32_t find(int bx, int by, int bz) const {
+        if (per[0]) bx = wrap(bx, nbx);
+        if (per[1]) by = wrap(by, nby);
+        if (per[2]) bz = wrap(bz, nbz);
+        if (bx < 0 || by < 0 || bz < 0 || bx >= nbx || by >= nby || bz >= nbz) return -1;
+        const auto it = index.find(key(bx, by, bz));
+        return it == index.end() ? -1 : it->second;
+    }
+    // cell index of global cell coordinates, -1 = outside the extent or no block; *blk = its block
+    int64_t cell(int gx, int gy, int gz, int32_t* blk) const {
+        if (per[0]) gx = wrap(gx, ext[0]);
+        if (per[1]) gy = wrap(gy, ext[1]);
+        if (per[2]) gz = wrap(gz, ext[2]);
+        if (gx < 0 || gy < 0 || gz < 0 || gx >= ext[0] || gy >= ext[1] || gz >= ext[2]) return -1;
+        const int32_t b = find(gx / BS, gy / BS, gz / BS);
+        if (b < 0) return -1;
+        *blk = b;
+        return (int64_t)b * BS3 + (gz % BS) * BS2 + (gy % BS) * BS + gx % BS;
+    }
+    int nb() const { return (int)mx.size(); }
+};
+
+struct XferHost {
+    std::vector<uint32_t> src, dst;   // donor cells (bit 31: not blended in time), written cells
+    std::vector<int32_t> slot;        // [64][n_dst] index into src, -1 = unused node
+    std::vector<float> w;             // [64][n_dst]
+};
+
+// written cells inside the destination level, donors inside the source level, slots inside the donor list
+static bool xfer_in_bounds(const XferHost& X, uint64_t ncell_dst, uint64_t ncell_src) {
+    for (uint32_t c : X.dst)
+        if (c >= ncell_dst) return false;
+    for (uint32_t c : X.src)
+        if ((c & 0x7fffffffu) >= ncell_src) return false;
+    if (X.slot.size() != 64 * X.dst.size() || X.w.size() != X.slot.size()) return false;
+    for (int32_t s : X.slot)
+        if (s < -1 || s >= (int64_t)X.src.size()) return false;
+    return true;
+}
+
+static void cubic_weights(double t, double w[4]) {   // octree.jl:84
+    w[0] = -t * (1 - t) * (2 - t) / 6;
+    w[1] = (1 + t) * (1 - t) * (2 - t) / 2;
+    w[2] = (1 + t) * t * (2 - t) / 2;
+    w[3] = -(1 + t) * t * (1 - t) / 6;
+}
+
+// append one written cell with its stencil (per-cell arrays; transposed to node-major at the end)
+struct StencilBuilder {
+    std::unordered_map<uint32_t, int32_t> slot_of;
+    std::vector<std::array<int32_t, 64>> sl;
+    std::vector<std::array<float, 64>> ww;
+    XferHost& X;
+    explicit StencilBuilder(XferHost& x) : X(x) {}
+    int32_t slot(uint32_t cell, bool no_blend) {
+        const auto it = slot_of.find(cell);
+        if (it != slot_of.end()) return it->second;
+        const int32_t s = (int32_t)X.src.size();
+        X.src.push_back(cell | (no_blend ? 0x80000000u : 0u));
+        slot_of.emplace(cell, s);
+        return s;
+    }
+    void finish() {
+        const size_t n = X.dst.size();
+        X.slot.assign(64 * n, -1);
+        X.w.assign(64 * n, 0.0f);
+        for (size_t i = 0; i < n; ++i)
+            for (int q = 0; q < 64; ++q) {
+                X.slot[(size_t)q * n + i] = sl[i][q];
+                X.w[(size_t)q * n + i] = ww[i][q];
+            }
+    }
+};
+
+// Fill of level F's ghost cells (those next to a stepped cell) from level C = F - 1.  A ghost cell at
+// global fine coordinate g sits at coarse coordinate g / 2 - 1/4 (cell centres); its donors are the 4^3
+// coarse cells around it with cubic weights when all are stepped or covered fluid cells, else the
+// admissible ones of the inner 2^3 with trilinear weights renormalised; none: the cell is left out.
+static int build_fill(const OctLevel& F, const OctLevel& C, XferHost& X, int64_t* n_cubic) {
+    StencilBuilder B(X);
+    *n_cubic = 0;
+    for (int gb = F.n_caller; gb < F.nb(); ++gb) {
+        const int bx = F.mx[gb] - 1, by = F.my[gb] - 1, bz = F.mz[gb] - 1;
+        bool stepped[27];
+        bool any = false;
+        for (int k = 0; k < 27; ++k) {
+            const int32_t nbk = k == 13 ? -1 : F.find(bx + k % 3 - 1, by + (k / 3) % 3 - 1, bz + k / 9 - 1);
+            stepped[k] = nbk >= 0 && F.status[nbk] == OCT_STEPPED;
+            any = any || stepped[k];
+        }
+        if (!any) continue;
+        for (int lz = 0; lz < BS; ++lz)
+            for (int ly = 0; ly < BS; ++ly)
+                for (int lx = 0; lx < BS; ++lx) {
+                    if (lx > 0 && lx < BS - 1 && ly > 0 && ly < BS - 1 && lz > 0 && lz < BS - 1) continue;
+                    bool need = false;
+                    for (int d = 0; d < 27 && !need; ++d) {
+                        const int nx = lx + d % 3 - 1, ny = ly + (d / 3) % 3 - 1, nz = lz + d / 9 - 1;
+                        const int ox = nx < 0 ? -1 : (nx >= BS ? 1 : 0), oy = ny < 0 ? -1 : (ny >= BS ? 1 : 0),
+                                  oz = nz < 0 ? -1 : (nz >= BS ? 1 : 0);
+                        if ((ox | oy | oz) && stepped[(ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)]) need = true;
+                    }
+                    if (!need) continue;
+                    const int g[3] = {bx * BS + lx, by * BS + ly, bz * BS + lz};
+                    int i0[3];
+                    double t[3], wc[3][4];
+                    for (int a = 0; a < 3; ++a) {
+                        const double fx = g[a] / 2.0 - 0.25;
+                        i0[a] = (int)std::floor(fx);
+                        t[a] = fx - i0[a];
+                        cubic_weights(t[a], wc[a]);
+                    }
+                    std::array<int32_t, 64> sl;
+                    std::array<float, 64> ww;
+                    sl.fill(-1);
+                    ww.fill(0.0f);
+                    int64_t dc[64];
+                    bool cov[64];
+                    bool complete = true;
+                    for (int n = 0; n < 64 && complete; ++n) {
+                        const int di = n & 3, dj = (n >> 2) & 3, dk = n >> 4;
+                        int32_t blk = -1;
+                        const int64_t c = C.cell(i0[0] - 1 + di, i0[1] - 1 + dj, i0[2] - 1 + dk, &blk);
+                        complete = c >= 0 && C.status[blk] != OCT_GHOST && !C.solid[c];
+                        if (complete) { dc[n] = c; cov[n] = C.status[blk] == OCT_COVERED; }
+                    }
+                    if (complete) {
+                        ++*n_cubic;
+                        for (int n = 0; n < 64; ++n) {
+                            const int di = n & 3, dj = (n >> 2) & 3, dk = n >> 4;
+                            sl[n] = B.slot((uint32_t)dc[n], cov[n]);
+                            ww[n] = (float)(wc[0][di] * wc[1][dj] * wc[2][dk]);
+                        }
+                    } else {
+                        double wt[8], wsum = 0.0;
+                        int64_t tc[8];
+                        bool tcov[8];
+                        for (int n = 0; n < 8; ++n) {
+                            const int di = n & 1, dj = (n >> 1) & 1, dk = n >> 2;
+                            int32_t blk = -1;
+                            const int64_t c = C.cell(i0[0] + di, i0[1] + dj, i0[2] + dk, &blk);
+                            wt[n] = 0.0;
+                            tc[n] = -1;
+                            if (c < 0 || C.status[blk] == OCT_GHOST || C.solid[c]) continue;
+                            tc[n] = c;
+                            tcov[n] = C.status[blk] == OCT_COVERED;
+                            wt[n] = (di ? t[0] : 1 - t[0]) * (dj ? t[1] : 1 - t[1]) * (dk ? t[2] : 1 - t[2]);
+                            wsum += wt[n];
+                        }
+                        if (!(wsum > 0.0)) continue;
+                        for (int n = 0; n < 8; ++n) {
+                            if (tc[n] < 0) continue;
+                            const int di = n & 1, dj = (n >> 1) & 1, dk = n >> 2;
+                            const int q = (di + 1) + 4 * (dj + 1) + 16 * (dk + 1);
+                            sl[q] = B.slot((uint32_t)tc[n], tcov[n]);
+                            ww[q] = (float)(wt[n] / wsum);
+                        }
+                    }
+                    X.dst.push_back((uint32_t)gb * BS3 + (uint32_t)(lz * BS2 + ly * BS + lx));
+                    B.sl.push_back(sl);
+                    B.ww.push_back(ww);
+                }
+    }
+    B.finish();
+    return OAL_OK;
+}
+
+// Restriction of level F = C + 1 into the covered fluid cells of C: the coarse centre lies between fine
+// cells 2i and 2i + 1; cubic weights (-1, 9, 9, -1)/16 per axis over fine cells 2i - 1 .. 2i + 2 when all
+// are stepped or covered fluid cells, else the average of the admissible children (octree.jl:86,
+// coupling.jl:52-76).
+static int build_restrict(const OctLevel& C, const OctLevel& F, XferHost& X, int64_t* n_cubic) {
+    StencilBuilder B(X);
+    *n_cubic = 0;
+    double wr[4];
+    cubic_weights(0.5, wr);
+    for (int cb = 0; cb < C.n_caller; ++cb) {
+        if (C.status[cb] != OCT_COVERED) continue;
+        const int bx = C.mx[cb] - 1, by = C.my[cb] - 1, bz = C.mz[cb] - 1;
+        for (int lz = 0; lz < BS; ++lz)
+            for (int ly = 0; ly < BS; ++ly)
+                for (int lx = 0; lx < BS; ++lx) {
+                    const uint32_t c = (uint32_t)cb * BS3 + (uint32_t)(lz * BS2 + ly * BS + lx);
+                    if (C.solid[c]) continue;
+                    const int i[3] = {bx * BS + lx, by * BS + ly, bz * BS + lz};
+                    std::array<int32_t, 64> sl;
+                    std::array<float, 64> ww;
+                    sl.fill(-1);
+                    ww.fill(0.0f);
+                    int64_t fc[64];
+                    bool complete = true;
+                    for (int n = 0; n < 64 && complete; ++n) {
+                        const int di = n & 3, dj = (n >> 2) & 3, dk = n >> 4;
+                        int32_t blk = -1;
+                        const int64_t f = F.cell(2 * i[0] - 1 + di, 2 * i[1] - 1 + dj, 2 * i[2] - 1 + dk, &blk);
+                        complete = f >= 0 && F.status[blk] != OCT_GHOST && !F.solid[f];
+                        fc[n] = f;
+                    }
+                    if (complete) {
+                        ++*n_cubic;
+                        for (int n = 0; n < 64; ++n) {
+                            const int di = n & 3, dj = (n >> 2) & 3, dk = n >> 4;
+                            sl[n] = B.slot((uint32_t)fc[n], false);
+                            ww[n] = (float)(wr[di] * wr[dj] * wr[dk]);
+                        }
+                    } else {
+                        int64_t ch[8];
+                        int cnt = 0;
+                        for (int n = 0; n < 8; ++n) {
+                            const int di = n & 1, dj = (n >> 1) & 1, dk = n >> 2;
+                            int32_t blk = -1;
+                            const int64_t f = F.cell(2 * i[0] + di, 2 * i[1] + dj, 2 * i[2] + dk, &blk);
+                            ch[n] = (f >= 0 && F.status[blk] != OCT_GHOST && !F.solid[f]) ? f : -1;
+                            if (ch[n] >= 0) ++cnt;
+                        }
+                        if (!cnt) continue;
+                        for (int n = 0; n < 8; ++n) {
+                            if (ch[n] < 0) continue;
+                            const int di = n & 1, dj = (n >> 1) & 1, dk = n >> 2;
+                            const int q = (di + 1) + 4 * (dj + 1) + 16 * (dk + 1);
+                            sl[q] = B.slot((uint32_t)ch[n], false);
+                            ww[q] = (float)(1.0 / cnt);
+                        }
+                    }
+                    X.dst.push_back(c);
+                    B.sl.push_back(sl);
+                    B.ww.push_back(ww);
+                }
+    }
+    B.finish();
+    return OAL_OK;
+}
+
+// The octree of a refined plan from the callers' level descriptors (coarsest first).
+static int build_octree(const OalPlanDesc* desc, std::vector<OctLevel>& O) {
+    const int L = desc->n_levels;
+    O.assign(L, OctLevel{});
+    for (int l = 0; l < L; ++l) {
+        const OalLevelDesc& d = desc->levels[l];
+        if (d.n_domains != 1 || !d.domains) return fail(OAL_ERR_UNSUPPORTED, "each level of a refined plan holds one domain");
+        if (d.block_domain) return fail(OAL_ERR_UNSUPPORTED, "block_domain must be NULL on a refined plan");
+        if (d.n_blocks <= 0 || !d.map_x || !d.map_y || !d.map_z) return fail(OAL_ERR_ARG, "level block arrays missing");
+        OctLevel& o = O[l];
+        for (int a = 0; a < 3; ++a) {
+            o.ext[a] = d.domains[0].extent[a];
+            o.per[a] = d.domains[0].periodic[a] ? 1 : 0;
+            if (o.ext[a] <= 0 || o.ext[a] % BS) return fail(OAL_ERR_UNSUPPORTED, "refined plans need extents that are multiples of 16");
+            if (l > 0 && (o.ext[a] != 2 * O[l - 1].ext[a] || o.per[a] != O[l - 1].per[a]))
+                return fail(OAL_ERR_ARG, "level l + 1 must have twice the extent of level l and the same periodicity");
+        }
+        o.nbx = o.ext[0] / BS; o.nby = o.ext[1] / BS; o.nbz = o.ext[2] / BS;
+        o.n_caller = d.n_blocks;
+        o.mx.assign(d.map_x, d.map_x + d.n_blocks);
+        o.my.assign(d.map_y, d.map_y + d.n_blocks);
+        o.mz.assign(d.map_z, d.map_z + d.n_blocks);
+        for (int b = 0; b < d.n_blocks; ++b) {
+            const int bx = o.mx[b] - 1, by = o.my[b] - 1, bz = o.mz[b] - 1;
+            if (bx < 0 || by < 0 || bz < 0 || bx >= o.nbx || by >= o.nby || bz >= o.nbz)
+                return fail(OAL_ERR_ARG, "block coordinates outside the level extent");
+            if (!o.index.emplace(o.key(bx, by, bz), b).second) return fail(OAL_ERR_ARG, "duplicate block in a level");
+        }
+        o.status.assign(d.n_blocks, OCT_STEPPED);
+        o.solid.assign((size_t)d.n_blocks * BS3, 0);
+        if (d.obstacle)
+            for (size_t i = 0; i < o.solid.size(); ++i) o.solid[i] = d.obstacle[i] ? 1 : 0;
+    }
+    // nesting: every block of a finer level has its parent, and parents have all 8 children
+    for (int l = 1; l < L; ++l) {
+        OctLevel& o = O[l];
+        for (int b = 0; b < o.n_caller; ++b) {
+            const int bx = o.mx[b] - 1, by = o.my[b] - 1, bz = o.mz[b] - 1;
+            if (O[l - 1].find(bx / 2, by / 2, bz / 2) < 0) return fail(OAL_ERR_ARG, "a refined block has no parent block");
+            for (int c = 0; c < 8; ++c)
+                if (o.find((bx & ~1) + (c & 1), (by & ~1) + ((c >> 1) & 1), (bz & ~1) + (c >> 2)) < 0)
+                    return fail(OAL_ERR_ARG, "a refined block lacks one of its 7 siblings");
+        }
+        // covered parents
+        OctLevel& c = O[l - 1];
+        for (int b = 0; b < c.n_caller; ++b)
+            if (o.find(2 * (c.mx[b] - 1), 2 * (c.my[b] - 1), 2 * (c.mz[b] - 1)) >= 0) c.status[b] = OCT_COVERED;
+    }
+    // ghost blocks around every finer level; their parents must exist (a margin of one coarse block)
+    for (int l = 1; l < L; ++l) {
+        OctLevel& o = O[l];
+        for (int b = 0; b < o.n_caller; ++b) {
+            const int bx = o.mx[b] - 1, by = o.my[b] - 1, bz = o.mz[b] - 1;
+            for (int k = 0; k < 27; ++k) {
+                if (k == 13) continue;
+                int nx = bx + k % 3 - 1, ny = by + (k / 3) % 3 - 1, nz = bz + k / 9 - 1;
+                if (o.per[0]) nx = OctLevel::wrap(nx, o.nbx);
+                if (o.per[1]) ny = OctLevel::wrap(ny, o.nby);
+                if (o.per[2]) nz = OctLevel::wrap(nz, o.nbz);
+                if (nx < 0 || ny < 0 || nz < 0 || nx >= o.nbx || ny >= o.nby || nz >= o.nbz) continue;
+                if (o.find(nx, ny, nz) >= 0) continue;
+                const int32_t pb = O[l - 1].find(nx / 2, ny / 2, nz / 2);
+                if (pb < 0 || O[l - 1].status[pb] == OCT_GHOST)
+                    return fail(OAL_ERR_ARG, "a refined region needs a margin of one coarser block around it");
+                const int32_t g = o.nb();
+                o.index.emplace(o.key(nx, ny, nz), g);
+                o.mx.push_back(nx + 1);
+                o.my.push_back(ny + 1);
+                o.mz.push_back(nz + 1);
+                o.status.push_back(OCT_GHOST);
+                o.solid.resize(o.solid.size() + BS3, 0);
+            }
+        }
+    }
+    for (int l = 0; l < L; ++l) {
+        OctLevel& o = O[l];
+        const int nb = o.nb();
+        o.nbr.assign((size_t)27 * nb, 0);
+        for (int b = 0; b < nb; ++b)
+            for (int k = 0; k < 27; ++k) {
+                const int32_t n = o.find(o.mx[b] - 1 + k % 3 - 1, o.my[b] - 1 + (k / 3) % 3 - 1, o.mz[b] - 1 + k / 9 - 1);
+                o.nbr[(size_t)k * nb + b] = n + 1;
+            }
     }
     return OAL_OK;
 }
@@ -471,6 +846,79 @@ int level_step(OalPlan* p, int level, int32_t time_step, int flags, hipStream_t 
 
 int permute_of(const OalPlan* p) { return p->stream_mode == OAL_STREAM_AA ? 1 : 0; }
 
+// ---- refined plans: transfers and the recursive step (src/octree_q19/src/coupling.jl:134-268) ----
+// Fill level l's ghost cells from level l - 1 at weight w1 along the coarse step: 1 = its current state
+// (fA), else the blend with its previous one (fB); covered donors (the finer level's restriction)
+// enter unblended.
+int ml_fill(OalPlan* p, int l, double w1, hipStream_t s) {
+    Level& F = p->levels[l];
+    Level& C = p->levels[l - 1];
+    if (!F.n_gdst) return OAL_OK;
+    OAL_HIP(launch_xfer_gather(p->storage, C.fA.p, w1 == 1.0 ? nullptr : C.fB.p, F.g_src.p, F.n_gsrc, C.ncell, w1,
+                                F.g_m.p, s));
+    OAL_HIP(launch_xfer_rebuild(p->storage, F.fA.p, F.ncell, F.g_dst.p, F.n_gdst, F.g_slot.p, F.g_w.p, F.g_m.p,
+                                F.fill_scale, F.fill_keep, s));
+    return OAL_OK;
+}
+
+// Restrict level l + 1's current state into level l's covered cells.
+int ml_restrict(OalPlan* p, int l, hipStream_t s) {
+    Level& C = p->levels[l];
+    Level& F = p->levels[l + 1];
+    if (!C.n_rdst) return OAL_OK;
+    OAL_HIP(launch_xfer_gather(p->storage, F.fA.p, nullptr, C.r_src.p, C.n_rsrc, F.ncell, 1.0, C.r_m.p, s));
+    OAL_HIP(launch_xfer_rebuild(p->storage, C.fA.p, C.ncell, C.r_dst.p, C.n_rdst, C.r_slot.p, C.r_w.p, C.r_m.p,
+                                C.restrict_scale, C.restrict_keep, s));
+    return OAL_OK;
+}
+
+// After a step and its swap the new read buffer's ghost cells are two fills old: take the last fill
+// from the previous read buffer (coupling.jl:236-251).
+int ml_copy_ghosts(OalPlan* p, int l, hipStream_t s) {
+    Level& L = p->levels[l];
+    if (!L.n_gdst) return OAL_OK;
+    OAL_HIP(launch_copy_cells(p->storage, L.fA.p, L.fB.p, L.g_dst.p, L.n_gdst, L.ncell, s));
+    return OAL_OK;
+}
+
+// Initial consistency (couple_initial!, coupling.jl:253-268): restrict every level into its parent,
+// then fill every level's ghosts; both buffers of a level hold the result.
+int ml_couple(OalPlan* p, hipStream_t s) {
+    const int nl = (int)p->levels.size();
+    for (int l = 0; l < nl; ++l)
+        if (!p->levels[l].have_f) return fail(OAL_ERR_STATE, "every level of a refined plan needs populations");
+    for (int l = nl - 2; l >= 0; --l) {
+        if (int rc = ml_restrict(p, l, s)) return rc;
+        OAL_HIP(hipMemcpyAsync(p->levels[l].fB.p, p->levels[l].fA.p, p->levels[l].fA.n, hipMemcpyDeviceToDevice, s));
+    }
+    for (int l = 1; l < nl; ++l) {
+        if (int rc = ml_fill(p, l, 1.0, s)) return rc;
+        OAL_HIP(hipMemcpyAsync(p->levels[l].fB.p, p->levels[l].fA.p, p->levels[l].fA.n, hipMemcpyDeviceToDevice, s));
+    }
+    p->coupled = true;
+    return OAL_OK;
+}
+
+// One step of level l and, recursively, two of each finer level (step_recursive!, coupling.jl:236-251).
+// Level l's sub-steps are numbered t = 2^l (coarse step) + k.  The force is collected on the last
+// sub-step of every level that holds obstacle cells when `last`.
+int ml_step_rec(OalPlan* p, int l, int32_t t, int flags, bool last, hipStream_t s) {
+    Level& L = p->levels[l];
+    const int f = (last && L.has_solid) ? flags : (flags & ~OAL_STEP_FORCE);
+    if (int rc = level_step(p, l, t, f, s)) return rc;
+    if (l > 0)
+        if (int rc = ml_copy_ghosts(p, l, s)) return rc;
+    if (l + 1 < (int)p->levels.size()) {
+        if (int rc = ml_step_rec(p, l + 1, 2 * t, flags, false, s)) return rc;
+        if (int rc = ml_restrict(p, l, s)) return rc;
+        if (int rc = ml_fill(p, l + 1, 0.5, s)) return rc;
+        if (int rc = ml_step_rec(p, l + 1, 2 * t + 1, flags, last, s)) return rc;
+        if (int rc = ml_restrict(p, l, s)) return rc;
+        if (int rc = ml_fill(p, l + 1, 1.0, s)) return rc;
+    }
+    return OAL_OK;
+}
+
 // AA pattern: the outlet side buffer mirrors the freshly set state (the first step is an odd one)
 hipError_t aa_prep(OalPlan* p, Level& L) {
     if (p->stream_mode != OAL_STREAM_AA || !L.obuf.p) return hipSuccess;
@@ -488,7 +936,7 @@ int require_plain_state(const OalPlan* p, const Level& L) {
 // WALE needs the previous step's velocity: rebuild it from the populations just set
 int refresh_velocity(OalPlan* p, Level& L) {
     if (!p->wale) return OAL_OK;
-    OAL_HIP(launch_macro(p->storage, L.fA.p, nullptr, nullptr, L.ncell, permute_of(p), L.velA.p, p->stream));
+    OAL_HIP(launch_macro(p->storage, L.fA.p, nullptr, nullptr, L.ncell, L.ncell, permute_of(p), L.velA.p, p->stream));
     return OAL_OK;
 }
 
@@ -523,9 +971,12 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
     if (desc->bb_mode != OAL_BB_REFERENCE && desc->bb_mode != OAL_BB_HALFWAY) return fail(OAL_ERR_ARG, "unknown bb_mode");
     if (desc->stream_mode == OAL_STREAM_AA && desc->bb_mode != OAL_BB_HALFWAY)
         return fail(OAL_ERR_UNSUPPORTED, "the AA pattern needs halfway bounce-back");
-    if (desc->n_levels != 1) return fail(OAL_ERR_UNSUPPORTED, "plans of more than one level are not supported in this version");
+    if (desc->n_levels < 1 || desc->n_levels > 12) return fail(OAL_ERR_ARG, "n_levels must be 1 to 12");
     if (!desc->levels) return fail(OAL_ERR_ARG, "no level descriptors");
     const OalCollision& c = desc->collision;
+    const bool ml = desc->n_levels > 1;
+    if (ml && desc->stream_mode != OAL_STREAM_AB) return fail(OAL_ERR_UNSUPPORTED, "refined plans use the AB pattern");
+    if (ml && c.c_wale != 0.0f) return fail(OAL_ERR_UNSUPPORTED, "WALE is not supported on refined plans");
     if (c.limiter < 0 || c.limiter > 2) return fail(OAL_ERR_ARG, "unknown limiter");
     if (desc->rounding != OAL_ROUND_STOCHASTIC && desc->rounding != OAL_ROUND_NEAREST)
         return fail(OAL_ERR_ARG, "rounding must be OAL_ROUND_STOCHASTIC or OAL_ROUND_NEAREST");
@@ -555,11 +1006,81 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
         return rc;
     };
     p->levels.resize(desc->n_levels);
+    p->multilevel = ml;
     std::vector<HostLevel> host(desc->n_levels);
+    std::vector<OctLevel> oct;
+    std::vector<XferHost> fillX(desc->n_levels), restX(desc->n_levels);
+    if (ml) {
+        if (int rc = build_octree(desc, oct)) return bail(rc);
+    }
     for (int l = 0; l < desc->n_levels; ++l) {
-        const int rc = prepare_level(desc->levels[l], desc->collision, desc->stream_mode == OAL_STREAM_AB, p->levels[l],
-                                     host[l]);
+        OalLevelDesc d = desc->levels[l];
+        std::vector<uint8_t> inactive;
+        if (ml) {
+            const OctLevel& o = oct[l];
+            d.n_blocks = o.nb();
+            d.map_x = o.mx.data(); d.map_y = o.my.data(); d.map_z = o.mz.data();
+            d.neighbor_table = o.nbr.data();
+            d.obstacle = o.solid.data();
+            inactive.resize(o.nb());
+            for (int b = 0; b < o.nb(); ++b) inactive[b] = o.status[b] != OCT_STEPPED;
+        }
+        const int rc = prepare_level(d, desc->collision, desc->stream_mode == OAL_STREAM_AB, p->levels[l], host[l],
+                                     ml ? &inactive : nullptr);
         if (rc) return bail(rc);
+        if (ml) {
+            Level& L = p->levels[l];
+            const OctLevel& o = oct[l];
+            L.n_caller_blocks = (uint32_t)o.n_caller;
+            L.ncell_host = (uint32_t)o.n_caller * BS3;
+            L.n_ghost_blocks = (uint32_t)(o.nb() - o.n_caller);
+            L.n_covered_blocks = 0;
+            for (int b = 0; b < o.n_caller; ++b) L.n_covered_blocks += o.status[b] == OCT_COVERED;
+        }
+    }
+    if (ml) {
+        const int nl = desc->n_levels;
+        int64_t ncub = 0;
+        for (int l = 1; l < nl; ++l) build_fill(oct[l], oct[l - 1], fillX[l], &ncub);
+        for (int l = 0; l + 1 < nl; ++l) build_restrict(oct[l], oct[l + 1], restX[l], &ncub);
+        // every index the transfer kernels follow stays inside its buffer
+        for (int l = 0; l < nl; ++l) {
+            const uint64_t here = (uint64_t)oct[l].nb() * BS3;
+            if (l > 0 && !xfer_in_bounds(fillX[l], here, (uint64_t)oct[l - 1].nb() * BS3))
+                return bail(fail(OAL_ERR_ARG, "fill transfer list out of bounds"));
+            if (l + 1 < nl && !xfer_in_bounds(restX[l], here, (uint64_t)oct[l + 1].nb() * BS3))
+                return bail(fail(OAL_ERR_ARG, "restriction transfer list out of bounds"));
+        }
+        // rescale of the non-equilibrium moments between levels (coupling.jl:134-178, 188-193): tau of
+        // each level in its own lattice units, 1/(1 - omega) of the sending level (0 near omega = 1)
+        const bool dbl = p->storage == OAL_F64;
+        for (int l = 0; l < nl; ++l) {
+            const DevDomain& d = p->levels[l].doms_h[0];
+            if (l > 0) {
+                const DevDomain& dc = p->levels[l - 1].doms_h[0];
+                if (dbl) {
+                    const double pc = std::fabs(1.0 - dc.omega1_d) > 0.05 ? 1.0 / (1.0 - dc.omega1_d) : 0.0;
+                    p->levels[l].fill_scale = ((double)d.tau / (2.0 * (double)dc.tau)) * pc;
+                    p->levels[l].fill_keep = 1.0 - d.omega1_d;
+                } else {
+                    const float pc = std::fabs(1.0f - dc.omega1) > 0.05f ? 1.0f / (1.0f - dc.omega1) : 0.0f;
+                    p->levels[l].fill_scale = (double)((d.tau / (2.0f * dc.tau)) * pc);
+                    p->levels[l].fill_keep = (double)(1.0f - d.omega1);
+                }
+            }
+            if (l + 1 < nl) {
+                const DevDomain& df = p->levels[l + 1].doms_h[0];
+                if (dbl) {
+                    const double pf = std::fabs(1.0 - df.omega1_d) > 0.05 ? 1.0 / (1.0 - df.omega1_d) : 0.0;
+                    p->levels[l].restrict_scale = ((2.0 * (double)d.tau) / (double)df.tau) * pf;
+                    p->levels[l].restrict_keep = 1.0 - d.omega1_d;
+                } else {
+                    const float pf = std::fabs(1.0f - df.omega1) > 0.05f ? 1.0f / (1.0f - df.omega1) : 0.0f;
+                    p->levels[l].restrict_scale = (double)(((2.0f * d.tau) / df.tau) * pf);
+                    p->levels[l].restrict_keep = (double)(1.0f - d.omega1);
+                }
+            }
+        }
     }
     hipError_t e = hipSetDevice(p->device);
     if (e != hipSuccess) return bail(fail(OAL_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)));
@@ -572,6 +1093,21 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
     for (int l = 0; l < desc->n_levels; ++l) {
         const int rc = upload_level(p, p->levels[l], host[l]);
         if (rc) return bail(rc);
+        if (ml) {
+            Level& L = p->levels[l];
+            const XferHost& F = fillX[l];
+            const XferHost& R = restX[l];
+            int r2;
+            if ((r2 = upload(L.g_src, F.src)) || (r2 = upload(L.g_dst, F.dst)) || (r2 = upload(L.g_slot, F.slot)) ||
+                (r2 = upload(L.g_w, F.w)) || (r2 = upload(L.r_src, R.src)) || (r2 = upload(L.r_dst, R.dst)) ||
+                (r2 = upload(L.r_slot, R.slot)) || (r2 = upload(L.r_w, R.w)))
+                return bail(r2);
+            L.n_gsrc = (uint32_t)F.src.size(); L.n_gdst = (uint32_t)F.dst.size();
+            L.n_rsrc = (uint32_t)R.src.size(); L.n_rdst = (uint32_t)R.dst.size();
+            if (L.g_m.alloc((size_t)L.n_gsrc * 27 * work_bytes(p->storage)) != hipSuccess ||
+                L.r_m.alloc((size_t)L.n_rsrc * 27 * work_bytes(p->storage)) != hipSuccess)
+                return bail(fail(OAL_ERR_NOMEM, "transfer buffer allocation failed"));
+        }
     }
     *out = p;
     return OAL_OK;
@@ -599,7 +1135,7 @@ int oal_plan_destroy(OalPlan* p) {
 int oal_level_cells(const OalPlan* p, int level, int64_t* n) {
     if (int rc = check_level(p, level)) return rc;
     if (!n) return fail(OAL_ERR_ARG, "NULL output");
-    *n = p->levels[level].ncell;
+    *n = p->levels[level].ncell_host;
     return OAL_OK;
 }
 
@@ -608,9 +1144,11 @@ int oal_level_info(const OalPlan* p, int level, OalLevelInfo* info) {
     if (!info) return fail(OAL_ERR_ARG, "NULL output");
     const Level& L = p->levels[level];
     *info = OalLevelInfo{};
-    info->n_cells = L.ncell;
+    info->n_cells = L.ncell_host;
     info->n_fluid_cells = L.n_fluid;
-    info->n_blocks = (int32_t)L.nb;
+    info->n_blocks = (int32_t)L.n_caller_blocks;
+    info->n_ghost_blocks = (int32_t)L.n_ghost_blocks;
+    info->n_covered_blocks = (int32_t)L.n_covered_blocks;
     info->n_regular = (int32_t)L.n_regular;
     info->n_interior = (int32_t)L.n_interior;
     info->n_boundary_regular = (int32_t)L.n_boundary_regular;
@@ -624,11 +1162,15 @@ int oal_upload_f(OalPlan* p, int level, const void* host_f, int dt) {
     if (!host_f || (dt != OAL_F32 && dt != OAL_F64)) return fail(OAL_ERR_ARG, "host_f must be OAL_F32 or OAL_F64 data");
     OAL_HIP(hipSetDevice(p->device));
     Level& L = p->levels[level];
-    const size_t bytes = (size_t)L.ncell * 27 * (dt == OAL_F64 ? 8 : 4);
+    const size_t bytes = (size_t)L.ncell_host * 27 * (dt == OAL_F64 ? 8 : 4);
     void* stage = nullptr;
     OAL_HIP(hipMalloc(&stage, bytes));
     hipError_t e = hipMemcpyAsync(stage, host_f, bytes, hipMemcpyHostToDevice, p->stream);
-    if (e == hipSuccess) e = launch_convert_in(p->storage, L.fA.p, stage, dt, L.ncell, permute_of(p), p->stream);
+    // ghost cells of a refined level start at rest (those no fill reaches keep that state)
+    if (e == hipSuccess && L.ncell > L.ncell_host)
+        e = launch_init_eq(p->storage, L.fA.p, L.ncell, 0, nullptr, nullptr, 1.0, 0.0, 0.0, 0.0, permute_of(p), nullptr,
+                           p->stream);
+    if (e == hipSuccess) e = launch_convert_in(p->storage, L.fA.p, stage, dt, L.ncell, L.ncell_host, permute_of(p), p->stream);
     if (e == hipSuccess && L.fB.p) e = hipMemcpyAsync(L.fB.p, L.fA.p, L.fA.n, hipMemcpyDeviceToDevice, p->stream);
     if (e == hipSuccess) e = aa_prep(p, L);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
@@ -637,6 +1179,7 @@ int oal_upload_f(OalPlan* p, int level, const void* host_f, int dt) {
     L.have_f = true;
     L.aa_phase = 0;
     L.force_valid = false;
+    p->coupled = false;
     return refresh_velocity(p, L);
 }
 
@@ -646,10 +1189,10 @@ int oal_download_f(OalPlan* p, int level, void* host_f, int dt) {
     Level& L = p->levels[level];
     if (int rc = require_plain_state(p, L)) return rc;
     OAL_HIP(hipSetDevice(p->device));
-    const size_t bytes = (size_t)L.ncell * 27 * (dt == OAL_F64 ? 8 : 4);
+    const size_t bytes = (size_t)L.ncell_host * 27 * (dt == OAL_F64 ? 8 : 4);
     void* stage = nullptr;
     OAL_HIP(hipMalloc(&stage, bytes));
-    hipError_t e = launch_convert_out(p->storage, L.fA.p, stage, dt, L.ncell, permute_of(p), p->stream);
+    hipError_t e = launch_convert_out(p->storage, L.fA.p, stage, dt, L.ncell, L.ncell_host, permute_of(p), p->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(host_f, stage, bytes, hipMemcpyDeviceToHost, p->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
     (void)hipFree(stage);
@@ -665,16 +1208,16 @@ int oal_init_equilibrium(OalPlan* p, int level, const double* rho, const double*
     double* du = nullptr;
     hipError_t e = hipSuccess;
     if (rho) {
-        e = hipMalloc((void**)&drho, (size_t)L.ncell * 8);
-        if (e == hipSuccess) e = hipMemcpyAsync(drho, rho, (size_t)L.ncell * 8, hipMemcpyHostToDevice, p->stream);
+        e = hipMalloc((void**)&drho, (size_t)L.ncell_host * 8);
+        if (e == hipSuccess) e = hipMemcpyAsync(drho, rho, (size_t)L.ncell_host * 8, hipMemcpyHostToDevice, p->stream);
     }
     if (e == hipSuccess && u) {
-        e = hipMalloc((void**)&du, (size_t)L.ncell * 24);
-        if (e == hipSuccess) e = hipMemcpyAsync(du, u, (size_t)L.ncell * 24, hipMemcpyHostToDevice, p->stream);
+        e = hipMalloc((void**)&du, (size_t)L.ncell_host * 24);
+        if (e == hipSuccess) e = hipMemcpyAsync(du, u, (size_t)L.ncell_host * 24, hipMemcpyHostToDevice, p->stream);
     }
     const double ux0 = u0 ? u0[0] : 0.0, uy0 = u0 ? u0[1] : 0.0, uz0 = u0 ? u0[2] : 0.0;
     if (e == hipSuccess)
-        e = launch_init_eq(p->storage, L.fA.p, L.ncell, drho, du, rho0, ux0, uy0, uz0, permute_of(p),
+        e = launch_init_eq(p->storage, L.fA.p, L.ncell, L.ncell_host, drho, du, rho0, ux0, uy0, uz0, permute_of(p),
                            p->wale ? (void*)L.velA.p : nullptr, p->stream);
     if (e == hipSuccess && L.fB.p) e = hipMemcpyAsync(L.fB.p, L.fA.p, L.fA.n, hipMemcpyDeviceToDevice, p->stream);
     if (e == hipSuccess) e = aa_prep(p, L);
@@ -685,6 +1228,7 @@ int oal_init_equilibrium(OalPlan* p, int level, const double* rho, const double*
     L.have_f = true;
     L.aa_phase = 0;
     L.force_valid = false;
+    p->coupled = false;
     return OAL_OK;
 }
 
@@ -699,6 +1243,13 @@ int oal_step(OalPlan* p, int n_steps, int32_t first_time_step, int flags, void* 
     if (n_steps < 0) return fail(OAL_ERR_ARG, "negative step count");
     OAL_HIP(hipSetDevice(p->device));
     hipStream_t s = pick_stream(p, stream);
+    if (p->multilevel) {
+        if (!p->coupled)
+            if (int rc = ml_couple(p, s)) return rc;
+        for (int i = 0; i < n_steps; ++i)
+            if (int rc = ml_step_rec(p, 0, first_time_step + i, flags, i == n_steps - 1, s)) return rc;
+        return OAL_OK;
+    }
     for (int i = 0; i < n_steps; ++i) {
         const int f = (i == n_steps - 1) ? flags : 0;
         if (int rc = level_step(p, 0, first_time_step + i, f, s)) return rc;
@@ -727,7 +1278,7 @@ int oal_mass(OalPlan* p, int level, double* out, void* stream) {
     if (int rc = require_plain_state(p, L)) return rc;
     OAL_HIP(hipSetDevice(p->device));
     hipStream_t s = pick_stream(p, stream);
-    OAL_HIP(launch_mass(p->storage, L.fA.p, L.solid.p, L.borg.p, L.bdom.p, L.doms.p, L.nb, permute_of(p), L.mpart.p, s));
+    OAL_HIP(launch_mass(p->storage, L.fA.p, L.solid.p, L.borg.p, L.bdom.p, L.bflags.p, L.doms.p, L.nb, permute_of(p), L.mpart.p, s));
     // the reduction walks blocks here: one "item" per block
     ReduceArgs r{L.mpart.p, L.dom_block0.p, L.red_out.p, L.n_domains, 4};
     OAL_HIP(launch_reduce(r, s));
@@ -743,11 +1294,11 @@ int oal_download_macro(OalPlan* p, int level, double* rho, double* u) {
     if (int rc = require_plain_state(p, L)) return rc;
     OAL_HIP(hipSetDevice(p->device));
     double* d = nullptr;
-    OAL_HIP(hipMalloc((void**)&d, (size_t)L.ncell * 32));
-    hipError_t e = launch_macro(p->storage, L.fA.p, d, d + L.ncell, L.ncell, permute_of(p), nullptr, p->stream);
+    OAL_HIP(hipMalloc((void**)&d, (size_t)L.ncell_host * 32));
+    hipError_t e = launch_macro(p->storage, L.fA.p, d, d + L.ncell_host, L.ncell, L.ncell_host, permute_of(p), nullptr, p->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
-    if (e == hipSuccess && rho) e = hipMemcpy(rho, d, (size_t)L.ncell * 8, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && u) e = hipMemcpy(u, d + L.ncell, (size_t)L.ncell * 24, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && rho) e = hipMemcpy(rho, d, (size_t)L.ncell_host * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && u) e = hipMemcpy(u, d + L.ncell_host, (size_t)L.ncell_host * 24, hipMemcpyDeviceToHost);
     (void)hipFree(d);
     if (e != hipSuccess) return fail(OAL_ERR_HIP, std::string("oal_download_macro: ") + hipGetErrorString(e));
     return OAL_OK;
@@ -760,7 +1311,7 @@ int oal_guard(OalPlan* p, int level, float rho_min, float u_max, int64_t* bad) {
     OAL_HIP(hipSetDevice(p->device));
     unsigned long long h = 0;
     OAL_HIP(hipMemsetAsync(L.guard.p, 0, sizeof(unsigned long long), p->stream));
-    OAL_HIP(launch_guard(p->storage, L.fA.p, L.solid.p, L.borg.p, L.bdom.p, L.doms.p, L.nb, rho_min, u_max, L.guard.p, p->stream));
+    OAL_HIP(launch_guard(p->storage, L.fA.p, L.solid.p, L.borg.p, L.bdom.p, L.bflags.p, L.doms.p, L.nb, rho_min, u_max, L.guard.p, p->stream));
     OAL_HIP(hipStreamSynchronize(p->stream));
     OAL_HIP(hipMemcpy(&h, L.guard.p, sizeof(h), hipMemcpyDeviceToHost));
     if (bad) *bad = (int64_t)h;

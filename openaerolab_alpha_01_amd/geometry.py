@@ -149,3 +149,92 @@ def channel_walls(grid: BlockGrid, axis: int = 1) -> np.ndarray:
     g = grid.cell_coords()
     m = (g[:, axis] == 0) | (g[:, axis] == grid.n[axis] - 1)
     return (m & grid.inside()).astype(np.uint8)
+
+
+def block_octree(coarse_extent, refined, periodic=(False, False, False), bs: int = BS) -> list[BlockGrid]:
+    """Levels of a block octree from the refined blocks of each level but the finest.
+
+    coarse_extent: level-0 extent in cells; level l has extent coarse_extent * 2^l.  refined[l] is a set of
+    0-based block coordinates of level l whose eight children exist on level l + 1 (level 0 holds every
+    block of its extent).  The sets are first closed for the nesting a refined plan needs: the blocks next
+    to a block of level l >= 1 exist on level l - 1's refined set's children, i.e. their parents are
+    refined too.  Returns the levels coarsest first, blocks in lexicographic order (z slowest), each with
+    its neighbour table and no obstacle cells.
+    """
+    ext0 = np.asarray(coarse_extent, dtype=np.int64)
+    if np.any(ext0 % bs):
+        raise ValueError("the coarse extent must be a multiple of the block size")
+    L = len(refined) + 1
+    nblk = [tuple(int(v) for v in (ext0 << l) // bs) for l in range(L)]
+    ref = [set(map(tuple, r)) for r in refined]
+    for l in range(L - 2, 0, -1):
+        extra = set()
+        for (x, y, z) in ref[l]:
+            for dz in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dx in (-1, 0, 1):
+                        n = [x + dx, y + dy, z + dz]
+                        ok = True
+                        for a in range(3):
+                            if periodic[a]:
+                                n[a] %= nblk[l][a]
+                            elif not 0 <= n[a] < nblk[l][a]:
+                                ok = False
+                        if ok:
+                            extra.add((n[0] // 2, n[1] // 2, n[2] // 2))
+        ref[l - 1] |= extra
+    # each finer level only refines blocks that exist on it
+    for l in range(1, L - 1):
+        ref[l] = {b for b in ref[l] if (b[0] // 2, b[1] // 2, b[2] // 2) in ref[l - 1]}
+    levels = []
+    for l in range(L):
+        if l == 0:
+            blocks = np.stack(np.meshgrid(*[np.arange(v) for v in nblk[0]], indexing="ij"), -1).reshape(-1, 3)
+        else:
+            par = np.array(sorted(ref[l - 1]), dtype=np.int64).reshape(-1, 3)
+            ch = np.array([(i, j, k) for k in (0, 1) for j in (0, 1) for i in (0, 1)], dtype=np.int64)
+            blocks = (2 * par[:, None, :] + ch[None, :, :]).reshape(-1, 3)
+        order = np.lexsort((blocks[:, 0], blocks[:, 1], blocks[:, 2]))     # z slowest, x fastest
+        coords = blocks[order].astype(np.int32) + 1
+        n = tuple(int(v) for v in ext0 << l)
+        table = build_neighbor_table(coords, nblk[l], tuple(bool(p) for p in periodic))
+        levels.append(BlockGrid(n=n, periodic=tuple(bool(p) for p in periodic), coords=coords, neighbor=table,
+                                bs=bs))
+    return levels
+
+
+def sphere_octree(coarse_extent, n_levels: int, center_finest, radius_finest: float, thresholds,
+                  periodic=(False, False, False), bs: int = BS) -> list[BlockGrid]:
+    """Block octree refined around a sphere by distance to its surface (block-AMR set-up).
+
+    coarse_extent: level-0 extent in cells; level l has extent coarse_extent * 2^l.  center_finest and
+    radius_finest are in cells of the finest level.  thresholds[l - 1] (finest cells) selects the
+    blocks of level l >= 1: a level-(l-1) block is refined when its box comes within thresholds[l - 1]
+    of the sphere's surface (or lies inside the sphere).  The refined sets are closed for nesting
+    (block_octree).  Returns the levels coarsest first, each with its sphere mask (cell centres j + 1/2
+    at that level's spacing).
+    """
+    L = n_levels
+    ext0 = np.asarray(coarse_extent, dtype=np.int64)
+    cf = np.asarray(center_finest, dtype=np.float64)
+    nblk = [tuple(int(v) for v in (ext0 << l) // bs) for l in range(L)]
+
+    def box_dist(coords_0b: np.ndarray, l: int) -> np.ndarray:
+        """distance (finest cells) from the sphere centre to the nearest point of each level-l block"""
+        sc = float(2 ** (L - 1 - l) * bs)            # block edge in finest cells
+        lo = coords_0b * sc
+        hi = lo + sc
+        q = np.clip(cf[None, :], lo, hi)
+        return np.sqrt(((q - cf[None, :]) ** 2).sum(1))
+
+    ref = []
+    for l in range(L - 1):
+        nb = nblk[l]
+        g = np.stack(np.meshgrid(*[np.arange(v) for v in nb], indexing="ij"), -1).reshape(-1, 3)
+        sel = box_dist(g, l) <= radius_finest + float(thresholds[l])
+        ref.append({tuple(v) for v in g[sel]})
+    levels = block_octree(coarse_extent, ref, periodic, bs)
+    for l, g in enumerate(levels):
+        s = 2.0 ** (L - 1 - l)
+        g.solid = sphere_mask(g, cf / s, radius_finest / s)
+    return levels

@@ -309,3 +309,21 @@ def uniform_plan(grid: BlockGrid, tau: float, storage: str = "f32", stream: str 
     dom = Domain(extent=grid.n, tau=tau, periodic=grid.periodic, u_inlet=u_inlet, body_force=tuple(body_force),
                  ymin_wall_bc=ymin_wall_bc, zwall_bc=zwall_bc)
     return Plan(Level(grid, [dom]), storage=storage, stream=stream, bb_mode=bb_mode, params=params, rounding=rounding)
+
+
+def acoustic_taus(tau_finest: float, n_levels: int) -> list[float]:
+    """Relaxation times of the levels (coarsest first) under acoustic scaling, dx and dt halving per level:
+    tau_l - 1/2 = (tau_finest - 1/2) / 2^(L-1-l) (src/octree_q19/src/config.jl:328-335)."""
+    return [0.5 + (tau_finest - 0.5) / 2.0 ** (n_levels - 1 - l) for l in range(n_levels)]
+
+
+def octree_plan(grids: list[BlockGrid], tau_finest: float, storage: str = "f32", params: StepParams | None = None,
+                u_inlet: float = 0.0, body_force=(0.0, 0.0, 0.0), rounding: str = "stochastic",
+                bb_mode: str = "halfway") -> Plan:
+    """A refined plan over the levels of a block octree (coarsest first, e.g. geometry.sphere_octree):
+    one domain per level with the acoustic-scaling relaxation times; the inlet velocity is the same in
+    every level's lattice units."""
+    taus = acoustic_taus(tau_finest, len(grids))
+    levels = [Level(g, [Domain(extent=g.n, tau=t, periodic=g.periodic, u_inlet=u_inlet, body_force=tuple(body_force))])
+              for g, t in zip(grids, taus)]
+    return Plan(levels, storage=storage, stream="ab", bb_mode=bb_mode, params=params, rounding=rounding)

@@ -294,6 +294,7 @@ struct OrGrid {
     int32_t n[3];             // domain extent in cells
     int32_t periodic[3];
     const uint8_t* solid;     // [nb*bs^3] or null
+    const uint8_t* active;    // [nb]: 0 = not stepped (covered or ghost block of a refined level), null = all
 };
 
 namespace {
@@ -772,6 +773,7 @@ void step_impl(const OrGrid& g, const OrParams& prm, const typename P::st* fin, 
 
 #pragma omp parallel for schedule(static)
     for (int b = 0; b < g.nb; ++b) {
+        if (g.active && !g.active[b]) continue;   // not stepped: its cells keep what the output held
         double fb[3] = {0.0, 0.0, 0.0};
         const int bx = g.bcoord[3 * b], by = g.bcoord[3 * b + 1], bz = g.bcoord[3 * b + 2];
         for (int z = 0; z < bs; ++z)

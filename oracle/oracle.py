@@ -61,6 +61,7 @@ class OrGrid(C.Structure):
     _fields_ = [
         ("bs", C.c_int32), ("nb", C.c_int32), ("bcoord", C.POINTER(C.c_int32)), ("nbr", C.POINTER(C.c_int32)),
         ("n", C.c_int32 * 3), ("periodic", C.c_int32 * 3), ("solid", C.POINTER(C.c_uint8)),
+        ("active", C.POINTER(C.c_uint8)),
     ]
 
 
@@ -148,6 +149,9 @@ class Grid:
         s.n[:] = [int(v) for v in g.n]
         s.periodic[:] = [1 if p else 0 for p in g.periodic]
         s.solid = None if self.solid is None else self.solid.ctypes.data_as(C.POINTER(C.c_uint8))
+        act = getattr(g, "active", None)
+        self.active = None if act is None else np.ascontiguousarray(act, dtype=np.uint8)
+        s.active = None if self.active is None else self.active.ctypes.data_as(C.POINTER(C.c_uint8))
         self.c = s
 
 
