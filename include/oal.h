@@ -235,7 +235,9 @@ int oal_sync(OalPlan* plan);
 /* Times n steps of one level on the plan's stream with HIP events: *ms_total is the time of the n
  * steps (all their launches), *ms_kernel the mean duration of the bulk launches of one step (the
  * regular and interior block lists, events on the plan's stream around them; the mean step time
- * when the level has no such blocks). */
+ * when the level has no such blocks).  level OAL_ALL_LEVELS times n steps of oal_step (time steps
+ * 0 .. n-1, every level, transfers and graph replay included); *ms_kernel is then the mean step time. */
+#define OAL_ALL_LEVELS (-1)
 int oal_time_steps(OalPlan* plan, int level, int n, int flags, double* ms_total, double* ms_kernel);
 
 /* Replaces: nothing.  Bandwidth of a streaming 16-byte device copy of `bytes` (read + write counted),
@@ -248,12 +250,18 @@ int oal_copy_bandwidth(int device, int64_t bytes, int reps, double* gbs);
  * pipelining; 2, 4, 8 or 16).  overlap: 1 = boundary and face launches on a second stream,
  * concurrent with the bulk launch; 0 = every launch in order on the plan's stream.  face_kernel
  * (pull/AB pattern, blocks on a domain face): 0 = the pipelined kernel applies the domain-face rules
- * itself; 1 = it skips the cells on the faces and a separate face-cell kernel steps them. */
+ * itself; 1 = it skips the cells on the faces and a separate face-cell kernel steps them.  graph:
+ * 1 = oal_step on the plan's own stream (stream NULL) replays a hipGraph of one (coarse) step,
+ * captured on first use for each arrangement of the levels' buffers, with the step index read from
+ * device memory (a step that collects the force runs as ordinary launches); 0 = the host issues
+ * every launch.  Refined plans start with graph 1, one-level plans with 0.  Setting the tuning drops
+ * the captured graphs. */
 typedef struct {
     int32_t pipe_planes;
     int32_t overlap;
     int32_t face_kernel;
-    int32_t reserved[5];
+    int32_t graph;
+    int32_t reserved[4];
 } OalTuning;
 
 /* number of launch shapes the autotuner times: pipe_planes {1, 2, 4, 8, 16} x overlap {1, 0} x

@@ -74,7 +74,10 @@ class OalLevelInfo(C.Structure):
 
 class OalTuning(C.Structure):
     _fields_ = [("pipe_planes", C.c_int32), ("overlap", C.c_int32), ("face_kernel", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("graph", C.c_int32), ("reserved", C.c_int32 * 4)]
+
+
+OAL_ALL_LEVELS = -1
 
 
 OAL_TUNE_SHAPES = 20

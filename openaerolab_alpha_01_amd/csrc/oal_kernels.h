@@ -72,9 +72,14 @@ struct StepArgs {
     uint32_t nb;                 // blocks
     uint32_t n_items;            // workgroups launched (nb * PLANES)
     int32_t time_step;
+    // a step replayed from a captured graph reads its index from device memory: ts_dev[0] * ts_mul + ts_add
+    // (the coarse step, 2^level, the sub-step within it); null: time_step
+    const int32_t* ts_dev;
+    int32_t ts_mul, ts_add;
     int32_t parity;              // AA pattern: 1 odd step (neighbour slots), 0 even step (own slots)
     int32_t pipe_planes;         // planes per workgroup of the pipelined interior kernel (1 = off; divides 16)
     CollParams cp;
+    __device__ __forceinline__ int32_t step() const { return ts_dev ? ts_dev[0] * ts_mul + ts_add : time_step; }
 };
 
 struct ReduceArgs {
@@ -90,6 +95,8 @@ struct ReduceArgs {
 // interior block lists), the kernels that carry most of the cells.
 hipError_t launch_step(int storage, int stream_mode, bool wale, const StepArgs& a, hipStream_t stream, hipStream_t side,
                        hipEvent_t bulk0 = nullptr, hipEvent_t bulk1 = nullptr);
+// p[0] = v, in stream order (the step index of a replayed step graph)
+hipError_t launch_set_i32(int32_t* p, int32_t v, hipStream_t stream);
 // per-storage step launchers (oal_step_<storage>.hip)
 hipError_t launch_step_f32(int stream_mode, bool wale, const StepArgs& a, hipStream_t stream, hipStream_t side,
                            hipEvent_t bulk0, hipEvent_t bulk1);

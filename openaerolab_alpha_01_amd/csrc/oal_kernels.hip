@@ -444,6 +444,15 @@ __global__ void __launch_bounds__(WG) k_xfer_rebuild(typename SP::st* f, uint32_
     });
 }
 
+__global__ void k_set_i32(int32_t* p, int32_t v) {
+    if (threadIdx.x == 0) p[0] = v;
+}
+
+hipError_t launch_set_i32(int32_t* p, int32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(64), 0, s, p, v);
+    return hipGetLastError();
+}
+
 template <class SP>
 __global__ void __launch_bounds__(WG) k_copy_cells(typename SP::st* dst, const typename SP::st* src, const uint32_t* cells,
                                                    uint32_t n, uint32_t ncell) {

@@ -125,6 +125,13 @@ struct OalPlan {
     bool multilevel = false;                // a refined plan: levels coupled by ghost fill and restriction
     bool coupled = false;                   // the initial restriction and fill have run since the last upload
     std::vector<Level> levels;
+    // captured steps: oal_step replays a hipGraph of one (coarse) step; the step index comes from ts_dev
+    bool graph = false;
+    bool capturing = false;
+    int32_t* ts_dev = nullptr;
+    struct LevelState { const void* fA; const void* velA; int aa_phase; };
+    struct StepGraph { hipGraphExec_t exec = nullptr; std::vector<LevelState> before, after; };
+    std::vector<StepGraph> graphs;          // one per buffer arrangement met so far
 };
 
 namespace {
@@ -807,6 +814,16 @@ StepArgs make_args(OalPlan* p, Level& L, int32_t time_step, bool want_force) {
     a.nb = L.nb;
     a.n_items = L.n_items;
     a.time_step = time_step;
+    if (p->capturing) {
+        // replayed steps: the coarse step from device memory, times 2^level, plus the sub-step
+        a.ts_dev = p->ts_dev;
+        a.ts_mul = 1 << (int)(&L - p->levels.data());
+        a.ts_add = time_step;
+    } else {
+        a.ts_dev = nullptr;
+        a.ts_mul = 0;
+        a.ts_add = 0;
+    }
     a.parity = p->stream_mode == OAL_STREAM_AA ? (L.aa_phase == 0 ? 1 : 0) : 0;
     a.pipe_planes = p->pipe_planes;
     a.cp = p->cp;
@@ -916,6 +933,78 @@ int ml_step_rec(OalPlan* p, int l, int32_t t, int flags, bool last, hipStream_t 
         if (int rc = ml_restrict(p, l, s)) return rc;
         if (int rc = ml_fill(p, l + 1, 1.0, s)) return rc;
     }
+    return OAL_OK;
+}
+
+// ---- captured steps ----
+// The host state a step changes: which buffer of each level is read next, the AA phase.
+std::vector<OalPlan::LevelState> level_states(const OalPlan* p) {
+    std::vector<OalPlan::LevelState> v;
+    for (const Level& L : p->levels) v.push_back({L.fA.p, L.velA.p, L.aa_phase});
+    return v;
+}
+
+void set_level_states(OalPlan* p, const std::vector<OalPlan::LevelState>& v) {
+    for (size_t l = 0; l < v.size(); ++l) {
+        Level& L = p->levels[l];
+        if (L.fA.p != v[l].fA) std::swap(L.fA, L.fB);
+        if (L.velA.p != v[l].velA) std::swap(L.velA, L.velB);
+        L.aa_phase = v[l].aa_phase;
+        L.force_valid = false;
+    }
+}
+
+bool same_states(const std::vector<OalPlan::LevelState>& a, const std::vector<OalPlan::LevelState>& b) {
+    if (a.size() != b.size()) return false;
+    for (size_t l = 0; l < a.size(); ++l)
+        if (a[l].fA != b[l].fA || a[l].velA != b[l].velA || a[l].aa_phase != b[l].aa_phase) return false;
+    return true;
+}
+
+void drop_graphs(OalPlan* p) {
+    for (auto& g : p->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    p->graphs.clear();
+}
+
+// Capture one (coarse) step from the current buffer arrangement into a graph; the host state is left
+// as it was (the capture only records launches).
+int capture_step(OalPlan* p, OalPlan::StepGraph& g) {
+    g.before = level_states(p);
+    hipStream_t s = p->stream;
+    OAL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    p->capturing = true;
+    int rc = p->multilevel ? ml_step_rec(p, 0, 0, 0, false, s) : level_step(p, 0, 0, 0, s);
+    p->capturing = false;
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    g.after = level_states(p);
+    set_level_states(p, g.before);
+    if (rc == OAL_OK && e != hipSuccess) rc = fail(OAL_ERR_HIP, std::string("step capture: ") + hipGetErrorString(e));
+    if (rc == OAL_OK) {
+        const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        if (ei != hipSuccess) rc = fail(OAL_ERR_HIP, std::string("step graph instantiation: ") + hipGetErrorString(ei));
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+}
+
+// One (coarse) step with index t by graph replay.
+int replay_step(OalPlan* p, int32_t t) {
+    const auto now = level_states(p);
+    OalPlan::StepGraph* g = nullptr;
+    for (auto& c : p->graphs)
+        if (same_states(c.before, now)) g = &c;
+    if (!g) {
+        if (p->graphs.size() >= 4) drop_graphs(p);
+        OalPlan::StepGraph ng;
+        if (int rc = capture_step(p, ng)) return rc;
+        p->graphs.push_back(ng);
+        g = &p->graphs.back();
+    }
+    OAL_HIP(launch_set_i32(p->ts_dev, t, p->stream));
+    OAL_HIP(hipGraphLaunch(g->exec, p->stream));
+    set_level_states(p, g->after);
     return OAL_OK;
 }
 
@@ -1088,8 +1177,10 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
         (e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&p->ev0)) != hipSuccess || (e = hipEventCreate(&p->ev1)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&p->join, hipEventDisableTiming)) != hipSuccess)
+        (e = hipEventCreateWithFlags(&p->join, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipMalloc((void**)&p->ts_dev, 256)) != hipSuccess)
         return bail(fail(OAL_ERR_HIP, std::string("stream/event creation: ") + hipGetErrorString(e)));
+    p->graph = ml;
     for (int l = 0; l < desc->n_levels; ++l) {
         const int rc = upload_level(p, p->levels[l], host[l]);
         if (rc) return bail(rc);
@@ -1119,6 +1210,8 @@ int oal_plan_destroy(OalPlan* p) {
         (void)hipSetDevice(p->device);
         (void)hipStreamSynchronize(p->stream);
     }
+    drop_graphs(p);
+    if (p->ts_dev) (void)hipFree(p->ts_dev);
     for (auto& L : p->levels) L.release();
     if (p->side) (void)hipStreamSynchronize(p->side);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
@@ -1243,16 +1336,20 @@ int oal_step(OalPlan* p, int n_steps, int32_t first_time_step, int flags, void* 
     if (n_steps < 0) return fail(OAL_ERR_ARG, "negative step count");
     OAL_HIP(hipSetDevice(p->device));
     hipStream_t s = pick_stream(p, stream);
-    if (p->multilevel) {
-        if (!p->coupled)
-            if (int rc = ml_couple(p, s)) return rc;
-        for (int i = 0; i < n_steps; ++i)
-            if (int rc = ml_step_rec(p, 0, first_time_step + i, flags, i == n_steps - 1, s)) return rc;
-        return OAL_OK;
-    }
+    if (p->multilevel && !p->coupled)
+        if (int rc = ml_couple(p, s)) return rc;
     for (int i = 0; i < n_steps; ++i) {
-        const int f = (i == n_steps - 1) ? flags : 0;
-        if (int rc = level_step(p, 0, first_time_step + i, f, s)) return rc;
+        const bool last = i == n_steps - 1;
+        const int f = last ? flags : 0;
+        if (p->graph && s == p->stream && !(f & OAL_STEP_FORCE)) {
+            for (const Level& L : p->levels)
+                if (!L.have_f) return fail(OAL_ERR_STATE, "no populations: call oal_upload_f or oal_init_equilibrium first");
+            if (int rc = replay_step(p, first_time_step + i)) return rc;
+        } else if (p->multilevel) {
+            if (int rc = ml_step_rec(p, 0, first_time_step + i, flags, last, s)) return rc;
+        } else {
+            if (int rc = level_step(p, 0, first_time_step + i, f, s)) return rc;
+        }
     }
     return OAL_OK;
 }
@@ -1326,6 +1423,22 @@ int oal_sync(OalPlan* p) {
 }
 
 int oal_time_steps(OalPlan* p, int level, int n, int flags, double* ms_total, double* ms_kernel) {
+    if (p && level == OAL_ALL_LEVELS) {
+        if (n <= 0) return fail(OAL_ERR_ARG, "n must be positive");
+        OAL_HIP(hipSetDevice(p->device));
+        if (p->multilevel && !p->coupled)
+            if (int rc = ml_couple(p, p->stream)) return rc;
+        OAL_HIP(hipStreamSynchronize(p->stream));
+        OAL_HIP(hipEventRecord(p->ev0, p->stream));
+        if (int rc = oal_step(p, n, 0, flags, nullptr)) return rc;
+        OAL_HIP(hipEventRecord(p->ev1, p->stream));
+        OAL_HIP(hipEventSynchronize(p->ev1));
+        float ms = 0.0f;
+        OAL_HIP(hipEventElapsedTime(&ms, p->ev0, p->ev1));
+        if (ms_total) *ms_total = ms;
+        if (ms_kernel) *ms_kernel = (double)ms / n;
+        return OAL_OK;
+    }
     if (int rc = check_level(p, level)) return rc;
     if (n <= 0) return fail(OAL_ERR_ARG, "n must be positive");
     OAL_HIP(hipSetDevice(p->device));
@@ -1395,6 +1508,7 @@ int oal_get_tuning(const OalPlan* p, OalTuning* t) {
     t->pipe_planes = p->pipe_planes;
     t->overlap = p->overlap ? 1 : 0;
     t->face_kernel = p->face_kernel ? 1 : 0;
+    t->graph = p->graph ? 1 : 0;
     return OAL_OK;
 }
 
@@ -1404,9 +1518,14 @@ int oal_set_tuning(OalPlan* p, const OalTuning* t) {
     if (np != 1 && np != 2 && np != 4 && np != 8 && np != 16) return fail(OAL_ERR_ARG, "pipe_planes must be 1, 2, 4, 8 or 16");
     if (t->overlap != 0 && t->overlap != 1) return fail(OAL_ERR_ARG, "overlap must be 0 or 1");
     if (t->face_kernel != 0 && t->face_kernel != 1) return fail(OAL_ERR_ARG, "face_kernel must be 0 or 1");
+    if (t->graph != 0 && t->graph != 1) return fail(OAL_ERR_ARG, "graph must be 0 or 1");
+    OAL_HIP(hipSetDevice(p->device));
+    OAL_HIP(hipStreamSynchronize(p->stream));
+    drop_graphs(p);
     p->pipe_planes = np;
     p->overlap = t->overlap != 0;
     p->face_kernel = t->face_kernel != 0;
+    p->graph = t->graph != 0;
     return OAL_OK;
 }
 
@@ -1426,6 +1545,8 @@ int oal_autotune(OalPlan* p, int level, int n_steps, OalTuning* best, double* ms
         if (e == hipSuccess) e = hipMemcpyAsync(vsave, L.velA.p, L.velA.n, hipMemcpyDeviceToDevice, p->stream);
     }
     const int aa_phase = L.aa_phase;
+    OAL_HIP(hipStreamSynchronize(p->stream));
+    drop_graphs(p);
     const int cand[5] = {1, 2, 4, 8, 16};
     int bestc = p->pipe_planes;
     bool besto = p->overlap, bestf = p->face_kernel;
@@ -1462,6 +1583,7 @@ int oal_autotune(OalPlan* p, int level, int n_steps, OalTuning* best, double* ms
         best->pipe_planes = bestc;
         best->overlap = besto ? 1 : 0;
         best->face_kernel = bestf ? 1 : 0;
+        best->graph = p->graph ? 1 : 0;
     }
     return rc;
 }

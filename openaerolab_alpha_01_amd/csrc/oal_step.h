@@ -325,7 +325,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
             bool inlet = false;
             if constexpr (GEN && dcx(K) > 0) inlet = (c.flags & BF_EDGE) && !c.per0 && c.gx == 0;
             if (inlet)
-                val = boundary_value<K, T>(SRC_INLET, a, c, u_in, a.time_step, [&](auto) { return T(0); });
+                val = boundary_value<K, T>(SRC_INLET, a, c, u_in, a.step(), [&](auto) { return T(0); });
             else
                 val = SP::ld(fi + (size_t)K * n + c.own, w);
         } else {
@@ -342,7 +342,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
                 // the west neighbour overwrites during this step; the even step saved a copy
                 val = SP::ld((const st*)a.obuf + ((size_t)a.oslot[b] * 9 + K / 3) * BS2 + c.ly * BS + c.z, w);
             } else {
-                val = boundary_value<K, T>(kind, a, c, u_in, a.time_step,
+                val = boundary_value<K, T>(kind, a, c, u_in, a.step(),
                                            [&](auto jc) { return own_prev<SP, PAT, decltype(jc)::value>(a, c); });
             }
         }
@@ -385,7 +385,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
     // rounding of the stored post-collision populations: population j of this cell takes the random
     // bits of (cell, j, step) whichever slot it goes to, so every pattern stores the same values
     const bool dith = SP::dithered && a.cp.dither;
-    const uint32_t hc = SP::dithered ? round_cell(round_key(a.time_step), c.own - dm->cell0) : 0u;
+    const uint32_t hc = SP::dithered ? round_cell(round_key(a.step()), c.own - dm->cell0) : 0u;
     auto rb = [&](int j) { return round_bits(hc, j); };
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
@@ -407,7 +407,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
             if (kind == SRC_REG) {
                 SP::sto_r((st*)(ra.base(fo, n, K, c.z, -dcz(K)) + ra.v[1 - dcx(K)][1 - dcy(K)]), f[K], wK, rb(K), dith);
             } else if (kind != SRC_INLET) {
-                const T v = boundary_value<J, T>(kind, a, c, u_in, a.time_step + 1,
+                const T v = boundary_value<J, T>(kind, a, c, u_in, a.step() + 1,
                                                  [&](auto jc) { return f[decltype(jc)::value]; });
                 SP::sto_r(fo + (size_t)J * n + c.own, v, wgt<T>(J), rb(rule_dir<J>(kind)), dith);
             }
@@ -425,7 +425,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
             if (kind == SRC_REG) {
                 SP::sto_r(fo + (size_t)K * n + dst, f[K], wK, rb(K), dith);
             } else if (kind != SRC_INLET) {   // inlet links are evaluated by the even step itself
-                const T v = boundary_value<J, T>(kind, a, c, u_in, a.time_step + 1,
+                const T v = boundary_value<J, T>(kind, a, c, u_in, a.step() + 1,
                                                  [&](auto jc) { return f[decltype(jc)::value]; });
                 SP::sto_r(fo + (size_t)J * n + c.own, v, wgt<T>(J), rb(rule_dir<J>(kind)), dith);
             }
@@ -549,13 +549,13 @@ __device__ __forceinline__ void reg_collide_store(const StepArgs& a, const RegAd
         for_dirs([&](auto kc) {
             constexpr int K = decltype(kc)::value;
             if constexpr (dcx(K) > 0)
-                f[K] = boundary_value<K, T>(SRC_INLET, a, c, inlet->u_in, a.time_step, [&](auto) { return T(0); });
+                f[K] = boundary_value<K, T>(SRC_INLET, a, c, inlet->u_in, a.step(), [&](auto) { return T(0); });
         });
     }
     T rho_o, ux_o, uy_o, uz_o;
     collide_cell<T>(f, q, a.cp, rho_o, ux_o, uy_o, uz_o);
     const bool dith = SP::dithered && a.cp.dither;
-    const uint32_t hc = SP::dithered ? round_cell(round_key(a.time_step), cell) : 0u;
+    const uint32_t hc = SP::dithered ? round_cell(round_key(a.step()), cell) : 0u;
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
         SP::sto_r((st*)(ra.base(a.fout, a.ncell, K, z, 0) + ra.v[1][1]), f[K], wgt<T>(K), round_bits(hc, K), dith);

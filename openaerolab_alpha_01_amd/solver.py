@@ -219,7 +219,9 @@ class Plan:
         return {name: getattr(info, name) for name, _ in _lib.OalLevelInfo._fields_ if name != "reserved"}
 
     def time_steps(self, n: int, level: int = 0, force: bool = False) -> tuple[float, float]:
-        """Times n steps with HIP events: (ms for all n steps, mean ms of one step's bulk launches)."""
+        """Times n steps with HIP events: (ms for all n steps, mean ms of one step's bulk launches).
+        level None: n whole (coarse) steps of every level through oal_step: (ms, mean ms per step)."""
+        level = _lib.OAL_ALL_LEVELS if level is None else level
         tot, ker = C.c_double(), C.c_double()
         check(self.lib.oal_time_steps(self.h, level, n, _lib.OAL_STEP_FORCE if force else 0, C.byref(tot),
                                       C.byref(ker)), "oal_time_steps")
@@ -228,18 +230,20 @@ class Plan:
     # -- launch shape ------------------------------------------------------------------------------
     @property
     def tuning(self) -> dict:
-        """Launch shape: pipe_planes (1, 2, 4, 8, 16), overlap (0 or 1), face_kernel (0 or 1)."""
+        """Launch shape: pipe_planes (1, 2, 4, 8, 16), overlap (0 or 1), face_kernel (0 or 1), graph (0 or 1:
+        step() replays a captured graph of one coarse step)."""
         t = _lib.OalTuning()
         check(self.lib.oal_get_tuning(self.h, C.byref(t)), "oal_get_tuning")
-        return {"pipe_planes": t.pipe_planes, "overlap": t.overlap, "face_kernel": t.face_kernel}
+        return {"pipe_planes": t.pipe_planes, "overlap": t.overlap, "face_kernel": t.face_kernel, "graph": t.graph}
 
     def set_tuning(self, pipe_planes: int | None = None, overlap: int | None = None,
-                   face_kernel: int | None = None) -> None:
+                   face_kernel: int | None = None, graph: int | None = None) -> None:
         cur = self.tuning
         t = _lib.OalTuning()
         t.pipe_planes = int(cur["pipe_planes"] if pipe_planes is None else pipe_planes)
         t.overlap = int(cur["overlap"] if overlap is None else overlap)
         t.face_kernel = int(cur["face_kernel"] if face_kernel is None else face_kernel)
+        t.graph = int(cur["graph"] if graph is None else graph)
         check(self.lib.oal_set_tuning(self.h, C.byref(t)), "oal_set_tuning")
 
     @property
@@ -256,7 +260,8 @@ class Plan:
         best = _lib.OalTuning()
         ms = (C.c_double * _lib.OAL_TUNE_SHAPES)()
         check(self.lib.oal_autotune(self.h, level, n_steps, C.byref(best), ms), "oal_autotune")
-        choice = {"pipe_planes": best.pipe_planes, "overlap": best.overlap, "face_kernel": best.face_kernel}
+        choice = {"pipe_planes": best.pipe_planes, "overlap": best.overlap, "face_kernel": best.face_kernel,
+                  "graph": best.graph}
         return choice, dict(zip(_lib.TUNE_KEYS, list(ms)))
 
     # -- results -----------------------------------------------------------------------------------

@@ -132,3 +132,71 @@ def test_octree_level_info(octree):
             assert info["n_cells"] == g.n_cells
             assert info["n_ghost_blocks"] == int((A.status == octree.GHOST).sum())
             assert info["n_covered_blocks"] == int((A.status == octree.COVERED).sum())
+
+
+def _states(plan, n_levels):
+    return [plan.download_f(level=l, dtype=np.float64) for l in range(n_levels)]
+
+
+@pytest.mark.parametrize("case,storage,overlap", [("sphere_channel", "f16s", 1), ("three_levels", "bf16s", 0),
+                                                  ("sphere_channel", "f32", 0)])
+def test_graph_replay_matches_eager(case, storage, overlap):
+    """oal_step replaying captured coarse-step graphs gives the eager launches' bits, call after call
+    (with downloads and odd step counts between the replays, so both buffer arrangements of level 0
+    are captured), and the force of a last step that collects it."""
+    cfg = CASES[case]
+    grids = cfg["grids"]()
+    fields = [level_fields(g, 3 + l, cfg["u0"]) for l, g in enumerate(grids)]
+    params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
+    plans = []
+    for graph in (1, 0):
+        p = solver.octree_plan(grids, cfg["tau"], storage=storage, params=params, u_inlet=cfg["u_inlet"])
+        p.set_tuning(graph=graph, overlap=overlap)
+        for l, (rho, u) in enumerate(fields):
+            p.init_equilibrium(rho, u, level=l)
+        plans.append(p)
+    g, e = plans
+    assert g.tuning["graph"] == 1 and e.tuning["graph"] == 0
+    t = 0
+    for n in (2, 3, 1, 2):
+        print(f"graph replay {case} {storage} overlap {overlap}: steps {t}..{t + n - 1}", flush=True)
+        for p in plans:
+            p.step(n, first_time_step=t)
+            p.sync()
+        t += n
+        for a, b in zip(_states(g, len(grids)), _states(e, len(grids))):
+            np.testing.assert_array_equal(a, b)
+    if grids[-1].solid is not None:
+        for p in plans:
+            p.step(2, first_time_step=t, force=True)
+            p.sync()
+        np.testing.assert_array_equal(g.force(level=len(grids) - 1), e.force(level=len(grids) - 1))
+        for a, b in zip(_states(g, len(grids)), _states(e, len(grids))):
+            np.testing.assert_array_equal(a, b)
+    for p in plans:
+        p.close()
+
+
+def test_graph_replay_one_level():
+    """a one-level plan with the graph switched on: same bits as the eager launches (FP16 stochastic
+    rounding keys on the replayed step index)"""
+    from openaerolab_alpha_01_amd.geometry import sphere_mask, uniform_grid
+    grid = uniform_grid((80, 48, 48))
+    grid.solid = sphere_mask(grid, (20.0, 24.0, 24.0), 5.0)
+    rho, u = level_fields(grid, 11, (0.05, 0.0, 0.0))
+    plans = []
+    for graph in (1, 0):
+        p = solver.uniform_plan(grid, 0.53, storage="f16s", u_inlet=0.05,
+                                params=solver.StepParams(c_wale=0.0, inlet_turbulence=0.0))
+        p.set_tuning(graph=graph)
+        p.init_equilibrium(rho, u)
+        plans.append(p)
+    t = 0
+    for n in (3, 2, 4):
+        for p in plans:
+            p.step(n, first_time_step=t)
+            p.sync()
+        t += n
+        np.testing.assert_array_equal(plans[0].download_f(), plans[1].download_f())
+    for p in plans:
+        p.close()

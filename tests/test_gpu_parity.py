@@ -313,7 +313,7 @@ def test_interior_pipeline_shapes(oracle, storage, planes, face_kernel):
     steps = 3
     f_or, _, _ = oracle_run(oracle, grid, storage, steps, okw, rho, u)
     plan = gpu_plan(grid, storage, okw)
-    shape = {"pipe_planes": planes, "overlap": int(planes % 4 == 0), "face_kernel": face_kernel}
+    shape = {"pipe_planes": planes, "overlap": int(planes % 4 == 0), "face_kernel": face_kernel, "graph": 0}
     plan.set_tuning(**shape)
     assert plan.tuning == shape
     plan.init_equilibrium(rho, u)
