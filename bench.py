@@ -179,6 +179,152 @@ def validation(steps: int, seed: int, chunk: int = 1000) -> dict:
     return out
 
 
+# ---- block-AMR form: octree of 16^3 blocks around a sphere -------------------------------------------
+AMR_COARSE = (64, 32, 32)          # level-0 extent; finest-equivalent 512 x 256 x 256
+AMR_LEVELS = 4                     # coarse + 3 refinement levels
+AMR_CENTER = (192.0, 128.0, 128.0)  # finest cells
+AMR_D = 48                         # sphere diameter, finest cells
+AMR_THRESHOLDS = (64, 24, 8)       # distance to the surface (finest cells) refining into level 1, 2, 3
+
+
+def amr_grids():
+    from openaerolab_alpha_01_amd.geometry import sphere_octree
+    return sphere_octree(AMR_COARSE, AMR_LEVELS, AMR_CENTER, AMR_D / 2, AMR_THRESHOLDS)
+
+
+def amr_tau_finest():
+    return 0.5 + 3.0 * U_IN * AMR_D / RE
+
+
+def stepped_masks(grids):
+    """per level, cell mask of the caller's blocks that are stepped (blocks whose children exist are not)"""
+    out = []
+    for l, g in enumerate(grids):
+        blk = np.ones(g.n_blocks, dtype=bool)
+        if l + 1 < len(grids):
+            fine = {tuple(v) for v in (np.asarray(grids[l + 1].coords) - 1).tolist()}
+            c0 = (np.asarray(g.coords, dtype=np.int64) - 1) * 2
+            blk = np.array([tuple(c) not in fine for c in c0.tolist()])
+        out.append(np.repeat(blk, 16 ** 3))
+    return out
+
+
+def amr_plan(grids, storage, u0_levels=None):
+    from openaerolab_alpha_01_amd import solver
+    params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
+    plan = solver.octree_plan(grids, amr_tau_finest(), storage=storage, params=params, u_inlet=U_IN)
+    for l, g in enumerate(grids):
+        if u0_levels is None:
+            plan.init_equilibrium(None, None, 1.0, (U_IN, 0.0, 0.0), level=l)
+        else:
+            plan.init_equilibrium(np.ones(g.n_cells), u0_levels[l], level=l)
+    return plan
+
+
+def amr_leg(storage: str, steps: int, warmup: int, autotune: bool, graph: int):
+    grids = amr_grids()
+    plan = amr_plan(grids, storage)
+    L = len(grids)
+    tuning = None
+    if autotune:
+        best, _ = plan.autotune(6, level=L - 1)      # the finest level carries most of the work
+        tuning = best
+    plan.set_tuning(graph=graph)
+    tuning = plan.tuning
+    if warmup:
+        plan.time_steps(warmup, level=None)
+    plan.sync()
+    t0 = time.perf_counter()
+    ms_total, _ = plan.time_steps(steps, level=None)
+    wall = time.perf_counter() - t0
+    plan.step(1, steps + warmup, force=True)
+    F = plan.force(level=L - 1)[0]
+    bad = sum(plan.guard(0.2, 0.5, level=l) for l in range(L))
+    infos = [plan.level_info(l) for l in range(L)]
+    plan.close()
+    masks = stepped_masks(grids)
+    updates = sum(int(m.sum()) * 2 ** l for l, m in enumerate(masks))
+    fe = int(np.prod(np.asarray(AMR_COARSE) << (L - 1))) * 2 ** (L - 1)
+    return {"storage": storage, "graph": graph, "ms_per_coarse_step": round(ms_total / steps, 4),
+            "mlups": round(updates * steps / (ms_total * 1e-3) / 1e6, 1),
+            "finest_equivalent_mlups": round(fe * steps / (ms_total * 1e-3) / 1e6, 1),
+            "cell_updates_per_coarse_step": updates, "wall_s": round(wall, 3), "tuning": tuning,
+            "levels": [{k: i[k] for k in ("n_blocks", "n_covered_blocks", "n_ghost_blocks")} for i in infos],
+            "cd_transient": round(float(F[0] / (0.5 * U_IN ** 2 * np.pi * (AMR_D / 2) ** 2)), 5), "guard_bad": bad}
+
+
+def amr_validation(steps: int, seed: int, chunk: int = 250) -> dict:
+    """Cd and L2(u) of the refined run in FP32 / FP16 / BF16 storage against the FP64 run of the same
+    kernels, from the same start: u = (u_in, 0, 0) plus a seeded uniform [-1e-3, 1e-3] perturbation per
+    component (numpy PCG64, `seed`, one stream per level).  steps are coarse steps (8 finest steps each);
+    Cd from the finest level's momentum exchange, L2 over the stepped cells of every level."""
+    grids = amr_grids()
+    masks = stepped_masks(grids)
+    area = np.pi * (AMR_D / 2) ** 2
+    u0 = []
+    for l, g in enumerate(grids):
+        u = np.zeros((3, g.n_cells))
+        u[0] = U_IN
+        u += np.random.default_rng(seed + l).uniform(-1e-3, 1e-3, size=(3, g.n_cells))
+        if g.solid is not None:
+            u[:, np.asarray(g.solid) != 0] = 0.0
+        u0.append(u)
+    fluid = [m & (np.asarray(g.solid) == 0) for m, g in zip(masks, grids)]
+    out = {"coarse_steps": steps, "finest_steps": steps * 2 ** (len(grids) - 1), "seed": seed,
+           "cd_reference_area": round(area, 4)}
+    ref = None
+    for st in ("f64", "f32", "f16s", "bf16s"):
+        t0 = time.perf_counter()
+        plan = amr_plan(grids, st, u0)
+        t = 0
+        while t < steps - 1:
+            k = min(chunk, steps - 1 - t)
+            plan.step(k, t)
+            plan.sync()
+            t += k
+            print(f"[validate-amr] {st} coarse step {t}/{steps} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
+                  flush=True)
+        plan.step(1, t, force=True)
+        F = plan.force(level=len(grids) - 1)[0]
+        us = np.concatenate([plan.macro(level=l)[1][:, fluid[l]] for l in range(len(grids))], axis=1)
+        bad = sum(plan.guard(0.2, 0.5, level=l) for l in range(len(grids)))
+        plan.close()
+        cd = float(F[0] / (0.5 * U_IN ** 2 * area))
+        entry = {"cd": round(cd, 6), "guard_bad": bad}
+        if ref is None:
+            ref = (cd, us)
+        else:
+            entry["cd_rel_err_vs_f64"] = float((cd - ref[0]) / ref[0])
+            entry["l2_u_vs_f64"] = float(np.linalg.norm(us - ref[1]) / np.linalg.norm(ref[1]))
+        out[st] = entry
+    return out
+
+
+def amr_main(args) -> int:
+    """The block-AMR form: one JSON line (finest-equivalent MLUPS of FP16 storage, graph replay)."""
+    if args.validate:
+        print(json.dumps({"validation_amr": amr_validation(args.val_steps, args.seed)}), flush=True)
+        return 0
+    legs = []
+    for item in args.amr_legs.split(","):
+        st, gr = item.split(":")
+        legs.append(amr_leg(st, args.steps, args.warmup, not args.no_autotune, int(gr)))
+        print(f"[amr] leg {item}: {legs[-1]['ms_per_coarse_step']} ms per coarse step", file=sys.stderr, flush=True)
+    head = legs[0]
+    out = {"metric": "finest-equivalent MLUPS (D3Q27 cumulant block-AMR, FP16 shifted storage, FP32 compute)",
+           "value": head["finest_equivalent_mlups"], "unit": "MLUPS", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": head["ms_per_coarse_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f16-storage/f32-compute", "data": "synthetic",
+           "config": {"workload": f"block octree of 16^3 blocks, {AMR_LEVELS} levels (coarse "
+                                  f"{'x'.join(map(str, AMR_COARSE))}, finest-equivalent "
+                                  f"{'x'.join(str(v << (AMR_LEVELS - 1)) for v in AMR_COARSE)}), sphere D={AMR_D} finest "
+                                  f"cells at {AMR_CENTER}, refinement thresholds {AMR_THRESHOLDS} finest cells, "
+                                  f"acoustic sub-cycling, inlet ux={U_IN}, Re={RE:g}", "parallelism": "1 GPU"},
+           "legs": legs}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +346,11 @@ def main(argv=None) -> int:
                     help="accuracy run instead of the benchmark: Cd and L2(u) of FP32 / FP16 / BF16 vs FP64")
     ap.add_argument("--val-steps", type=int, default=20000)
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--amr", action="store_true",
+                    help="the block-AMR form (4-level sphere octree) instead of the uniform box; with --validate "
+                         "its accuracy run (--val-steps counts coarse steps)")
+    ap.add_argument("--amr-legs", default="f16s:1,f16s:0,f32:1",
+                    help="block-AMR legs timed, storage:graph comma-separated; the first is the reported value")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -209,6 +360,8 @@ def main(argv=None) -> int:
                                ("face_kernel", args.face_kernel)) if v is not None}
     if fixed:
         args.no_autotune = True
+    if args.amr:
+        return amr_main(args)
     if args.validate:
         print(json.dumps({"validation": validation(args.val_steps, args.seed)}), flush=True)
         return 0

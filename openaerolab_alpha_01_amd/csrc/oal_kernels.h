@@ -95,6 +95,9 @@ struct ReduceArgs {
 // interior block lists), the kernels that carry most of the cells.
 hipError_t launch_step(int storage, int stream_mode, bool wale, const StepArgs& a, hipStream_t stream, hipStream_t side,
                        hipEvent_t bulk0 = nullptr, hipEvent_t bulk1 = nullptr);
+// values per gathered donor record of the level transfers: 27 populations padded to 28 (16-byte rows)
+constexpr int XFER_STRIDE = 28;
+
 // p[0] = v, in stream order (the step index of a replayed step graph)
 hipError_t launch_set_i32(int32_t* p, int32_t v, hipStream_t stream);
 // per-storage step launchers (oal_step_<storage>.hip)

@@ -355,6 +355,17 @@ hipError_t launch_guard(int storage, const void* f, const uint8_t* solid, const 
 // of its weight takes the equilibrium of its rho and u (kernel.jl:94-100).  Every operation is written
 // out in a fixed order: the CPU oracle restates the same sequence and must agree bit for bit.
 // ------------------------------------------------------------------------------------------------
+// 16-byte vectors of the working type (records of the level transfers)
+template <class T> struct Vec16;
+template <> struct Vec16<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+    static constexpr int n = 4;
+};
+template <> struct Vec16<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+    static constexpr int n = 2;
+};
+
 template <class SP>
 __global__ void __launch_bounds__(WG) k_xfer_gather(const typename SP::st* fnew, const typename SP::st* fold,
                                                     const uint32_t* cells, uint32_t n, uint32_t ncell, typename SP::T w1,
@@ -366,12 +377,25 @@ __global__ void __launch_bounds__(WG) k_xfer_gather(const typename SP::st* fnew,
     const uint32_t c = e & 0x7fffffffu;
     const bool blend = fold && !(e >> 31);
     const T w0 = T(1) - w1;
+    T vals[XFER_STRIDE];
+    vals[XFER_STRIDE - 1] = T(0);
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
         T v = SP::ld(fnew + (size_t)K * ncell + c, wgt<T>(K));
         if (blend) v = SP::ld(fold + (size_t)K * ncell + c, wgt<T>(K)) * w0 + v * w1;
-        out[(size_t)i * 27 + K] = v;
+        vals[K] = v;
     });
+    // one 16-byte-aligned record per donor, written as whole 16-byte vectors
+    using V = typename Vec16<T>::type;
+    constexpr int VN = Vec16<T>::n;
+    V* o = reinterpret_cast<V*>(out + (size_t)i * XFER_STRIDE);
+#pragma unroll
+    for (int j = 0; j < XFER_STRIDE / VN; ++j) {
+        V v;
+#pragma unroll
+        for (int e = 0; e < VN; ++e) v[e] = vals[j * VN + e];
+        o[j] = v;
+    }
 }
 
 template <class SP>
@@ -384,12 +408,20 @@ __global__ void __launch_bounds__(WG) k_xfer_rebuild(typename SP::st* f, uint32_
     T acc[27];
 #pragma unroll
     for (int k = 0; k < 27; ++k) acc[k] = T(0);
+    using V = typename Vec16<T>::type;
+    constexpr int VN = Vec16<T>::n;
     for (int nn = 0; nn < 64; ++nn) {
         const int32_t s = slot[(size_t)nn * n + i];
         if (s < 0) continue;
         const T wn = T(wt[(size_t)nn * n + i]);
+        const V* row = reinterpret_cast<const V*>(m + (size_t)s * XFER_STRIDE);
 #pragma unroll
-        for (int k = 0; k < 27; ++k) acc[k] = acc[k] + wn * m[(size_t)s * 27 + k];
+        for (int j = 0; j < XFER_STRIDE / VN; ++j) {
+            const V v = row[j];
+#pragma unroll
+            for (int e = 0; e < VN; ++e)
+                if (j * VN + e < 27) acc[j * VN + e] = acc[j * VN + e] + wn * v[e];
+        }
     }
     // moments of the interpolated state
     T rho = T(0), jx = T(0), jy = T(0), jz = T(0), pxx = T(0), pyy = T(0), pzz = T(0), pxy = T(0), pxz = T(0), pyz = T(0);

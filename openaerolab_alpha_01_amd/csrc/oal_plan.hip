@@ -8,7 +8,9 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <algorithm>
 #include <array>
+#include <numeric>
 #include <unordered_map>
 #include <vector>
 
@@ -89,6 +91,12 @@ struct Level {
     DevBuf<float> r_w;
     DevBuf<unsigned char> r_m;
     uint32_t n_rsrc = 0, n_rdst = 0;
+    // r_*: the covered cells the stepping reads (next to stepped cells, fill donors, donors of the
+    // coarser level's restriction); rf_*: every covered fluid cell (before the state is read out)
+    DevBuf<uint32_t> rf_src, rf_dst;
+    DevBuf<int32_t> rf_slot;
+    DevBuf<float> rf_w;
+    uint32_t n_rfsrc = 0, n_rfdst = 0;
     double restrict_scale = 0.0, restrict_keep = 0.0;
     bool has_solid = false;
     int64_t n_fluid = 0;
@@ -102,6 +110,7 @@ struct Level {
         lslot.release(); lmask.release();
         g_src.release(); g_dst.release(); g_slot.release(); g_w.release(); g_m.release();
         r_src.release(); r_dst.release(); r_slot.release(); r_w.release(); r_m.release();
+        rf_src.release(); rf_dst.release(); rf_slot.release(); rf_w.release();
     }
 };
 
@@ -124,6 +133,7 @@ struct OalPlan {
     std::vector<hipEvent_t> bulk_ev;        // pairs around the bulk launches of timed steps
     bool multilevel = false;                // a refined plan: levels coupled by ghost fill and restriction
     bool coupled = false;                   // the initial restriction and fill have run since the last upload
+    bool covered_stale = false;             // steps restricted only the covered cells the stepping reads
     std::vector<Level> levels;
     // captured steps: oal_step replays a hipGraph of one (coarse) step; the step index comes from ts_dev
     bool graph = false;
@@ -483,16 +493,61 @@ struct StencilBuilder {
         return s;
     }
     void finish() {
+        // donors in ascending cell order: the gather then reads each direction's array in order
+        std::vector<int32_t> order(X.src.size()), remap(X.src.size());
+        std::iota(order.begin(), order.end(), 0);
+        std::sort(order.begin(), order.end(),
+                  [&](int32_t a, int32_t b) { return (X.src[a] & 0x7fffffffu) < (X.src[b] & 0x7fffffffu); });
+        std::vector<uint32_t> src(X.src.size());
+        for (size_t r = 0; r < order.size(); ++r) {
+            remap[order[r]] = (int32_t)r;
+            src[r] = X.src[order[r]];
+        }
+        X.src.swap(src);
         const size_t n = X.dst.size();
         X.slot.assign(64 * n, -1);
         X.w.assign(64 * n, 0.0f);
         for (size_t i = 0; i < n; ++i)
             for (int q = 0; q < 64; ++q) {
-                X.slot[(size_t)q * n + i] = sl[i][q];
+                X.slot[(size_t)q * n + i] = sl[i][q] < 0 ? -1 : remap[sl[i][q]];
                 X.w[(size_t)q * n + i] = ww[i][q];
             }
     }
 };
+
+// The entries of X whose written cell is marked in keep (indexed by entry), with only their donors.
+static XferHost xfer_subset(const XferHost& X, const std::vector<char>& keep) {
+    XferHost Y;
+    const size_t n = X.dst.size();
+    std::vector<size_t> rows;
+    for (size_t i = 0; i < n; ++i)
+        if (keep[i]) rows.push_back(i);
+    std::vector<char> used(X.src.size(), 0);
+    for (size_t r : rows)
+        for (int q = 0; q < 64; ++q) {
+            const int32_t sl = X.slot[(size_t)q * n + r];
+            if (sl >= 0) used[sl] = 1;
+        }
+    std::vector<int32_t> remap(X.src.size(), -1);
+    for (size_t i = 0; i < X.src.size(); ++i)
+        if (used[i]) {
+            remap[i] = (int32_t)Y.src.size();
+            Y.src.push_back(X.src[i]);
+        }
+    const size_t m = rows.size();
+    Y.slot.assign(64 * m, -1);
+    Y.w.assign(64 * m, 0.0f);
+    for (size_t j = 0; j < m; ++j) {
+        const size_t r = rows[j];
+        Y.dst.push_back(X.dst[r]);
+        for (int q = 0; q < 64; ++q) {
+            const int32_t sl = X.slot[(size_t)q * n + r];
+            Y.slot[(size_t)q * m + j] = sl < 0 ? -1 : remap[sl];
+            Y.w[(size_t)q * m + j] = X.w[(size_t)q * n + r];
+        }
+    }
+    return Y;
+}
 
 // Fill of level F's ghost cells (those next to a stepped cell) from level C = F - 1.  A ghost cell at
 // global fine coordinate g sits at coarse coordinate g / 2 - 1/4 (cell centres); its donors are the 4^3
@@ -878,14 +933,26 @@ int ml_fill(OalPlan* p, int l, double w1, hipStream_t s) {
     return OAL_OK;
 }
 
-// Restrict level l + 1's current state into level l's covered cells.
-int ml_restrict(OalPlan* p, int l, hipStream_t s) {
+// Restrict level l + 1's current state into level l's covered cells: those the stepping reads, or
+// (full) every covered fluid cell.
+int ml_restrict(OalPlan* p, int l, hipStream_t s, bool full = false) {
     Level& C = p->levels[l];
     Level& F = p->levels[l + 1];
-    if (!C.n_rdst) return OAL_OK;
-    OAL_HIP(launch_xfer_gather(p->storage, F.fA.p, nullptr, C.r_src.p, C.n_rsrc, F.ncell, 1.0, C.r_m.p, s));
-    OAL_HIP(launch_xfer_rebuild(p->storage, C.fA.p, C.ncell, C.r_dst.p, C.n_rdst, C.r_slot.p, C.r_w.p, C.r_m.p,
-                                C.restrict_scale, C.restrict_keep, s));
+    const uint32_t nd = full ? C.n_rfdst : C.n_rdst, ns = full ? C.n_rfsrc : C.n_rsrc;
+    if (!nd) return OAL_OK;
+    OAL_HIP(launch_xfer_gather(p->storage, F.fA.p, nullptr, full ? C.rf_src.p : C.r_src.p, ns, F.ncell, 1.0, C.r_m.p, s));
+    OAL_HIP(launch_xfer_rebuild(p->storage, C.fA.p, C.ncell, full ? C.rf_dst.p : C.r_dst.p, nd,
+                                full ? C.rf_slot.p : C.r_slot.p, full ? C.rf_w.p : C.r_w.p, C.r_m.p, C.restrict_scale,
+                                C.restrict_keep, s));
+    return OAL_OK;
+}
+
+// Before the state of a refined plan is read out: every covered cell restricted, finest first.
+int ml_sync_covered(OalPlan* p, hipStream_t s) {
+    if (!p->multilevel || !p->covered_stale) return OAL_OK;
+    for (int l = (int)p->levels.size() - 2; l >= 0; --l)
+        if (int rc = ml_restrict(p, l, s, true)) return rc;
+    p->covered_stale = false;
     return OAL_OK;
 }
 
@@ -905,7 +972,7 @@ int ml_couple(OalPlan* p, hipStream_t s) {
     for (int l = 0; l < nl; ++l)
         if (!p->levels[l].have_f) return fail(OAL_ERR_STATE, "every level of a refined plan needs populations");
     for (int l = nl - 2; l >= 0; --l) {
-        if (int rc = ml_restrict(p, l, s)) return rc;
+        if (int rc = ml_restrict(p, l, s, true)) return rc;
         OAL_HIP(hipMemcpyAsync(p->levels[l].fB.p, p->levels[l].fA.p, p->levels[l].fA.n, hipMemcpyDeviceToDevice, s));
     }
     for (int l = 1; l < nl; ++l) {
@@ -913,6 +980,7 @@ int ml_couple(OalPlan* p, hipStream_t s) {
         OAL_HIP(hipMemcpyAsync(p->levels[l].fB.p, p->levels[l].fA.p, p->levels[l].fA.n, hipMemcpyDeviceToDevice, s));
     }
     p->coupled = true;
+    p->covered_stale = false;
     return OAL_OK;
 }
 
@@ -1098,7 +1166,7 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
     p->multilevel = ml;
     std::vector<HostLevel> host(desc->n_levels);
     std::vector<OctLevel> oct;
-    std::vector<XferHost> fillX(desc->n_levels), restX(desc->n_levels);
+    std::vector<XferHost> fillX(desc->n_levels), restX(desc->n_levels), restFull(desc->n_levels);
     if (ml) {
         if (int rc = build_octree(desc, oct)) return bail(rc);
     }
@@ -1132,12 +1200,47 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
         int64_t ncub = 0;
         for (int l = 1; l < nl; ++l) build_fill(oct[l], oct[l - 1], fillX[l], &ncub);
         for (int l = 0; l + 1 < nl; ++l) build_restrict(oct[l], oct[l + 1], restX[l], &ncub);
+        // covered cells the stepping reads, coarsest level first: next to a stepped cell, donors of the
+        // finer level's fill, donors of the coarser level's (reduced) restriction
+        for (int l = 0; l + 1 < nl; ++l) {
+            const OctLevel& o = oct[l];
+            std::vector<char> need((size_t)o.nb() * BS3, 0);
+            for (int b = 0; b < o.nb(); ++b) {
+                if (o.status[b] != OCT_COVERED) continue;
+                const int bx = (o.mx[b] - 1) * BS, by = (o.my[b] - 1) * BS, bz = (o.mz[b] - 1) * BS;
+                for (int z = 0; z < BS; ++z)
+                    for (int y = 0; y < BS; ++y)
+                        for (int x = 0; x < BS; ++x) {
+                            if (x > 0 && x < BS - 1 && y > 0 && y < BS - 1 && z > 0 && z < BS - 1) continue;
+                            bool hit = false;
+                            for (int k = 0; k < 27 && !hit; ++k) {
+                                if (k == 13) continue;
+                                int32_t nb = -1;
+                                const int64_t c = o.cell(bx + x + k % 3 - 1, by + y + (k / 3) % 3 - 1, bz + z + k / 9 - 1, &nb);
+                                hit = c >= 0 && o.status[nb] == OCT_STEPPED;
+                            }
+                            if (hit) need[(size_t)b * BS3 + (size_t)z * BS2 + (size_t)y * BS + x] = 1;
+                        }
+            }
+            for (uint32_t e : fillX[l + 1].src)
+                if (e >> 31) need[e & 0x7fffffffu] = 1;
+            if (l > 0)
+                for (uint32_t e : restX[l - 1].src) {
+                    const uint32_t c = e & 0x7fffffffu;
+                    if (o.status[c / BS3] == OCT_COVERED) need[c] = 1;
+                }
+            restFull[l] = restX[l];
+            std::vector<char> keep(restX[l].dst.size());
+            for (size_t i = 0; i < keep.size(); ++i) keep[i] = need[restX[l].dst[i]];
+            restX[l] = xfer_subset(restFull[l], keep);
+        }
         // every index the transfer kernels follow stays inside its buffer
         for (int l = 0; l < nl; ++l) {
             const uint64_t here = (uint64_t)oct[l].nb() * BS3;
             if (l > 0 && !xfer_in_bounds(fillX[l], here, (uint64_t)oct[l - 1].nb() * BS3))
                 return bail(fail(OAL_ERR_ARG, "fill transfer list out of bounds"));
-            if (l + 1 < nl && !xfer_in_bounds(restX[l], here, (uint64_t)oct[l + 1].nb() * BS3))
+            if (l + 1 < nl && (!xfer_in_bounds(restX[l], here, (uint64_t)oct[l + 1].nb() * BS3) ||
+                               !xfer_in_bounds(restFull[l], here, (uint64_t)oct[l + 1].nb() * BS3)))
                 return bail(fail(OAL_ERR_ARG, "restriction transfer list out of bounds"));
         }
         // rescale of the non-equilibrium moments between levels (coupling.jl:134-178, 188-193): tau of
@@ -1188,15 +1291,18 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
             Level& L = p->levels[l];
             const XferHost& F = fillX[l];
             const XferHost& R = restX[l];
+            const XferHost& RF = restFull[l];
             int r2;
             if ((r2 = upload(L.g_src, F.src)) || (r2 = upload(L.g_dst, F.dst)) || (r2 = upload(L.g_slot, F.slot)) ||
                 (r2 = upload(L.g_w, F.w)) || (r2 = upload(L.r_src, R.src)) || (r2 = upload(L.r_dst, R.dst)) ||
-                (r2 = upload(L.r_slot, R.slot)) || (r2 = upload(L.r_w, R.w)))
+                (r2 = upload(L.r_slot, R.slot)) || (r2 = upload(L.r_w, R.w)) || (r2 = upload(L.rf_src, RF.src)) ||
+                (r2 = upload(L.rf_dst, RF.dst)) || (r2 = upload(L.rf_slot, RF.slot)) || (r2 = upload(L.rf_w, RF.w)))
                 return bail(r2);
             L.n_gsrc = (uint32_t)F.src.size(); L.n_gdst = (uint32_t)F.dst.size();
             L.n_rsrc = (uint32_t)R.src.size(); L.n_rdst = (uint32_t)R.dst.size();
-            if (L.g_m.alloc((size_t)L.n_gsrc * 27 * work_bytes(p->storage)) != hipSuccess ||
-                L.r_m.alloc((size_t)L.n_rsrc * 27 * work_bytes(p->storage)) != hipSuccess)
+            L.n_rfsrc = (uint32_t)RF.src.size(); L.n_rfdst = (uint32_t)RF.dst.size();
+            if (L.g_m.alloc((size_t)L.n_gsrc * XFER_STRIDE * work_bytes(p->storage)) != hipSuccess ||
+                L.r_m.alloc((size_t)std::max(L.n_rsrc, L.n_rfsrc) * XFER_STRIDE * work_bytes(p->storage)) != hipSuccess)
                 return bail(fail(OAL_ERR_NOMEM, "transfer buffer allocation failed"));
         }
     }
@@ -1282,6 +1388,7 @@ int oal_download_f(OalPlan* p, int level, void* host_f, int dt) {
     Level& L = p->levels[level];
     if (int rc = require_plain_state(p, L)) return rc;
     OAL_HIP(hipSetDevice(p->device));
+    if (int rc = ml_sync_covered(p, p->stream)) return rc;
     const size_t bytes = (size_t)L.ncell_host * 27 * (dt == OAL_F64 ? 8 : 4);
     void* stage = nullptr;
     OAL_HIP(hipMalloc(&stage, bytes));
@@ -1341,6 +1448,7 @@ int oal_step(OalPlan* p, int n_steps, int32_t first_time_step, int flags, void* 
     for (int i = 0; i < n_steps; ++i) {
         const bool last = i == n_steps - 1;
         const int f = last ? flags : 0;
+        if (p->multilevel) p->covered_stale = true;
         if (p->graph && s == p->stream && !(f & OAL_STEP_FORCE)) {
             for (const Level& L : p->levels)
                 if (!L.have_f) return fail(OAL_ERR_STATE, "no populations: call oal_upload_f or oal_init_equilibrium first");
@@ -1375,6 +1483,7 @@ int oal_mass(OalPlan* p, int level, double* out, void* stream) {
     if (int rc = require_plain_state(p, L)) return rc;
     OAL_HIP(hipSetDevice(p->device));
     hipStream_t s = pick_stream(p, stream);
+    if (int rc = ml_sync_covered(p, s)) return rc;
     OAL_HIP(launch_mass(p->storage, L.fA.p, L.solid.p, L.borg.p, L.bdom.p, L.bflags.p, L.doms.p, L.nb, permute_of(p), L.mpart.p, s));
     // the reduction walks blocks here: one "item" per block
     ReduceArgs r{L.mpart.p, L.dom_block0.p, L.red_out.p, L.n_domains, 4};
@@ -1390,6 +1499,7 @@ int oal_download_macro(OalPlan* p, int level, double* rho, double* u) {
     Level& L = p->levels[level];
     if (int rc = require_plain_state(p, L)) return rc;
     OAL_HIP(hipSetDevice(p->device));
+    if (int rc = ml_sync_covered(p, p->stream)) return rc;
     double* d = nullptr;
     OAL_HIP(hipMalloc((void**)&d, (size_t)L.ncell_host * 32));
     hipError_t e = launch_macro(p->storage, L.fA.p, d, d + L.ncell_host, L.ncell, L.ncell_host, permute_of(p), nullptr, p->stream);
@@ -1406,6 +1516,7 @@ int oal_guard(OalPlan* p, int level, float rho_min, float u_max, int64_t* bad) {
     Level& L = p->levels[level];
     if (!L.have_f) return fail(OAL_ERR_STATE, "no populations on this level");
     OAL_HIP(hipSetDevice(p->device));
+    if (int rc = ml_sync_covered(p, p->stream)) return rc;
     unsigned long long h = 0;
     OAL_HIP(hipMemsetAsync(L.guard.p, 0, sizeof(unsigned long long), p->stream));
     OAL_HIP(launch_guard(p->storage, L.fA.p, L.solid.p, L.borg.p, L.bdom.p, L.bflags.p, L.doms.p, L.nb, rho_min, u_max, L.guard.p, p->stream));
