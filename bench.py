@@ -179,6 +179,183 @@ def validation(steps: int, seed: int, chunk: int = 1000) -> dict:
     return out
 
 
+# ---- Taylor-Green vortex, 64^3 periodic, FP64 (the reference's CPU-runnable case) ---------------------
+TGV_N = 64
+TGV_U0 = 0.05
+TGV_RE = 1600.0
+
+
+def tgv_fields(grid):
+    """u = u0 sin x cos y cos z, v = -u0 cos x sin y cos z, w = 0 at x = 2 pi (i - 1/2) / 64 (cell
+    centres), rho = 1 + 3 p' with p' = (u0^2 / 16)(cos 2x + cos 2y)(cos 2z + 2); nu = u0 L / Re with
+    L = 64 (nu = 0.002, tau = 0.506)."""
+    x = 2 * np.pi * (grid.cell_coords().astype(np.float64) + 0.5) / TGV_N
+    X, Y, Z = x[:, 0], x[:, 1], x[:, 2]
+    u = np.stack([TGV_U0 * np.sin(X) * np.cos(Y) * np.cos(Z), -TGV_U0 * np.cos(X) * np.sin(Y) * np.cos(Z),
+                  np.zeros_like(X)])
+    rho = 1.0 + 3.0 * (TGV_U0 ** 2 / 16.0) * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0)
+    return rho, u
+
+
+def tgv_tau():
+    return 0.5 + 3.0 * TGV_U0 * TGV_N / TGV_RE
+
+
+def tgv_main(args) -> int:
+    """C1: FP64 (and FP32) on the GPU, kinetic energy every 100 steps and MLUPS; the CPU oracle (FP64,
+    literal form) on the same case, timed for a bounded number of steps, with its kinetic energy at
+    those steps."""
+    from openaerolab_alpha_01_amd import solver
+    from openaerolab_alpha_01_amd.geometry import uniform_grid
+    grid = uniform_grid((TGV_N,) * 3, periodic=(True, True, True))
+    rho, u = tgv_fields(grid)
+    n = grid.n_cells
+    steps = args.tgv_steps
+    params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
+    out = {"workload": f"Taylor-Green vortex, {TGV_N}^3 periodic, u0={TGV_U0}, Re={TGV_RE:g} on L={TGV_N} "
+                       f"(nu={TGV_U0 * TGV_N / TGV_RE:g}, tau={tgv_tau():g}), equilibrium init, {steps} steps",
+           "every": 100}
+    for st in ("f64", "f32"):
+        plan = solver.uniform_plan(grid, tgv_tau(), storage=st, params=params)
+        plan.init_equilibrium(rho, u)
+        ke, ms = [], 0.0
+        for k in range(0, steps + 1, 100):
+            r, v = plan.macro()
+            ke.append(float(0.5 * np.sum(r * (v * v).sum(0))))
+            if k < steps:
+                tot, _ = plan.time_steps(min(100, steps - k))
+                ms += tot
+        plan.close()
+        out[st] = {"ke": ke, "ms_total": round(ms, 3), "mlups": round(n * steps / (ms * 1e-3) / 1e6, 1)}
+    out["f32_ke_rel_err_vs_f64"] = float(np.max(np.abs(np.array(out["f32"]["ke"]) / np.array(out["f64"]["ke"]) - 1)))
+    # viscous decay rate of the initial mode, 2 nu |k|^2 with |k|^2 = 3 (2 pi / 64)^2, against the first 100 steps
+    nu = (np.float32(tgv_tau()) - 0.5) / 3.0
+    k2 = 3 * (2 * np.pi / TGV_N) ** 2
+    ke = out["f64"]["ke"]
+    # steps 0-100 include the start-up transient of the equilibrium initialisation
+    out["decay_rate"] = {"steps_0_100": float(-np.log(ke[1] / ke[0]) / 100),
+                         "steps_100_200": float(-np.log(ke[2] / ke[1]) / 100), "viscous_initial": float(2 * nu * k2)}
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        G = O.Grid(grid)
+        f = O.init_eq("f64", G, rho, u)
+        g = f.copy()
+        p = O.make_params(tgv_tau(), arith=O.LITERAL)
+        cpu_steps = min(steps, args.tgv_cpu_steps)
+        t0 = time.perf_counter()
+        for _ in range(cpu_steps):
+            O.step("f64", G, p, f, out=g)
+            f, g = g, f
+        secs = time.perf_counter() - t0
+        _, r, v = O.moments("f64", G, f)
+        ke_cpu = float(0.5 * np.sum(r * (v * v).sum(0)))
+        threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+        ke_gpu = out["f64"]["ke"][cpu_steps // 100] if cpu_steps % 100 == 0 else None
+        out["cpu_baseline"] = {"value": round(n * cpu_steps / secs / 1e6, 2), "unit": "MLUPS", "cores": threads,
+                               "kind": "port", "sample": f"{cpu_steps} FP64 steps of the same case in {secs:.1f} s "
+                                                         "(CPU oracle, literal form)",
+                               "ke": ke_cpu,
+                               "ke_rel_diff_vs_gpu_f64": None if ke_gpu is None else float(ke_cpu / ke_gpu - 1)}
+    print(json.dumps({"tgv": out}), flush=True)
+    return 0
+
+
+# ---- many-instance form: 48 independent boxes with ellipsoids, one launch per step ------------------
+MANY_COUNT = 48
+MANY_TAU = 0.60      # fixed relaxation time of every box (the configuration leaves nu open; at 0.52 and 0.56
+                     # boxes whose wake meets the zero-gradient outlet early diverge within 5000 steps, in FP64 too)
+
+
+def many_instances(seed: int):
+    """48 boxes: edges uniform in [48, 160] cells per axis, an ellipsoid with semi-axes uniform in [6, 20]
+    cells and a uniformly random orientation (unit quaternion), inlet velocity uniform in [0.02, 0.08];
+    numpy PCG64 with `seed`.  The ellipsoid's centre sits at a third of the box length (at least its
+    largest semi-axis + 4 cells from the inlet), mid-height and mid-width."""
+    from openaerolab_alpha_01_amd.geometry import ellipsoid_mask, random_rotation, uniform_grid
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(MANY_COUNT):
+        n = tuple(int(v) for v in rng.integers(48, 161, 3))
+        semi = rng.uniform(6.0, 20.0, 3)
+        R = random_rotation(rng)
+        u_in = float(rng.uniform(0.02, 0.08))
+        c = (max(n[0] / 3.0, semi.max() + 4.0), n[1] / 2.0, n[2] / 2.0)
+        g = uniform_grid(n)
+        g.solid = ellipsoid_mask(g, c, semi, R)
+        out.append(dict(grid=g, n=n, semi=semi, R=R, u_in=u_in, center=c))
+    return out
+
+
+def many_plan(inst, storage):
+    from openaerolab_alpha_01_amd import solver
+    params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
+    plan = solver.batched_plan([i["grid"] for i in inst], [MANY_TAU] * len(inst), [i["u_in"] for i in inst],
+                               storage=storage, params=params)
+    rho = np.concatenate([np.ones(i["grid"].n_cells) for i in inst])
+    u = np.concatenate([np.stack([np.where(i["grid"].solid != 0, 0.0, i["u_in"]), np.zeros(i["grid"].n_cells),
+                                  np.zeros(i["grid"].n_cells)]) for i in inst], axis=1)
+    plan.init_equilibrium(rho, u)
+    return plan
+
+
+def many_main(args) -> int:
+    """C5: aggregate MLUPS of the 48 boxes in FP16 shifted storage (FP32 compute); with --validate, the
+    per-box Cd after --many-steps steps against the FP64 run of the same kernels."""
+    from openaerolab_alpha_01_amd.geometry import ellipsoid_frontal_area
+    inst = many_instances(args.seed)
+    cells = sum(int(np.prod(i["n"])) for i in inst)
+    stored = sum(i["grid"].n_cells for i in inst)
+    out = {"workload": f"{MANY_COUNT} independent inlet/outlet/free-slip boxes, edges U[48,160], ellipsoids with "
+                       f"semi-axes U[6,20] at random orientation, u_in U[0.02,0.08], tau={MANY_TAU}, seed {args.seed}; "
+                       "all boxes in one block list, one launch per kernel kind per step",
+           "cells": cells, "stored_cells": stored}
+    if args.validate:
+        res = {}
+        for st in ("f64", "f16s"):
+            t0 = time.perf_counter()
+            plan = many_plan(inst, st)
+            t = 0
+            while t < args.many_steps - 1:
+                k = min(1000, args.many_steps - 1 - t)
+                plan.step(k, t)
+                plan.sync()
+                t += k
+                print(f"[validate-many] {st} step {t}/{args.many_steps} ({time.perf_counter() - t0:.1f} s)",
+                      file=sys.stderr, flush=True)
+            plan.level_step(0, t, force=True)
+            F = plan.force()
+            mass = plan.mass()
+            plan.close()
+            res[st] = [float(F[k][0] / (0.5 * i["u_in"] ** 2 * ellipsoid_frontal_area(i["semi"], i["R"])))
+                       for k, i in enumerate(inst)]
+            res[st + "_finite"] = [bool(np.isfinite(mass[k]).all() and np.isfinite(F[k]).all())
+                                   for k in range(len(inst))]
+        ok = [a and b for a, b in zip(res["f64_finite"], res["f16s_finite"])]
+        err = [(a - b) / b for a, b, o in zip(res["f16s"], res["f64"], ok) if o]
+        out["validation"] = {"steps": args.many_steps, "cd_f64": [round(v, 5) for v in res["f64"]],
+                             "cd_f16s": [round(v, 5) for v in res["f16s"]],
+                             "stable_boxes": int(sum(ok)),
+                             "cd_rel_err": [float(e) for e in err],
+                             "max_abs_cd_rel_err": float(np.max(np.abs(err))) if err else None,
+                             "median_abs_cd_rel_err": float(np.median(np.abs(err))) if err else None}
+    else:
+        legs = []
+        for st in ("f16s", "f32"):
+            plan = many_plan(inst, st)
+            tun = plan.tuning
+            if not args.no_autotune:
+                tun, _ = plan.autotune(6)
+            plan.time_steps(args.warmup)
+            ms, ker = plan.time_steps(args.steps)
+            plan.close()
+            legs.append({"storage": st, "ms_per_step": round(ms / args.steps, 4),
+                         "mlups": round(cells * args.steps / (ms * 1e-3) / 1e6, 1),
+                         "stored_mlups": round(stored * args.steps / (ms * 1e-3) / 1e6, 1), "tuning": tun})
+        out["legs"] = legs
+    print(json.dumps({"many": out}), flush=True)
+    return 0
+
+
 # ---- block-AMR form: octree of 16^3 blocks around a sphere -------------------------------------------
 AMR_COARSE = (64, 32, 32)          # level-0 extent; finest-equivalent 512 x 256 x 256
 AMR_LEVELS = 4                     # coarse + 3 refinement levels
@@ -349,6 +526,13 @@ def main(argv=None) -> int:
     ap.add_argument("--amr", action="store_true",
                     help="the block-AMR form (4-level sphere octree) instead of the uniform box; with --validate "
                          "its accuracy run (--val-steps counts coarse steps)")
+    ap.add_argument("--many", action="store_true",
+                    help="the many-instance form (48 boxes with ellipsoids, FP16 storage) instead; with --validate "
+                         "its per-box Cd against FP64 after --many-steps steps")
+    ap.add_argument("--many-steps", type=int, default=5000)
+    ap.add_argument("--tgv", action="store_true", help="the Taylor-Green vortex case (64^3 periodic, FP64) instead")
+    ap.add_argument("--tgv-steps", type=int, default=1000)
+    ap.add_argument("--tgv-cpu-steps", type=int, default=200, help="steps of the CPU oracle in --tgv")
     ap.add_argument("--amr-legs", default="f16s:1,f16s:0,f32:1",
                     help="block-AMR legs timed, storage:graph comma-separated; the first is the reported value")
     args = ap.parse_args(argv)
@@ -362,6 +546,10 @@ def main(argv=None) -> int:
         args.no_autotune = True
     if args.amr:
         return amr_main(args)
+    if args.tgv:
+        return tgv_main(args)
+    if args.many:
+        return many_main(args)
     if args.validate:
         print(json.dumps({"validation": validation(args.val_steps, args.seed)}), flush=True)
         return 0

@@ -144,6 +144,45 @@ def ellipsoid_mask(grid: BlockGrid, center, semi_axes, rotation: np.ndarray) -> 
     return ((q <= 1.0) & grid.inside()).astype(np.uint8)
 
 
+def batch_grids(grids: list[BlockGrid]) -> BlockGrid:
+    """Independent boxes as one block list: the blocks of grids[i] follow those of grids[i - 1], each
+    box's neighbour table is offset to the combined numbering (no links between boxes), and
+    block_domain gives each block's box (the domain index of a batched level)."""
+    coords, nbrs, solids, bdom = [], [], [], []
+    off = 0
+    for i, g in enumerate(grids):
+        coords.append(np.asarray(g.coords, dtype=np.int32))
+        nb = np.asarray(g.neighbor, dtype=np.int32).copy()
+        nb[nb > 0] += off
+        nbrs.append(nb)
+        solids.append(np.zeros(g.n_cells, dtype=np.uint8) if g.solid is None else np.asarray(g.solid, dtype=np.uint8))
+        bdom.append(np.full(g.n_blocks, i, dtype=np.int32))
+        off += g.n_blocks
+    return BlockGrid(n=(0, 0, 0), periodic=(False, False, False), coords=np.concatenate(coords),
+                     neighbor=np.concatenate(nbrs), solid=np.concatenate(solids), bs=grids[0].bs,
+                     block_domain=np.concatenate(bdom))
+
+
+def random_rotation(rng: np.random.Generator) -> np.ndarray:
+    """A rotation matrix from a uniformly distributed unit quaternion (Shoemake's method)."""
+    u1, u2, u3 = rng.uniform(0.0, 1.0, 3)
+    a, b = np.sqrt(1 - u1), np.sqrt(u1)
+    w, x, y, z = a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2), b * np.sin(2 * np.pi * u3), b * np.cos(2 * np.pi * u3)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def ellipsoid_frontal_area(semi_axes, rotation: np.ndarray, direction=(1.0, 0.0, 0.0)) -> float:
+    """Area of the ellipsoid's projection on the plane normal to `direction`: pi a b c sqrt(n^T M n) with
+    M = R diag(1/a^2, 1/b^2, 1/c^2) R^T."""
+    a = np.asarray(semi_axes, dtype=np.float64)
+    R = np.asarray(rotation, dtype=np.float64)
+    n = np.asarray(direction, dtype=np.float64)
+    M = R @ np.diag(1.0 / a ** 2) @ R.T
+    return float(np.pi * a.prod() * np.sqrt(n @ M @ n))
+
+
 def channel_walls(grid: BlockGrid, axis: int = 1) -> np.ndarray:
     """Solid layers at the first and last cell along `axis` (a plane channel)."""
     g = grid.cell_coords()

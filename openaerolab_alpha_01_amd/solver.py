@@ -332,3 +332,15 @@ def octree_plan(grids: list[BlockGrid], tau_finest: float, storage: str = "f32",
     levels = [Level(g, [Domain(extent=g.n, tau=t, periodic=g.periodic, u_inlet=u_inlet, body_force=tuple(body_force))])
               for g, t in zip(grids, taus)]
     return Plan(levels, storage=storage, stream="ab", bb_mode=bb_mode, params=params, rounding=rounding)
+
+
+def batched_plan(grids: list[BlockGrid], taus, u_inlets, storage: str = "f16s", params: StepParams | None = None,
+                 rounding: str = "stochastic", bb_mode: str = "halfway") -> Plan:
+    """Independent boxes (inlet / outlet / free-slip channels) advanced together: one level whose block
+    list holds every box (geometry.batch_grids), one domain per box with its own relaxation time and inlet
+    velocity.  Every launch of a step covers all boxes; force and mass are per box (per domain)."""
+    from .geometry import batch_grids
+    big = batch_grids(grids)
+    doms = [Domain(extent=g.n, tau=float(t), u_inlet=float(u)) for g, t, u in zip(grids, taus, u_inlets)]
+    return Plan(Level(big, doms, block_domain=big.block_domain), storage=storage, stream="ab", bb_mode=bb_mode,
+                params=params, rounding=rounding)
