@@ -134,10 +134,11 @@ hipError_t launch_copy_cells(int storage, void* dst, const void* src, const uint
                              hipStream_t stream);
 hipError_t launch_aa_outlet_prep(int storage, const StepArgs& a, hipStream_t stream);
 hipError_t launch_mass(int storage, const void* f, const uint8_t* solid, const int32_t* borg, const int32_t* bdom,
-                       const uint32_t* bflags, const DevDomain* doms, uint32_t nb, int permute, double* part, hipStream_t stream);
+                       const uint32_t* bflags, const DevDomain* doms, uint32_t nb, uint32_t stride, int permute, double* part,
+                       hipStream_t stream);
 hipError_t launch_reduce(const ReduceArgs& r, hipStream_t stream);
 hipError_t launch_guard(int storage, const void* f, const uint8_t* solid, const int32_t* borg, const int32_t* bdom,
-                        const uint32_t* bflags, const DevDomain* doms, uint32_t nb, float rho_min, float u_max,
-                        unsigned long long* count, hipStream_t stream);
+                        const uint32_t* bflags, const DevDomain* doms, uint32_t nb, uint32_t stride, float rho_min,
+                        float u_max, unsigned long long* count, hipStream_t stream);
 
 }  // namespace oal

@@ -218,7 +218,7 @@ hipError_t launch_aa_outlet_prep(int storage, const StepArgs& a, hipStream_t s) 
 template <class SP>
 __global__ void __launch_bounds__(WG) k_mass(const typename SP::st* f, const uint8_t* solid, const int32_t* borg,
                                              const int32_t* bdom, const uint32_t* bflags, const DevDomain* doms,
-                                             uint32_t nb, int permute, double* part) {
+                                             uint32_t stride, int permute, double* part) {
     using T = typename SP::T;
     const uint32_t b = blockIdx.x;
     if (bflags[b] & BF_INACTIVE) {   // covered or ghost block of a refined level: counted on another level
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(WG) k_mass(const typename SP::st* f, const uin
         return;
     }
     const DevDomain* dm = doms + bdom[b];
-    const uint32_t ncell = nb * BS3;
+    const uint32_t ncell = stride;   // population stride between directions
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int z = 0; z < BS; ++z) {
         const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
@@ -259,12 +259,13 @@ __global__ void __launch_bounds__(WG) k_mass(const typename SP::st* f, const uin
 }
 
 hipError_t launch_mass(int storage, const void* f, const uint8_t* solid, const int32_t* borg, const int32_t* bdom,
-                       const uint32_t* bflags, const DevDomain* doms, uint32_t nb, int permute, double* part, hipStream_t s) {
+                       const uint32_t* bflags, const DevDomain* doms, uint32_t nb, uint32_t stride, int permute, double* part,
+                       hipStream_t s) {
     switch (storage) {
-        case 0: hipLaunchKernelGGL((k_mass<StF32>), dim3(nb), dim3(WG), 0, s, (const float*)f, solid, borg, bdom, bflags, doms, nb, permute, part); break;
-        case 1: hipLaunchKernelGGL((k_mass<StF64>), dim3(nb), dim3(WG), 0, s, (const double*)f, solid, borg, bdom, bflags, doms, nb, permute, part); break;
-        case 2: hipLaunchKernelGGL((k_mass<StF16S>), dim3(nb), dim3(WG), 0, s, (const _Float16*)f, solid, borg, bdom, bflags, doms, nb, permute, part); break;
-        case 3: hipLaunchKernelGGL((k_mass<StBF16S>), dim3(nb), dim3(WG), 0, s, (const uint16_t*)f, solid, borg, bdom, bflags, doms, nb, permute, part); break;
+        case 0: hipLaunchKernelGGL((k_mass<StF32>), dim3(nb), dim3(WG), 0, s, (const float*)f, solid, borg, bdom, bflags, doms, stride, permute, part); break;
+        case 1: hipLaunchKernelGGL((k_mass<StF64>), dim3(nb), dim3(WG), 0, s, (const double*)f, solid, borg, bdom, bflags, doms, stride, permute, part); break;
+        case 2: hipLaunchKernelGGL((k_mass<StF16S>), dim3(nb), dim3(WG), 0, s, (const _Float16*)f, solid, borg, bdom, bflags, doms, stride, permute, part); break;
+        case 3: hipLaunchKernelGGL((k_mass<StBF16S>), dim3(nb), dim3(WG), 0, s, (const uint16_t*)f, solid, borg, bdom, bflags, doms, stride, permute, part); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -302,12 +303,12 @@ hipError_t launch_reduce(const ReduceArgs& r, hipStream_t s) {
 template <class SP>
 __global__ void __launch_bounds__(WG) k_guard(const typename SP::st* f, const uint8_t* solid, const int32_t* borg,
                                               const int32_t* bdom, const uint32_t* bflags, const DevDomain* doms,
-                                              uint32_t nb, float rho_min, float u_max, unsigned long long* count) {
+                                              uint32_t stride, float rho_min, float u_max, unsigned long long* count) {
     using T = typename SP::T;
     const uint32_t b = blockIdx.x;
     if (bflags[b] & BF_INACTIVE) return;
     const DevDomain* dm = doms + bdom[b];
-    const uint32_t ncell = nb * BS3;
+    const uint32_t ncell = stride;   // population stride between directions
     unsigned bad = 0;
     for (int z = 0; z < BS; ++z) {
         const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
@@ -330,13 +331,13 @@ __global__ void __launch_bounds__(WG) k_guard(const typename SP::st* f, const ui
 }
 
 hipError_t launch_guard(int storage, const void* f, const uint8_t* solid, const int32_t* borg, const int32_t* bdom,
-                        const uint32_t* bflags, const DevDomain* doms, uint32_t nb, float rho_min, float u_max,
-                        unsigned long long* count, hipStream_t s) {
+                        const uint32_t* bflags, const DevDomain* doms, uint32_t nb, uint32_t stride, float rho_min,
+                        float u_max, unsigned long long* count, hipStream_t s) {
     switch (storage) {
-        case 0: hipLaunchKernelGGL((k_guard<StF32>), dim3(nb), dim3(WG), 0, s, (const float*)f, solid, borg, bdom, bflags, doms, nb, rho_min, u_max, count); break;
-        case 1: hipLaunchKernelGGL((k_guard<StF64>), dim3(nb), dim3(WG), 0, s, (const double*)f, solid, borg, bdom, bflags, doms, nb, rho_min, u_max, count); break;
-        case 2: hipLaunchKernelGGL((k_guard<StF16S>), dim3(nb), dim3(WG), 0, s, (const _Float16*)f, solid, borg, bdom, bflags, doms, nb, rho_min, u_max, count); break;
-        case 3: hipLaunchKernelGGL((k_guard<StBF16S>), dim3(nb), dim3(WG), 0, s, (const uint16_t*)f, solid, borg, bdom, bflags, doms, nb, rho_min, u_max, count); break;
+        case 0: hipLaunchKernelGGL((k_guard<StF32>), dim3(nb), dim3(WG), 0, s, (const float*)f, solid, borg, bdom, bflags, doms, stride, rho_min, u_max, count); break;
+        case 1: hipLaunchKernelGGL((k_guard<StF64>), dim3(nb), dim3(WG), 0, s, (const double*)f, solid, borg, bdom, bflags, doms, stride, rho_min, u_max, count); break;
+        case 2: hipLaunchKernelGGL((k_guard<StF16S>), dim3(nb), dim3(WG), 0, s, (const _Float16*)f, solid, borg, bdom, bflags, doms, stride, rho_min, u_max, count); break;
+        case 3: hipLaunchKernelGGL((k_guard<StBF16S>), dim3(nb), dim3(WG), 0, s, (const uint16_t*)f, solid, borg, bdom, bflags, doms, stride, rho_min, u_max, count); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

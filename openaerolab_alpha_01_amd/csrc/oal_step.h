@@ -629,8 +629,11 @@ enum : int { FACE_NONE = 0, FACE_RULES = 1, FACE_SKIP = 2 };
 // Register budget: with one plane of populations in flight beside the collision of another the kernel
 // fits the 168 VGPRs of 3 waves per SIMD; the in-kernel face rules add per-lane address selects and
 // scalar offsets that only fit the 256 VGPRs of 2 waves.
+#ifndef OAL_FACE_RULES_WAVES
+#define OAL_FACE_RULES_WAVES 2
+#endif
 template <int FACE>
-constexpr int pipe_waves() { return FACE == FACE_RULES ? 2 : 3; }
+constexpr int pipe_waves() { return FACE == FACE_RULES ? OAL_FACE_RULES_WAVES : 3; }
 
 template <class SP, int NP, int FACE>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(pipe_waves<FACE>(), pipe_waves<FACE>())))
