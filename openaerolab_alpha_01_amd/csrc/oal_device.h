@@ -77,6 +77,13 @@ __host__ __device__ __forceinline__ uint32_t round_bits(uint32_t hc, int k) {
     return (hc ^ ((uint32_t)k * 0x9e3779b9u)) * 0x85ebca6bu;
 }
 
+// Population storage inside the library: block-major.  Each 16^3 block holds its 27 direction arrays
+// of 4096 cells contiguously, so the loads and stores of a workgroup stay within one 432-element-wide
+// region per block (the host layout f[x,y,z,b,k] is converted on upload and download).
+__host__ __device__ __forceinline__ size_t pop_index(uint32_t cell, int k) {
+    return ((size_t)(cell >> 12) * 27u + (size_t)k) * 4096u + (cell & 4095u);
+}
+
 struct StF32 {
     using st = float;
     using T = float;

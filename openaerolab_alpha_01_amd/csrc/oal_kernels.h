@@ -68,7 +68,7 @@ struct StepArgs {
                                  // them; 0: it applies the face rules itself
     const int32_t* oslot;        // [nb] AA pattern: outlet side-buffer slot of the block, -1 = none
     void* obuf;                  // [n_oslots * 9 * BS2] AA pattern: outlet-face populations with cx = -1
-    uint32_t ncell;              // cells per direction (stride between directions)
+    uint32_t ncell;              // cells of the level (the stride of the velocity fields' components)
     uint32_t nb;                 // blocks
     uint32_t n_items;            // workgroups launched (nb * PLANES)
     int32_t time_step;
@@ -134,11 +134,10 @@ hipError_t launch_copy_cells(int storage, void* dst, const void* src, const uint
                              hipStream_t stream);
 hipError_t launch_aa_outlet_prep(int storage, const StepArgs& a, hipStream_t stream);
 hipError_t launch_mass(int storage, const void* f, const uint8_t* solid, const int32_t* borg, const int32_t* bdom,
-                       const uint32_t* bflags, const DevDomain* doms, uint32_t nb, uint32_t stride, int permute, double* part,
-                       hipStream_t stream);
+                       const uint32_t* bflags, const DevDomain* doms, uint32_t nb, int permute, double* part, hipStream_t stream);
 hipError_t launch_reduce(const ReduceArgs& r, hipStream_t stream);
 hipError_t launch_guard(int storage, const void* f, const uint8_t* solid, const int32_t* borg, const int32_t* bdom,
-                        const uint32_t* bflags, const DevDomain* doms, uint32_t nb, uint32_t stride, float rho_min,
-                        float u_max, unsigned long long* count, hipStream_t stream);
+                        const uint32_t* bflags, const DevDomain* doms, uint32_t nb, float rho_min, float u_max,
+                        unsigned long long* count, hipStream_t stream);
 
 }  // namespace oal

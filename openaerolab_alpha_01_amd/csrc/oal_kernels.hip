@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(WG) k_init_eq(typename SP::st* f, uint32_t nce
     for (int k = 0; k < 27; ++k) {
         const T cu = T(dcx(k)) * ux + T(dcy(k)) * uy + T(dcz(k)) * uz;
         const T feq = r * wgt<T>(k) * (T(1) + inv_cs2 * cu + inv_2cs4 * cu * cu - inv_2cs2 * usq);
-        SP::sto(f + (size_t)(permute ? 26 - k : k) * ncell + cidx, feq, wgt<T>(k));
+        SP::sto(f + pop_index(cidx, permute ? 26 - k : k), feq, wgt<T>(k));
     }
     if (vel) { vel[cidx] = ux; vel[ncell + cidx] = uy; vel[2 * (size_t)ncell + cidx] = uz; }
 }
@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(WG) k_convert_in(typename SP::st* f, const H* 
     const uint32_t cidx = blockIdx.x * WG + threadIdx.x;
     const int k = blockIdx.y;
     if (cidx >= nhost) return;
-    SP::sto(f + (size_t)(permute ? 26 - k : k) * ncell + cidx, T(src[(size_t)k * nhost + cidx]), wgt<T>(k));
+    SP::sto(f + pop_index(cidx, permute ? 26 - k : k), T(src[(size_t)k * nhost + cidx]), wgt<T>(k));
 }
 
 template <class SP, class H>
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(WG) k_convert_out(const typename SP::st* f, H*
     const uint32_t cidx = blockIdx.x * WG + threadIdx.x;
     const int k = blockIdx.y;
     if (cidx >= nhost) return;
-    dst[(size_t)k * nhost + cidx] = H(SP::ld(f + (size_t)(permute ? 26 - k : k) * ncell + cidx, wgt<T>(k)));
+    dst[(size_t)k * nhost + cidx] = H(SP::ld(f + pop_index(cidx, permute ? 26 - k : k), wgt<T>(k)));
 }
 
 template <class SP>
@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(WG) k_macro(const typename SP::st* f, double* 
     double dr = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
 #pragma unroll
     for (int k = 0; k < 27; ++k) {
-        const double s = (double)SP::ld(f + (size_t)(permute ? 26 - k : k) * ncell + cidx, wgt<T>(k)) - (double)wgt<T>(k);
+        const double s = (double)SP::ld(f + pop_index(cidx, permute ? 26 - k : k), wgt<T>(k)) - (double)wgt<T>(k);
         dr += s; jx += s * dcx(k); jy += s * dcy(k); jz += s * dcz(k);
     }
     const double r = 1.0 + dr;
@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(WG) k_aa_outlet_prep(const StepArgs a) {
     const st* f = (const st*)a.fin;
     st* o = (st*)a.obuf;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) o[((size_t)os * 9 + q) * BS2 + ly * BS + z] = f[(size_t)(26 - 3 * q) * a.ncell + own];
+    for (int q = 0; q < 9; ++q) o[((size_t)os * 9 + q) * BS2 + ly * BS + z] = f[pop_index(own, 26 - 3 * q)];
 }
 
 hipError_t launch_aa_outlet_prep(int storage, const StepArgs& a, hipStream_t s) {
@@ -218,7 +218,7 @@ hipError_t launch_aa_outlet_prep(int storage, const StepArgs& a, hipStream_t s) 
 template <class SP>
 __global__ void __launch_bounds__(WG) k_mass(const typename SP::st* f, const uint8_t* solid, const int32_t* borg,
                                              const int32_t* bdom, const uint32_t* bflags, const DevDomain* doms,
-                                             uint32_t stride, int permute, double* part) {
+                                             int permute, double* part) {
     using T = typename SP::T;
     const uint32_t b = blockIdx.x;
     if (bflags[b] & BF_INACTIVE) {   // covered or ghost block of a refined level: counted on another level
@@ -226,7 +226,6 @@ __global__ void __launch_bounds__(WG) k_mass(const typename SP::st* f, const uin
         return;
     }
     const DevDomain* dm = doms + bdom[b];
-    const uint32_t ncell = stride;   // population stride between directions
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int z = 0; z < BS; ++z) {
         const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
@@ -236,7 +235,7 @@ __global__ void __launch_bounds__(WG) k_mass(const typename SP::st* f, const uin
         double dr = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
 #pragma unroll
         for (int k = 0; k < 27; ++k) {
-            const double s = (double)SP::ld(f + (size_t)(permute ? 26 - k : k) * ncell + cidx, wgt<T>(k)) - (double)wgt<T>(k);
+            const double s = (double)SP::ld(f + pop_index(cidx, permute ? 26 - k : k), wgt<T>(k)) - (double)wgt<T>(k);
             dr += s; jx += s * dcx(k); jy += s * dcy(k); jz += s * dcz(k);
         }
         acc[0] += 1.0 + dr; acc[1] += jx; acc[2] += jy; acc[3] += jz;
@@ -259,13 +258,12 @@ __global__ void __launch_bounds__(WG) k_mass(const typename SP::st* f, const uin
 }
 
 hipError_t launch_mass(int storage, const void* f, const uint8_t* solid, const int32_t* borg, const int32_t* bdom,
-                       const uint32_t* bflags, const DevDomain* doms, uint32_t nb, uint32_t stride, int permute, double* part,
-                       hipStream_t s) {
+                       const uint32_t* bflags, const DevDomain* doms, uint32_t nb, int permute, double* part, hipStream_t s) {
     switch (storage) {
-        case 0: hipLaunchKernelGGL((k_mass<StF32>), dim3(nb), dim3(WG), 0, s, (const float*)f, solid, borg, bdom, bflags, doms, stride, permute, part); break;
-        case 1: hipLaunchKernelGGL((k_mass<StF64>), dim3(nb), dim3(WG), 0, s, (const double*)f, solid, borg, bdom, bflags, doms, stride, permute, part); break;
-        case 2: hipLaunchKernelGGL((k_mass<StF16S>), dim3(nb), dim3(WG), 0, s, (const _Float16*)f, solid, borg, bdom, bflags, doms, stride, permute, part); break;
-        case 3: hipLaunchKernelGGL((k_mass<StBF16S>), dim3(nb), dim3(WG), 0, s, (const uint16_t*)f, solid, borg, bdom, bflags, doms, stride, permute, part); break;
+        case 0: hipLaunchKernelGGL((k_mass<StF32>), dim3(nb), dim3(WG), 0, s, (const float*)f, solid, borg, bdom, bflags, doms, permute, part); break;
+        case 1: hipLaunchKernelGGL((k_mass<StF64>), dim3(nb), dim3(WG), 0, s, (const double*)f, solid, borg, bdom, bflags, doms, permute, part); break;
+        case 2: hipLaunchKernelGGL((k_mass<StF16S>), dim3(nb), dim3(WG), 0, s, (const _Float16*)f, solid, borg, bdom, bflags, doms, permute, part); break;
+        case 3: hipLaunchKernelGGL((k_mass<StBF16S>), dim3(nb), dim3(WG), 0, s, (const uint16_t*)f, solid, borg, bdom, bflags, doms, permute, part); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -303,12 +301,11 @@ hipError_t launch_reduce(const ReduceArgs& r, hipStream_t s) {
 template <class SP>
 __global__ void __launch_bounds__(WG) k_guard(const typename SP::st* f, const uint8_t* solid, const int32_t* borg,
                                               const int32_t* bdom, const uint32_t* bflags, const DevDomain* doms,
-                                              uint32_t stride, float rho_min, float u_max, unsigned long long* count) {
+                                              float rho_min, float u_max, unsigned long long* count) {
     using T = typename SP::T;
     const uint32_t b = blockIdx.x;
     if (bflags[b] & BF_INACTIVE) return;
     const DevDomain* dm = doms + bdom[b];
-    const uint32_t ncell = stride;   // population stride between directions
     unsigned bad = 0;
     for (int z = 0; z < BS; ++z) {
         const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
@@ -319,7 +316,7 @@ __global__ void __launch_bounds__(WG) k_guard(const typename SP::st* f, const ui
         bool finite = true;
 #pragma unroll
         for (int k = 0; k < 27; ++k) {
-            const float v = (float)SP::ld(f + (size_t)k * ncell + cidx, wgt<T>(k));
+            const float v = (float)SP::ld(f + pop_index(cidx, k), wgt<T>(k));
             finite = finite && isfinite(v);
             r += v; jx += v * dcx(k); jy += v * dcy(k); jz += v * dcz(k);
         }
@@ -331,13 +328,13 @@ __global__ void __launch_bounds__(WG) k_guard(const typename SP::st* f, const ui
 }
 
 hipError_t launch_guard(int storage, const void* f, const uint8_t* solid, const int32_t* borg, const int32_t* bdom,
-                        const uint32_t* bflags, const DevDomain* doms, uint32_t nb, uint32_t stride, float rho_min,
-                        float u_max, unsigned long long* count, hipStream_t s) {
+                        const uint32_t* bflags, const DevDomain* doms, uint32_t nb, float rho_min, float u_max,
+                        unsigned long long* count, hipStream_t s) {
     switch (storage) {
-        case 0: hipLaunchKernelGGL((k_guard<StF32>), dim3(nb), dim3(WG), 0, s, (const float*)f, solid, borg, bdom, bflags, doms, stride, rho_min, u_max, count); break;
-        case 1: hipLaunchKernelGGL((k_guard<StF64>), dim3(nb), dim3(WG), 0, s, (const double*)f, solid, borg, bdom, bflags, doms, stride, rho_min, u_max, count); break;
-        case 2: hipLaunchKernelGGL((k_guard<StF16S>), dim3(nb), dim3(WG), 0, s, (const _Float16*)f, solid, borg, bdom, bflags, doms, stride, rho_min, u_max, count); break;
-        case 3: hipLaunchKernelGGL((k_guard<StBF16S>), dim3(nb), dim3(WG), 0, s, (const uint16_t*)f, solid, borg, bdom, bflags, doms, stride, rho_min, u_max, count); break;
+        case 0: hipLaunchKernelGGL((k_guard<StF32>), dim3(nb), dim3(WG), 0, s, (const float*)f, solid, borg, bdom, bflags, doms, rho_min, u_max, count); break;
+        case 1: hipLaunchKernelGGL((k_guard<StF64>), dim3(nb), dim3(WG), 0, s, (const double*)f, solid, borg, bdom, bflags, doms, rho_min, u_max, count); break;
+        case 2: hipLaunchKernelGGL((k_guard<StF16S>), dim3(nb), dim3(WG), 0, s, (const _Float16*)f, solid, borg, bdom, bflags, doms, rho_min, u_max, count); break;
+        case 3: hipLaunchKernelGGL((k_guard<StBF16S>), dim3(nb), dim3(WG), 0, s, (const uint16_t*)f, solid, borg, bdom, bflags, doms, rho_min, u_max, count); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -382,8 +379,8 @@ __global__ void __launch_bounds__(WG) k_xfer_gather(const typename SP::st* fnew,
     vals[XFER_STRIDE - 1] = T(0);
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
-        T v = SP::ld(fnew + (size_t)K * ncell + c, wgt<T>(K));
-        if (blend) v = SP::ld(fold + (size_t)K * ncell + c, wgt<T>(K)) * w0 + v * w1;
+        T v = SP::ld(fnew + pop_index(c, K), wgt<T>(K));
+        if (blend) v = SP::ld(fold + pop_index(c, K), wgt<T>(K)) * w0 + v * w1;
         vals[K] = v;
     });
     // one 16-byte-aligned record per donor, written as whole 16-byte vectors
@@ -473,7 +470,7 @@ __global__ void __launch_bounds__(WG) k_xfer_rebuild(typename SP::st* f, uint32_
     const uint32_t c = tcells[i];
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
-        SP::sto(f + (size_t)K * ncell + c, bad ? eq[K] : out[K], wgt<T>(K));
+        SP::sto(f + pop_index(c, K), bad ? eq[K] : out[K], wgt<T>(K));
     });
 }
 
@@ -493,7 +490,7 @@ __global__ void __launch_bounds__(WG) k_copy_cells(typename SP::st* dst, const t
     if (i >= n) return;
     const uint32_t c = cells[i];
 #pragma unroll
-    for (int k = 0; k < 27; ++k) dst[(size_t)k * ncell + c] = src[(size_t)k * ncell + c];
+    for (int k = 0; k < 27; ++k) dst[pop_index(c, k)] = src[pop_index(c, k)];
 }
 
 template <class SP>

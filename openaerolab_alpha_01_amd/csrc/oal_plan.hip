@@ -76,7 +76,6 @@ struct Level {
     DevBuf<unsigned long long> guard;
     // refined plans: the caller's blocks come first (ncell_host cells), the level's ghost blocks after them
     uint32_t ncell_host = 0;
-    uint32_t fstride = 0;           // population stride between directions (ncell plus a pad)
     uint32_t n_caller_blocks = 0, n_ghost_blocks = 0, n_covered_blocks = 0;
     // fill of this level's ghost cells from the coarser level: moments of g_src cells (coarser level),
     // stencils (g_slot, g_w: [64][n_gdst]) onto g_dst cells (this level)
@@ -162,17 +161,6 @@ int upload(DevBuf<T>& d, const std::vector<T>& h) {
     return OAL_OK;
 }
 
-
-// Direction stride of a level's population arrays: its cells plus a pad of OAL_DIR_PAD_BYTES, so that
-// the 27 direction arrays of a level whose size is a power of two do not all start at the same offset
-// modulo the HBM channel interleave.  No pad where it would break 32-bit cell indexing.
-#ifndef OAL_DIR_PAD_BYTES
-#define OAL_DIR_PAD_BYTES 256
-#endif
-uint32_t dir_stride(uint32_t ncell, int storage) {
-    const uint64_t s = (uint64_t)ncell + (uint64_t)(OAL_DIR_PAD_BYTES / storage_bytes(storage));
-    return s * 27ull < (1ull << 32) ? (uint32_t)s : ncell;
-}
 
 // Host tables of one level, built and validated before any device call
 struct HostLevel {
@@ -826,13 +814,12 @@ int upload_level(OalPlan* p, Level& L, const HostLevel& H) {
         if (H.n_oslots && L.obuf.alloc((size_t)H.n_oslots * 9 * BS2 * storage_bytes(p->storage)) != hipSuccess)
             return fail(OAL_ERR_NOMEM, "outlet buffer allocation failed");
     }
-    L.fstride = dir_stride(L.ncell, p->storage);
-    const size_t fbytes = (size_t)L.fstride * 27 * storage_bytes(p->storage);
+    const size_t fbytes = (size_t)L.ncell * 27 * storage_bytes(p->storage);
     if (L.fA.alloc(fbytes) != hipSuccess) return fail(OAL_ERR_NOMEM, "population buffer allocation failed");
     if (p->stream_mode == OAL_STREAM_AB && L.fB.alloc(fbytes) != hipSuccess)
         return fail(OAL_ERR_NOMEM, "population buffer allocation failed");
     if (p->wale) {
-        const size_t vbytes = (size_t)L.fstride * 3 * work_bytes(p->storage);
+        const size_t vbytes = (size_t)L.ncell * 3 * work_bytes(p->storage);
         if (L.velA.alloc(vbytes) != hipSuccess || L.velB.alloc(vbytes) != hipSuccess)
             return fail(OAL_ERR_NOMEM, "velocity buffer allocation failed");
         OAL_HIP(hipMemset(L.velA.p, 0, vbytes));
@@ -875,11 +862,14 @@ StepArgs make_args(OalPlan* p, Level& L, int32_t time_step, bool want_force) {
     a.face_cells = L.face_cells.p;
     a.n_face = L.n_face;
     a.regular_face = L.regular_face ? 1 : 0;
-    // the in-kernel face rules address by 32-bit byte offsets: levels of 4 GiB or more take the face-cell kernel
-    const bool small = (uint64_t)L.fstride * 27u * (uint64_t)storage_bytes(p->storage) <= 0xffffffffull;
-    a.face_kernel = (p->face_kernel || !small) ? 1 : 0;
+    // the in-kernel face rules address by signed 32-bit byte offsets from a block: a domain whose
+    // layer of blocks (plus a row and a block) spans 2 GiB or more takes the face-cell kernel
+    bool near = true;
+    for (const DevDomain& dm : L.doms_h)
+        near = near && ((uint64_t)dm.nbx * dm.nby + dm.nbx + 1) * 27u * BS3 * (uint64_t)storage_bytes(p->storage) < (1ull << 31);
+    a.face_kernel = (p->face_kernel || !near) ? 1 : 0;
     a.obuf = L.obuf.p;
-    a.ncell = L.fstride;
+    a.ncell = L.ncell;
     a.nb = L.nb;
     a.n_items = L.n_items;
     a.time_step = time_step;
@@ -940,9 +930,9 @@ int ml_fill(OalPlan* p, int l, double w1, hipStream_t s) {
     Level& F = p->levels[l];
     Level& C = p->levels[l - 1];
     if (!F.n_gdst) return OAL_OK;
-    OAL_HIP(launch_xfer_gather(p->storage, C.fA.p, w1 == 1.0 ? nullptr : C.fB.p, F.g_src.p, F.n_gsrc, C.fstride, w1,
+    OAL_HIP(launch_xfer_gather(p->storage, C.fA.p, w1 == 1.0 ? nullptr : C.fB.p, F.g_src.p, F.n_gsrc, C.ncell, w1,
                                 F.g_m.p, s));
-    OAL_HIP(launch_xfer_rebuild(p->storage, F.fA.p, F.fstride, F.g_dst.p, F.n_gdst, F.g_slot.p, F.g_w.p, F.g_m.p,
+    OAL_HIP(launch_xfer_rebuild(p->storage, F.fA.p, F.ncell, F.g_dst.p, F.n_gdst, F.g_slot.p, F.g_w.p, F.g_m.p,
                                 F.fill_scale, F.fill_keep, s));
     return OAL_OK;
 }
@@ -954,8 +944,8 @@ int ml_restrict(OalPlan* p, int l, hipStream_t s, bool full = false) {
     Level& F = p->levels[l + 1];
     const uint32_t nd = full ? C.n_rfdst : C.n_rdst, ns = full ? C.n_rfsrc : C.n_rsrc;
     if (!nd) return OAL_OK;
-    OAL_HIP(launch_xfer_gather(p->storage, F.fA.p, nullptr, full ? C.rf_src.p : C.r_src.p, ns, F.fstride, 1.0, C.r_m.p, s));
-    OAL_HIP(launch_xfer_rebuild(p->storage, C.fA.p, C.fstride, full ? C.rf_dst.p : C.r_dst.p, nd,
+    OAL_HIP(launch_xfer_gather(p->storage, F.fA.p, nullptr, full ? C.rf_src.p : C.r_src.p, ns, F.ncell, 1.0, C.r_m.p, s));
+    OAL_HIP(launch_xfer_rebuild(p->storage, C.fA.p, C.ncell, full ? C.rf_dst.p : C.r_dst.p, nd,
                                 full ? C.rf_slot.p : C.r_slot.p, full ? C.rf_w.p : C.r_w.p, C.r_m.p, C.restrict_scale,
                                 C.restrict_keep, s));
     return OAL_OK;
@@ -975,7 +965,7 @@ int ml_sync_covered(OalPlan* p, hipStream_t s) {
 int ml_copy_ghosts(OalPlan* p, int l, hipStream_t s) {
     Level& L = p->levels[l];
     if (!L.n_gdst) return OAL_OK;
-    OAL_HIP(launch_copy_cells(p->storage, L.fA.p, L.fB.p, L.g_dst.p, L.n_gdst, L.fstride, s));
+    OAL_HIP(launch_copy_cells(p->storage, L.fA.p, L.fB.p, L.g_dst.p, L.n_gdst, L.ncell, s));
     return OAL_OK;
 }
 
@@ -1107,7 +1097,7 @@ int require_plain_state(const OalPlan* p, const Level& L) {
 // WALE needs the previous step's velocity: rebuild it from the populations just set
 int refresh_velocity(OalPlan* p, Level& L) {
     if (!p->wale) return OAL_OK;
-    OAL_HIP(launch_macro(p->storage, L.fA.p, nullptr, nullptr, L.fstride, L.ncell, permute_of(p), L.velA.p, p->stream));
+    OAL_HIP(launch_macro(p->storage, L.fA.p, nullptr, nullptr, L.ncell, L.ncell, permute_of(p), L.velA.p, p->stream));
     return OAL_OK;
 }
 
@@ -1381,9 +1371,9 @@ int oal_upload_f(OalPlan* p, int level, const void* host_f, int dt) {
     hipError_t e = hipMemcpyAsync(stage, host_f, bytes, hipMemcpyHostToDevice, p->stream);
     // ghost cells of a refined level start at rest (those no fill reaches keep that state)
     if (e == hipSuccess && L.ncell > L.ncell_host)
-        e = launch_init_eq(p->storage, L.fA.p, L.fstride, 0, nullptr, nullptr, 1.0, 0.0, 0.0, 0.0, permute_of(p), nullptr,
+        e = launch_init_eq(p->storage, L.fA.p, L.ncell, 0, nullptr, nullptr, 1.0, 0.0, 0.0, 0.0, permute_of(p), nullptr,
                            p->stream);
-    if (e == hipSuccess) e = launch_convert_in(p->storage, L.fA.p, stage, dt, L.fstride, L.ncell_host, permute_of(p), p->stream);
+    if (e == hipSuccess) e = launch_convert_in(p->storage, L.fA.p, stage, dt, L.ncell, L.ncell_host, permute_of(p), p->stream);
     if (e == hipSuccess && L.fB.p) e = hipMemcpyAsync(L.fB.p, L.fA.p, L.fA.n, hipMemcpyDeviceToDevice, p->stream);
     if (e == hipSuccess) e = aa_prep(p, L);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
@@ -1406,7 +1396,7 @@ int oal_download_f(OalPlan* p, int level, void* host_f, int dt) {
     const size_t bytes = (size_t)L.ncell_host * 27 * (dt == OAL_F64 ? 8 : 4);
     void* stage = nullptr;
     OAL_HIP(hipMalloc(&stage, bytes));
-    hipError_t e = launch_convert_out(p->storage, L.fA.p, stage, dt, L.fstride, L.ncell_host, permute_of(p), p->stream);
+    hipError_t e = launch_convert_out(p->storage, L.fA.p, stage, dt, L.ncell, L.ncell_host, permute_of(p), p->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(host_f, stage, bytes, hipMemcpyDeviceToHost, p->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
     (void)hipFree(stage);
@@ -1431,7 +1421,7 @@ int oal_init_equilibrium(OalPlan* p, int level, const double* rho, const double*
     }
     const double ux0 = u0 ? u0[0] : 0.0, uy0 = u0 ? u0[1] : 0.0, uz0 = u0 ? u0[2] : 0.0;
     if (e == hipSuccess)
-        e = launch_init_eq(p->storage, L.fA.p, L.fstride, L.ncell_host, drho, du, rho0, ux0, uy0, uz0, permute_of(p),
+        e = launch_init_eq(p->storage, L.fA.p, L.ncell, L.ncell_host, drho, du, rho0, ux0, uy0, uz0, permute_of(p),
                            p->wale ? (void*)L.velA.p : nullptr, p->stream);
     if (e == hipSuccess && L.fB.p) e = hipMemcpyAsync(L.fB.p, L.fA.p, L.fA.n, hipMemcpyDeviceToDevice, p->stream);
     if (e == hipSuccess) e = aa_prep(p, L);
@@ -1498,8 +1488,7 @@ int oal_mass(OalPlan* p, int level, double* out, void* stream) {
     OAL_HIP(hipSetDevice(p->device));
     hipStream_t s = pick_stream(p, stream);
     if (int rc = ml_sync_covered(p, s)) return rc;
-    OAL_HIP(launch_mass(p->storage, L.fA.p, L.solid.p, L.borg.p, L.bdom.p, L.bflags.p, L.doms.p, L.nb, L.fstride, permute_of(p),
-                        L.mpart.p, s));
+    OAL_HIP(launch_mass(p->storage, L.fA.p, L.solid.p, L.borg.p, L.bdom.p, L.bflags.p, L.doms.p, L.nb, permute_of(p), L.mpart.p, s));
     // the reduction walks blocks here: one "item" per block
     ReduceArgs r{L.mpart.p, L.dom_block0.p, L.red_out.p, L.n_domains, 4};
     OAL_HIP(launch_reduce(r, s));
@@ -1517,7 +1506,7 @@ int oal_download_macro(OalPlan* p, int level, double* rho, double* u) {
     if (int rc = ml_sync_covered(p, p->stream)) return rc;
     double* d = nullptr;
     OAL_HIP(hipMalloc((void**)&d, (size_t)L.ncell_host * 32));
-    hipError_t e = launch_macro(p->storage, L.fA.p, d, d + L.ncell_host, L.fstride, L.ncell_host, permute_of(p), nullptr, p->stream);
+    hipError_t e = launch_macro(p->storage, L.fA.p, d, d + L.ncell_host, L.ncell, L.ncell_host, permute_of(p), nullptr, p->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
     if (e == hipSuccess && rho) e = hipMemcpy(rho, d, (size_t)L.ncell_host * 8, hipMemcpyDeviceToHost);
     if (e == hipSuccess && u) e = hipMemcpy(u, d + L.ncell_host, (size_t)L.ncell_host * 24, hipMemcpyDeviceToHost);
@@ -1534,8 +1523,8 @@ int oal_guard(OalPlan* p, int level, float rho_min, float u_max, int64_t* bad) {
     if (int rc = ml_sync_covered(p, p->stream)) return rc;
     unsigned long long h = 0;
     OAL_HIP(hipMemsetAsync(L.guard.p, 0, sizeof(unsigned long long), p->stream));
-    OAL_HIP(launch_guard(p->storage, L.fA.p, L.solid.p, L.borg.p, L.bdom.p, L.bflags.p, L.doms.p, L.nb, L.fstride, rho_min, u_max,
-                         L.guard.p, p->stream));
+    OAL_HIP(launch_guard(p->storage, L.fA.p, L.solid.p, L.borg.p, L.bdom.p, L.bflags.p, L.doms.p, L.nb, rho_min, u_max, L.guard.p,
+                         p->stream));
     OAL_HIP(hipStreamSynchronize(p->stream));
     OAL_HIP(hipMemcpy(&h, L.guard.p, sizeof(h), hipMemcpyDeviceToHost));
     if (bad) *bad = (int64_t)h;

@@ -114,17 +114,21 @@ __device__ __forceinline__ int classify(const StepArgs& a, const CellCtx& c, uin
 }
 
 // Regular interior blocks: the block's neighbours sit at lexicographic offsets (b +- 1, +- nbx,
-// +- nbx*nby), so a source cell's index is arithmetic.  Each lane forms, once, the byte offsets of its
-// nine in-plane source classes (cx, cy) - own cell shifted by -c in x and y, into the neighbour block
-// where it crosses - and every direction's access adds a uniform z part to its scalar base pointer.
+// +- nbx*nby), so a source cell's address is arithmetic.  Each lane forms, once, the byte offsets of
+// its nine in-plane source classes (cx, cy) - own cell shifted by -c in x and y, into the neighbour
+// block where it crosses - relative to the start of its own block; every direction's access adds a
+// uniform part (the block, the direction slot, the z plane) to its scalar base pointer.
 template <class SP>
 struct RegAddr {
-    uint32_t v[3][3];          // [cx+1][cy+1]: byte offset of cell (x - cx, y - cy, zref)
-    int64_t plane_cells;       // cells between consecutive block layers (nbx * nby * BS3)
+    int32_t v[3][3];           // [cx+1][cy+1]: byte offset of cell (x - cx, y - cy, zref) from the own block
+    int64_t blk0;              // element offset of the own block: b * 27 * BS3
+    int64_t plane_elems;       // elements between consecutive block layers (nbx * nby * 27 * BS3)
     int zref;                  // the plane the offsets were formed for (other planes: uniform shift)
     __device__ __forceinline__ void init(const CellCtx& c, int nbx, int nby) {
-        plane_cells = (int64_t)nbx * nby * BS3;
+        blk0 = (int64_t)c.b * (27 * BS3);
+        plane_elems = (int64_t)nbx * nby * (27 * BS3);
         zref = c.z;
+        const int32_t local = (int32_t)(c.own - c.b * BS3);   // zref * BS2 + ly * BS + lx
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -133,16 +137,16 @@ struct RegAddr {
                 const int sx = c.lx - cx, sy = c.ly - cy;
                 const int dx = sx < 0 ? -1 : (sx >= BS ? 1 : 0);
                 const int dy = sy < 0 ? -1 : (sy >= BS ? 1 : 0);
-                const int32_t idx = (int32_t)c.own + dx * BS3 + dy * nbx * BS3 + (sx - BS * dx - c.lx) + BS * (sy - BS * dy - c.ly);
-                v[i][j] = (uint32_t)idx * (uint32_t)sizeof(typename SP::st);
+                const int32_t idx = (dx + dy * nbx) * (27 * BS3) + local + (sx - BS * dx - c.lx) + BS * (sy - BS * dy - c.ly);
+                v[i][j] = idx * (int32_t)sizeof(typename SP::st);
             }
     }
     // uniform base of direction-slot `slot` for plane z of the block and a source shifted by -cz in z
-    __device__ __forceinline__ const char* base(const void* f, uint32_t n, int slot, int z, int cz) const {
+    __device__ __forceinline__ const char* base(const void* f, int slot, int z, int cz) const {
         const int sz = z - cz;
         const int dz = sz < 0 ? -1 : (sz >= BS ? 1 : 0);
-        const int64_t cells = (int64_t)slot * n + dz * plane_cells + (int64_t)(sz - BS * dz - zref) * BS2;
-        return (const char*)f + cells * (int64_t)sizeof(typename SP::st);
+        const int64_t e = blk0 + (int64_t)slot * BS3 + dz * plane_elems + (int64_t)(sz - BS * dz - zref) * BS2;
+        return (const char*)f + e * (int64_t)sizeof(typename SP::st);
     }
 };
 
@@ -180,7 +184,7 @@ __device__ __forceinline__ bool face_plane(const FaceBits& f, int z) {
 template <class SP, int PAT, int J>
 __device__ __forceinline__ typename SP::T own_prev(const StepArgs& a, const CellCtx& c) {
     constexpr int slot = PAT == PAT_AB ? J : dopp(J);
-    return SP::ld((const typename SP::st*)a.fin + (size_t)slot * a.ncell + c.own, wgt<typename SP::T>(J));
+    return SP::ld((const typename SP::st*)a.fin + pop_index(c.own, slot), wgt<typename SP::T>(J));
 }
 
 template <int J>
@@ -315,10 +319,10 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
         constexpr T w = wgt<T>(K);
         T val;
         if constexpr (REG && !GEN && PAT == PAT_AA_EVEN) {
-            val = SP::ld((const st*)(ra.base(fi, n, K, c.z, 0) + ra.v[1][1]), w);
+            val = SP::ld((const st*)(ra.base(fi, K, c.z, 0) + ra.v[1][1]), w);
         } else if constexpr (REG && !GEN) {
             constexpr int slot = PAT == PAT_AB ? K : dopp(K);
-            val = SP::ld((const st*)(ra.base(fi, n, slot, c.z, dcz(K)) + ra.v[dcx(K) + 1][dcy(K) + 1]), w);
+            val = SP::ld((const st*)(ra.base(fi, slot, c.z, dcz(K)) + ra.v[dcx(K) + 1][dcy(K) + 1]), w);
         } else if constexpr (PAT == PAT_AA_EVEN) {
             // inlet links are evaluated here in full precision, as the pull step does, rather than
             // read back from a slot that went through the storage rounding
@@ -327,16 +331,16 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
             if (inlet)
                 val = boundary_value<K, T>(SRC_INLET, a, c, u_in, a.step(), [&](auto) { return T(0); });
             else
-                val = SP::ld(fi + (size_t)K * n + c.own, w);
+                val = SP::ld(fi + pop_index(c.own, K), w);
         } else {
             uint32_t src = 0;
             const int kind = classify<K, GEN, REG>(a, c, src);
             if (kind == SRC_REG) {
                 constexpr int slot = PAT == PAT_AB ? K : dopp(K);
                 if constexpr (REG)
-                    val = SP::ld((const st*)(ra.base(fi, n, slot, c.z, dcz(K)) + ra.v[dcx(K) + 1][dcy(K) + 1]), w);
+                    val = SP::ld((const st*)(ra.base(fi, slot, c.z, dcz(K)) + ra.v[dcx(K) + 1][dcy(K) + 1]), w);
                 else
-                    val = SP::ld(fi + (size_t)slot * n + src, w);
+                    val = SP::ld(fi + pop_index(src, slot), w);
             } else if (PAT == PAT_AA_ODD && kind == SRC_OWN) {
                 // outlet: the own population of direction K (cx = -1) sits in slot (x, opp K), which
                 // the west neighbour overwrites during this step; the even step saved a copy
@@ -392,29 +396,29 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
         constexpr int J = dopp(K);   // the source of direction J is x + c_K, where f*_K goes
         constexpr T wK = wgt<T>(K);
         if constexpr (REG && PAT == PAT_AB) {
-            SP::sto_r((st*)(ra.base(fo, n, K, c.z, 0) + ra.v[1][1]), f[K], wK, rb(K), dith);
+            SP::sto_r((st*)(ra.base(fo, K, c.z, 0) + ra.v[1][1]), f[K], wK, rb(K), dith);
         } else if constexpr (REG && PAT == PAT_AA_EVEN) {
-            SP::sto_r((st*)(ra.base(fo, n, J, c.z, 0) + ra.v[1][1]), f[K], wK, rb(K), dith);
+            SP::sto_r((st*)(ra.base(fo, J, c.z, 0) + ra.v[1][1]), f[K], wK, rb(K), dith);
             if constexpr (GEN && dcx(K) < 0) {
                 if (outlet_cell)
                     SP::sto_r((st*)a.obuf + ((size_t)a.oslot[b] * 9 + K / 3) * BS2 + c.ly * BS + c.z, f[K], wK, rb(K), dith);
             }
         } else if constexpr (REG && !GEN) {
-            SP::sto_r((st*)(ra.base(fo, n, K, c.z, -dcz(K)) + ra.v[1 - dcx(K)][1 - dcy(K)]), f[K], wK, rb(K), dith);
+            SP::sto_r((st*)(ra.base(fo, K, c.z, -dcz(K)) + ra.v[1 - dcx(K)][1 - dcy(K)]), f[K], wK, rb(K), dith);
         } else if constexpr (REG) {
             uint32_t dst = 0;
             const int kind = classify<J, GEN, REG>(a, c, dst);
             if (kind == SRC_REG) {
-                SP::sto_r((st*)(ra.base(fo, n, K, c.z, -dcz(K)) + ra.v[1 - dcx(K)][1 - dcy(K)]), f[K], wK, rb(K), dith);
+                SP::sto_r((st*)(ra.base(fo, K, c.z, -dcz(K)) + ra.v[1 - dcx(K)][1 - dcy(K)]), f[K], wK, rb(K), dith);
             } else if (kind != SRC_INLET) {
                 const T v = boundary_value<J, T>(kind, a, c, u_in, a.step() + 1,
                                                  [&](auto jc) { return f[decltype(jc)::value]; });
-                SP::sto_r(fo + (size_t)J * n + c.own, v, wgt<T>(J), rb(rule_dir<J>(kind)), dith);
+                SP::sto_r(fo + pop_index(c.own, J), v, wgt<T>(J), rb(rule_dir<J>(kind)), dith);
             }
         } else if constexpr (PAT == PAT_AB) {
-            SP::sto_r(fo + (size_t)K * n + c.own, f[K], wK, rb(K), dith);
+            SP::sto_r(fo + pop_index(c.own, K), f[K], wK, rb(K), dith);
         } else if constexpr (PAT == PAT_AA_EVEN) {
-            SP::sto_r(fo + (size_t)J * n + c.own, f[K], wK, rb(K), dith);
+            SP::sto_r(fo + pop_index(c.own, J), f[K], wK, rb(K), dith);
             if constexpr (GEN && dcx(K) < 0) {
                 if (outlet_cell)
                     SP::sto_r((st*)a.obuf + ((size_t)a.oslot[b] * 9 + K / 3) * BS2 + c.ly * BS + c.z, f[K], wK, rb(K), dith);
@@ -423,11 +427,11 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
             uint32_t dst = 0;
             const int kind = classify<J, GEN>(a, c, dst);
             if (kind == SRC_REG) {
-                SP::sto_r(fo + (size_t)K * n + dst, f[K], wK, rb(K), dith);
+                SP::sto_r(fo + pop_index(dst, K), f[K], wK, rb(K), dith);
             } else if (kind != SRC_INLET) {   // inlet links are evaluated by the even step itself
                 const T v = boundary_value<J, T>(kind, a, c, u_in, a.step() + 1,
                                                  [&](auto jc) { return f[decltype(jc)::value]; });
-                SP::sto_r(fo + (size_t)J * n + c.own, v, wgt<T>(J), rb(rule_dir<J>(kind)), dith);
+                SP::sto_r(fo + pop_index(c.own, J), v, wgt<T>(J), rb(rule_dir<J>(kind)), dith);
             }
         }
         if constexpr (GEN) {
@@ -515,7 +519,7 @@ __device__ __forceinline__ void reg_load_raw(const StepArgs& a, const RegAddr<SP
     using st = typename SP::st;
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
-        r[K] = SP::ld_raw((const st*)(ra.base(a.fin, a.ncell, K, z, dcz(K)) + ra.v[dcx(K) + 1][dcy(K) + 1]));
+        r[K] = SP::ld_raw((const st*)(ra.base(a.fin, K, z, dcz(K)) + ra.v[dcx(K) + 1][dcy(K) + 1]));
     });
 }
 
@@ -558,7 +562,7 @@ __device__ __forceinline__ void reg_collide_store(const StepArgs& a, const RegAd
     const uint32_t hc = SP::dithered ? round_cell(round_key(a.step()), cell) : 0u;
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
-        SP::sto_r((st*)(ra.base(a.fout, a.ncell, K, z, 0) + ra.v[1][1]), f[K], wgt<T>(K), round_bits(hc, K), dith);
+        SP::sto_r((st*)(ra.base(a.fout, K, z, 0) + ra.v[1][1]), f[K], wgt<T>(K), round_bits(hc, K), dith);
     });
 }
 
@@ -573,32 +577,32 @@ struct FaceRules {
 };
 
 // Gather with the domain-face rules.  Every read is addressed by its byte offset from the start of
-// fin, formed in 32-bit arithmetic modulo 2^32: the host uses this form only for levels whose
-// populations span less than 4 GiB, where every offset read is exact.  The rules pick, per lane, one
+// the own block, formed in 32-bit arithmetic modulo 2^32 and read as a signed offset: the host uses
+// this form only where a layer of blocks spans less than 2 GiB, so every offset read is exact.  The rules pick, per lane, one
 // of a few candidate offsets that are all computed (uniform parts in scalar registers), so that the
 // choice is a select rather than a branch.
 template <class SP>
 __device__ __forceinline__ void reg_load_raw_face(const StepArgs& a, const RegAddr<SP>& ra, const FaceRules& fr, int z,
                                                   typename SP::raw (&r)[27]) {
     using st = typename SP::st;
-    const char* f0 = (const char*)a.fin;
-    const uint32_t nbytes = a.ncell * (uint32_t)sizeof(st);   // one direction slot
+    const char* f0 = (const char*)a.fin + ra.blk0 * (int64_t)sizeof(st);   // the own block (uniform)
+    const uint32_t nbytes = BS3 * (uint32_t)sizeof(st);                    // one direction slot
     uint32_t zs[3];                                           // plane shift of sources with cz = -1, 0, 1
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         const int sz = z - (q - 1);
         const int dz = sz < 0 ? -1 : (sz >= BS ? 1 : 0);
-        zs[q] = (uint32_t)((dz * ra.plane_cells + (int64_t)(sz - BS * dz - ra.zref) * BS2) * (int64_t)sizeof(st));
+        zs[q] = (uint32_t)((dz * ra.plane_elems + (int64_t)(sz - BS * dz - ra.zref) * BS2) * (int64_t)sizeof(st));
     }
     const bool zlo = fr.zlo && z == 0, zhi = fr.zhi && z == BS - 1;   // uniform
     // uniform parts pass through readfirstlane, which the optimiser cannot see through: it must not
     // fold them into per-lane sums and hoist 27 of those out of the plane loop
     auto uni = [](uint32_t u) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)u); };
-    const uint32_t v11 = ra.v[1][1];
+    const uint32_t v11 = (uint32_t)ra.v[1][1];
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
         constexpr int cx = dcx(K), cy = dcy(K), cz = dcz(K);
-        uint32_t o = ra.v[cx + 1][cy + 1] + uni((uint32_t)K * nbytes + zs[cz + 1]);
+        uint32_t o = (uint32_t)ra.v[cx + 1][cy + 1] + uni((uint32_t)K * nbytes + zs[cz + 1]);
         auto own = [&](uint32_t slot) { return v11 + uni(slot * nbytes + zs[1]); };
         // lowest precedence first; each later rule overrides (kernel_cumulant.jl:233-345)
         if constexpr (cz != 0) {
@@ -619,7 +623,7 @@ __device__ __forceinline__ void reg_load_raw_face(const StepArgs& a, const RegAd
             const uint32_t o_x = own(K);
             o = (cx > 0 ? fr.xl : fr.xh) ? o_x : o;    // outlet: own population; inlet: any valid address
         }
-        r[K] = SP::ld_raw((const st*)(f0 + o));
+        r[K] = SP::ld_raw((const st*)(f0 + (int32_t)o));   // within +-2 GiB of the own block
     });
 }
 
