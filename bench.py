@@ -92,11 +92,10 @@ def measured_traffic(kernel: str):
     if not os.path.exists(path):
         return None
     text = open(path).read()
-    k = re.search(r"^\s*kernel = (.+)$", text, re.M)
-    b = re.search(r"^\s*hbm_bytes_per_launch = (\d+)$", text, re.M)
-    if not (k and b) or k.group(1).strip() != kernel:
-        return None
-    return int(b.group(1))
+    for k, b in re.findall(r"^\s*kernel = (.+)\n\s*hbm_bytes_per_launch = (\d+)$", text, re.M):
+        if k.strip() == kernel:
+            return int(b)
+    return None
 
 
 def cpu_baseline(grid, budget_s: float = 20.0):
