@@ -26,6 +26,15 @@ Parity status of each oracle function and what pins it (tests/test_oracle_*.py):
   + momentum exchange   velocity profile vs the parabola
   domain-face rules     pinned: uniform inflow through inlet/outlet/free-slip stays uniform; mirror
                         rules by symmetry of a mirrored run
+  WALE eddy viscosity   pinned: the model's tensor definition evaluated independently (einsum), zero in
+                        pure shear, degree-1 homogeneity, frame invariance (test_oracle_wale.py)
+  refined runs          pinned (octree.py, test_oracle_octree.py): cubic weights reproduce cubics,
+                        stencils are partitions of unity and reproduce linear fields, uniform flow
+                        across 2 and 3 levels, restriction of the prolongation on a freestream, shear
+                        wave across a refined slab at the physical viscosity with composite mass drift
+                        under 1e-4; the exact rounding of the transfers is parity unpinned (no
+                        reference vector)
+  golden fixtures       tests/golden/ (written from this oracle; regression pins, test_golden.py)
 """
 from __future__ import annotations
 
