@@ -364,7 +364,11 @@ template <> struct Vec16<double> {
     static constexpr int n = 2;
 };
 
-template <class SP>
+// RAW (16-bit storage, no blend): records of the 27 stored codes padded to 32 (64 bytes), decoded by
+// the rebuild; the decoded values are the ones a float record would hold.
+constexpr int XFER_RAW_STRIDE = 32;
+
+template <class SP, bool RAW>
 __global__ void __launch_bounds__(WG) k_xfer_gather(const typename SP::st* fnew, const typename SP::st* fold,
                                                     const uint32_t* cells, uint32_t n, uint32_t ncell, typename SP::T w1,
                                                     typename SP::T* out) {
@@ -373,6 +377,21 @@ __global__ void __launch_bounds__(WG) k_xfer_gather(const typename SP::st* fnew,
     if (i >= n) return;
     const uint32_t e = cells[i];
     const uint32_t c = e & 0x7fffffffu;
+    if constexpr (RAW) {
+        uint16_t codes[XFER_RAW_STRIDE];
+#pragma unroll
+        for (int k = 27; k < XFER_RAW_STRIDE; ++k) codes[k] = 0;
+        for_dirs([&](auto kc) {
+            constexpr int K = decltype(kc)::value;
+            codes[K] = (uint16_t)SP::ld_raw(fnew + pop_index(c, K));
+        });
+        uint4* o = reinterpret_cast<uint4*>(reinterpret_cast<uint16_t*>(out) + (size_t)i * XFER_RAW_STRIDE);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            o[j] = make_uint4(codes[8 * j] | ((uint32_t)codes[8 * j + 1] << 16), codes[8 * j + 2] | ((uint32_t)codes[8 * j + 3] << 16),
+                              codes[8 * j + 4] | ((uint32_t)codes[8 * j + 5] << 16), codes[8 * j + 6] | ((uint32_t)codes[8 * j + 7] << 16));
+        return;
+    }
     const bool blend = fold && !(e >> 31);
     const T w0 = T(1) - w1;
     T vals[XFER_STRIDE];
@@ -396,7 +415,7 @@ __global__ void __launch_bounds__(WG) k_xfer_gather(const typename SP::st* fnew,
     }
 }
 
-template <class SP>
+template <class SP, bool RAW>
 __global__ void __launch_bounds__(WG) k_xfer_rebuild(typename SP::st* f, uint32_t ncell, const uint32_t* tcells, uint32_t n,
                                                      const int32_t* slot, const float* wt, const typename SP::T* m,
                                                      typename SP::T scale, typename SP::T keep) {
@@ -412,13 +431,27 @@ __global__ void __launch_bounds__(WG) k_xfer_rebuild(typename SP::st* f, uint32_
         const int32_t s = slot[(size_t)nn * n + i];
         if (s < 0) continue;
         const T wn = T(wt[(size_t)nn * n + i]);
-        const V* row = reinterpret_cast<const V*>(m + (size_t)s * XFER_STRIDE);
+        if constexpr (RAW) {
+            const uint4* row = reinterpret_cast<const uint4*>(reinterpret_cast<const uint16_t*>(m) + (size_t)s * XFER_RAW_STRIDE);
 #pragma unroll
-        for (int j = 0; j < XFER_STRIDE / VN; ++j) {
-            const V v = row[j];
+            for (int j = 0; j < 4; ++j) {
+                const uint4 q = row[j];
+                const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-            for (int e = 0; e < VN; ++e)
-                if (j * VN + e < 27) acc[j * VN + e] = acc[j * VN + e] + wn * v[e];
+                for (int e = 0; e < 8; ++e) {
+                    const int k = 8 * j + e;
+                    if (k < 27) acc[k] = acc[k] + wn * SP::from_raw((wd[e >> 1] >> (16 * (e & 1))) & 0xffffu, wgt<T>(k));
+                }
+            }
+        } else {
+            const V* row = reinterpret_cast<const V*>(m + (size_t)s * XFER_STRIDE);
+#pragma unroll
+            for (int j = 0; j < XFER_STRIDE / VN; ++j) {
+                const V v = row[j];
+#pragma unroll
+                for (int e = 0; e < VN; ++e)
+                    if (j * VN + e < 27) acc[j * VN + e] = acc[j * VN + e] + wn * v[e];
+            }
         }
     }
     // moments of the interpolated state
@@ -493,21 +526,21 @@ __global__ void __launch_bounds__(WG) k_copy_cells(typename SP::st* dst, const t
     for (int k = 0; k < 27; ++k) dst[pop_index(c, k)] = src[pop_index(c, k)];
 }
 
-template <class SP>
+template <class SP, bool RAW>
 static hipError_t xfer_sp(const void* fnew, const void* fold, const uint32_t* cells, uint32_t n, uint32_t ncell, double w1,
                           void* out, hipStream_t s) {
     using T = typename SP::T;
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL((k_xfer_gather<SP>), dim3((n + WG - 1) / WG), dim3(WG), 0, s, (const typename SP::st*)fnew,
+    hipLaunchKernelGGL((k_xfer_gather<SP, RAW>), dim3((n + WG - 1) / WG), dim3(WG), 0, s, (const typename SP::st*)fnew,
                        (const typename SP::st*)fold, cells, n, ncell, T(w1), (T*)out);
     return hipGetLastError();
 }
-template <class SP>
+template <class SP, bool RAW>
 static hipError_t rebuild_sp(void* f, uint32_t ncell, const uint32_t* tcells, uint32_t n, const int32_t* slot, const float* w,
                              const void* m, double scale, double keep, hipStream_t s) {
     using T = typename SP::T;
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL((k_xfer_rebuild<SP>), dim3((n + WG - 1) / WG), dim3(WG), 0, s, (typename SP::st*)f, ncell, tcells, n,
+    hipLaunchKernelGGL((k_xfer_rebuild<SP, RAW>), dim3((n + WG - 1) / WG), dim3(WG), 0, s, (typename SP::st*)f, ncell, tcells, n,
                        slot, w, (const T*)m, T(scale), T(keep));
     return hipGetLastError();
 }
@@ -519,23 +552,29 @@ static hipError_t copy_cells_sp(void* dst, const void* src, const uint32_t* cell
     return hipGetLastError();
 }
 
+// raw records when the storage is 16-bit and no time blend is asked for
 hipError_t launch_xfer_gather(int storage, const void* fnew, const void* fold, const uint32_t* cells, uint32_t n,
                                uint32_t ncell, double w1, void* out, hipStream_t s) {
+    const bool raw = fold == nullptr;
     switch (storage) {
-        case 0: return xfer_sp<StF32>(fnew, fold, cells, n, ncell, w1, out, s);
-        case 1: return xfer_sp<StF64>(fnew, fold, cells, n, ncell, w1, out, s);
-        case 2: return xfer_sp<StF16S>(fnew, fold, cells, n, ncell, w1, out, s);
-        case 3: return xfer_sp<StBF16S>(fnew, fold, cells, n, ncell, w1, out, s);
+        case 0: return xfer_sp<StF32, false>(fnew, fold, cells, n, ncell, w1, out, s);
+        case 1: return xfer_sp<StF64, false>(fnew, fold, cells, n, ncell, w1, out, s);
+        case 2: return raw ? xfer_sp<StF16S, true>(fnew, fold, cells, n, ncell, w1, out, s)
+                           : xfer_sp<StF16S, false>(fnew, fold, cells, n, ncell, w1, out, s);
+        case 3: return raw ? xfer_sp<StBF16S, true>(fnew, fold, cells, n, ncell, w1, out, s)
+                           : xfer_sp<StBF16S, false>(fnew, fold, cells, n, ncell, w1, out, s);
         default: return hipErrorInvalidValue;
     }
 }
-hipError_t launch_xfer_rebuild(int storage, void* f, uint32_t ncell, const uint32_t* tcells, uint32_t n, const int32_t* slot,
-                               const float* w, const void* m, double scale, double keep, hipStream_t s) {
+hipError_t launch_xfer_rebuild(int storage, bool raw, void* f, uint32_t ncell, const uint32_t* tcells, uint32_t n,
+                               const int32_t* slot, const float* w, const void* m, double scale, double keep, hipStream_t s) {
     switch (storage) {
-        case 0: return rebuild_sp<StF32>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
-        case 1: return rebuild_sp<StF64>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
-        case 2: return rebuild_sp<StF16S>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
-        case 3: return rebuild_sp<StBF16S>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
+        case 0: return rebuild_sp<StF32, false>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
+        case 1: return rebuild_sp<StF64, false>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
+        case 2: return raw ? rebuild_sp<StF16S, true>(f, ncell, tcells, n, slot, w, m, scale, keep, s)
+                           : rebuild_sp<StF16S, false>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
+        case 3: return raw ? rebuild_sp<StBF16S, true>(f, ncell, tcells, n, slot, w, m, scale, keep, s)
+                           : rebuild_sp<StBF16S, false>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -932,7 +932,7 @@ int ml_fill(OalPlan* p, int l, double w1, hipStream_t s) {
     if (!F.n_gdst) return OAL_OK;
     OAL_HIP(launch_xfer_gather(p->storage, C.fA.p, w1 == 1.0 ? nullptr : C.fB.p, F.g_src.p, F.n_gsrc, C.ncell, w1,
                                 F.g_m.p, s));
-    OAL_HIP(launch_xfer_rebuild(p->storage, F.fA.p, F.ncell, F.g_dst.p, F.n_gdst, F.g_slot.p, F.g_w.p, F.g_m.p,
+    OAL_HIP(launch_xfer_rebuild(p->storage, w1 == 1.0, F.fA.p, F.ncell, F.g_dst.p, F.n_gdst, F.g_slot.p, F.g_w.p, F.g_m.p,
                                 F.fill_scale, F.fill_keep, s));
     return OAL_OK;
 }
@@ -945,7 +945,7 @@ int ml_restrict(OalPlan* p, int l, hipStream_t s, bool full = false) {
     const uint32_t nd = full ? C.n_rfdst : C.n_rdst, ns = full ? C.n_rfsrc : C.n_rsrc;
     if (!nd) return OAL_OK;
     OAL_HIP(launch_xfer_gather(p->storage, F.fA.p, nullptr, full ? C.rf_src.p : C.r_src.p, ns, F.ncell, 1.0, C.r_m.p, s));
-    OAL_HIP(launch_xfer_rebuild(p->storage, C.fA.p, C.ncell, full ? C.rf_dst.p : C.r_dst.p, nd,
+    OAL_HIP(launch_xfer_rebuild(p->storage, true, C.fA.p, C.ncell, full ? C.rf_dst.p : C.r_dst.p, nd,
                                 full ? C.rf_slot.p : C.r_slot.p, full ? C.rf_w.p : C.r_w.p, C.r_m.p, C.restrict_scale,
                                 C.restrict_keep, s));
     return OAL_OK;
