@@ -122,14 +122,14 @@ struct StF16S {
         return (float)__builtin_bit_cast(_Float16, (uint16_t)r) * inv_scale + w;
     }
     __device__ static __forceinline__ T ld(const st* p, T w) { return (float)(*p) * inv_scale + w; }
-    // FP32 -> FP16: to nearest (hardware conversion), or stochastic in the normal range: the 13
-    // dropped mantissa bits plus 13 random bits, truncated (carries into the exponent are exact)
+    // FP32 -> FP16: to nearest, or stochastic with the hardware conversion: the random bits r >> 19
+    // added below the result's last place, then truncated (normal and subnormal results alike)
     __device__ static __forceinline__ uint16_t enc(float x, uint32_t r, bool sr) {
-        const uint32_t u = __float_as_uint(x);
-        const uint32_t mag = u & 0x7fffffffu;
-        const uint32_t m = mag + (r >> 19);
-        if (sr && mag >= 0x38800000u && m < 0x47800000u)
-            return (uint16_t)(((u >> 16) & 0x8000u) | ((m >> 13) - (112u << 10)));
+        if (sr) {
+            typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+            const h2 v = __builtin_amdgcn_cvt_sr_f16_f32(h2{0, 0}, x, r, false);
+            return __builtin_bit_cast(uint16_t, v[0]);
+        }
         return __builtin_bit_cast(uint16_t, (_Float16)x);
     }
     __device__ static __forceinline__ float dec(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
@@ -158,8 +158,11 @@ struct StBF16S {
     // FP32 -> BF16: to nearest even, or stochastic for finite values: the 16 dropped bits plus 16
     // random bits, truncated (subnormals included: their encoding is linear too)
     __device__ static __forceinline__ uint16_t enc(float x, uint32_t r, bool sr) {
-        const uint32_t u = __float_as_uint(x);
-        if (sr && (u & 0x7f800000u) != 0x7f800000u) return (uint16_t)((u + (r >> 16)) >> 16);
+        if (sr) {   // the hardware conversion: (u + (r >> 16)) >> 16 for finite values
+            typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+            const b2 v = __builtin_amdgcn_cvt_sr_bf16_f32(b2{0, 0}, x, r, false);
+            return __builtin_bit_cast(uint16_t, v[0]);
+        }
         return f32_to_bf16_rne(x);
     }
     __device__ static __forceinline__ void sto(st* p, T f, T w) { *p = f32_to_bf16_rne(f - w); }

@@ -137,22 +137,29 @@ def _decode(oracle, storage, codes):
 
 def test_f16_stochastic_rounding_is_exact_in_mean(oracle):
     """Over all 2^13 values of the random bits, the stochastically rounded FP16 code averages to the
-    FP32 input exactly, and every result is one of the two neighbouring FP16 values."""
+    FP32 input exactly, and every result is one of the two neighbouring FP16 values: in the normal range,
+    and for subnormal results whenever the input is a multiple of 2^-37 (the 13 random bits sit below
+    the subnormal spacing 2^-24); other subnormal inputs average to within 2^-37."""
     rng = np.random.default_rng(11)
     sign = np.where(rng.random(40) < 0.5, -1.0, 1.0)
     x = (sign * rng.uniform(1, 2, 40) * 2.0 ** rng.integers(-14, 15, 40)).astype(np.float32)   # FP16 normal range
     x = np.concatenate([x, np.float32([2.0 ** -14, -(2.0 ** -14), 1.0, 65000.0])])
+    grid = (rng.integers(1, 2 ** 23, 30) * 2.0 ** -37 * np.where(rng.random(30) < 0.5, -1.0, 1.0)).astype(np.float32)
     d = np.arange(8192, dtype=np.uint32) << 19
-    # below the normal range the conversion rounds to nearest, whatever the random bits
-    tiny = np.float32([3e-7, -1.7630312e-07, 5e-8, 0.0])
-    for v in tiny:
-        assert set(oracle.f32_to_f16_sr(np.full(64, v, np.float32), d[::128]).tolist()) == {int(np.float16(v).view(np.uint16))}
-    for v in x:
+    for v in np.concatenate([x, grid]):
         codes = oracle.f32_to_f16_sr(np.full(8192, v, dtype=np.float32), d)
         vals = _decode(oracle, "f16", codes)
         assert vals.mean() == float(v)
-        # only the two FP16 neighbours of v occur, less than one FP16 spacing (2^-10 relative) apart
-        assert len(set(vals.tolist())) <= 2 and np.abs(vals - float(v)).max() <= abs(float(v)) * 2.0 ** -10
+        # only the two FP16 neighbours of v occur, less than one FP16 spacing apart
+        spacing = max(abs(float(v)) * 2.0 ** -10, 2.0 ** -24)
+        assert len(set(vals.tolist())) <= 2 and np.abs(vals - float(v)).max() < spacing
+    for v in np.float32([3e-7, -1.7630312e-07, 5e-8, 1e-12]):
+        vals = _decode(oracle, "f16", oracle.f32_to_f16_sr(np.full(8192, v, dtype=np.float32), d))
+        assert abs(vals.mean() - float(v)) <= 2.0 ** -37
+    # beyond the largest finite value: 65504 or infinity; NaN stays NaN
+    vals = _decode(oracle, "f16", oracle.f32_to_f16_sr(np.full(8192, np.float32(65519.0)), d))
+    assert set(vals.tolist()) == {65504.0, np.inf}
+    assert np.isnan(_decode(oracle, "f16", oracle.f32_to_f16_sr(np.float32([np.nan]), d[:1]))).all()
 
 
 def test_bf16_stochastic_rounding_is_exact_in_mean(oracle):
