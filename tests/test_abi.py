@@ -60,3 +60,39 @@ def test_operator_registry():
         get_collision_kernel("bgk")
     with pytest.raises(NotImplementedError):
         get_collision_kernel("entropic")
+
+
+def test_refined_plan_validation_without_gpu():
+    """Refined-plan descriptors are checked before any device call: nesting, margin, pattern, WALE."""
+    import numpy as np
+    from openaerolab_alpha_01_amd import _lib, solver
+    from openaerolab_alpha_01_amd.geometry import BlockGrid, block_octree, build_neighbor_table
+    params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
+    grids = block_octree((64, 16, 64), [{(1, 0, 1)}, {(2, 0, 2)}], periodic=(True, True, True))
+
+    def level_from(g, keep):
+        coords = np.asarray(g.coords)[keep]
+        nblk = tuple(int(v) // 16 for v in g.n)
+        return BlockGrid(n=g.n, periodic=g.periodic, coords=coords.astype(np.int32),
+                         neighbor=build_neighbor_table(coords, nblk, g.periodic))
+
+    def code_of(levels, **kw):
+        with pytest.raises(_lib.OalError) as e:
+            solver.octree_plan(levels, 0.56, params=kw.pop("params", params), **kw)
+        return e.value.code
+
+    # level 1 cut down to the 8 children around the finest block: no margin for level 2's ghosts
+    c1 = (np.asarray(grids[1].coords) - 1).tolist()
+    core = {(x, y, z) for x in (2, 3) for y in (0, 1) for z in (2, 3)}
+    assert code_of([grids[0], level_from(grids[1], [i for i, c in enumerate(c1) if tuple(c) in core]), grids[2]]) \
+        == _lib.OAL_ERR_ARG
+    # a child without its 7 siblings
+    c2 = (np.asarray(grids[2].coords) - 1).tolist()
+    assert code_of([grids[0], grids[1], level_from(grids[2], list(range(1, len(c2))))]) == _lib.OAL_ERR_ARG
+    # WALE and the AA pattern are not offered on refined plans
+    assert code_of(grids, params=solver.StepParams(c_wale=0.2, inlet_turbulence=0.0)) == _lib.OAL_ERR_UNSUPPORTED
+    levels = [solver.Level(g, [solver.Domain(extent=g.n, tau=t, periodic=g.periodic)])
+              for g, t in zip(grids, solver.acoustic_taus(0.56, 3))]
+    with pytest.raises(_lib.OalError) as e:
+        solver.Plan(levels, stream="aa", params=params)
+    assert e.value.code == _lib.OAL_ERR_UNSUPPORTED
