@@ -516,15 +516,6 @@ hipError_t launch_set_i32(int32_t* p, int32_t v, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <class SP>
-__global__ void __launch_bounds__(WG) k_copy_cells(typename SP::st* dst, const typename SP::st* src, const uint32_t* cells,
-                                                   uint32_t n, uint32_t ncell) {
-    const uint32_t i = blockIdx.x * WG + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t c = cells[i];
-#pragma unroll
-    for (int k = 0; k < 27; ++k) dst[pop_index(c, k)] = src[pop_index(c, k)];
-}
 
 template <class SP, bool RAW>
 static hipError_t xfer_sp(const void* fnew, const void* fold, const uint32_t* cells, uint32_t n, uint32_t ncell, double w1,
@@ -542,13 +533,6 @@ static hipError_t rebuild_sp(void* f, uint32_t ncell, const uint32_t* tcells, ui
     if (!n) return hipSuccess;
     hipLaunchKernelGGL((k_xfer_rebuild<SP, RAW>), dim3((n + WG - 1) / WG), dim3(WG), 0, s, (typename SP::st*)f, ncell, tcells, n,
                        slot, w, (const T*)m, T(scale), T(keep));
-    return hipGetLastError();
-}
-template <class SP>
-static hipError_t copy_cells_sp(void* dst, const void* src, const uint32_t* cells, uint32_t n, uint32_t ncell, hipStream_t s) {
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL((k_copy_cells<SP>), dim3((n + WG - 1) / WG), dim3(WG), 0, s, (typename SP::st*)dst,
-                       (const typename SP::st*)src, cells, n, ncell);
     return hipGetLastError();
 }
 
@@ -575,16 +559,6 @@ hipError_t launch_xfer_rebuild(int storage, bool raw, void* f, uint32_t ncell, c
                            : rebuild_sp<StF16S, false>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
         case 3: return raw ? rebuild_sp<StBF16S, true>(f, ncell, tcells, n, slot, w, m, scale, keep, s)
                            : rebuild_sp<StBF16S, false>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
-        default: return hipErrorInvalidValue;
-    }
-}
-hipError_t launch_copy_cells(int storage, void* dst, const void* src, const uint32_t* cells, uint32_t n, uint32_t ncell,
-                             hipStream_t s) {
-    switch (storage) {
-        case 0: return copy_cells_sp<StF32>(dst, src, cells, n, ncell, s);
-        case 1: return copy_cells_sp<StF64>(dst, src, cells, n, ncell, s);
-        case 2: return copy_cells_sp<StF16S>(dst, src, cells, n, ncell, s);
-        case 3: return copy_cells_sp<StBF16S>(dst, src, cells, n, ncell, s);
         default: return hipErrorInvalidValue;
     }
 }

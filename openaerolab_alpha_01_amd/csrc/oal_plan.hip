@@ -960,15 +960,6 @@ int ml_sync_covered(OalPlan* p, hipStream_t s) {
     return OAL_OK;
 }
 
-// After a step and its swap the new read buffer's ghost cells are two fills old: take the last fill
-// from the previous read buffer (coupling.jl:236-251).
-int ml_copy_ghosts(OalPlan* p, int l, hipStream_t s) {
-    Level& L = p->levels[l];
-    if (!L.n_gdst) return OAL_OK;
-    OAL_HIP(launch_copy_cells(p->storage, L.fA.p, L.fB.p, L.g_dst.p, L.n_gdst, L.ncell, s));
-    return OAL_OK;
-}
-
 // Initial consistency (couple_initial!, coupling.jl:253-268): restrict every level into its parent,
 // then fill every level's ghosts; both buffers of a level hold the result.
 int ml_couple(OalPlan* p, hipStream_t s) {
@@ -995,8 +986,9 @@ int ml_step_rec(OalPlan* p, int l, int32_t t, int flags, bool last, hipStream_t 
     Level& L = p->levels[l];
     const int f = (last && L.has_solid) ? flags : (flags & ~OAL_STEP_FORCE);
     if (int rc = level_step(p, l, t, f, s)) return rc;
-    if (l > 0)
-        if (int rc = ml_copy_ghosts(p, l, s)) return rc;
+    // the ghost cells of level l in its new read buffer are refilled by the parent before level l's
+    // next sub-step reads them; nothing reads them in between (transfers take stepped or covered
+    // donors only), so they need no copy from the other buffer
     if (l + 1 < (int)p->levels.size()) {
         if (int rc = ml_step_rec(p, l + 1, 2 * t, flags, false, s)) return rc;
         if (int rc = ml_restrict(p, l, s)) return rc;

@@ -487,9 +487,7 @@ class OctreeRun:
         p = O.make_params(self.taus[l], time_step=t, arith=O.FAST, **self.kw)
         _, F, _ = O.step(self.storage, self.G[l], p, self.A[l], out=self.B[l], want_force=want_force)
         self.A[l], self.B[l] = self.B[l], self.A[l]
-        if l > 0 and self.fill[l] is not None and len(self.fill[l].dst):
-            d = self.fill[l].dst
-            self.A[l][:, d] = self.B[l][:, d]
+        # ghost cells of the new read buffer are refilled before the level's next sub-step reads them
         return F
 
     def step_rec(self, l, t, last, force):
