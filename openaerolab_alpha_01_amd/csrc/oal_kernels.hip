@@ -427,30 +427,62 @@ __global__ void __launch_bounds__(WG) k_xfer_rebuild(typename SP::st* f, uint32_
     for (int k = 0; k < 27; ++k) acc[k] = T(0);
     using V = typename Vec16<T>::type;
     constexpr int VN = Vec16<T>::n;
-    for (int nn = 0; nn < 64; ++nn) {
-        const int32_t s = slot[(size_t)nn * n + i];
-        if (s < 0) continue;
-        const T wn = T(wt[(size_t)nn * n + i]);
+    // the 64 nodes in groups of G: the group's slots and weights, then all of its records, are in
+    // flight at once (the compiler would otherwise wait for each node's loads in turn); an unused node
+    // reads record 0 and leaves acc unchanged through a select; the sums are formed in node order
+    constexpr int G = sizeof(T) == 8 && !RAW ? 2 : 4;
+    for (int nb = 0; nb < 64; nb += G) {
+        int32_t sr[G];
+        T wn[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            sr[g] = slot[(size_t)(nb + g) * n + i];
+            wn[g] = T(wt[(size_t)(nb + g) * n + i]);
+        }
         if constexpr (RAW) {
-            const uint4* row = reinterpret_cast<const uint4*>(reinterpret_cast<const uint16_t*>(m) + (size_t)s * XFER_RAW_STRIDE);
+            uint4 rec[G][4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const uint4 q = row[j];
-                const uint32_t wd[4] = {q.x, q.y, q.z, q.w};
+            for (int g = 0; g < G; ++g) {
+                const uint4* row = reinterpret_cast<const uint4*>(reinterpret_cast<const uint16_t*>(m) +
+                                                                  (size_t)(sr[g] >= 0 ? sr[g] : 0) * XFER_RAW_STRIDE);
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int k = 8 * j + e;
-                    if (k < 27) acc[k] = acc[k] + wn * SP::from_raw((wd[e >> 1] >> (16 * (e & 1))) & 0xffffu, wgt<T>(k));
+                for (int j = 0; j < 4; ++j) rec[g][j] = row[j];
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const bool use = sr[g] >= 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t wd[4] = {rec[g][j].x, rec[g][j].y, rec[g][j].z, rec[g][j].w};
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const int k = 8 * j + e;
+                        if (k < 27) {
+                            const T add = acc[k] + wn[g] * SP::from_raw((wd[e >> 1] >> (16 * (e & 1))) & 0xffffu, wgt<T>(k));
+                            acc[k] = use ? add : acc[k];
+                        }
+                    }
                 }
             }
         } else {
-            const V* row = reinterpret_cast<const V*>(m + (size_t)s * XFER_STRIDE);
+            V rec[G][XFER_STRIDE / VN];
 #pragma unroll
-            for (int j = 0; j < XFER_STRIDE / VN; ++j) {
-                const V v = row[j];
+            for (int g = 0; g < G; ++g) {
+                const V* row = reinterpret_cast<const V*>(m + (size_t)(sr[g] >= 0 ? sr[g] : 0) * XFER_STRIDE);
 #pragma unroll
-                for (int e = 0; e < VN; ++e)
-                    if (j * VN + e < 27) acc[j * VN + e] = acc[j * VN + e] + wn * v[e];
+                for (int j = 0; j < XFER_STRIDE / VN; ++j) rec[g][j] = row[j];
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const bool use = sr[g] >= 0;
+#pragma unroll
+                for (int j = 0; j < XFER_STRIDE / VN; ++j)
+#pragma unroll
+                    for (int e = 0; e < VN; ++e)
+                        if (j * VN + e < 27) {
+                            const T add = acc[j * VN + e] + wn[g] * rec[g][j][e];
+                            acc[j * VN + e] = use ? add : acc[j * VN + e];
+                        }
             }
         }
     }
