@@ -430,7 +430,7 @@ __global__ void __launch_bounds__(WG) k_xfer_rebuild(typename SP::st* f, uint32_
     // the 64 nodes in groups of G: the group's slots and weights, then all of its records, are in
     // flight at once (the compiler would otherwise wait for each node's loads in turn); an unused node
     // reads record 0 and leaves acc unchanged through a select; the sums are formed in node order
-    constexpr int G = sizeof(T) == 8 && !RAW ? 2 : 4;
+    constexpr int G = RAW ? 8 : (sizeof(T) == 8 ? 2 : 4);
     for (int nb = 0; nb < 64; nb += G) {
         int32_t sr[G];
         T wn[G];
