@@ -314,6 +314,8 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
     if constexpr (REG) ra.init(c, dm->nbx, dm->nby);
 
     T f[27];
+    typename SP::raw raw[27];     // AB pattern: the loaded populations, decoded after all loads
+    uint32_t inlet_links = 0, rest_links = 0;
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
         constexpr T w = wgt<T>(K);
@@ -332,6 +334,24 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
                 val = boundary_value<K, T>(SRC_INLET, a, c, u_in, a.step(), [&](auto) { return T(0); });
             else
                 val = SP::ld(fi + pop_index(c.own, K), w);
+        } else if constexpr (PAT == PAT_AB) {
+            // branch-free gather: each direction reads one population - its source, or the own-cell
+            // population its boundary rule returns (an inlet or rest link reads its own slot, unused) -
+            // so the 27 loads are all in flight together; the inlet and rest rules replace the value
+            // afterwards.  The rules' directions (opposite, mirrors) have the weight of K.
+            uint32_t src = 0;
+            const int kind = classify<K, GEN, REG>(a, c, src);
+            const int jd = kind == SRC_REG ? K : rule_dir<K>(kind);
+            const st* p;
+            if constexpr (REG)
+                p = kind == SRC_REG ? (const st*)(ra.base(fi, K, c.z, dcz(K)) + ra.v[dcx(K) + 1][dcy(K) + 1])
+                                    : fi + pop_index(c.own, jd);
+            else
+                p = fi + pop_index(kind == SRC_REG ? src : c.own, jd);
+            raw[K] = SP::ld_raw(p);
+            inlet_links |= (kind == SRC_INLET ? 1u : 0u) << K;
+            rest_links |= (kind == SRC_REST ? 1u : 0u) << K;
+            val = T(0);   // set below
         } else {
             uint32_t src = 0;
             const int kind = classify<K, GEN, REG>(a, c, src);
@@ -352,6 +372,20 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
         }
         f[K] = val;
     });
+    if constexpr (PAT == PAT_AB && !(REG && !GEN)) {
+        for_dirs([&](auto kc) {
+            constexpr int K = decltype(kc)::value;
+            constexpr T w = wgt<T>(K);
+            T val = SP::from_raw(raw[K], w);
+            if constexpr (GEN) {
+                if ((rest_links >> K) & 1u) val = w;
+                if constexpr (dcx(K) > 0)
+                    if ((inlet_links >> K) & 1u)
+                        val = boundary_value<K, T>(SRC_INLET, a, c, u_in, a.step(), [&](auto) { return T(0); });
+            }
+            f[K] = val;
+        });
+    }
 
     T rho_o, ux_o, uy_o, uz_o;
     if (GEN && solid_own) {
