@@ -69,6 +69,14 @@ enum {
     OAL_BB_HALFWAY = 1     /* halfway bounce-back at the fluid cell, wall half a cell out */
 };
 
+/* interpolated bounce-back: the value of a q < 1/2 link whose cell behind (x - c_k) is not a fluid
+ * cell of the level (BOUZIDI_FF_FALLBACK_MODE, src/io/config_loader.jl:2515-2527) */
+enum {
+    OAL_BFL_FALLBACK_HALFWAY = 0,      /* "halfway_bb": f_ff = f_k (the reference default) */
+    OAL_BFL_FALLBACK_WALL_EQ = 1,      /* "wall_eq": f_ff = rho_f w_k */
+    OAL_BFL_FALLBACK_BOUNCE_BACK = 2   /* "bounce_back": f_ff = f_opp(k) of the cell */
+};
+
 /* limiter_type of the reference kernel (kernel_cumulant.jl:725-735, 849-884) */
 enum { OAL_LIMITER_NONE = 0, OAL_LIMITER_FACTORED = 1, OAL_LIMITER_POSITIVITY = 2 };
 
@@ -215,6 +223,26 @@ int oal_step(OalPlan* plan, int n_steps, int32_t first_time_step, int flags, voi
  * i toward the wall, shifted by w_i (src/octree_q19/src/stream.jl:248-256, reduce :382-389); the
  * reference collects the same sum while gathering the next step. */
 int oal_force(OalPlan* plan, int level, double* force);
+
+/* Interpolated (curved-wall) bounce-back on the solid links of a level: the linear Bouzidi rule
+ * (BFL) of bouzidi_correction_kernel_fixed! (src/geometry/bouzidi_kernel.jl:150-380, launched by
+ * apply_bouzidi_correction!, :382-432, from src/collision/timestep.jl:1752-1770) with its D3Q27
+ * settings (no lattice-frame shift, no wall correction, no population limiter), and the Float16 q-map
+ * of compute_bouzidi_qmap_sparse (src/geometry/bouzidi_setup.jl:67-163).
+ *   q_map [n_cells * 27] Float16 bits in the reference layout q_map[x,y,z,b,k]: the distance from
+ *         the cell centre to the wall along c_k in units of |c_k|, for links from a fluid cell to a
+ *         solid cell (other entries are ignored); NULL switches the level back to halfway only.
+ *   ff_fallback OAL_BFL_FALLBACK_*; q_min: links with q <= q_min or q > 1 keep halfway bounce-back
+ *         (Q_MIN_THRESHOLD, 0.001).
+ * The rule is applied in pull form inside the step: for link k from fluid cell x toward the wall,
+ *   q < 1/2:  f_opp(k)(x, t+1) = 2q f_k*(x) + (1 - 2q) f_k*(x - c_k)
+ *   q >= 1/2: f_opp(k)(x, t+1) = f_k*(x) / (2q) + (2q - 1) / (2q) f_opp(k)*(x),
+ * so the reflected population arrives at x.  The reference's post-pass writes the same value into
+ * x's own outgoing slot opp(k), which its pull kernel delivers to x - c_k on the next step.  With a
+ * q-map, oal_force of a level sums each solid link's exchange (f_k*(x) + f_opp(k)(x, t+1)) c_k when
+ * the step gathers it: the force collected in step t is that of the streaming into step t.
+ * Requires the AB pattern and OAL_BB_HALFWAY.  Drops captured step graphs. */
+int oal_set_bouzidi(OalPlan* plan, int level, const uint16_t* q_map, int32_t ff_fallback, float q_min);
 
 /* FP64 mass and momentum over fluid cells of each domain: out[4 * n_domains] = (mass, jx, jy, jz)
  * (src/octree_q19/src/state.jl:333-355). */

@@ -19,6 +19,7 @@ OAL_BB_REFERENCE, OAL_BB_HALFWAY = 0, 1
 OAL_LIMITER_NONE, OAL_LIMITER_FACTORED, OAL_LIMITER_POSITIVITY = 0, 1, 2
 OAL_ROUND_STOCHASTIC, OAL_ROUND_NEAREST = 0, 1
 OAL_STEP_FORCE = 1
+OAL_BFL_FALLBACK_HALFWAY, OAL_BFL_FALLBACK_WALL_EQ, OAL_BFL_FALLBACK_BOUNCE_BACK = 0, 1, 2
 
 STORAGE_CODES = {"f32": OAL_F32, "f64": OAL_F64, "f16s": OAL_F16S, "bf16s": OAL_BF16S}
 
@@ -27,7 +28,7 @@ EXPORTS = [
     "oal_abi_version", "oal_strerror", "oal_last_error", "oal_plan_create", "oal_plan_destroy",
     "oal_level_cells", "oal_level_info", "oal_upload_f", "oal_download_f", "oal_init_equilibrium", "oal_level_step",
     "oal_step", "oal_force", "oal_mass", "oal_download_macro", "oal_guard", "oal_sync", "oal_time_steps",
-    "oal_get_tuning", "oal_set_tuning", "oal_autotune", "oal_copy_bandwidth",
+    "oal_get_tuning", "oal_set_tuning", "oal_autotune", "oal_copy_bandwidth", "oal_set_bouzidi",
 ]
 
 
@@ -126,6 +127,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "oal_set_tuning": ([vp, C.POINTER(OalTuning)], C.c_int),
         "oal_autotune": ([vp, C.c_int, C.c_int, C.POINTER(OalTuning), dp], C.c_int),
         "oal_copy_bandwidth": ([C.c_int, C.c_int64, C.c_int, dp], C.c_int),
+        "oal_set_bouzidi": ([vp, C.c_int, vp, i32, C.c_float], C.c_int),
     }
     for name, (argt, rest) in sig.items():
         fn = getattr(lib, name)

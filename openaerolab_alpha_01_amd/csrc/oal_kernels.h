@@ -78,6 +78,11 @@ struct StepArgs {
     int32_t ts_mul, ts_add;
     int32_t parity;              // AA pattern: 1 odd step (neighbour slots), 0 even step (own slots)
     int32_t pipe_planes;         // planes per workgroup of the pipelined interior kernel (1 = off; divides 16)
+    // interpolated bounce-back: per link-mask slot, [27][BS3] Float16 bits of the wall distance q of the
+    // link from a cell along c_k (units of |c_k|); null = halfway bounce-back on every solid link
+    const uint16_t* qmap;
+    float q_min;                 // links with q <= q_min (or q > 1) keep halfway bounce-back
+    int32_t bfl_fallback;        // q < 1/2 without a fluid cell behind: 0 halfway, 1 wall equilibrium, 2 own opposite
     CollParams cp;
     __device__ __forceinline__ int32_t step() const { return ts_dev ? ts_dev[0] * ts_mul + ts_add : time_step; }
 };

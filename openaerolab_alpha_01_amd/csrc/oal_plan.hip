@@ -72,6 +72,12 @@ struct Level {
     bool regular_face = false;         // AB: the regular list holds blocks on a domain face
     DevBuf<int32_t> lslot;             // link-mask slot per block
     DevBuf<uint32_t> lmask;            // per-cell solid-link masks of blocks near solid cells
+    std::vector<int32_t> lslot_h;      // host copy of lslot
+    int32_t n_lslots = 0;
+    // interpolated bounce-back: per link-mask slot [27][BS3] Float16 q codes (empty: halfway only)
+    DevBuf<uint16_t> qmap;
+    int32_t bfl_fallback = 0;
+    float q_min = 0.001f;
     DevBuf<unsigned char> obuf;        // AA: outlet-face populations with cx = -1
     DevBuf<unsigned long long> guard;
     // refined plans: the caller's blocks come first (ncell_host cells), the level's ghost blocks after them
@@ -107,7 +113,7 @@ struct Level {
         nbr.release(); borg.release(); bdom.release(); dom_item0.release(); dom_block0.release();
         bflags.release(); solid.release(); doms.release(); fA.release(); fB.release(); velA.release(); velB.release();
         fpart.release(); mpart.release(); red_out.release(); guard.release(); oslot.release(); obuf.release(); blist_rg.release(); blist_in.release(); blist_br.release(); blist_bd.release(); face_cells.release();
-        lslot.release(); lmask.release();
+        lslot.release(); lmask.release(); qmap.release();
         g_src.release(); g_dst.release(); g_slot.release(); g_w.release(); g_m.release();
         r_src.release(); r_dst.release(); r_slot.release(); r_w.release(); r_m.release();
         rf_src.release(); rf_dst.release(); rf_slot.release(); rf_w.release();
@@ -377,6 +383,8 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, Leve
     for (uint32_t b = 0; b < nb; ++b)
         if (flags[b] & BF_SOLID) H.lslot[b] = nls++;
     H.lmask.assign((size_t)nls * BS3, 0u);
+    L.lslot_h = H.lslot;
+    L.n_lslots = nls;
     for (uint32_t b = 0; b < nb; ++b) {
         if (H.lslot[b] < 0) continue;
         const DevDomain& dm = L.doms_h[bdom[b]];
@@ -885,6 +893,9 @@ StepArgs make_args(OalPlan* p, Level& L, int32_t time_step, bool want_force) {
     }
     a.parity = p->stream_mode == OAL_STREAM_AA ? (L.aa_phase == 0 ? 1 : 0) : 0;
     a.pipe_planes = p->pipe_planes;
+    a.qmap = L.qmap.p;
+    a.q_min = L.q_min;
+    a.bfl_fallback = L.bfl_fallback;
     a.cp = p->cp;
     return a;
 }
@@ -1607,6 +1618,37 @@ int oal_copy_bandwidth(int device, int64_t bytes, int reps, double* gbs) {
     if (b) (void)hipFree(b);
     if (e != hipSuccess) return fail(OAL_ERR_HIP, std::string("oal_copy_bandwidth: ") + hipGetErrorString(e));
     *gbs = 2.0 * (double)nb * reps / (ms * 1e-3) / 1e9;
+    return OAL_OK;
+}
+
+int oal_set_bouzidi(OalPlan* p, int level, const uint16_t* q_map, int32_t ff_fallback, float q_min) {
+    if (int rc = check_level(p, level)) return rc;
+    if (p->stream_mode != OAL_STREAM_AB || p->bb_mode != OAL_BB_HALFWAY)
+        return fail(OAL_ERR_ARG, "interpolated bounce-back needs the AB pattern and halfway bounce-back");
+    if (ff_fallback < OAL_BFL_FALLBACK_HALFWAY || ff_fallback > OAL_BFL_FALLBACK_BOUNCE_BACK)
+        return fail(OAL_ERR_ARG, "ff_fallback must be OAL_BFL_FALLBACK_HALFWAY, _WALL_EQ or _BOUNCE_BACK");
+    if (!(q_min >= 0.0f && q_min < 1.0f)) return fail(OAL_ERR_ARG, "q_min must lie in [0, 1)");
+    Level& L = p->levels[level];
+    if (q_map && !L.has_solid) return fail(OAL_ERR_STATE, "the level has no obstacle cells");
+    OAL_HIP(hipSetDevice(p->device));
+    OAL_HIP(hipStreamSynchronize(p->stream));
+    drop_graphs(p);
+    L.qmap.release();
+    L.bfl_fallback = ff_fallback;
+    L.q_min = q_min;
+    if (!q_map || L.n_lslots == 0) return OAL_OK;
+    // the caller's q-map [x,y,z,b,k] regrouped per link-mask slot (blocks with solid cells near);
+    // ghost blocks of a refined level (not stepped) keep q = 0
+    std::vector<uint16_t> h((size_t)L.n_lslots * 27 * BS3, 0);
+    for (uint32_t b = 0; b < L.n_caller_blocks; ++b) {
+        const int32_t ls = L.lslot_h[b];
+        if (ls < 0) continue;
+        for (int k = 0; k < 27; ++k)
+            std::memcpy(h.data() + ((size_t)ls * 27 + k) * BS3, q_map + (size_t)k * L.ncell_host + (size_t)b * BS3,
+                        BS3 * sizeof(uint16_t));
+    }
+    OAL_HIP(L.qmap.alloc(h.size()));
+    OAL_HIP(hipMemcpy(L.qmap.p, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
     return OAL_OK;
 }
 

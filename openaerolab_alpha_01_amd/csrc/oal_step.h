@@ -299,6 +299,63 @@ __device__ __forceinline__ T wale_of(const StepArgs& a, const CellCtx& c) {
 // produced, i.e. the exchange of the streaming that follows it: 2 (f_i* - w_i) c_i for halfway
 // bounce-back, (f_i*(t) + f_i*(t-1) - 2 w_i) c_i for the reference's solid reflection (the solid cell
 // returns last step's population), i toward a solid cell (src/octree_q19/src/stream.jl:248-256).
+// Interpolated bounce-back (the linear BFL rule of Bouzidi, Firdaouss and Lallemand 2001;
+// bouzidi_kernel.jl:204-370 with its D3Q27 settings) in pull form, on the populations of one cell whose
+// sources are solid (bits of solid_links).  For such a direction K the link toward the wall is
+// I = opp(K), f[K] holds its halfway value f*_I(x) and q is the wall distance of link I:
+//   q < 1/2:  2q f*_I(x) + (1 - 2q) f*_I(x - c_I)      (x - c_I: the pull source of direction I,
+//             used when it is a fluid cell of the level; otherwise the fallback of a.bfl_fallback)
+//   q >= 1/2: f*_I(x) / (2q) + (2q - 1) / (2q) f*_K(x)
+// so the reflected population arrives at x itself.  q outside (q_min, 1] keeps halfway bounce-back.
+// The link's momentum exchange (f*_I(x) + arriving value) c_I is summed here, at the gather.
+template <class SP>
+__device__ __forceinline__ void bfl_links(const StepArgs& a, const CellCtx& c, typename SP::T (&f)[27],
+                                          uint32_t solid_links, uint32_t reg_links, bool want_force,
+                                          double& F0, double& F1, double& F2) {
+    using T = typename SP::T;
+    using st = typename SP::st;
+    const st* fi = (const st*)a.fin;
+    const uint16_t* qb = a.qmap + (size_t)a.lslot[c.b] * (27 * BS3) + (c.own - c.b * BS3);
+    T rho_f = T(0);
+    if (a.bfl_fallback == 1) {
+        // density of the cell's own post-collision populations (the wall-equilibrium fallback)
+        for_dirs([&](auto kc) {
+            constexpr int K = decltype(kc)::value;
+            rho_f += SP::ld(fi + pop_index(c.own, K), wgt<T>(K));
+        });
+    }
+    for_dirs([&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        constexpr int I = dopp(K);
+        constexpr T w = wgt<T>(K);
+        if ((solid_links >> K) & 1u) {
+            const T fk = f[K];
+            T val = fk;
+            const float q = (float)__builtin_bit_cast(_Float16, qb[(size_t)I * BS3]);
+            if (q > a.q_min && q <= 1.0f) {
+                const T tq = T(q);
+                if (q < 0.5f) {
+                    T fff = fk;
+                    if ((reg_links >> I) & 1u) fff = f[I];
+                    else if (a.bfl_fallback == 1) fff = rho_f * w;
+                    else if (a.bfl_fallback == 2) fff = SP::ld(fi + pop_index(c.own, K), w);
+                    const T c1 = T(2) * tq;
+                    val = c1 * fk + (T(1) - c1) * fff;
+                } else {
+                    const T inv = T(1) / (T(2) * tq);
+                    const T c2 = (T(2) * tq - T(1)) * inv;
+                    val = inv * fk + c2 * SP::ld(fi + pop_index(c.own, K), w);
+                }
+            }
+            if (want_force) {
+                const double m = ((double)fk - (double)w) + ((double)val - (double)w);
+                F0 += m * dcx(I); F1 += m * dcy(I); F2 += m * dcz(I);
+            }
+            f[K] = val;
+        }
+    });
+}
+
 template <class SP, int PAT, bool WALE, bool GEN, bool REG = false>
 __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, const DevDomain* dm, bool active,
                                           bool solid_own, double& F0, double& F1, double& F2) {
@@ -315,7 +372,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
 
     T f[27];
     typename SP::raw raw[27];     // AB pattern: the loaded populations, decoded after all loads
-    uint32_t inlet_links = 0, rest_links = 0;
+    uint32_t inlet_links = 0, rest_links = 0, solid_links = 0, reg_links = 0;
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
         constexpr T w = wgt<T>(K);
@@ -351,6 +408,8 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
             raw[K] = SP::ld_raw(p);
             inlet_links |= (kind == SRC_INLET ? 1u : 0u) << K;
             rest_links |= (kind == SRC_REST ? 1u : 0u) << K;
+            solid_links |= (kind == SRC_SOLID ? 1u : 0u) << K;
+            reg_links |= (kind == SRC_REG ? 1u : 0u) << K;
             val = T(0);   // set below
         } else {
             uint32_t src = 0;
@@ -386,6 +445,10 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
             f[K] = val;
         });
     }
+    if constexpr (PAT == PAT_AB && GEN) {
+        if (a.qmap && (c.flags & BF_SOLID) && solid_links)
+            bfl_links<SP>(a, c, f, solid_links, reg_links, a.fpart != nullptr, F0, F1, F2);
+    }
 
     T rho_o, ux_o, uy_o, uz_o;
     if (GEN && solid_own) {
@@ -418,7 +481,8 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
     if (a.rho_out) ((T*)a.rho_out)[c.own] = rho_o;
 
     const bool want_force = a.fpart != nullptr;
-    const bool link_force = GEN && want_force && (c.flags & BF_SOLID) && !solid_own;
+    // (with a q-map the links' momentum exchange was taken at the gather)
+    const bool link_force = GEN && want_force && (c.flags & BF_SOLID) && !solid_own && !a.qmap;
     const bool outlet_cell = GEN && PAT == PAT_AA_EVEN && (c.flags & BF_EDGE) && !c.per0 && c.gx == c.ext0 - 1;
     // rounding of the stored post-collision populations: population j of this cell takes the random
     // bits of (cell, j, step) whichever slot it goes to, so every pattern stores the same values

@@ -135,6 +135,75 @@ def sphere_mask(grid: BlockGrid, center, radius: float) -> np.ndarray:
     return (m & grid.inside()).astype(np.uint8)
 
 
+def link_solid(grid: BlockGrid) -> np.ndarray:
+    """[27, n_cells] bool: the cell x + c_k is a solid cell of the level (inside the extent, in a present
+    block; periodic axes wrap through the neighbour table)."""
+    c, _, _, _, _ = lattice_d3q27()
+    bs = grid.bs
+    nc = grid.n_cells
+    out = np.zeros((27, nc), dtype=bool)
+    if grid.solid is None:
+        return out
+    solid = np.asarray(grid.solid) != 0
+    g = grid.cell_coords()
+    b = np.repeat(np.arange(grid.n_blocks), bs ** 3)
+    loc = g - (np.asarray(grid.coords, dtype=np.int64)[b] - 1) * bs
+    nbr = np.asarray(grid.neighbor, dtype=np.int64)
+    for k in range(27):
+        t = loc + c[k]
+        o = np.floor_divide(t, bs)
+        col = (o[:, 0] + 1) + 3 * (o[:, 1] + 1) + 9 * (o[:, 2] + 1)
+        nb = np.where(col == 13, b + 1, nbr[b, col])
+        ok = nb > 0
+        gt = g + c[k]
+        for a in range(3):
+            if not grid.periodic[a]:
+                ok &= (gt[:, a] >= 0) & (gt[:, a] < grid.n[a])
+        tl = t - o * bs
+        cell = (nb - 1) * bs ** 3 + (tl[:, 2] * bs + tl[:, 1]) * bs + tl[:, 0]
+        out[k, ok] = solid[cell[ok]]
+    return out
+
+
+def sphere_qmap(grid: BlockGrid, center, radius: float) -> np.ndarray:
+    """Wall distances q of the links from fluid cells to solid cells of a sphere (float16 [27, n_cells],
+    0 elsewhere), for interpolated bounce-back.  q is the distance from the cell centre to the sphere
+    along c_k in units of |c_k|: the first ray-sphere hit beyond 1e-10 max(R, 1)
+    (ray_sphere_first_hit, bouzidi_math.jl:127-150; compute_q_for_sphere_cell :159-204, without the
+    symmetry-plane rule), kept for 0 < q <= 1 on links whose end cell is solid
+    (_bouzidi_qmap_block!, bouzidi_setup.jl:91-139), rounded to float16 as the reference's q-map."""
+    c, _, _, _, _ = lattice_d3q27()
+    nc = grid.n_cells
+    q = np.zeros((27, nc), dtype=np.float16)
+    ends = link_solid(grid)
+    fluid = grid.inside() & ((np.asarray(grid.solid) == 0) if grid.solid is not None else True)
+    p = grid.cell_coords().astype(np.float64) + 0.5
+    oc = p - np.asarray(center, dtype=np.float64)[None, :]
+    cc = (oc * oc).sum(1) - radius * radius
+    eps = 1.0e-10 * max(radius, 1.0)
+    for k in range(27):
+        if k == 13:
+            continue
+        sel = ends[k] & fluid
+        if not sel.any():
+            continue
+        cm = float(np.sqrt((c[k] * c[k]).sum()))
+        d = c[k].astype(np.float64) / cm
+        bq = oc[sel] @ d
+        disc = bq * bq - cc[sel]
+        with np.errstate(invalid="ignore"):
+            sq = np.sqrt(disc)
+        t1, t2 = -bq - sq, -bq + sq
+        t = np.where(t1 > eps, t1, np.where(t2 > eps, t2, np.inf))
+        t = np.where(disc < 0.0, np.inf, t)
+        qk = t / cm
+        keep = (qk > 0.0) & (qk <= 1.0)
+        vals = np.zeros(int(sel.sum()), dtype=np.float64)
+        vals[keep] = qk[keep]
+        q[k, sel] = vals.astype(np.float16)
+    return q
+
+
 def ellipsoid_mask(grid: BlockGrid, center, semi_axes, rotation: np.ndarray) -> np.ndarray:
     """Solid iff the cell centre lies inside the ellipsoid with the given semi-axes along the columns
     of `rotation` (a 3x3 rotation matrix)."""

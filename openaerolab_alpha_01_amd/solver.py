@@ -246,6 +246,30 @@ class Plan:
         t.graph = int(cur["graph"] if graph is None else graph)
         check(self.lib.oal_set_tuning(self.h, C.byref(t)), "oal_set_tuning")
 
+    BFL_FALLBACKS = {"halfway_bb": _lib.OAL_BFL_FALLBACK_HALFWAY, "wall_eq": _lib.OAL_BFL_FALLBACK_WALL_EQ,
+                     "bounce_back": _lib.OAL_BFL_FALLBACK_BOUNCE_BACK}
+
+    def set_bouzidi(self, q_map, ff_fallback: str = "halfway_bb", q_min: float = 0.001, level: int = 0) -> None:
+        """Interpolated bounce-back on the solid links of a level (apply_bouzidi_correction!,
+        src/geometry/bouzidi_kernel.jl:382-432, in pull form).  q_map: [27, n_cells] wall distances
+        (float16, or their uint16 bits; e.g. geometry.sphere_qmap), None to return to halfway
+        bounce-back.  ff_fallback: "halfway_bb", "wall_eq" or "bounce_back" (advanced.boundary.ff_fallback,
+        src/io/config_loader.jl:2519-2527)."""
+        if ff_fallback not in self.BFL_FALLBACKS:
+            raise ValueError(f"ff_fallback must be one of {sorted(self.BFL_FALLBACKS)}")
+        ptr = None
+        if q_map is not None:
+            q = np.asarray(q_map)
+            if q.dtype != np.uint16:
+                q = q.astype(np.float16).view(np.uint16)
+            n = self.n_cells(level)
+            if q.shape != (27, n):
+                raise ValueError(f"q_map must be [27, {n}]")
+            q = np.ascontiguousarray(q)   # copied by the call
+            ptr = q.ctypes.data
+        check(self.lib.oal_set_bouzidi(self.h, level, ptr, self.BFL_FALLBACKS[ff_fallback], C.c_float(q_min)),
+              "oal_set_bouzidi")
+
     @property
     def pipe_planes(self) -> int:
         return self.tuning["pipe_planes"]

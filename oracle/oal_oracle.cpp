@@ -36,7 +36,14 @@
  *     no-slip wall form (kernel_cumulant.jl:270-271, 340-341); bb_mode 0 is the reference's own
  *     solid-cell reflection (kernel_cumulant.jl:405-417);
  *   - a uniform body force enters through the reference's Guo-forcing slot (the wall-model
- *     force, kernel_cumulant.jl:452-486 and 889-905), with the wall model itself off.
+ *     force, kernel_cumulant.jl:452-486 and 889-905), with the wall model itself off;
+ *   - interpolated bounce-back (BFL, the linear Bouzidi rule of bouzidi_kernel.jl:150-380 with its
+ *     D3Q27 settings: no frame shift, no wall correction, no limiter) in pull form: with a q-map, the
+ *     population arriving at fluid cell x from a solid source is the BFL value of the link
+ *     i = opp(k) from x toward the wall, so the reflected population arrives at x itself.  The
+ *     reference's post-pass writes the same value into x's outgoing slot opp(i), which its pull
+ *     kernel hands to x - c_i on the next step; here it is where BFL defines it.  The momentum
+ *     exchange of such a level is taken at the gather, for the links that fed this step.
  *
  * Parity status of each function is documented in the header of oracle/oracle.py.
  */
@@ -298,6 +305,10 @@ struct OrParams {
     int32_t time_step_seed;
     int32_t arith;            // 0: literal reference arithmetic, 1: the library's fused arithmetic
     int32_t rounding;         // 16-bit storage: 0 stochastic (the library's default), 1 to nearest
+    int32_t bfl_fallback;     // q < 1/2 link without a fluid cell behind: 0 halfway (f_ff = f_k),
+                              // 1 wall equilibrium (rho_f w_k), 2 own opposite population
+                              // (BOUZIDI_FF_FALLBACK_MODE, config_loader.jl:2526-2527)
+    double q_min;             // links with q <= q_min keep halfway bounce-back (Q_MIN_THRESHOLD)
 };
 
 struct OrGrid {
@@ -308,6 +319,8 @@ struct OrGrid {
     int32_t periodic[3];
     const uint8_t* solid;     // [nb*bs^3] or null
     const uint8_t* active;    // [nb]: 0 = not stepped (covered or ghost block of a refined level), null = all
+    const uint16_t* qmap;     // [27*ncell] Float16 bits: q of the link from a cell along c_k, in units
+                              // of |c_k| (bouzidi_setup.jl:139); null = halfway bounce-back only
 };
 
 namespace {
@@ -798,6 +811,39 @@ void step_impl(const OrGrid& g, const OrParams& prm, const typename P::st* fin, 
                     const bool is_obs = g.solid && g.solid[own];
                     if (is_obs && prm.bb_mode == 1) continue;
                     auto own_f = [&](int k) { return P::ld(fin[(int64_t)k * ncell + own], w[k]); };
+                    // BFL value of the population k arriving from a solid source; i = opp(k) is the link
+                    // toward the wall, fk = f*_i(x) its halfway value (bouzidi_kernel.jl:204-370)
+                    auto bfl = [&](int k, T fk) -> T {
+                        const int i = LAT.opp[k];
+                        const float q = f16_to_f32(g.qmap[(int64_t)i * ncell + own]);
+                        if (!(q > (float)prm.q_min && q <= 1.0f)) return fk;
+                        const T tq = (T)q;
+                        if (q < 0.5f) {
+                            // x_ff = x - c_i: the pull source of direction i, when a fluid cell of the level
+                            T fff = fk;
+                            bool support = false;
+                            const int fx = gx - LAT.cx[i], fy = gy - LAT.cy[i], fz = gz - LAT.cz[i];
+                            if (!geo.outside(0, fx) && !geo.outside(1, fy) && !geo.outside(2, fz)) {
+                                const int64_t s2 = geo.locate(b, x - LAT.cx[i], y - LAT.cy[i], z - LAT.cz[i]);
+                                if (s2 >= 0 && !(g.solid && g.solid[s2])) {
+                                    fff = P::ld(fin[(int64_t)i * ncell + s2], w[i]);
+                                    support = true;
+                                }
+                            }
+                            if (!support && prm.bfl_fallback == 1) {
+                                T r = T(0);
+                                for (int kk = 0; kk < 27; ++kk) r += own_f(kk);
+                                fff = r * w[i];
+                            } else if (!support && prm.bfl_fallback == 2) {
+                                fff = own_f(k);
+                            }
+                            const T c1 = T(2) * tq;
+                            return c1 * fk + (T(1) - c1) * fff;
+                        }
+                        const T inv = T(1) / (T(2) * tq);
+                        const T c2 = (T(2) * tq - T(1)) * inv;
+                        return inv * fk + c2 * own_f(k);
+                    };
 
                     T rho = T(0), jx = T(0), jy = T(0), jz = T(0);
                     T fs[27];
@@ -833,6 +879,15 @@ void step_impl(const OrGrid& g, const OrParams& prm, const typename P::st* fin, 
                                 val = w[k];   // no neighbour on level 1: rest weight (kernel_cumulant.jl:389-390)
                             } else if (g.solid && g.solid[src] && prm.bb_mode == 1) {
                                 val = own_f(LAT.opp[k]);   // halfway bounce-back: own population toward the wall
+                                if (g.qmap) {
+                                    const T fk = val;
+                                    val = bfl(k, fk);
+                                    const int i = LAT.opp[k];
+                                    if (force) {
+                                        const double m = ((double)fk - (double)w[i]) + ((double)val - (double)w[i]);
+                                        fb[0] += m * LAT.cx[i]; fb[1] += m * LAT.cy[i]; fb[2] += m * LAT.cz[i];
+                                    }
+                                }
                             } else {
                                 val = P::ld(fin[(int64_t)k * ncell + src], w[k]);
                             }
@@ -893,7 +948,7 @@ void step_impl(const OrGrid& g, const OrParams& prm, const typename P::st* fin, 
                     // (src/octree_q19/src/stream.jl:248-256), shifted by w_i (the w terms cancel over
                     // a closed surface).  Halfway: f_opp(i)(x, t+1) = f_i*(x).  Reference reflection:
                     // the solid cell returns f_i*(x) of the previous step.
-                    if (force && g.solid) {
+                    if (force && g.solid && !g.qmap) {
                         for (int i = 0; i < 27; ++i) {
                             const int dgx = gx + LAT.cx[i], dgy = gy + LAT.cy[i], dgz = gz + LAT.cz[i];
                             if (geo.outside(0, dgx) || geo.outside(1, dgy) || geo.outside(2, dgz)) continue;

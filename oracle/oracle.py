@@ -62,7 +62,7 @@ class OrParams(C.Structure):
         ("u_inlet", C.c_double), ("inlet_turb", C.c_double), ("body_force", C.c_double * 3),
         ("adaptive_omega4", C.c_int32), ("limiter", C.c_int32), ("compressible", C.c_int32),
         ("ymin_wall_bc", C.c_int32), ("zwall_bc", C.c_int32), ("bb_mode", C.c_int32), ("time_step_seed", C.c_int32),
-        ("arith", C.c_int32), ("rounding", C.c_int32),
+        ("arith", C.c_int32), ("rounding", C.c_int32), ("bfl_fallback", C.c_int32), ("q_min", C.c_double),
     ]
 
 
@@ -70,7 +70,7 @@ class OrGrid(C.Structure):
     _fields_ = [
         ("bs", C.c_int32), ("nb", C.c_int32), ("bcoord", C.POINTER(C.c_int32)), ("nbr", C.POINTER(C.c_int32)),
         ("n", C.c_int32 * 3), ("periodic", C.c_int32 * 3), ("solid", C.POINTER(C.c_uint8)),
-        ("active", C.POINTER(C.c_uint8)),
+        ("active", C.POINTER(C.c_uint8)), ("qmap", C.POINTER(C.c_uint16)),
     ]
 
 
@@ -122,11 +122,13 @@ def f32(v: float) -> float:
 def make_params(tau, *, omega_bulk=1.0, omega_3=1.0, omega_4=1.0, lambda_=1.0 / 6.0, adaptive_omega4=1, limiter=1,
                 compressible=0, c_wale=0.0, nu_sgs_background=0.0, amr_sgs_scale=1.0, u_inlet=0.0,
                 inlet_turbulence=0.0, body_force=(0.0, 0.0, 0.0), ymin_wall_bc=0, zwall_bc=0, bb_mode=1,
-                time_step=0, arith=LITERAL, rounding=STOCHASTIC) -> OrParams:
+                time_step=0, arith=LITERAL, rounding=STOCHASTIC, bfl_fallback=0, q_min=0.001) -> OrParams:
     """Step parameters; arith LITERAL restates the reference arithmetic operation for operation, FAST the
     library's fused arithmetic (same mathematics, fused multiply-adds, rho and j from the transform's
     partial sums).  rounding: how the 16-bit forms store a step's populations (STOCHASTIC, the
-    library's default, or NEAREST)."""
+    library's default, or NEAREST).  bfl_fallback and q_min apply to a grid with a q-map (interpolated
+    bounce-back): the rule of a q < 1/2 link with no fluid cell behind (0 halfway, 1 wall equilibrium,
+    2 own opposite population) and the smallest q that is interpolated."""
     p = OrParams()
     p.tau, p.omega_bulk, p.omega3, p.omega4 = f32(tau), f32(omega_bulk), f32(omega_3), f32(omega_4)
     p.lambda_, p.c_wale, p.nu_sgs_bg, p.amr_sgs_scale = f32(lambda_), f32(c_wale), f32(nu_sgs_background), f32(amr_sgs_scale)
@@ -136,13 +138,16 @@ def make_params(tau, *, omega_bulk=1.0, omega_3=1.0, omega_4=1.0, lambda_=1.0 / 
     p.ymin_wall_bc, p.zwall_bc, p.bb_mode, p.time_step_seed = ymin_wall_bc, zwall_bc, bb_mode, time_step
     p.arith = arith
     p.rounding = rounding
+    p.bfl_fallback = bfl_fallback
+    p.q_min = f32(q_min)
     return p
 
 
 class Grid:
-    """The oracle's view of a block grid (any object with n, periodic, coords, neighbor, solid, bs)."""
+    """The oracle's view of a block grid (any object with n, periodic, coords, neighbor, solid, bs).
+    qmap: [27, n_cells] float16 wall distances of the links (interpolated bounce-back), or None."""
 
-    def __init__(self, g):
+    def __init__(self, g, qmap=None):
         self.src = g
         self.bs = int(g.bs)
         self.nb = int(g.coords.shape[0])
@@ -161,6 +166,15 @@ class Grid:
         act = getattr(g, "active", None)
         self.active = None if act is None else np.ascontiguousarray(act, dtype=np.uint8)
         s.active = None if self.active is None else self.active.ctypes.data_as(C.POINTER(C.c_uint8))
+        self.qmap = None
+        if qmap is not None:
+            q = np.asarray(qmap)
+            if q.dtype != np.uint16:
+                q = q.astype(np.float16).view(np.uint16)
+            if q.shape != (27, self.ncell):
+                raise ValueError(f"qmap must be [27, {self.ncell}]")
+            self.qmap = np.ascontiguousarray(q)
+        s.qmap = None if self.qmap is None else self.qmap.ctypes.data_as(C.POINTER(C.c_uint16))
         self.c = s
 
 
