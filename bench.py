@@ -37,6 +37,12 @@ def workload_grid():
     return g
 
 
+def wall_qmap(grid):
+    """wall distances of the sphere's links for interpolated bounce-back (--bouzidi)"""
+    from openaerolab_alpha_01_amd.geometry import sphere_qmap
+    return sphere_qmap(grid, (96.0, 128.0, 128.0), D / 2)
+
+
 def tau_of():
     nu = U_IN * D / RE
     return 0.5 + 3.0 * nu
@@ -46,10 +52,13 @@ def bytes_per_lup(storage: str) -> int:
     return 2 * 27 * {"f32": 4, "f64": 8, "f16s": 2, "bf16s": 2}[storage]
 
 
-def gpu_leg(grid, storage: str, stream: str, steps: int, warmup: int, autotune: bool = True, fixed=None):
+def gpu_leg(grid, storage: str, stream: str, steps: int, warmup: int, autotune: bool = True, fixed=None,
+            qmap=None):
     from openaerolab_alpha_01_amd import solver
     params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
     plan = solver.uniform_plan(grid, tau_of(), storage=storage, stream=stream, params=params, u_inlet=U_IN)
+    if qmap is not None:
+        plan.set_bouzidi(qmap)
     plan.init_equilibrium(None, None, 1.0, (U_IN, 0.0, 0.0))
     info = plan.level_info()
     tuning = None
@@ -122,7 +131,7 @@ def cpu_baseline(grid, budget_s: float = 20.0):
     return n * done / t_used / 1e6, threads, done, t_used
 
 
-def validation(steps: int, seed: int, chunk: int = 1000) -> dict:
+def validation(steps: int, seed: int, chunk: int = 1000, bouzidi: bool = False) -> dict:
     """Accuracy of the storage forms on the same workload against an FP64 run of the same kernels (the
     FP64 path equals the FP64 oracle bit for bit in the parity tests).
 
@@ -130,14 +139,17 @@ def validation(steps: int, seed: int, chunk: int = 1000) -> dict:
     velocity carries a seeded perturbation, uniform in [-1e-3, 1e-3] per component (numpy PCG64,
     `seed`), with the FP64 run started from the same field.  After `steps` steps (the last one
     collecting the momentum exchange): Cd = F_x / (1/2 rho0 u_in^2 pi R^2) and
-    L2(u) = |u - u_64| / |u_64| over the fluid cells."""
+    L2(u) = |u - u_64| / |u_64| over the fluid cells.  bouzidi: the sphere's links take interpolated
+    bounce-back (every storage form, the FP64 run included)."""
     from openaerolab_alpha_01_amd import solver
     grid = workload_grid()
+    qmap = wall_qmap(grid) if bouzidi else None
     fluid = grid.solid == 0
     n = grid.n_cells
     area = np.pi * (D / 2) ** 2
     params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
-    out = {"steps": steps, "seed": seed, "cd_reference_area": round(area, 4)}
+    out = {"steps": steps, "seed": seed, "cd_reference_area": round(area, 4),
+           "wall": "interpolated bounce-back (BFL)" if bouzidi else "halfway bounce-back"}
     for case, sd, forms in (("C2", None, ("f32",)), ("C3", seed, ("f16s", "bf16s"))):
         u0 = np.zeros((3, n))
         u0[0] = U_IN
@@ -149,6 +161,8 @@ def validation(steps: int, seed: int, chunk: int = 1000) -> dict:
         for st in ("f64",) + forms:
             t0 = time.perf_counter()
             plan = solver.uniform_plan(grid, tau_of(), storage=st, params=params, u_inlet=U_IN)
+            if qmap is not None:
+                plan.set_bouzidi(qmap)
             plan.init_equilibrium(rho0, u0)
             t = 0
             while t < steps - 1:
@@ -522,6 +536,9 @@ def main(argv=None) -> int:
                     help="accuracy run instead of the benchmark: Cd and L2(u) of FP32 / FP16 / BF16 vs FP64")
     ap.add_argument("--val-steps", type=int, default=20000)
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--bouzidi", action="store_true",
+                    help="interpolated (BFL) bounce-back on the sphere's links instead of halfway bounce-back "
+                         "(the default leg and --validate)")
     ap.add_argument("--amr", action="store_true",
                     help="the block-AMR form (4-level sphere octree) instead of the uniform box; with --validate "
                          "its accuracy run (--val-steps counts coarse steps)")
@@ -550,8 +567,10 @@ def main(argv=None) -> int:
     if args.many:
         return many_main(args)
     if args.validate:
-        print(json.dumps({"validation": validation(args.val_steps, args.seed)}), flush=True)
+        print(json.dumps({"validation": validation(args.val_steps, args.seed, bouzidi=args.bouzidi)}), flush=True)
         return 0
+    if args.bouzidi:
+        args.no_cpu_baseline = True   # the CPU baseline times the halfway form
     if args.steps % 2 or args.warmup % 2:
         args.steps += args.steps % 2
         args.warmup += args.warmup % 2
@@ -559,8 +578,10 @@ def main(argv=None) -> int:
     grid = workload_grid()
     n_cells = N ** 3
     n_fluid = int(n_cells - grid.solid.sum())
+    qmap = wall_qmap(grid) if args.bouzidi else None
+    wall_rule = "interpolated (BFL) bounce-back" if args.bouzidi else "halfway bounce-back"
     ms_total, ms_kernel, wall, F, tun, info = gpu_leg(grid, "f32", args.stream, args.steps, args.warmup,
-                                                      not args.no_autotune, fixed)
+                                                      not args.no_autotune, fixed, qmap)
     mlups = n_cells * args.steps / (ms_total * 1e-3) / 1e6
     ms_step = ms_total / args.steps
     pp = tun["pipe_planes"]
@@ -591,7 +612,7 @@ def main(argv=None) -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "256^3 box, inlet ux=0.05, zero-gradient outlet, free-slip sides, sphere D=32 at "
-                               "(96,128,128), halfway bounce-back, Re=200 (tau=0.524), cumulant, WALE off",
+                               f"(96,128,128), {wall_rule}, Re=200 (tau=0.524), cumulant, WALE off",
                    "cells": n_cells, "fluid_cells": n_fluid, "stream": args.stream, "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -607,7 +628,7 @@ def main(argv=None) -> int:
     }
     if not args.no_fp16:
         ms_t16, ms_k16, _, F16, tun16, _ = gpu_leg(grid, "f16s", args.stream, args.steps, args.warmup,
-                                                   not args.no_autotune, fixed)
+                                                   not args.no_autotune, fixed, qmap)
         m16 = n_cells * args.steps / (ms_t16 * 1e-3) / 1e6
         face16 = args.stream == "ab" and tun16["pipe_planes"] > 1 and not tun16["face_kernel"]
         bulk16 = (info["n_regular"] + info["n_interior"]) * 16 ** 3 - (0 if face16 else info["n_face_cells"])

@@ -137,18 +137,30 @@ def sphere_mask(grid: BlockGrid, center, radius: float) -> np.ndarray:
 
 def link_solid(grid: BlockGrid) -> np.ndarray:
     """[27, n_cells] bool: the cell x + c_k is a solid cell of the level (inside the extent, in a present
-    block; periodic axes wrap through the neighbour table)."""
+    block; periodic axes wrap through the neighbour table).  Only blocks holding solid cells and their
+    neighbours can have such links; the others are skipped."""
     c, _, _, _, _ = lattice_d3q27()
     bs = grid.bs
-    nc = grid.n_cells
-    out = np.zeros((27, nc), dtype=bool)
+    bs3 = bs ** 3
+    out = np.zeros((27, grid.n_cells), dtype=bool)
     if grid.solid is None:
         return out
     solid = np.asarray(grid.solid) != 0
-    g = grid.cell_coords()
-    b = np.repeat(np.arange(grid.n_blocks), bs ** 3)
-    loc = g - (np.asarray(grid.coords, dtype=np.int64)[b] - 1) * bs
     nbr = np.asarray(grid.neighbor, dtype=np.int64)
+    has = solid.reshape(grid.n_blocks, bs3).any(1)
+    cand = has.copy()
+    for col in range(27):
+        nb = nbr[has, col]
+        cand[nb[nb > 0] - 1] = True
+    blocks = np.nonzero(cand)[0]
+    if blocks.size == 0:
+        return out
+    z, y, x = np.meshgrid(np.arange(bs), np.arange(bs), np.arange(bs), indexing="ij")
+    loc0 = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    b = np.repeat(blocks, bs3)
+    loc = np.tile(loc0, (blocks.size, 1))
+    g = (np.asarray(grid.coords, dtype=np.int64)[b] - 1) * bs + loc
+    cells = b * bs3 + (loc[:, 2] * bs + loc[:, 1]) * bs + loc[:, 0]
     for k in range(27):
         t = loc + c[k]
         o = np.floor_divide(t, bs)
@@ -160,8 +172,8 @@ def link_solid(grid: BlockGrid) -> np.ndarray:
             if not grid.periodic[a]:
                 ok &= (gt[:, a] >= 0) & (gt[:, a] < grid.n[a])
         tl = t - o * bs
-        cell = (nb - 1) * bs ** 3 + (tl[:, 2] * bs + tl[:, 1]) * bs + tl[:, 0]
-        out[k, ok] = solid[cell[ok]]
+        cell = (nb - 1) * bs3 + (tl[:, 2] * bs + tl[:, 1]) * bs + tl[:, 0]
+        out[k, cells[ok]] = solid[cell[ok]]
     return out
 
 
@@ -177,14 +189,22 @@ def sphere_qmap(grid: BlockGrid, center, radius: float) -> np.ndarray:
     q = np.zeros((27, nc), dtype=np.float16)
     ends = link_solid(grid)
     fluid = grid.inside() & ((np.asarray(grid.solid) == 0) if grid.solid is not None else True)
-    p = grid.cell_coords().astype(np.float64) + 0.5
+    sel_any = ends.any(0) & fluid
+    idx = np.nonzero(sel_any)[0]
+    if idx.size == 0:
+        return q
+    bs = grid.bs
+    bidx = idx // bs ** 3
+    r = idx % bs ** 3
+    loc = np.stack([r % bs, (r // bs) % bs, r // bs ** 2], 1)
+    p = ((np.asarray(grid.coords, dtype=np.int64)[bidx] - 1) * bs + loc).astype(np.float64) + 0.5
     oc = p - np.asarray(center, dtype=np.float64)[None, :]
     cc = (oc * oc).sum(1) - radius * radius
     eps = 1.0e-10 * max(radius, 1.0)
     for k in range(27):
         if k == 13:
             continue
-        sel = ends[k] & fluid
+        sel = ends[k, idx]
         if not sel.any():
             continue
         cm = float(np.sqrt((c[k] * c[k]).sum()))
@@ -200,7 +220,7 @@ def sphere_qmap(grid: BlockGrid, center, radius: float) -> np.ndarray:
         keep = (qk > 0.0) & (qk <= 1.0)
         vals = np.zeros(int(sel.sum()), dtype=np.float64)
         vals[keep] = qk[keep]
-        q[k, sel] = vals.astype(np.float16)
+        q[k, idx[sel]] = vals.astype(np.float16)
     return q
 
 

@@ -34,6 +34,11 @@ Parity status of each oracle function and what pins it (tests/test_oracle_*.py):
                         wave across a refined slab at the physical viscosity with composite mass drift
                         under 1e-4; the exact rounding of the transfers is parity unpinned (no
                         reference vector)
+  interpolated          pinned (test_oracle_bfl.py): q = 1/2 reproduces halfway bounce-back bit for bit,
+  bounce-back (BFL)     with the gather-time force equal to the halfway force of the step before;
+                        Poiseuille walls off the half-link (q = 1/4, 3/4) follow the parabola of the
+                        q-placed walls; rest state kept under every fallback.  The values of fallbacks
+                        1 and 2 away from rest are parity unpinned (no reference vector)
   golden fixtures       tests/golden/ (written from this oracle; regression pins, test_golden.py)
 """
 from __future__ import annotations
