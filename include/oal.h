@@ -244,6 +244,20 @@ int oal_force(OalPlan* plan, int level, double* force);
  * Requires the AB pattern and OAL_BB_HALFWAY.  Drops captured step graphs. */
 int oal_set_bouzidi(OalPlan* plan, int level, const uint16_t* q_map, int32_t ff_fallback, float q_min);
 
+/* The raw level state of dump_raw_state (src/io/raw_state_dump.jl:10-47, the reference's "LBM1"
+ * file, written per level as <output_dir>/raw_state/step_<NNNNNN>/level_<L>.bin), little-endian:
+ *   Int32 magic 0x4C424D31; Int32 level_id, n_blocks, block_size, grid_dim_x, grid_dim_y, grid_dim_z;
+ *   Float64 dx; Int32 coords[3, n_blocks] (1-based block coordinates); Float32 rho[bs,bs,bs,n_blocks];
+ *   Float32 vel[bs,bs,bs,n_blocks,3]; UInt8 obstacle[bs,bs,bs,n_blocks].
+ * oal_write_raw_state writes the level's current density and velocity (obstacle cells as rho = 1,
+ * u = 0, as the reference's kernel leaves them); grid_dim is the largest block coordinate per axis.
+ * oal_read_raw_state checks the file against the level (block count and size, coordinates, obstacle
+ * mask) and starts the level at the equilibrium of the stored fields (init_eq!, src/main.jl:546-572):
+ * the format keeps no non-equilibrium part.  level_id and dx receive the header's values (either may
+ * be NULL).  The caller's blocks of a refined level are written and read. */
+int oal_write_raw_state(OalPlan* plan, int level, int32_t level_id, double dx, const char* path);
+int oal_read_raw_state(OalPlan* plan, int level, const char* path, int32_t* level_id, double* dx);
+
 /* FP64 mass and momentum over fluid cells of each domain: out[4 * n_domains] = (mass, jx, jy, jz)
  * (src/octree_q19/src/state.jl:333-355). */
 int oal_mass(OalPlan* plan, int level, double* out, void* stream);

@@ -29,6 +29,7 @@ EXPORTS = [
     "oal_level_cells", "oal_level_info", "oal_upload_f", "oal_download_f", "oal_init_equilibrium", "oal_level_step",
     "oal_step", "oal_force", "oal_mass", "oal_download_macro", "oal_guard", "oal_sync", "oal_time_steps",
     "oal_get_tuning", "oal_set_tuning", "oal_autotune", "oal_copy_bandwidth", "oal_set_bouzidi",
+    "oal_write_raw_state", "oal_read_raw_state",
 ]
 
 
@@ -128,6 +129,8 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "oal_autotune": ([vp, C.c_int, C.c_int, C.POINTER(OalTuning), dp], C.c_int),
         "oal_copy_bandwidth": ([C.c_int, C.c_int64, C.c_int, dp], C.c_int),
         "oal_set_bouzidi": ([vp, C.c_int, vp, i32, C.c_float], C.c_int),
+        "oal_write_raw_state": ([vp, C.c_int, i32, C.c_double, C.c_char_p], C.c_int),
+        "oal_read_raw_state": ([vp, C.c_int, C.c_char_p, C.POINTER(C.c_int32), dp], C.c_int),
     }
     for name, (argt, rest) in sig.items():
         fn = getattr(lib, name)

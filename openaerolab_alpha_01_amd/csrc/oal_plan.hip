@@ -73,6 +73,8 @@ struct Level {
     DevBuf<int32_t> lslot;             // link-mask slot per block
     DevBuf<uint32_t> lmask;            // per-cell solid-link masks of blocks near solid cells
     std::vector<int32_t> lslot_h;      // host copy of lslot
+    std::vector<int32_t> map_h;        // [3 * nb] 1-based block coordinates (caller's blocks first)
+    std::vector<uint8_t> obst_h;       // [nb * BS3] obstacle of the caller's layout, empty = none
     int32_t n_lslots = 0;
     // interpolated bounce-back: per link-mask slot [27][BS3] Float16 q codes (empty: halfway only)
     DevBuf<uint16_t> qmap;
@@ -200,8 +202,17 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, Leve
     borg.assign(3 * (size_t)nb, 0);
     bdom.assign(nb, 0);
     nbr.assign(27 * (size_t)nb, -1);
+    L.map_h.assign(3 * (size_t)nb, 0);
+    L.obst_h.clear();
+    if (d.obstacle) {
+        L.obst_h.resize((size_t)nb * BS3);
+        for (size_t i = 0; i < L.obst_h.size(); ++i) L.obst_h[i] = d.obstacle[i] ? 1 : 0;
+    }
     for (uint32_t b = 0; b < nb; ++b) {
         if (d.map_x[b] < 1 || d.map_y[b] < 1 || d.map_z[b] < 1) return fail(OAL_ERR_ARG, "block coordinates are 1-based");
+        L.map_h[3 * b] = d.map_x[b];
+        L.map_h[3 * b + 1] = d.map_y[b];
+        L.map_h[3 * b + 2] = d.map_z[b];
         borg[3 * b] = (d.map_x[b] - 1) * BS;
         borg[3 * b + 1] = (d.map_y[b] - 1) * BS;
         borg[3 * b + 2] = (d.map_z[b] - 1) * BS;
@@ -1649,6 +1660,89 @@ int oal_set_bouzidi(OalPlan* p, int level, const uint16_t* q_map, int32_t ff_fal
     }
     OAL_HIP(L.qmap.alloc(h.size()));
     OAL_HIP(hipMemcpy(L.qmap.p, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    return OAL_OK;
+}
+
+// ---- LBM1 raw level state (src/io/raw_state_dump.jl:10-47) ------------------------------------------
+namespace {
+constexpr int32_t LBM1_MAGIC = 0x4C424D31;   // "LBM1"
+
+struct FileCloser {
+    FILE* f;
+    ~FileCloser() { if (f) std::fclose(f); }
+};
+}  // namespace
+
+int oal_write_raw_state(OalPlan* p, int level, int32_t level_id, double dx, const char* path) {
+    if (int rc = check_level(p, level)) return rc;
+    if (!path) return fail(OAL_ERR_ARG, "NULL path");
+    Level& L = p->levels[level];
+    if (!L.have_f) return fail(OAL_ERR_STATE, "no populations on this level");
+    const size_t nc = L.ncell_host;
+    const uint32_t nb = L.n_caller_blocks;
+    std::vector<double> rho(nc), u(3 * nc);
+    if (int rc = oal_download_macro(p, level, rho.data(), u.data())) return rc;
+    // the reference's obstacle cells carry rho = 1, u = 0 (kernel_cumulant.jl:405-410)
+    std::vector<float> fr(nc), fv(3 * nc);
+    std::vector<uint8_t> ob(nc, 0);
+    for (size_t c = 0; c < nc; ++c) {
+        const bool solid = !L.obst_h.empty() && L.obst_h[c];
+        ob[c] = solid ? 1 : 0;
+        fr[c] = solid ? 1.0f : (float)rho[c];
+        for (int a = 0; a < 3; ++a) fv[a * nc + c] = solid ? 0.0f : (float)u[a * nc + c];
+    }
+    int32_t head[7] = {LBM1_MAGIC, level_id, (int32_t)nb, BS, 0, 0, 0};
+    for (uint32_t b = 0; b < nb; ++b)
+        for (int a = 0; a < 3; ++a) head[4 + a] = std::max(head[4 + a], L.map_h[3 * b + a]);
+    FileCloser fc{std::fopen(path, "wb")};
+    if (!fc.f) return fail(OAL_ERR_ARG, std::string("cannot open for writing: ") + path);
+    bool ok = std::fwrite(head, sizeof(head), 1, fc.f) == 1 && std::fwrite(&dx, sizeof(dx), 1, fc.f) == 1 &&
+              std::fwrite(L.map_h.data(), sizeof(int32_t), 3 * (size_t)nb, fc.f) == 3 * (size_t)nb &&
+              std::fwrite(fr.data(), sizeof(float), nc, fc.f) == nc &&
+              std::fwrite(fv.data(), sizeof(float), 3 * nc, fc.f) == 3 * nc &&
+              std::fwrite(ob.data(), 1, nc, fc.f) == nc;
+    ok = (std::fclose(fc.f) == 0) && ok;
+    fc.f = nullptr;
+    if (!ok) return fail(OAL_ERR_ARG, std::string("write failed: ") + path);
+    return OAL_OK;
+}
+
+int oal_read_raw_state(OalPlan* p, int level, const char* path, int32_t* level_id, double* dx) {
+    if (int rc = check_level(p, level)) return rc;
+    if (!path) return fail(OAL_ERR_ARG, "NULL path");
+    Level& L = p->levels[level];
+    const size_t nc = L.ncell_host;
+    const uint32_t nb = L.n_caller_blocks;
+    FileCloser fc{std::fopen(path, "rb")};
+    if (!fc.f) return fail(OAL_ERR_ARG, std::string("cannot open: ") + path);
+    int32_t head[7];
+    double h_dx = 0.0;
+    if (std::fread(head, sizeof(head), 1, fc.f) != 1 || head[0] != LBM1_MAGIC)
+        return fail(OAL_ERR_ARG, "not an LBM1 raw state file");
+    if (std::fread(&h_dx, sizeof(h_dx), 1, fc.f) != 1) return fail(OAL_ERR_ARG, "truncated LBM1 header");
+    if (head[2] != (int32_t)nb || head[3] != BS)
+        return fail(OAL_ERR_ARG, "LBM1 file: block count or block size differs from the level's");
+    std::vector<int32_t> coords(3 * (size_t)nb);
+    std::vector<float> fr(nc), fv(3 * nc);
+    std::vector<uint8_t> ob(nc);
+    if (std::fread(coords.data(), sizeof(int32_t), coords.size(), fc.f) != coords.size() ||
+        std::fread(fr.data(), sizeof(float), nc, fc.f) != nc || std::fread(fv.data(), sizeof(float), 3 * nc, fc.f) != 3 * nc ||
+        std::fread(ob.data(), 1, nc, fc.f) != nc)
+        return fail(OAL_ERR_ARG, "truncated LBM1 file");
+    unsigned char extra;
+    if (std::fread(&extra, 1, 1, fc.f) != 0) return fail(OAL_ERR_ARG, "LBM1 file has trailing data");
+    if (!std::equal(coords.begin(), coords.end(), L.map_h.begin()))
+        return fail(OAL_ERR_ARG, "LBM1 file: block coordinates differ from the level's");
+    for (size_t c = 0; c < nc; ++c) {
+        const uint8_t want = !L.obst_h.empty() && L.obst_h[c] ? 1 : 0;
+        if ((ob[c] != 0) != (want != 0)) return fail(OAL_ERR_ARG, "LBM1 file: obstacle mask differs from the level's");
+    }
+    std::vector<double> rho(nc), u(3 * nc);
+    for (size_t c = 0; c < nc; ++c) rho[c] = fr[c];
+    for (size_t i = 0; i < 3 * nc; ++i) u[i] = fv[i];
+    if (int rc = oal_init_equilibrium(p, level, rho.data(), u.data(), 1.0, nullptr)) return rc;
+    if (level_id) *level_id = head[1];
+    if (dx) *dx = h_dx;
     return OAL_OK;
 }
 

@@ -270,6 +270,21 @@ class Plan:
         check(self.lib.oal_set_bouzidi(self.h, level, ptr, self.BFL_FALLBACKS[ff_fallback], C.c_float(q_min)),
               "oal_set_bouzidi")
 
+    def write_raw_state(self, path, level: int = 0, level_id: int | None = None, dx: float = 1.0) -> None:
+        """The level's density and velocity as the reference's LBM1 raw state file (dump_raw_state,
+        src/io/raw_state_dump.jl:21-47); level_id defaults to the reference's 1-based level + 1."""
+        lid = level + 1 if level_id is None else int(level_id)
+        check(self.lib.oal_write_raw_state(self.h, level, lid, float(dx), str(path).encode()), "oal_write_raw_state")
+
+    def read_raw_state(self, path, level: int = 0) -> tuple[int, float]:
+        """Start the level at the equilibrium of an LBM1 raw state file of the same blocks and obstacle;
+        returns the file's (level_id, dx)."""
+        lid = C.c_int32(0)
+        dx = C.c_double(0.0)
+        check(self.lib.oal_read_raw_state(self.h, level, str(path).encode(), C.byref(lid), C.byref(dx)),
+              "oal_read_raw_state")
+        return int(lid.value), float(dx.value)
+
     @property
     def pipe_planes(self) -> int:
         return self.tuning["pipe_planes"]
