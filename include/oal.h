@@ -169,7 +169,8 @@ typedef struct OalLevelInfo {
     int64_t n_cells;         /* n_blocks * bs^3 */
     int64_t n_fluid_cells;   /* inside the domain extent and not solid */
     int32_t n_blocks;
-    int32_t n_regular;
+    int32_t n_regular;       /* includes, in the AB pattern, regular blocks that overhang the extent
+                                (the pipelined kernel with the face rules at the extent's last cells) */
     int32_t n_interior;
     int32_t n_boundary_regular;
     int32_t n_boundary;

@@ -59,6 +59,8 @@ struct StepArgs {
     const int32_t* blist_boundary_regular;  // [n_boundary_regular] boundary blocks with lexicographic neighbours
     const int32_t* blist_boundary;   // [n_boundary] the other blocks
     uint32_t n_regular, n_interior, n_boundary_regular, n_boundary;
+    const int32_t* blist_partial;    // [n_partial] AB pattern: regular blocks that overhang the extent, no solid near
+    uint32_t n_partial;
     const int32_t* lslot;        // [nb] link-mask slot of a block with solid cells near, -1 = none
     const uint32_t* lmask;       // [n_lslots * BS3] per cell: bit k source of direction k solid, bit 27 cell solid
     const uint32_t* face_cells;  // [n_face] AB pattern: cells of regular-list blocks with a source outside the extent

@@ -67,6 +67,8 @@ struct Level {
     DevBuf<int32_t> oslot;             // AA: outlet side-buffer slot per block
     DevBuf<int32_t> blist_rg, blist_in, blist_br, blist_bd;  // block lists per kernel kind
     uint32_t n_regular = 0, n_interior = 0, n_boundary_regular = 0, n_boundary = 0;
+    DevBuf<int32_t> blist_pr;          // regular blocks that overhang the extent (pipelined, AB)
+    uint32_t n_partial = 0;
     DevBuf<uint32_t> face_cells;       // AB: face cells of the regular-list blocks
     uint32_t n_face = 0;
     bool regular_face = false;         // AB: the regular list holds blocks on a domain face
@@ -114,7 +116,7 @@ struct Level {
     void release() {
         nbr.release(); borg.release(); bdom.release(); dom_item0.release(); dom_block0.release();
         bflags.release(); solid.release(); doms.release(); fA.release(); fB.release(); velA.release(); velB.release();
-        fpart.release(); mpart.release(); red_out.release(); guard.release(); oslot.release(); obuf.release(); blist_rg.release(); blist_in.release(); blist_br.release(); blist_bd.release(); face_cells.release();
+        fpart.release(); mpart.release(); red_out.release(); guard.release(); oslot.release(); obuf.release(); blist_rg.release(); blist_in.release(); blist_br.release(); blist_bd.release(); blist_pr.release(); face_cells.release();
         lslot.release(); lmask.release(); qmap.release();
         g_src.release(); g_dst.release(); g_slot.release(); g_w.release(); g_m.release();
         r_src.release(); r_dst.release(); r_slot.release(); r_w.release(); r_m.release();
@@ -172,7 +174,7 @@ int upload(DevBuf<T>& d, const std::vector<T>& h) {
 
 // Host tables of one level, built and validated before any device call
 struct HostLevel {
-    std::vector<int32_t> borg, bdom, nbr, nbr_bm, oslot, blist_rg, blist_in, blist_br, blist_bd, lslot;
+    std::vector<int32_t> borg, bdom, nbr, nbr_bm, oslot, blist_rg, blist_in, blist_br, blist_bd, blist_pr, lslot;
     std::vector<uint32_t> lmask;
     std::vector<uint32_t> face_cells;
     int32_t n_oslots = 0;
@@ -181,7 +183,7 @@ struct HostLevel {
 };
 
 // Host-side construction of one level's tables from the reference-convention descriptor.
-int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, Level& L, HostLevel& H,
+int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, bool pipe_ok, Level& L, HostLevel& H,
                   const std::vector<uint8_t>* inactive = nullptr) {
     if (d.n_blocks <= 0) return fail(OAL_ERR_ARG, "level with no blocks");
     if (!d.map_x || !d.map_y || !d.map_z || !d.neighbor_table) return fail(OAL_ERR_ARG, "level block arrays missing");
@@ -348,6 +350,10 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, Leve
                         for (int a = 0; a < 3; ++a) face = face || (lo[a] && l[a] == 0) || (hi[a] && l[a] == BS - 1);
                         if (face) H.face_cells.push_back(b * BS3 + (uint32_t)(x + BS * (y + BS * z)));
                     }
+        } else if (ab && pipe_ok && regular && flags[b] == (BF_EDGE | BF_PARTIAL)) {
+            // AB pattern: a block that overhangs the extent, no solid near: the pipelined kernel with the
+            // face rules at the extent's last cells
+            H.blist_pr.push_back((int32_t)b);
         } else {
             (regular ? H.blist_br : H.blist_bd).push_back((int32_t)b);
         }
@@ -356,6 +362,7 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, Leve
     L.n_regular = (uint32_t)H.blist_rg.size();
     L.n_interior = (uint32_t)H.blist_in.size();
     L.n_boundary_regular = (uint32_t)H.blist_br.size();
+    L.n_partial = (uint32_t)H.blist_pr.size();
     L.n_boundary = (uint32_t)H.blist_bd.size();
     // block-major copy of the neighbour table: one 32-entry row per block (scalar loads)
     H.nbr_bm.assign((size_t)nb * 32, -1);
@@ -825,7 +832,7 @@ int upload_level(OalPlan* p, Level& L, const HostLevel& H) {
         return rc;
     if (!H.solid.empty() && (rc = upload(L.solid, H.solid))) return rc;
     if ((rc = upload(L.blist_rg, H.blist_rg)) || (rc = upload(L.blist_in, H.blist_in)) ||
-        (rc = upload(L.blist_br, H.blist_br)) || (rc = upload(L.blist_bd, H.blist_bd)) ||
+        (rc = upload(L.blist_br, H.blist_br)) || (rc = upload(L.blist_bd, H.blist_bd)) || (rc = upload(L.blist_pr, H.blist_pr)) ||
         (rc = upload(L.lslot, H.lslot)) || (rc = upload(L.lmask, H.lmask)) || (rc = upload(L.face_cells, H.face_cells)))
         return rc;
     if (p->stream_mode == OAL_STREAM_AA) {
@@ -873,6 +880,8 @@ StepArgs make_args(OalPlan* p, Level& L, int32_t time_step, bool want_force) {
     a.blist_boundary_regular = L.blist_br.p;
     a.blist_boundary = L.blist_bd.p;
     a.n_regular = L.n_regular;
+    a.blist_partial = L.blist_pr.p;
+    a.n_partial = L.n_partial;
     a.n_interior = L.n_interior;
     a.n_boundary_regular = L.n_boundary_regular;
     a.n_boundary = L.n_boundary;
@@ -918,7 +927,8 @@ int level_step(OalPlan* p, int level, int32_t time_step, int flags, hipStream_t 
     const bool want_force = (flags & OAL_STEP_FORCE) != 0;
     if (want_force && !L.has_solid) return fail(OAL_ERR_STATE, "force requested on a level without obstacle cells");
     StepArgs a = make_args(p, L, time_step, want_force);
-    const bool split = p->overlap && L.n_regular + L.n_interior > 0 && L.n_boundary_regular + L.n_boundary + L.n_face > 0;
+    const bool split = p->overlap && L.n_regular + L.n_partial + L.n_interior > 0 &&
+                       L.n_boundary_regular + L.n_boundary + L.n_face > 0;
     if (split) {
         // fork: the side stream starts after everything queued on s, then s waits for the side stream
         OAL_HIP(hipEventRecord(p->fork, s));
@@ -1200,8 +1210,10 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
             inactive.resize(o.nb());
             for (int b = 0; b < o.nb(); ++b) inactive[b] = o.status[b] != OCT_STEPPED;
         }
-        const int rc = prepare_level(d, desc->collision, desc->stream_mode == OAL_STREAM_AB, p->levels[l], host[l],
-                                     ml ? &inactive : nullptr);
+        // the pipelined kernels cover FP32-arithmetic storage without WALE
+        const bool pipe_ok = desc->storage != OAL_F64 && !p->wale;
+        const int rc = prepare_level(d, desc->collision, desc->stream_mode == OAL_STREAM_AB, pipe_ok, p->levels[l],
+                                     host[l], ml ? &inactive : nullptr);
         if (rc) return bail(rc);
         if (ml) {
             Level& L = p->levels[l];
@@ -1366,7 +1378,7 @@ int oal_level_info(const OalPlan* p, int level, OalLevelInfo* info) {
     info->n_blocks = (int32_t)L.n_caller_blocks;
     info->n_ghost_blocks = (int32_t)L.n_ghost_blocks;
     info->n_covered_blocks = (int32_t)L.n_covered_blocks;
-    info->n_regular = (int32_t)L.n_regular;
+    info->n_regular = (int32_t)(L.n_regular + L.n_partial);
     info->n_interior = (int32_t)L.n_interior;
     info->n_boundary_regular = (int32_t)L.n_boundary_regular;
     info->n_boundary = (int32_t)L.n_boundary;
@@ -1574,7 +1586,7 @@ int oal_time_steps(OalPlan* p, int level, int n, int flags, double* ms_total, do
     OAL_HIP(hipSetDevice(p->device));
     OAL_HIP(hipStreamSynchronize(p->stream));
     const Level& L = p->levels[level];
-    const bool bulk = L.n_regular + L.n_interior > 0;
+    const bool bulk = L.n_regular + L.n_partial + L.n_interior > 0;
     if (bulk) {
         while (p->bulk_ev.size() < 2 * (size_t)n) {
             hipEvent_t e = nullptr;

@@ -636,7 +636,7 @@ struct InletRule {
 template <class SP>
 __device__ __forceinline__ void reg_collide_store(const StepArgs& a, const RegAddr<SP>& ra, const CellRates<typename SP::T>& q,
                                                   int z, uint32_t cell, const typename SP::raw (&r)[27],
-                                                  const InletRule* inlet = nullptr) {
+                                                  const InletRule* inlet = nullptr, bool store = true) {
     using T = typename SP::T;
     using st = typename SP::st;
     T f[27];
@@ -658,10 +658,12 @@ __device__ __forceinline__ void reg_collide_store(const StepArgs& a, const RegAd
     collide_cell<T>(f, q, a.cp, rho_o, ux_o, uy_o, uz_o);
     const bool dith = SP::dithered && a.cp.dither;
     const uint32_t hc = SP::dithered ? round_cell(round_key(a.step()), cell) : 0u;
-    for_dirs([&](auto kc) {
-        constexpr int K = decltype(kc)::value;
-        SP::sto_r((st*)(ra.base(a.fout, K, z, 0) + ra.v[1][1]), f[K], wgt<T>(K), round_bits(hc, K), dith);
-    });
+    if (store) {
+        for_dirs([&](auto kc) {
+            constexpr int K = decltype(kc)::value;
+            SP::sto_r((st*)(ra.base(a.fout, K, z, 0) + ra.v[1][1]), f[K], wgt<T>(K), round_bits(hc, K), dith);
+        });
+    }
 }
 
 // Domain-face rules inside the pipelined kernel (AB pattern).  A source outside the extent is replaced
@@ -672,6 +674,11 @@ struct FaceRules {
     bool xl, xh, yl, yh;          // per lane: the cell lies on the x-min / x-max / y-min / y-max face
     bool zlo, zhi;                // the block touches the z-min / z-max face (uniform)
     bool zwall, ymin_wall;        // no-slip variants of the z faces and of the y-min face (uniform)
+    // blocks that overhang the extent (FACE_PARTIAL): the planes of the z faces inside the block
+    // (-1: none) and, per lane, a cell beyond the extent in x or y (its in-plane source offsets all
+    // point at its own cell; never stored)
+    int zlo_plane, zhi_plane;
+    bool dead;
 };
 
 // Gather with the domain-face rules.  Every read is addressed by its byte offset from the start of
@@ -679,7 +686,7 @@ struct FaceRules {
 // this form only where a layer of blocks spans less than 2 GiB, so every offset read is exact.  The rules pick, per lane, one
 // of a few candidate offsets that are all computed (uniform parts in scalar registers), so that the
 // choice is a select rather than a branch.
-template <class SP>
+template <class SP, bool PARTIAL = false>
 __device__ __forceinline__ void reg_load_raw_face(const StepArgs& a, const RegAddr<SP>& ra, const FaceRules& fr, int z,
                                                   typename SP::raw (&r)[27]) {
     using st = typename SP::st;
@@ -692,7 +699,8 @@ __device__ __forceinline__ void reg_load_raw_face(const StepArgs& a, const RegAd
         const int dz = sz < 0 ? -1 : (sz >= BS ? 1 : 0);
         zs[q] = (uint32_t)((dz * ra.plane_elems + (int64_t)(sz - BS * dz - ra.zref) * BS2) * (int64_t)sizeof(st));
     }
-    const bool zlo = fr.zlo && z == 0, zhi = fr.zhi && z == BS - 1;   // uniform
+    const bool zlo = PARTIAL ? z == fr.zlo_plane : (fr.zlo && z == 0);       // uniform
+    const bool zhi = PARTIAL ? z == fr.zhi_plane : (fr.zhi && z == BS - 1);
     // uniform parts pass through readfirstlane, which the optimiser cannot see through: it must not
     // fold them into per-lane sums and hoist 27 of those out of the plane loop
     auto uni = [](uint32_t u) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)u); };
@@ -725,8 +733,9 @@ __device__ __forceinline__ void reg_load_raw_face(const StepArgs& a, const RegAd
     });
 }
 
-// FACE: how blocks on a domain face are handled
-enum : int { FACE_NONE = 0, FACE_RULES = 1, FACE_SKIP = 2 };
+// FACE: how blocks on a domain face are handled.  FACE_PARTIAL: blocks that overhang the extent, with
+// the face rules at the extent's last cells and the cells beyond it neither read through nor stored.
+enum : int { FACE_NONE = 0, FACE_RULES = 1, FACE_SKIP = 2, FACE_PARTIAL = 3 };
 
 // Register budget: with one plane of populations in flight beside the collision of another the kernel
 // fits the 168 VGPRs of 3 waves per SIMD; the in-kernel face rules add per-lane address selects and
@@ -735,7 +744,7 @@ enum : int { FACE_NONE = 0, FACE_RULES = 1, FACE_SKIP = 2 };
 #define OAL_FACE_RULES_WAVES 2
 #endif
 template <int FACE>
-constexpr int pipe_waves() { return FACE == FACE_RULES ? OAL_FACE_RULES_WAVES : 3; }
+constexpr int pipe_waves() { return FACE == FACE_RULES || FACE == FACE_PARTIAL ? OAL_FACE_RULES_WAVES : 3; }
 
 template <class SP, int NP, int FACE>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(pipe_waves<FACE>(), pipe_waves<FACE>())))
@@ -750,6 +759,14 @@ k_step_pipe(const StepArgs a) {
     const uint32_t litem = xcd_item(blockIdx.x, gridDim.x);
     const uint32_t b = (uint32_t)a.blist[litem / groups];
     const int z0 = (int)(litem % groups) * np;
+    // FACE_PARTIAL: planes from zmax on lie beyond the extent and are neither read nor stepped (the
+    // loop bound itself stays z0 + np: a bound the compiler can narrow lets it reorder the prefetch)
+    int zmax = BS;
+    if constexpr (FACE == FACE_PARTIAL) {
+        const DevDomain* d0 = a.doms + a.bdom[b];
+        zmax = d0->ext[2] - a.borg[3 * b + 2];
+        if (z0 >= zmax) return;
+    }
     CellCtx c;
     c.b = b;
     c.lx = threadIdx.x & (BS - 1);
@@ -777,6 +794,31 @@ k_step_pipe(const StepArgs a) {
         lane_face = face_lane(fskip, c.lx, c.ly);
     }
     auto live = [&](int z) { return FACE != FACE_SKIP || !(lane_face || face_plane(fskip, z)); };
+    if constexpr (FACE == FACE_PARTIAL) {
+        const int gx = a.borg[3 * b] + c.lx, gy = a.borg[3 * b + 1] + c.ly, oz = a.borg[3 * b + 2];
+        fr.xl = !dm->per[0] && gx == 0;
+        fr.xh = !dm->per[0] && gx == dm->ext[0] - 1;
+        fr.yl = !dm->per[1] && gy == 0;
+        fr.yh = !dm->per[1] && gy == dm->ext[1] - 1;
+        fr.zlo_plane = (!dm->per[2] && oz == 0) ? 0 : -1;
+        fr.zhi_plane = (!dm->per[2] && dm->ext[2] - 1 - oz < BS) ? dm->ext[2] - 1 - oz : -1;
+        fr.dead = gx >= dm->ext[0] || gy >= dm->ext[1];
+        if (fr.dead) {
+            // a cell beyond the extent reads only its own column (its x / y sources may lie in blocks
+            // that do not exist); the z rules keep its reads inside the level as for every lane
+            const int32_t v11 = ra.v[1][1];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) ra.v[i][j] = v11;
+        }
+        fr.zwall = dm->zwall != 0;
+        fr.ymin_wall = dm->ymin_wall == 1;
+        in.on = fr.xl;
+        in.u_in = dm->u_in;
+        in.gy = gy;
+        in.gz = oz;      // + z per plane
+    }
     if constexpr (FACE == FACE_RULES) {
         const FaceBits fb = face_bits(a, b, dm);
         fr.xl = fb.xlo && c.lx == 0;
@@ -794,6 +836,7 @@ k_step_pipe(const StepArgs a) {
     }
     auto load = [&](const RegAddr<SP>& r, int z, typename SP::raw (&dst)[27]) {
         if constexpr (FACE == FACE_RULES) reg_load_raw_face<SP>(a, r, fr, z, dst);
+        else if constexpr (FACE == FACE_PARTIAL) { if (z < zmax) reg_load_raw_face<SP, true>(a, r, fr, z, dst); }
         else if (live(z)) reg_load_raw<SP>(a, r, z, dst);
     };
     const uint32_t cell0 = dm->cell0;   // read before the first store, as every domain parameter
@@ -803,6 +846,11 @@ k_step_pipe(const StepArgs a) {
             InletRule iz = in;
             iz.gz += z;
             reg_collide_store<SP>(a, r, q, z, cell, src, &iz);
+        } else if constexpr (FACE == FACE_PARTIAL) {
+            if (z >= zmax) return;
+            InletRule iz = in;
+            iz.gz += z;
+            reg_collide_store<SP>(a, r, q, z, cell, src, &iz, !fr.dead);
         } else if (live(z)) {
             reg_collide_store<SP>(a, r, q, z, cell, src);
         }
@@ -898,6 +946,25 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
             }
         } else {
             hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_REGULAR>), dim3(a.n_regular * PLANES), dim3(WG), 0, stream, r);
+        }
+    }
+    if (a.n_partial) {
+        // regular blocks that overhang the extent (listed only where the pipelined kernel applies); at
+        // most 8 planes per workgroup (the 16-plane form of this kernel spills)
+        r.blist = a.blist_partial;
+        if constexpr (pipe_capable) {
+            StepArgs rp = r;
+            rp.pipe_planes = a.pipe_planes > 8 ? 8 : a.pipe_planes;
+            const dim3 g(a.n_partial * (PLANES / (rp.pipe_planes > 1 ? rp.pipe_planes : 1)));
+            switch (rp.pipe_planes) {
+                case 2: hipLaunchKernelGGL((k_step_pipe<SP, 2, FACE_PARTIAL>), g, dim3(WG), 0, stream, rp); break;
+                case 4: hipLaunchKernelGGL((k_step_pipe<SP, 4, FACE_PARTIAL>), g, dim3(WG), 0, stream, rp); break;
+                case 8: hipLaunchKernelGGL((k_step_pipe<SP, 8, FACE_PARTIAL>), g, dim3(WG), 0, stream, rp); break;
+                default: hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), g, dim3(WG), 0, stream, rp); break;
+            }
+        } else {
+            hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), dim3(a.n_partial * PLANES), dim3(WG), 0,
+                               stream, r);
         }
     }
     if (a.n_interior) {
