@@ -762,10 +762,12 @@ k_step_pipe(const StepArgs a) {
     // FACE_PARTIAL: planes from zmax on lie beyond the extent and are neither read nor stepped (the
     // loop bound itself stays z0 + np: a bound the compiler can narrow lets it reorder the prefetch)
     int zmax = BS;
-    if constexpr (FACE == FACE_PARTIAL) {
+    if constexpr (FACE == FACE_PARTIAL || FACE == FACE_RULES) {
+        // (FACE_RULES: whole blocks, zmax >= BS; the guard it adds to every plane keeps the compiler from
+        // hoisting the next plane's address arithmetic, which spills at 2 to 8 planes)
         const DevDomain* d0 = a.doms + a.bdom[b];
         zmax = d0->ext[2] - a.borg[3 * b + 2];
-        if (z0 >= zmax) return;
+        if (FACE == FACE_PARTIAL && z0 >= zmax) return;
     }
     CellCtx c;
     c.b = b;
@@ -835,7 +837,7 @@ k_step_pipe(const StepArgs a) {
         in.gz = a.borg[3 * b + 2];      // + z per plane
     }
     auto load = [&](const RegAddr<SP>& r, int z, typename SP::raw (&dst)[27]) {
-        if constexpr (FACE == FACE_RULES) reg_load_raw_face<SP>(a, r, fr, z, dst);
+        if constexpr (FACE == FACE_RULES) { if (z < zmax) reg_load_raw_face<SP>(a, r, fr, z, dst); }
         else if constexpr (FACE == FACE_PARTIAL) { if (z < zmax) reg_load_raw_face<SP, true>(a, r, fr, z, dst); }
         else if (live(z)) reg_load_raw<SP>(a, r, z, dst);
     };
@@ -843,6 +845,7 @@ k_step_pipe(const StepArgs a) {
     auto collide = [&](const RegAddr<SP>& r, int z, const typename SP::raw (&src)[27]) {
         const uint32_t cell = c.own + (uint32_t)(z - z0) * BS2 - cell0;
         if constexpr (FACE == FACE_RULES) {
+            if (z >= zmax) return;
             InletRule iz = in;
             iz.gz += z;
             reg_collide_store<SP>(a, r, q, z, cell, src, &iz);
