@@ -77,3 +77,36 @@ def test_bfl_off_again_is_halfway(oracle):
     np.testing.assert_array_equal(plans[0].download_f(), plans[1].download_f())
     for p in plans:
         p.close()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16s"])
+def test_bfl_refined_level_half_q_is_halfway(storage):
+    """on the finest level of a refined plan (caller blocks, then ghost blocks in the library's
+    numbering), q = 1/2 on every solid link gives the halfway populations bit for bit on every level,
+    and a sphere q-map runs with a finite state"""
+    from openaerolab_alpha_01_amd import solver
+    from openaerolab_alpha_01_amd.geometry import link_solid, sphere_octree, sphere_qmap
+    centre, R = (40.0, 32.0, 32.0), 5.0
+    grids = sphere_octree((48, 32, 32), 2, centre, R, (8,))
+    fine = grids[-1]
+    fl = (np.asarray(fine.solid) == 0) & fine.inside()
+    q_half = np.where(link_solid(fine) & fl[None, :], 0.5, 0.0).astype(np.float16)
+    params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
+    plans = [solver.octree_plan(grids, 0.54, storage=storage, params=params, u_inlet=0.04) for _ in range(3)]
+    plans[0].set_bouzidi(q_half, level=len(grids) - 1)
+    plans[2].set_bouzidi(sphere_qmap(fine, centre, R), level=len(grids) - 1)
+    for p in plans:
+        for l, g in enumerate(grids):
+            rho, u = fields(g, seed=30 + l, u0=(0.04, 0.0, 0.0))
+            p.init_equilibrium(rho, u, level=l)
+        print(f"bfl refined {storage}", flush=True)
+        p.step(3)
+        p.sync()
+    for l in range(len(grids)):
+        np.testing.assert_array_equal(plans[0].download_f(level=l, dtype=np.float64),
+                                      plans[1].download_f(level=l, dtype=np.float64))
+        assert plans[2].guard(0.2, 0.5, level=l) == 0
+        assert not np.array_equal(plans[2].download_f(level=l, dtype=np.float64),
+                                  plans[1].download_f(level=l, dtype=np.float64)) or l == 0
+    for p in plans:
+        p.close()
