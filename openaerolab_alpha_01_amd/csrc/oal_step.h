@@ -356,7 +356,9 @@ __device__ __forceinline__ void bfl_links(const StepArgs& a, const CellCtx& c, t
     });
 }
 
-template <class SP, int PAT, bool WALE, bool GEN, bool REG = false>
+// BFL: the level has a q-map (interpolated bounce-back); a separate instantiation, so that the boundary
+// kernels of levels without one keep their register budget.
+template <class SP, int PAT, bool WALE, bool GEN, bool REG = false, bool BFL = false>
 __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, const DevDomain* dm, bool active,
                                           bool solid_own, double& F0, double& F1, double& F2) {
     using T = typename SP::T;
@@ -445,8 +447,8 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
             f[K] = val;
         });
     }
-    if constexpr (PAT == PAT_AB && GEN) {
-        if (a.qmap && (c.flags & BF_SOLID) && solid_links)
+    if constexpr (PAT == PAT_AB && GEN && BFL) {
+        if ((c.flags & BF_SOLID) && solid_links)
             bfl_links<SP>(a, c, f, solid_links, reg_links, a.fpart != nullptr, F0, F1, F2);
     }
 
@@ -482,7 +484,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
 
     const bool want_force = a.fpart != nullptr;
     // (with a q-map the links' momentum exchange was taken at the gather)
-    const bool link_force = GEN && want_force && (c.flags & BF_SOLID) && !solid_own && !a.qmap;
+    const bool link_force = GEN && !BFL && want_force && (c.flags & BF_SOLID) && !solid_own;
     const bool outlet_cell = GEN && PAT == PAT_AA_EVEN && (c.flags & BF_EDGE) && !c.per0 && c.gx == c.ext0 - 1;
     // rounding of the stored post-collision populations: population j of this cell takes the random
     // bits of (cell, j, step) whichever slot it goes to, so every pattern stores the same values
@@ -557,7 +559,7 @@ enum : int { SEL_ALL = 0, SEL_INTERIOR = 1, SEL_BOUNDARY = 2, SEL_REGULAR = 3, S
 #ifndef OAL_STEP_WAVES
 #define OAL_STEP_WAVES 3
 #endif
-template <class SP, int PAT, bool WALE, int SEL = SEL_ALL>
+template <class SP, int PAT, bool WALE, int SEL = SEL_ALL, bool BFL = false>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(OAL_STEP_WAVES))) k_step(const StepArgs a) {
     const uint32_t litem = xcd_item(blockIdx.x, gridDim.x);
     const uint32_t lb = litem / PLANES;
@@ -596,9 +598,9 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(OAL_STE
         const bool solid_own = (c.lmask >> 27) & 1u;
         const bool active = inside && !(solid_own && a.cp.bb_mode == 1);
         if constexpr (SEL == SEL_BOUNDARY_REGULAR)
-            step_cell<SP, PAT, WALE, true, true>(a, c, dm, active, solid_own, F0, F1, F2);
+            step_cell<SP, PAT, WALE, true, true, BFL>(a, c, dm, active, solid_own, F0, F1, F2);
         else
-            step_cell<SP, PAT, WALE, true>(a, c, dm, active, solid_own, F0, F1, F2);
+            step_cell<SP, PAT, WALE, true, false, BFL>(a, c, dm, active, solid_own, F0, F1, F2);
     }
     if (a.fpart) wg_sum3(F0 * dm->force_scale, F1 * dm->force_scale, F2 * dm->force_scale, a.fpart, item);
 }
@@ -914,14 +916,27 @@ template <class SP, int PAT, bool WALE>
 static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t side, hipEvent_t bulk0, hipEvent_t bulk1) {
     StepArgs r = a;
     hipStream_t bs = side ? side : stream;
+    // (a q-map exists only in the AB pattern)
+    const bool bfl = PAT == PAT_AB && a.qmap != nullptr;
     if (a.n_boundary_regular) {
         r.blist = a.blist_boundary_regular;
-        hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), dim3(a.n_boundary_regular * PLANES), dim3(WG), 0,
-                           bs, r);
+        const dim3 g(a.n_boundary_regular * PLANES);
+        if constexpr (PAT == PAT_AB) {
+            if (bfl) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, true>), g, dim3(WG), 0, bs, r);
+            else hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), g, dim3(WG), 0, bs, r);
+        } else {
+            hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), g, dim3(WG), 0, bs, r);
+        }
     }
     if (a.n_boundary) {
         r.blist = a.blist_boundary;
-        hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY>), dim3(a.n_boundary * PLANES), dim3(WG), 0, bs, r);
+        const dim3 g(a.n_boundary * PLANES);
+        if constexpr (PAT == PAT_AB) {
+            if (bfl) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY, true>), g, dim3(WG), 0, bs, r);
+            else hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY>), g, dim3(WG), 0, bs, r);
+        } else {
+            hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY>), g, dim3(WG), 0, bs, r);
+        }
     }
     // face cells: the pipelined kernel applies the face rules itself, the one-plane regular kernel
     // leaves them to the face-cell kernel
