@@ -927,8 +927,9 @@ int level_step(OalPlan* p, int level, int32_t time_step, int flags, hipStream_t 
     const bool want_force = (flags & OAL_STEP_FORCE) != 0;
     if (want_force && !L.has_solid) return fail(OAL_ERR_STATE, "force requested on a level without obstacle cells");
     StepArgs a = make_args(p, L, time_step, want_force);
-    const bool split = p->overlap && L.n_regular + L.n_partial + L.n_interior > 0 &&
-                       L.n_boundary_regular + L.n_boundary + L.n_face > 0;
+    // side-stream work: boundary blocks, face cells and the blocks that overhang the extent
+    const bool split = p->overlap && L.n_regular + L.n_interior > 0 &&
+                       L.n_boundary_regular + L.n_boundary + L.n_face + L.n_partial > 0;
     if (split) {
         // fork: the side stream starts after everything queued on s, then s waits for the side stream
         OAL_HIP(hipEventRecord(p->fork, s));
@@ -1586,7 +1587,7 @@ int oal_time_steps(OalPlan* p, int level, int n, int flags, double* ms_total, do
     OAL_HIP(hipSetDevice(p->device));
     OAL_HIP(hipStreamSynchronize(p->stream));
     const Level& L = p->levels[level];
-    const bool bulk = L.n_regular + L.n_partial + L.n_interior > 0;
+    const bool bulk = L.n_regular + L.n_interior > 0;
     if (bulk) {
         while (p->bulk_ev.size() < 2 * (size_t)n) {
             hipEvent_t e = nullptr;

@@ -968,21 +968,22 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
     }
     if (a.n_partial) {
         // regular blocks that overhang the extent (listed only where the pipelined kernel applies); at
-        // most 8 planes per workgroup (the 16-plane form of this kernel spills)
+        // most 8 planes per workgroup (the 16-plane form of this kernel spills); on the side stream
+        // with the boundary launches, overlapping the bulk launch
         r.blist = a.blist_partial;
         if constexpr (pipe_capable) {
             StepArgs rp = r;
             rp.pipe_planes = a.pipe_planes > 8 ? 8 : a.pipe_planes;
             const dim3 g(a.n_partial * (PLANES / (rp.pipe_planes > 1 ? rp.pipe_planes : 1)));
             switch (rp.pipe_planes) {
-                case 2: hipLaunchKernelGGL((k_step_pipe<SP, 2, FACE_PARTIAL>), g, dim3(WG), 0, stream, rp); break;
-                case 4: hipLaunchKernelGGL((k_step_pipe<SP, 4, FACE_PARTIAL>), g, dim3(WG), 0, stream, rp); break;
-                case 8: hipLaunchKernelGGL((k_step_pipe<SP, 8, FACE_PARTIAL>), g, dim3(WG), 0, stream, rp); break;
-                default: hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), g, dim3(WG), 0, stream, rp); break;
+                case 2: hipLaunchKernelGGL((k_step_pipe<SP, 2, FACE_PARTIAL>), g, dim3(WG), 0, bs, rp); break;
+                case 4: hipLaunchKernelGGL((k_step_pipe<SP, 4, FACE_PARTIAL>), g, dim3(WG), 0, bs, rp); break;
+                case 8: hipLaunchKernelGGL((k_step_pipe<SP, 8, FACE_PARTIAL>), g, dim3(WG), 0, bs, rp); break;
+                default: hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), g, dim3(WG), 0, bs, rp); break;
             }
         } else {
             hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), dim3(a.n_partial * PLANES), dim3(WG), 0,
-                               stream, r);
+                               bs, r);
         }
     }
     if (a.n_interior) {
