@@ -77,6 +77,18 @@ struct Lattice {
 };
 const Lattice LAT;
 
+// Planted mistakes (test hook, oracle_set_plant): the pin tests switch one on to show that their
+// check fails on it.  0 = none, the only value outside those tests.
+//   1  5th-order rebuild: the c110 c111 term of c221 with factor 2 instead of 4
+//   2  6th-order rebuild: the c110 c101 c011 term of c222 with factor 4 instead of 8
+//   3  inlet hash: second xor-shift by 12 instead of 13
+//   4  inlet noise: Int32 combination formed in saturating arithmetic instead of wrapping
+//   5  WALE gradients: a solid neighbour's velocity used instead of the cell's own
+//   6  WALE gradients: an absent neighbour contributes zero velocity instead of the cell's own
+//   7  BFL fallback 1: the wall equilibrium at unit density instead of the cell's density
+//   8  BFL fallback 2: the own population of the wall-ward link instead of the reflected one
+int g_plant = 0;
+
 // weights as the reference builds them in the working precision (timestep.jl:2013)
 template <class T> T weight(int k) {
     switch (LAT.d2[k]) {
@@ -249,13 +261,17 @@ T feq_auto(T rho, T ux, T uy, T uz, T w, T cx, T cy, T cz, int compressible) {
 inline uint32_t gpu_hash(int32_t x) {
     uint32_t h = (uint32_t)x;
     h = (h ^ (h >> 16)) * 0x85ebca6bu;
-    h = (h ^ (h >> 13)) * 0xc2b2ae35u;
+    h = (h ^ (h >> (g_plant == 3 ? 12 : 13))) * 0xc2b2ae35u;
     return h ^ (h >> 16);
 }
 
 // Int32 arithmetic wraps in the reference (Julia Int32 * and +); unsigned arithmetic here.
 inline float gradient_noise(int32_t gx, int32_t gy, int32_t gz, int32_t seed) {
     uint32_t c = (uint32_t)gx * 374761393u + (uint32_t)gy * 668265263u + (uint32_t)gz * 1274126177u + (uint32_t)seed;
+    if (g_plant == 4) {
+        const int64_t w = (int64_t)gx * 374761393 + (int64_t)gy * 668265263 + (int64_t)gz * 1274126177 + seed;
+        c = (uint32_t)(int32_t)std::max<int64_t>(INT32_MIN, std::min<int64_t>(INT32_MAX, w));
+    }
     uint32_t h = gpu_hash((int32_t)c);
     return ((float)(h & 0xffffu) / 32768.0f) - 1.0f;
 }
@@ -479,7 +495,7 @@ void collide(const T fin[27], T rho, T jx, T jy, T jz, T nu_eddy, const Coll<T>&
     T c121_s = C_121_s + c020_s * c101_s + T(2) * c110_s * c011_s;
     T c112_s = C_112_s + c002_s * c110_s + T(2) * c101_s * c011_s;
     T c221_s = c200_s * c021_s + c020_s * c201_s +
-               T(4) * c110_s * c111_s + T(2) * c101_s * c120_s + T(2) * c011_s * c210_s;
+               T(g_plant == 1 ? 2 : 4) * c110_s * c111_s + T(2) * c101_s * c120_s + T(2) * c011_s * c210_s;
     T c212_s = c200_s * c012_s + c002_s * c210_s +
                T(2) * c110_s * c102_s + T(4) * c101_s * c111_s + T(2) * c011_s * c201_s;
     T c122_s = c020_s * c102_s + c002_s * c120_s +
@@ -489,7 +505,7 @@ void collide(const T fin[27], T rho, T jx, T jy, T jz, T nu_eddy, const Coll<T>&
     c222_s += T(2) * (c210_s * c012_s + c201_s * c021_s + c120_s * c102_s) + T(4) * c111_s * c111_s;
     c222_s += c200_s * c020_s * c002_s +
               T(2) * (c200_s * c011_s * c011_s + c020_s * c101_s * c101_s + c002_s * c110_s * c110_s) +
-              T(8) * c110_s * c101_s * c011_s;
+              T(g_plant == 2 ? 4 : 8) * c110_s * c101_s * c011_s;
 
     // write back (kernel_cumulant.jl:786-804)
     T r = k_000;
@@ -684,7 +700,7 @@ void collide_fast(T f[27], T nu_eddy, const Coll<T>& c, T& rho_o, T u_o[3]) {
     const T s211 = fma_(y110, s101, fma_(s200, s011, S211));
     const T s121 = fma_(y110, s011, fma_(s020, s101, S121));
     const T s112 = fma_(y101, s011, fma_(s002, s110, S112));
-    const T s221 = fma_(y011, s210, fma_(y101, s120, fma_(T(2) * y110, s111, fma_(s020, s201, s200 * s021))));
+    const T s221 = fma_(y011, s210, fma_(y101, s120, fma_(T(g_plant == 1 ? 1 : 2) * y110, s111, fma_(s020, s201, s200 * s021))));
     const T s212 = fma_(y011, s201, fma_(T(2) * y101, s111, fma_(y110, s102, fma_(s002, s210, s200 * s012))));
     const T s122 = fma_(y101, s021, fma_(T(2) * y011, s111, fma_(y110, s012, fma_(s002, s120, s020 * s102))));
     T e = s200 * S022;
@@ -701,7 +717,7 @@ void collide_fast(T f[27], T nu_eddy, const Coll<T>& c, T& rho_o, T u_o[3]) {
     e = fma_(s200 * y011, s011, e);
     e = fma_(s020 * y101, s101, e);
     e = fma_(s002 * y110, s110, e);
-    e = fma_((T(4) * y110) * s101, s011, e);
+    e = fma_((T(g_plant == 2 ? 2 : 4) * y110) * s101, s011, e);
     const T s222 = e;
 
     const T r = rho;
@@ -784,6 +800,37 @@ struct Geo {
     }
 };
 
+// WALE's velocity gradient g[3a + j] = d u_a / d x_j of cell (x, y, z) of block b: central differences
+// of the previous step's velocity over the six face neighbours, where a solid neighbour, or one that
+// does not exist (outside a non-periodic extent, or an absent block), contributes the cell's own
+// velocity (compute_velocity_gradients_fluid / get_velocity_neighbor_fluid, physics_utils.jl:1981-2035,
+// 2217-2233, as the cumulant kernel calls them: no symmetry-plane mirror)
+template <class T>
+void vel_gradients(const OrGrid& g, const Geo& geo, const T* vel_in, int b, int x, int y, int z, T gr[9]) {
+    const int64_t ncell = geo.ncell;
+    const int64_t own = (int64_t)b * geo.bs3 + ((int64_t)z * g.bs + y) * g.bs + x;
+    const int gx = g.bcoord[3 * b] * g.bs + x, gy = g.bcoord[3 * b + 1] * g.bs + y, gz = g.bcoord[3 * b + 2] * g.bs + z;
+    const T u0[3] = {vel_in[own], vel_in[ncell + own], vel_in[2 * ncell + own]};
+    T un[6][3];
+    const int dd[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    for (int q = 0; q < 6; ++q) {
+        const int ngx = gx + dd[q][0], ngy = gy + dd[q][1], ngz = gz + dd[q][2];
+        int64_t n = -1;
+        if (!geo.outside(0, ngx) && !geo.outside(1, ngy) && !geo.outside(2, ngz))
+            n = geo.locate(b, x + dd[q][0], y + dd[q][1], z + dd[q][2]);
+        const bool solid = n >= 0 && g.solid && g.solid[n] && g_plant != 5;
+        for (int a = 0; a < 3; ++a) {
+            if (n < 0) un[q][a] = g_plant == 6 ? T(0) : u0[a];
+            else un[q][a] = solid ? u0[a] : vel_in[a * ncell + n];
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        gr[3 * a + 0] = T(0.5) * (un[0][a] - un[1][a]);
+        gr[3 * a + 1] = T(0.5) * (un[2][a] - un[3][a]);
+        gr[3 * a + 2] = T(0.5) * (un[4][a] - un[5][a]);
+    }
+}
+
 template <class P>
 void step_impl(const OrGrid& g, const OrParams& prm, const typename P::st* fin, typename P::st* fout,
                const typename P::T* vel_in, typename P::T* vel_out, typename P::T* rho_out, double* force) {
@@ -833,9 +880,9 @@ void step_impl(const OrGrid& g, const OrParams& prm, const typename P::st* fin, 
                             if (!support && prm.bfl_fallback == 1) {
                                 T r = T(0);
                                 for (int kk = 0; kk < 27; ++kk) r += own_f(kk);
-                                fff = r * w[i];
+                                fff = (g_plant == 7 ? T(1) : r) * w[i];
                             } else if (!support && prm.bfl_fallback == 2) {
-                                fff = own_f(k);
+                                fff = own_f(g_plant == 8 ? i : k);
                             }
                             const T c1 = T(2) * tq;
                             return c1 * fk + (T(1) - c1) * fff;
@@ -911,24 +958,8 @@ void step_impl(const OrGrid& g, const OrParams& prm, const typename P::st* fin, 
 
                     T nu_eddy = T(0);
                     if (cp.c_wale != T(0) && vel_in) {
-                        // central differences of the previous step's velocity (physics_utils.jl:1981-2035, 2217-2233)
-                        T u0[3] = {vel_in[own], vel_in[ncell + own], vel_in[2 * ncell + own]};
-                        T un[6][3];
-                        const int dd[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
-                        for (int q = 0; q < 6; ++q) {
-                            const int ngx = gx + dd[q][0], ngy = gy + dd[q][1], ngz = gz + dd[q][2];
-                            int64_t n = -1;
-                            if (!geo.outside(0, ngx) && !geo.outside(1, ngy) && !geo.outside(2, ngz))
-                                n = geo.locate(b, x + dd[q][0], y + dd[q][1], z + dd[q][2]);
-                            const bool use_own = n < 0 || (g.solid && g.solid[n]);
-                            for (int a = 0; a < 3; ++a) un[q][a] = use_own ? u0[a] : vel_in[a * ncell + n];
-                        }
                         T gr[9];
-                        for (int a = 0; a < 3; ++a) {
-                            gr[3 * a + 0] = T(0.5) * (un[0][a] - un[1][a]);
-                            gr[3 * a + 1] = T(0.5) * (un[2][a] - un[3][a]);
-                            gr[3 * a + 2] = T(0.5) * (un[4][a] - un[5][a]);
-                        }
+                        vel_gradients(g, geo, vel_in, b, x, y, z, gr);
                         nu_eddy = wale_nu(gr, cp.c_wale);
                     }
 
@@ -1058,6 +1089,21 @@ void oracle_round_bits(const int64_t* cell, const int32_t* k, int32_t time_step,
 }
 
 float oracle_gradient_noise(int32_t gx, int32_t gy, int32_t gz, int32_t seed) { return gradient_noise(gx, gy, gz, seed); }
+// test hook: switch on planted mistake `which` (0 = none)
+void oracle_set_plant(int which) { g_plant = which; }
+// WALE velocity gradients of every cell of the grid from a velocity field [3][ncell] (FP64): out[9][ncell]
+void oracle_vel_gradients(const OrGrid* g, const double* vel_in, double* out) {
+    const Geo geo(*g);
+    for (int b = 0; b < g->nb; ++b)
+        for (int z = 0; z < g->bs; ++z)
+            for (int y = 0; y < g->bs; ++y)
+                for (int x = 0; x < g->bs; ++x) {
+                    double gr[9];
+                    vel_gradients(*g, geo, vel_in, b, x, y, z, gr);
+                    const int64_t own = (int64_t)b * geo.bs3 + ((int64_t)z * g->bs + y) * g->bs + x;
+                    for (int j = 0; j < 9; ++j) out[j * geo.ncell + own] = gr[j];
+                }
+}
 uint32_t oracle_gpu_hash(int32_t x) { return gpu_hash(x); }
 
 double oracle_feq_f64(double rho, double ux, double uy, double uz, int k, int compressible) {

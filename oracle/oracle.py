@@ -3,7 +3,9 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
 as the checker; the product (openaerolab_alpha_01_amd) never imports it.
 
-Parity status of each oracle function and what pins it (tests/test_oracle_*.py):
+Parity status of each oracle function and what pins it (tests/test_oracle_*.py).  The pins of
+test_oracle_pins.py each come with a planted mistake (`planted`, a test hook of the C++ oracle) that the
+same check is shown to catch.
 
   lattice tables        pinned: isotropy identities (sum w = 1, sum w c c = I/3, 4th-order isotropy),
                         opp / mirror involutions, index formula of lattice.jl:16-19
@@ -11,10 +13,15 @@ Parity status of each oracle function and what pins it (tests/test_oracle_*.py):
                         against numpy's float16 cast (round to nearest even) on random floats
   equilibria            pinned: exact moments rho, rho u, rho (u u + I/3) (2nd order); 3rd/4th-order
                         Hermite moments of the O(u^4) form
-  inlet noise hash      pinned: bijectivity of the murmur3 finaliser on a large sample, range [-1, 1);
-                        fmix32(0) = 0.  The exact hash values are parity unpinned (no reference vector).
+  inlet noise hash      pinned (test_oracle_pins.py): gpu_hash is MurmurHash3_x86_32's finaliser, checked
+                        against 13 published MurmurHash3_x86_32 test vectors (empty-key vectors are the
+                        finaliser of the seed); gradient_noise's Int32 combination wraps modulo 2^32 on
+                        overflowing coordinates; bijectivity and range [-1, 1) (test_oracle_lattice.py)
   chimera transforms    pinned: forward equals brute-force central moments sum f (c-u)^a (FP64);
                         backward(forward(f)) = f
+  5th/6th-order rebuild pinned (test_oracle_pins.py): the post-collision 5th- and 6th-order cumulants,
+                        computed from brute-force moments by the set-partition formula, vanish with
+                        omega_3, omega_4 != 1 and 3rd-order input
   collision (one cell)  pinned: mass and momentum conservation (no force); rest state fixed point;
                         omega1 = 1 with omega_bulk = omega_3 = omega_4 = 1 gives the cumulant
                         equilibrium (2nd-order cumulants I/3, higher cumulants zero) checked in moment
@@ -28,6 +35,9 @@ Parity status of each oracle function and what pins it (tests/test_oracle_*.py):
                         rules by symmetry of a mirrored run
   WALE eddy viscosity   pinned: the model's tensor definition evaluated independently (einsum), zero in
                         pure shear, degree-1 homogeneity, frame invariance (test_oracle_wale.py)
+  WALE gradient gather  pinned (test_oracle_pins.py): exact gradient of a linear field with fluid
+                        neighbours; one-sided half differences next to solid cells, the extent and an
+                        absent block, derived by hand; periodic wrap
   refined runs          pinned (octree.py, test_oracle_octree.py): cubic weights reproduce cubics,
                         stencils are partitions of unity and reproduce linear fields, uniform flow
                         across 2 and 3 levels, restriction of the prolongation on a freestream, shear
@@ -37,8 +47,9 @@ Parity status of each oracle function and what pins it (tests/test_oracle_*.py):
   interpolated          pinned (test_oracle_bfl.py): q = 1/2 reproduces halfway bounce-back bit for bit,
   bounce-back (BFL)     with the gather-time force equal to the halfway force of the step before;
                         Poiseuille walls off the half-link (q = 1/4, 3/4) follow the parabola of the
-                        q-placed walls; rest state kept under every fallback.  The values of fallbacks
-                        1 and 2 away from rest are parity unpinned (no reference vector)
+                        q-placed walls; rest state kept under every fallback; fallbacks 0, 1 and 2 away
+                        from rest on a single cut link, the arriving value read back from the link's
+                        momentum exchange and derived by hand (test_oracle_pins.py)
   golden fixtures       tests/golden/ (written from this oracle; regression pins, test_golden.py)
 """
 from __future__ import annotations
@@ -115,6 +126,8 @@ def load() -> C.CDLL:
     for nm in ("oracle_f32_to_f16_sr", "oracle_f32_to_bf16_sr"):
         getattr(lib, nm).argtypes = [vp, vp, vp, C.c_int64]
     lib.oracle_round_bits.argtypes = [vp, vp, C.c_int32, vp, C.c_int64]
+    lib.oracle_set_plant.argtypes = [C.c_int]
+    lib.oracle_vel_gradients.argtypes = [C.POINTER(OrGrid), vp, vp]
     _lib = lib
     return lib
 
@@ -220,6 +233,30 @@ def moments(storage: str, grid: Grid, f: np.ndarray):
     u = np.zeros((3, grid.ncell))
     lib.oracle_moments(STORAGE[storage], C.byref(grid.c), f.ctypes.data, out, rho.ctypes.data, u.ctypes.data)
     return np.array(out[:]), rho, u
+
+
+def vel_gradients(grid: Grid, vel: np.ndarray) -> np.ndarray:
+    """WALE's velocity gradients [3, 3, ncell] (g[a, j] = d u_a / d x_j) of a velocity field [3, ncell]."""
+    v = np.ascontiguousarray(vel, dtype=np.float64)
+    out = np.zeros((9, grid.ncell))
+    load().oracle_vel_gradients(C.byref(grid.c), v.ctypes.data, out.ctypes.data)
+    return out.reshape(3, 3, grid.ncell)
+
+
+class planted:
+    """Context manager that switches on one of the oracle's planted mistakes (test hook; see
+    oal_oracle.cpp, g_plant), so that a pin test can show its check fails on it."""
+
+    def __init__(self, which: int):
+        self.which = which
+
+    def __enter__(self):
+        load().oracle_set_plant(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        load().oracle_set_plant(0)
+        return False
 
 
 def to_unshifted(storage: str, f: np.ndarray) -> np.ndarray:
