@@ -163,14 +163,14 @@ int oal_level_cells(const OalPlan* plan, int level, int64_t* n_cells);
 /* Replaces: nothing (introspection).  How a level's blocks split over the step kernels: blocks on
  * a regular lexicographic grid with no boundary rule or solid link (n_regular, one arithmetic-address
  * launch; in the AB pattern also whole blocks on a domain face with no solid near), other blocks
- * with no boundary rule (n_interior), and the boundary blocks of both kinds, launched on a second
- * stream that overlaps the bulk launches. */
+ * with no boundary rule (n_interior), and the boundary blocks of both kinds plus the regular blocks
+ * that overhang the extent (n_partial), launched on a second stream that overlaps the bulk launches.
+ * The bulk launches that oal_time_steps times (ms_kernel) cover the n_regular and n_interior blocks. */
 typedef struct OalLevelInfo {
     int64_t n_cells;         /* n_blocks * bs^3 */
     int64_t n_fluid_cells;   /* inside the domain extent and not solid */
     int32_t n_blocks;
-    int32_t n_regular;       /* includes, in the AB pattern, regular blocks that overhang the extent
-                                (the pipelined kernel with the face rules at the extent's last cells) */
+    int32_t n_regular;
     int32_t n_interior;
     int32_t n_boundary_regular;
     int32_t n_boundary;
@@ -181,6 +181,10 @@ typedef struct OalLevelInfo {
                                 the coarser level, not stepped; not part of n_blocks or n_cells) */
     int32_t n_covered_blocks;/* refined plans: blocks of n_blocks whose children exist (written by
                                 restriction, not stepped) */
+    int32_t n_partial;       /* AB pattern: regular blocks that overhang the extent, no solid near (the
+                                pipelined kernel with the face rules at the extent's last cells, on the
+                                second stream; in a domain whose block layers span 2 GiB or more they
+                                are boundary blocks) */
 } OalLevelInfo;
 int oal_level_info(const OalPlan* plan, int level, OalLevelInfo* info);
 

@@ -183,7 +183,7 @@ struct HostLevel {
 };
 
 // Host-side construction of one level's tables from the reference-convention descriptor.
-int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, bool pipe_ok, Level& L, HostLevel& H,
+int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, bool pipe_ok, size_t sbytes, Level& L, HostLevel& H,
                   const std::vector<uint8_t>* inactive = nullptr) {
     if (d.n_blocks <= 0) return fail(OAL_ERR_ARG, "level with no blocks");
     if (!d.map_x || !d.map_y || !d.map_z || !d.neighbor_table) return fail(OAL_ERR_ARG, "level block arrays missing");
@@ -316,6 +316,13 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, bool
         dm.nbx = (dm.ext[0] + BS - 1) / BS;
         dm.nby = (dm.ext[1] + BS - 1) / BS;
     }
+    // the in-kernel face rules address by signed 32-bit byte offsets from a block: only domains whose
+    // layer of blocks (plus a row and a block) spans less than 2 GiB may use them
+    std::vector<char> near(L.doms_h.size());
+    for (size_t i = 0; i < L.doms_h.size(); ++i) {
+        const DevDomain& dm = L.doms_h[i];
+        near[i] = ((uint64_t)dm.nbx * dm.nby + dm.nbx + 1) * 27u * BS3 * (uint64_t)sbytes < (1ull << 31);
+    }
     // (a boundary block qualifies when no neighbour inside the extent is absent and every present one is
     // at its lexicographic offset: its out-of-extent sources take the face rules and are never read)
     for (uint32_t b = 0; b < nb; ++b) {
@@ -350,9 +357,10 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, bool
                         for (int a = 0; a < 3; ++a) face = face || (lo[a] && l[a] == 0) || (hi[a] && l[a] == BS - 1);
                         if (face) H.face_cells.push_back(b * BS3 + (uint32_t)(x + BS * (y + BS * z)));
                     }
-        } else if (ab && pipe_ok && regular && flags[b] == (BF_EDGE | BF_PARTIAL)) {
+        } else if (ab && pipe_ok && regular && flags[b] == (BF_EDGE | BF_PARTIAL) && near[bdom[b]]) {
             // AB pattern: a block that overhangs the extent, no solid near: the pipelined kernel with the
-            // face rules at the extent's last cells
+            // face rules at the extent's last cells (in a domain whose block layers span 2 GiB or more the
+            // one-plane boundary kernel, whose addresses are 64-bit)
             H.blist_pr.push_back((int32_t)b);
         } else {
             (regular ? H.blist_br : H.blist_bd).push_back((int32_t)b);
@@ -1213,7 +1221,8 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
         }
         // the pipelined kernels cover FP32-arithmetic storage without WALE
         const bool pipe_ok = desc->storage != OAL_F64 && !p->wale;
-        const int rc = prepare_level(d, desc->collision, desc->stream_mode == OAL_STREAM_AB, pipe_ok, p->levels[l],
+        const int rc = prepare_level(d, desc->collision, desc->stream_mode == OAL_STREAM_AB, pipe_ok,
+                                     storage_bytes(desc->storage), p->levels[l],
                                      host[l], ml ? &inactive : nullptr);
         if (rc) return bail(rc);
         if (ml) {
@@ -1379,7 +1388,8 @@ int oal_level_info(const OalPlan* p, int level, OalLevelInfo* info) {
     info->n_blocks = (int32_t)L.n_caller_blocks;
     info->n_ghost_blocks = (int32_t)L.n_ghost_blocks;
     info->n_covered_blocks = (int32_t)L.n_covered_blocks;
-    info->n_regular = (int32_t)(L.n_regular + L.n_partial);
+    info->n_regular = (int32_t)L.n_regular;
+    info->n_partial = (int32_t)L.n_partial;
     info->n_interior = (int32_t)L.n_interior;
     info->n_boundary_regular = (int32_t)L.n_boundary_regular;
     info->n_boundary = (int32_t)L.n_boundary;

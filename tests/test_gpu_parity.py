@@ -342,14 +342,43 @@ def test_partial_pipeline_shapes(oracle, storage, planes, rules):
     f_or, _, _ = oracle_run(oracle, grid, storage, steps, okw, rho, u)
     plan = gpu_plan(grid, storage, okw)
     info = plan.level_info()
-    # 5 x 3 x 3 blocks; those clear of the sphere's neighbourhood and not on a full face are regular
-    assert info["n_regular"] >= 20, info
+    # 5 x 3 x 3 blocks; those clear of the sphere's neighbourhood are regular, the ones that overhang
+    # the extent in the partial list
+    assert info["n_regular"] + info["n_partial"] >= 20 and info["n_partial"] > 0, info
     plan.set_tuning(pipe_planes=planes, overlap=1, face_kernel=0, graph=0)
     plan.init_equilibrium(rho, u)
     for t in range(steps):
         plan.level_step(0, t)
     compare(oracle, plan, storage, f_or, grid)
     plan.close()
+
+
+@pytest.mark.timeout(300)
+def test_partial_blocks_of_a_level_with_layers_beyond_2gib():
+    """FP32 1320 x 1000 x 20: a layer of 83 x 63 blocks spans 2.3 GB, beyond the signed 32-bit byte
+    offsets of the in-kernel face rules, and the z extent leaves partial blocks in the second layer.
+    Such blocks take the one-plane boundary kernel (64-bit addresses): every launch shape of the
+    pipelined regular kernel gives the bits of the one-plane kernels."""
+    grid = uniform_grid((1320, 1000, 20))
+    n = grid.n_cells
+    rng = np.random.default_rng(12)
+    rho = 1.0 + 0.01 * rng.standard_normal(n)
+    u = np.stack([0.05 + 0.01 * rng.standard_normal(n), 0.01 * rng.standard_normal(n), 0.01 * rng.standard_normal(n)])
+    okw = dict(tau=0.55, u_inlet=0.05)
+    plan = gpu_plan(grid, "f32", okw)
+    info = plan.level_info()
+    assert info["n_partial"] == 0 and info["n_boundary_regular"] >= 83 * 63, info
+    print(f"1320x1000x20 FP32: {n} cells, {info}", flush=True)
+    results = []
+    for planes in (4, 1):
+        plan.set_tuning(pipe_planes=planes, overlap=1, face_kernel=0, graph=0)
+        plan.init_equilibrium(rho, u)
+        for t in range(3):
+            plan.level_step(0, t)
+        results.append(plan.download_f(dtype=np.float32))
+    plan.close()
+    assert np.array_equal(results[0], results[1])
+    assert np.isfinite(results[0]).all()
 
 
 def test_autotune_restores_state():
