@@ -267,6 +267,17 @@ int oal_read_raw_state(OalPlan* plan, int level, const char* path, int32_t* leve
  * (src/octree_q19/src/state.jl:333-355). */
 int oal_mass(OalPlan* plan, int level, double* out, void* stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The sums of oal_mass, written
+ * asynchronously on `stream` into the caller's device memory out_dev[4 * n_domains] (FP64, the same
+ * values as oal_mass, bit for bit); nothing is allocated and nothing waits for the device.  The
+ * per-block partials live in a fixed device array of the library (one per GPU, 2 MiB, kept until the
+ * process ends), so levels of more than OAL_NOALLOC_MAX_BLOCKS blocks are rejected
+ * (OAL_ERR_UNSUPPORTED).  Every call on one GPU shares that array: calls that overlap on one GPU (on
+ * different streams, or from different threads) race. */
+#define OAL_NOALLOC_MAX_BLOCKS 65536
+int oal_mass_noalloc(OalPlan* plan, int level, double* out_dev, void* stream);
+
 /* rho [n_cells] and u [3 * n_cells] of the current populations, FP64 host arrays in the reference
  * layout (materialize_current_state_for_io!, src/solver/solver_control.jl:830-851). */
 int oal_download_macro(OalPlan* plan, int level, double* rho, double* u);

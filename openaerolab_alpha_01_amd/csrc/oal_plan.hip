@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <array>
 #include <numeric>
+#include <mutex>
 #include <unordered_map>
 #include <vector>
 
@@ -18,6 +19,9 @@
 #include "oal_kernels.h"
 
 using namespace oal;
+
+// the scratch array of oal_mass_noalloc (see scratch_init): the per-block partials of one level
+static __device__ __align__(256) double scratch[(size_t)OAL_NOALLOC_MAX_BLOCKS * 4];
 
 namespace {
 
@@ -33,6 +37,32 @@ int fail(int code, const std::string& msg) {
         hipError_t e_ = (call);                                                                         \
         if (e_ != hipSuccess) return fail(OAL_ERR_HIP, std::string(#call " : ") + hipGetErrorString(e_)); \
     } while (0)
+
+// Scratch of oal_mass_noalloc: the per-block FP64 partials (mass, jx, jy, jz) of one level, in a
+// file-scope device array so that the entry point allocates nothing.  Every device holds its own copy
+// at its own address, which hipGetSymbolAddress gives once per device when a plan is created there
+// (scratch_addr); host code passes that address to kernels, never the array's name.
+constexpr uint32_t NOALLOC_MAX_BLOCKS = OAL_NOALLOC_MAX_BLOCKS;
+
+std::mutex g_scratch_mu;
+std::unordered_map<int, double*> g_scratch_addr;   // device ordinal -> address of `scratch` there
+
+// call with `device` current
+int scratch_init(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_addr.count(device)) return OAL_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return fail(OAL_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_addr[device] = (double*)ptr;
+    return OAL_OK;
+}
+
+double* scratch_addr(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    return it == g_scratch_addr.end() ? nullptr : it->second;
+}
 
 size_t storage_bytes(int storage) { return storage == OAL_F64 ? 8 : storage == OAL_F32 ? 4 : 2; }
 size_t work_bytes(int storage) { return storage == OAL_F64 ? 8 : 4; }
@@ -1316,6 +1346,7 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
     }
     hipError_t e = hipSetDevice(p->device);
     if (e != hipSuccess) return bail(fail(OAL_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)));
+    if (int rc = scratch_init(p->device)) return bail(rc);
     if ((e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&p->ev0)) != hipSuccess || (e = hipEventCreate(&p->ev1)) != hipSuccess ||
@@ -1532,6 +1563,27 @@ int oal_mass(OalPlan* p, int level, double* out, void* stream) {
     OAL_HIP(hipStreamSynchronize(s));
     OAL_HIP(hipMemcpy(out, L.red_out.p, (size_t)L.n_domains * 32, hipMemcpyDeviceToHost));
     L.force_valid = false;   // red_out reused
+    return OAL_OK;
+}
+
+int oal_mass_noalloc(OalPlan* p, int level, double* out_dev, void* stream) {
+    if (int rc = check_level(p, level)) return rc;
+    if (!out_dev) return fail(OAL_ERR_ARG, "NULL output");
+    Level& L = p->levels[level];
+    if (int rc = require_plain_state(p, L)) return rc;
+    if (L.nb > NOALLOC_MAX_BLOCKS)
+        return fail(OAL_ERR_UNSUPPORTED, "oal_mass_noalloc: the level has " + std::to_string(L.nb) +
+                                             " blocks, its scratch holds the partials of at most " +
+                                             std::to_string(NOALLOC_MAX_BLOCKS));
+    OAL_HIP(hipSetDevice(p->device));
+    double* part = scratch_addr(p->device);
+    if (!part) return fail(OAL_ERR_STATE, "oal_mass_noalloc: no scratch address for this device");
+    hipStream_t s = pick_stream(p, stream);
+    if (int rc = ml_sync_covered(p, s)) return rc;
+    // every block writes its own partials (zero for blocks not counted), so nothing is cleared
+    OAL_HIP(launch_mass(p->storage, L.fA.p, L.solid.p, L.borg.p, L.bdom.p, L.bflags.p, L.doms.p, L.nb, permute_of(p), part, s));
+    ReduceArgs r{part, L.dom_block0.p, out_dev, L.n_domains, 4};
+    OAL_HIP(launch_reduce(r, s));
     return OAL_OK;
 }
 
