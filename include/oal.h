@@ -319,7 +319,11 @@ typedef struct {
     int32_t overlap;
     int32_t face_kernel;
     int32_t graph;
-    int32_t reserved[4];
+    int32_t pair_kernel;   /* 16-bit storage (OAL_F16S, OAL_BF16S), pipe_planes > 1, face_kernel 0: the
+                              regular blocks step two cells per lane (32-bit words holding both): 1 in one
+                              launch, 2 (default) with the blocks on a domain face in a launch of their own;
+                              0 = one cell per lane; the same bits every way */
+    int32_t reserved[3];
 } OalTuning;
 
 /* number of launch shapes the autotuner times: pipe_planes {1, 2, 4, 8, 16} x overlap {1, 0} x

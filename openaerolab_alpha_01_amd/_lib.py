@@ -77,7 +77,7 @@ class OalLevelInfo(C.Structure):
 
 class OalTuning(C.Structure):
     _fields_ = [("pipe_planes", C.c_int32), ("overlap", C.c_int32), ("face_kernel", C.c_int32),
-                ("graph", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("graph", C.c_int32), ("pair_kernel", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 OAL_ALL_LEVELS = -1

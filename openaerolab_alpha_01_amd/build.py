@@ -6,6 +6,7 @@ file so that it travels with the source tree.  hipcc cross-compiles for gfx950 w
 from __future__ import annotations
 
 import os
+import re
 import subprocess
 import sys
 import tempfile
@@ -73,9 +74,37 @@ def build_oracle(verbose: bool = False) -> Path:
     return oracle / "liboal_oracle.so"
 
 
+def scratch_kernels(report: str) -> list[tuple[str, int]]:
+    """(kernel, bytes per lane) of every kernel that the compiler's resource-usage remarks show with
+    scratch (register spills)"""
+    out = []
+    for block in report.split("remark: Function Name: ")[1:]:
+        name = block.split()[0]
+        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", block)
+        if m and int(m.group(1)) > 0:
+            out.append((name, int(m.group(1))))
+    return out
+
+
 def main(argv: list[str]) -> int:
     verbose = "-v" in argv
-    build_library(verbose=verbose, extra=["-Rpass-analysis=kernel-resource-usage"] if "--resources" in argv else None)
+    if "--resources" in argv:
+        # every kernel's registers and scratch; fails when any kernel spills to scratch
+        import io
+        import contextlib
+        err = io.StringIO()
+        with contextlib.redirect_stderr(err):
+            build_library(verbose=verbose, extra=["-Rpass-analysis=kernel-resource-usage"])
+        report = err.getvalue()
+        sys.stderr.write(report)
+        spills = scratch_kernels(report)
+        for name, b in spills:
+            print(f"scratch: {b} B/lane in {name}")
+        print(f"{report.count('Function Name: ')} kernels, {len(spills)} with scratch")
+        if spills:
+            return 1
+    else:
+        build_library(verbose=verbose)
     if "--no-oracle" not in argv:
         build_oracle(verbose=verbose)
     return 0

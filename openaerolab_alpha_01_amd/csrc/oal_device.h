@@ -118,10 +118,15 @@ struct StF16S {
     static constexpr bool dithered = true;
     static constexpr float scale = 16384.0f, inv_scale = 1.0f / 16384.0f;
     __device__ static __forceinline__ raw ld_raw(const st* p) { return (uint32_t)*(const uint16_t*)p; }
+    // decode h * 2^-14 + w: the product is exact (a power-of-two scaling of an FP16 value stays in FP32's
+    // normal range), so the fused form rounds once at the same place as the multiply-then-add form
     __device__ static __forceinline__ T from_raw(raw r, T w) {
-        return (float)__builtin_bit_cast(_Float16, (uint16_t)r) * inv_scale + w;
+        return __builtin_fmaf((float)__builtin_bit_cast(_Float16, (uint16_t)r), inv_scale, w);
     }
-    __device__ static __forceinline__ T ld(const st* p, T w) { return (float)(*p) * inv_scale + w; }
+    __device__ static __forceinline__ T ld(const st* p, T w) { return __builtin_fmaf((float)(*p), inv_scale, w); }
+    // encode (f - w) * 2^14 as f 2^14 - w 2^14, one rounding of the exact difference: the same value (scaling
+    // by a power of two commutes with rounding away from overflow and underflow)
+    __device__ static __forceinline__ float shifted(T f, T w) { return __builtin_fmaf(f, scale, -(w * scale)); }
     // FP32 -> FP16: to nearest, or stochastic with the hardware conversion: the random bits r >> 19
     // added below the result's last place, then truncated (normal and subnormal results alike)
     __device__ static __forceinline__ uint16_t enc(float x, uint32_t r, bool sr) {
@@ -135,9 +140,23 @@ struct StF16S {
     __device__ static __forceinline__ float dec(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
     __device__ static __forceinline__ void sto(st* p, T f, T w) { *p = (_Float16)((f - w) * scale); }
     __device__ static __forceinline__ void sto_r(st* p, T f, T w, uint32_t r, bool sr) {
-        __builtin_nontemporal_store(enc((f - w) * scale, r, sr), (uint16_t*)p);
+        __builtin_nontemporal_store(enc(shifted(f, w), r, sr), (uint16_t*)p);
     }
-    __device__ static __forceinline__ T rt_r(T f, T w, uint32_t r, bool sr) { return dec(enc((f - w) * scale, r, sr)) * inv_scale + w; }
+    __device__ static __forceinline__ T rt_r(T f, T w, uint32_t r, bool sr) {
+        return __builtin_fmaf(dec(enc(shifted(f, w), r, sr)), inv_scale, w);
+    }
+    // packed pairs (two cells along x in one 32-bit word, the lower cell in the low half): decode one
+    // half, or replace one half by a cell's rounded value (the same rounding as sto_r)
+    __device__ static __forceinline__ T dec_lo(uint32_t p, T w) { return __builtin_fmaf(dec((uint16_t)p), inv_scale, w); }
+    __device__ static __forceinline__ T dec_hi(uint32_t p, T w) { return __builtin_fmaf(dec((uint16_t)(p >> 16)), inv_scale, w); }
+    template <bool HI>
+    __device__ static __forceinline__ uint32_t put(uint32_t old, T f, T w, uint32_t r, bool sr) {
+        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+        const float x = shifted(f, w);
+        if (sr) return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(h2, old), x, r, HI));
+        const uint32_t h = __builtin_bit_cast(uint16_t, (_Float16)x);
+        return HI ? (old & 0xffffu) | (h << 16) : (old & 0xffff0000u) | h;
+    }
 };
 // bfloat16 round to nearest even, done on integers so that host and device agree bit for bit
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
@@ -171,6 +190,15 @@ struct StBF16S {
     }
     __device__ static __forceinline__ T rt_r(T f, T w, uint32_t r, bool sr) {
         return __uint_as_float((uint32_t)enc(f - w, r, sr) << 16) + w;
+    }
+    __device__ static __forceinline__ T dec_lo(uint32_t p, T w) { return __uint_as_float(p << 16) + w; }
+    __device__ static __forceinline__ T dec_hi(uint32_t p, T w) { return __uint_as_float(p & 0xffff0000u) + w; }
+    template <bool HI>
+    __device__ static __forceinline__ uint32_t put(uint32_t old, T f, T w, uint32_t r, bool sr) {
+        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+        if (sr) return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(b2, old), f - w, r, HI));
+        const uint32_t h = f32_to_bf16_rne(f - w);
+        return HI ? (old & 0xffffu) | (h << 16) : (old & 0xffff0000u) | h;
     }
 };
 
@@ -207,6 +235,7 @@ struct CollParams {
     int32_t adaptive_omega4, limiter, compressible, bb_mode;
     float c_wale, nu_sgs_bg, inlet_turb;
     int32_t dither;   // 16-bit storage forms: stochastic rounding of the step's stores
+    int32_t unit;     // omega_bulk = omega_3 = omega_4 = 1: the pipelined kernels use the UNIT collision
 };
 
 // WALE eddy viscosity of the gradient g_ij = d u_i / d x_j (kernel_cumulant.jl:500-535)
@@ -303,7 +332,13 @@ struct CellRates {
 // (m0, d = f+ - f-, s = f- + f+); rho and j are sums of those over rows; the x, y and z passes then
 // shift to central moments about u (k1 = d - u m0, k2 = s - 2 u d + u^2 m0).  The total of the last
 // pass equals rho bit for bit, so 1/max(k000, 0.01) is the 1/max(rho, 0.01) already formed.
-template <class T>
+//
+// UNIT: the reference's default rates omega_bulk = omega_3 = omega_4 = 1 (config_loader.jl:154-159), for
+// which the relaxed trace is 1 and the 3rd-order, mixed 4th-order and 5th-order moments are zero: the
+// terms they would contribute are left out, and the first backward pass uses the zeros it knows.  Each
+// term left out adds an exact zero in the general form, so both give the same values (only the sign of
+// a result that is exactly zero could differ).
+template <class T, bool UNIT = false>
 __device__ __forceinline__ void collide_cell(T (&f)[27], const CellRates<T>& q, const CollParams& p,
                                              T& rho_o, T& ux_o, T& uy_o, T& uz_o) {
     T A[27];
@@ -367,25 +402,18 @@ __device__ __forceinline__ void collide_cell(T (&f)[27], const CellRates<T>& q, 
     // normalised central moments (index a + 3b + 9c for order (a,b,c)) and 4th-order cumulants
     const T c200 = A[2] * inv, c020 = A[6] * inv, c002 = A[18] * inv;
     const T c110 = A[4] * inv, c101 = A[10] * inv, c011 = A[12] * inv;
-    const T c210 = A[5] * inv, c120 = A[7] * inv, c201 = A[11] * inv;
-    const T c021 = A[15] * inv, c102 = A[19] * inv, c012 = A[21] * inv;
-    const T c111 = A[13] * inv;
     const T c220n = A[8] * inv, c202n = A[20] * inv, c022n = A[24] * inv;
-    const T c211n = A[14] * inv, c121n = A[16] * inv, c112n = A[22] * inv;
     const T x110 = T(2) * c110, x101 = T(2) * c101, x011 = T(2) * c011;
     const T C220 = fmx(-x110, c110, fmx(-c200, c020, c220n));
     const T C202 = fmx(-x101, c101, fmx(-c200, c002, c202n));
     const T C022 = fmx(-x011, c011, fmx(-c020, c002, c022n));
-    const T C211 = fmx(-x110, c101, fmx(-c200, c011, c211n));
-    const T C121 = fmx(-x110, c011, fmx(-c020, c101, c121n));
-    const T C112 = fmx(-x101, c011, fmx(-c002, c110, c112n));
 
     // relaxation (kernel_cumulant.jl:705-754): trace toward 1 at omega_bulk, deviators at omega1,
     // realizability clamp, 3rd order at omega3, 4th-order cumulants at omega4 / omega4_diag
     const T third = T(1) / T(3);
     const T ob = T(p.omega_bulk);
     const T tr = (c200 + c020) + c002;
-    const T tr_s = fmx(T(1) - ob, tr, ob);
+    const T tr_s = UNIT ? T(1) : fmx(T(1) - ob, tr, ob);
     const T g1 = T(1) - q.omega1;
     const T d1s = g1 * (c200 - c020), d2s = g1 * (c200 - c002);
     const T t3 = tr_s * third;
@@ -399,51 +427,73 @@ __device__ __forceinline__ void collide_cell(T (&f)[27], const CellRates<T>& q, 
         if (s101 * s101 > q101) s101 = copysign(sqrt(q101), s101);
         if (s011 * s011 > q011) s011 = copysign(sqrt(q011), s011);
     }
-    const T g3 = T(1) - T(p.omega_3), g4 = T(1) - T(p.omega_4), g4d = T(1) - q.omega4_diag;
-    const T s210 = g3 * c210, s120 = g3 * c120, s201 = g3 * c201, s021 = g3 * c021;
-    const T s102 = g3 * c102, s012 = g3 * c012, s111 = g3 * c111;
+    const T g4d = T(1) - q.omega4_diag;
     const T S220 = g4d * C220, S202 = g4d * C202, S022 = g4d * C022;
-    const T S211 = g4 * C211, S121 = g4 * C121, S112 = g4 * C112;
-
-    // central moments from the relaxed cumulants; 5th and 6th order cumulants vanish
     const T y110 = T(2) * s110, y101 = T(2) * s101, y011 = T(2) * s011;
     const T s220 = fmx(y110, s110, fmx(s200, s020, S220));
     const T s202 = fmx(y101, s101, fmx(s200, s002, S202));
     const T s022 = fmx(y011, s011, fmx(s020, s002, S022));
-    const T s211 = fmx(y110, s101, fmx(s200, s011, S211));
-    const T s121 = fmx(y110, s011, fmx(s020, s101, S121));
-    const T s112 = fmx(y101, s011, fmx(s002, s110, S112));
-    const T s221 = fmx(y011, s210, fmx(y101, s120, fmx(T(2) * y110, s111, fmx(s020, s201, s200 * s021))));
-    const T s212 = fmx(y011, s201, fmx(T(2) * y101, s111, fmx(y110, s102, fmx(s002, s210, s200 * s012))));
-    const T s122 = fmx(y101, s021, fmx(T(2) * y011, s111, fmx(y110, s012, fmx(s002, s120, s020 * s102))));
-    T e = s200 * S022;
-    e = fmx(s020, S202, e);
-    e = fmx(s002, S220, e);
-    e = fmx(T(2) * y110, S112, e);
-    e = fmx(T(2) * y101, S121, e);
-    e = fmx(T(2) * y011, S211, e);
-    e = fmx(T(2) * s210, s012, e);
-    e = fmx(T(2) * s201, s021, e);
-    e = fmx(T(2) * s120, s102, e);
-    e = fmx(T(4) * s111, s111, e);
-    e = fmx(s200 * s020, s002, e);
-    e = fmx(s200 * y011, s011, e);
-    e = fmx(s020 * y101, s101, e);
-    e = fmx(s002 * y110, s110, e);
-    e = fmx((T(4) * y110) * s101, s011, e);
-
     const T r = rho;
     A[0] = r;
     A[1] = T(0); A[3] = T(0); A[9] = T(0);
     A[2] = r * s200; A[6] = r * s020; A[18] = r * s002;
     A[4] = r * s110; A[10] = r * s101; A[12] = r * s011;
-    A[5] = r * s210; A[7] = r * s120; A[11] = r * s201;
-    A[15] = r * s021; A[19] = r * s102; A[21] = r * s012;
-    A[13] = r * s111;
     A[8] = r * s220; A[20] = r * s202; A[24] = r * s022;
-    A[14] = r * s211; A[16] = r * s121; A[22] = r * s112;
-    A[17] = r * s221; A[23] = r * s212; A[25] = r * s122;
-    A[26] = r * e;
+    // central moments from the relaxed cumulants; 5th and 6th order cumulants vanish
+    if constexpr (UNIT) {
+        const T s211 = fmx(y110, s101, s200 * s011);
+        const T s121 = fmx(y110, s011, s020 * s101);
+        const T s112 = fmx(y101, s011, s002 * s110);
+        T e = s200 * S022;
+        e = fmx(s020, S202, e);
+        e = fmx(s002, S220, e);
+        e = fmx(s200 * s020, s002, e);
+        e = fmx(s200 * y011, s011, e);
+        e = fmx(s020 * y101, s101, e);
+        e = fmx(s002 * y110, s110, e);
+        e = fmx((T(4) * y110) * s101, s011, e);
+        A[14] = r * s211; A[16] = r * s121; A[22] = r * s112;
+        A[26] = r * e;
+    } else {
+        const T c210 = A[5] * inv, c120 = A[7] * inv, c201 = A[11] * inv;
+        const T c021 = A[15] * inv, c102 = A[19] * inv, c012 = A[21] * inv;
+        const T c111 = A[13] * inv;
+        const T c211n = A[14] * inv, c121n = A[16] * inv, c112n = A[22] * inv;
+        const T C211 = fmx(-x110, c101, fmx(-c200, c011, c211n));
+        const T C121 = fmx(-x110, c011, fmx(-c020, c101, c121n));
+        const T C112 = fmx(-x101, c011, fmx(-c002, c110, c112n));
+        const T g3 = T(1) - T(p.omega_3), g4 = T(1) - T(p.omega_4);
+        const T s210 = g3 * c210, s120 = g3 * c120, s201 = g3 * c201, s021 = g3 * c021;
+        const T s102 = g3 * c102, s012 = g3 * c012, s111 = g3 * c111;
+        const T S211 = g4 * C211, S121 = g4 * C121, S112 = g4 * C112;
+        const T s211 = fmx(y110, s101, fmx(s200, s011, S211));
+        const T s121 = fmx(y110, s011, fmx(s020, s101, S121));
+        const T s112 = fmx(y101, s011, fmx(s002, s110, S112));
+        const T s221 = fmx(y011, s210, fmx(y101, s120, fmx(T(2) * y110, s111, fmx(s020, s201, s200 * s021))));
+        const T s212 = fmx(y011, s201, fmx(T(2) * y101, s111, fmx(y110, s102, fmx(s002, s210, s200 * s012))));
+        const T s122 = fmx(y101, s021, fmx(T(2) * y011, s111, fmx(y110, s012, fmx(s002, s120, s020 * s102))));
+        T e = s200 * S022;
+        e = fmx(s020, S202, e);
+        e = fmx(s002, S220, e);
+        e = fmx(T(2) * y110, S112, e);
+        e = fmx(T(2) * y101, S121, e);
+        e = fmx(T(2) * y011, S211, e);
+        e = fmx(T(2) * s210, s012, e);
+        e = fmx(T(2) * s201, s021, e);
+        e = fmx(T(2) * s120, s102, e);
+        e = fmx(T(4) * s111, s111, e);
+        e = fmx(s200 * s020, s002, e);
+        e = fmx(s200 * y011, s011, e);
+        e = fmx(s020 * y101, s101, e);
+        e = fmx(s002 * y110, s110, e);
+        e = fmx((T(4) * y110) * s101, s011, e);
+        A[5] = r * s210; A[7] = r * s120; A[11] = r * s201;
+        A[15] = r * s021; A[19] = r * s102; A[21] = r * s012;
+        A[13] = r * s111;
+        A[14] = r * s211; A[16] = r * s121; A[22] = r * s112;
+        A[17] = r * s221; A[23] = r * s212; A[25] = r * s122;
+        A[26] = r * e;
+    }
 
     // backward passes z, y, x: (k0, k1, k2) -> (f-, f0, f+) about u
     auto back = [&](int i0, int i1, int i2, T u) {
@@ -456,8 +506,33 @@ __device__ __forceinline__ void collide_cell(T (&f)[27], const CellRates<T>& q, 
         A[i1] = a - m2;
         A[i2] = fmx(T(0.5), m2, h);
     };
+    if constexpr (UNIT) {
+        // the z pass with its known zeros: rows (r, r + 9, r + 18) with r even have k1 = 0 (1st, 3rd and
+        // 5th-order moments), rows with r odd have k0 = k2 = 0
+        const T u = uze, uu = u * u, t = T(2) * u;
 #pragma unroll
-    for (int r2 = 0; r2 < 9; ++r2) back(r2, r2 + 9, r2 + 18, uze);
+        for (int r2 = 0; r2 < 9; r2 += 2) {
+            const T a = A[r2], c = A[r2 + 18];
+            const T m1 = u * a;
+            const T m2 = fmx(uu, a, c);
+            const T h = T(0.5) * m1;
+            A[r2] = fmx(T(0.5), m2, -h);
+            A[r2 + 9] = a - m2;
+            A[r2 + 18] = fmx(T(0.5), m2, h);
+        }
+#pragma unroll
+        for (int r2 = 1; r2 < 9; r2 += 2) {
+            const T b = A[r2 + 9];
+            const T m2 = t * b;
+            const T h = T(0.5) * b;
+            A[r2] = fmx(T(0.5), m2, -h);
+            A[r2 + 9] = -m2;
+            A[r2 + 18] = fmx(T(0.5), m2, h);
+        }
+    } else {
+#pragma unroll
+        for (int r2 = 0; r2 < 9; ++r2) back(r2, r2 + 9, r2 + 18, uze);
+    }
 #pragma unroll
     for (int iz = 0; iz < 3; ++iz)
 #pragma unroll

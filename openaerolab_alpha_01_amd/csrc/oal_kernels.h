@@ -61,11 +61,16 @@ struct StepArgs {
     uint32_t n_regular, n_interior, n_boundary_regular, n_boundary;
     const int32_t* blist_partial;    // [n_partial] AB pattern: regular blocks that overhang the extent, no solid near
     uint32_t n_partial;
+    // the regular list split (AB pattern): blocks on no domain face, and whole blocks on a domain face
+    const int32_t* blist_regular_plain;
+    uint32_t n_regular_plain;
+    const int32_t* blist_regular_face;
+    uint32_t n_regular_face;
     const int32_t* lslot;        // [nb] link-mask slot of a block with solid cells near, -1 = none
     const uint32_t* lmask;       // [n_lslots * BS3] per cell: bit k source of direction k solid, bit 27 cell solid
     const uint32_t* face_cells;  // [n_face] AB pattern: cells of regular-list blocks with a source outside the extent
     uint32_t n_face;
-    int32_t regular_face;        // AB pattern: the regular list holds blocks on a domain face
+    int32_t regular_face;        // AB pattern: the blocks of this launch may lie on a domain face (set per launch)
     int32_t face_kernel;         // 1: the pipelined kernel skips face cells and the face-cell kernel steps
                                  // them; 0: it applies the face rules itself
     const int32_t* oslot;        // [nb] AA pattern: outlet side-buffer slot of the block, -1 = none
@@ -80,6 +85,7 @@ struct StepArgs {
     int32_t ts_mul, ts_add;
     int32_t parity;              // AA pattern: 1 odd step (neighbour slots), 0 even step (own slots)
     int32_t pipe_planes;         // planes per workgroup of the pipelined interior kernel (1 = off; divides 16)
+    int32_t pair_kernel;         // 16-bit storage: interior regular blocks two cells per lane (pipelined shapes)
     // interpolated bounce-back: per link-mask slot, [27][BS3] Float16 bits of the wall distance q of the
     // link from a cell along c_k (units of |c_k|); null = halfway bounce-back on every solid link
     const uint16_t* qmap;
@@ -88,6 +94,14 @@ struct StepArgs {
     CollParams cp;
     __device__ __forceinline__ int32_t step() const { return ts_dev ? ts_dev[0] * ts_mul + ts_add : time_step; }
 };
+
+// Whether a domain's block grid fits the 32-bit offsets of the two-cells-per-lane kernel of the 16-bit
+// forms (16-bit populations, one buffer resource per block): two layers of blocks, two rows and three
+// blocks below 4 GiB
+inline bool pair_fits(int nbx, int nby) {
+    const uint64_t blk = 27ull * BS3 * 2;
+    return 2ull * (uint64_t)nbx * (uint64_t)nby * blk + (2ull * (uint64_t)nbx + 3ull) * blk < (1ull << 32);
+}
 
 struct ReduceArgs {
     const double* part;          // [n_items * 4]

@@ -99,6 +99,9 @@ struct Level {
     uint32_t n_regular = 0, n_interior = 0, n_boundary_regular = 0, n_boundary = 0;
     DevBuf<int32_t> blist_pr;          // regular blocks that overhang the extent (pipelined, AB)
     uint32_t n_partial = 0;
+    // the regular blocks split in two (AB), for the kernels that step the ones on a domain face apart
+    DevBuf<int32_t> blist_rp, blist_rf;   // plain (no face) and on a domain face, each in lexicographic order
+    uint32_t n_regular_face = 0;          // (n_regular counts them too)
     DevBuf<uint32_t> face_cells;       // AB: face cells of the regular-list blocks
     uint32_t n_face = 0;
     bool regular_face = false;         // AB: the regular list holds blocks on a domain face
@@ -146,7 +149,7 @@ struct Level {
     void release() {
         nbr.release(); borg.release(); bdom.release(); dom_item0.release(); dom_block0.release();
         bflags.release(); solid.release(); doms.release(); fA.release(); fB.release(); velA.release(); velB.release();
-        fpart.release(); mpart.release(); red_out.release(); guard.release(); oslot.release(); obuf.release(); blist_rg.release(); blist_in.release(); blist_br.release(); blist_bd.release(); blist_pr.release(); face_cells.release();
+        fpart.release(); mpart.release(); red_out.release(); guard.release(); oslot.release(); obuf.release(); blist_rg.release(); blist_in.release(); blist_br.release(); blist_bd.release(); blist_pr.release(); blist_rp.release(); blist_rf.release(); face_cells.release();
         lslot.release(); lmask.release(); qmap.release();
         g_src.release(); g_dst.release(); g_slot.release(); g_w.release(); g_m.release();
         r_src.release(); r_dst.release(); r_slot.release(); r_w.release(); r_m.release();
@@ -161,6 +164,7 @@ struct OalPlan {
     int pipe_planes = 4;   // planes per workgroup of the pipelined interior kernel
     bool overlap = true;   // boundary and face launches on the side stream
     bool face_kernel = false;   // face cells by the face-cell kernel instead of rules in the pipelined one
+    int pair_kernel = 2;        // 16-bit storage: regular blocks two cells per lane (1 one launch, 2 face blocks apart)
     int storage = OAL_F32;
     int stream_mode = OAL_STREAM_AB;
     int bb_mode = OAL_BB_HALFWAY;
@@ -204,7 +208,7 @@ int upload(DevBuf<T>& d, const std::vector<T>& h) {
 
 // Host tables of one level, built and validated before any device call
 struct HostLevel {
-    std::vector<int32_t> borg, bdom, nbr, nbr_bm, oslot, blist_rg, blist_in, blist_br, blist_bd, blist_pr, lslot;
+    std::vector<int32_t> borg, bdom, nbr, nbr_bm, oslot, blist_rg, blist_in, blist_br, blist_bd, blist_pr, blist_rp, blist_rf, lslot;
     std::vector<uint32_t> lmask;
     std::vector<uint32_t> face_cells;
     int32_t n_oslots = 0;
@@ -368,10 +372,12 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, bool
         }
         if (flags[b] == 0) {
             (regular ? H.blist_rg : H.blist_in).push_back((int32_t)b);
+            if (regular) H.blist_rp.push_back((int32_t)b);
         } else if (ab && regular && flags[b] == BF_EDGE) {
             // AB pattern: a whole block on a domain face, no solid near, joins the regular list; the
             // cells on its faces (sources outside the extent) go to the face-cell list
             H.blist_rg.push_back((int32_t)b);
+            H.blist_rf.push_back((int32_t)b);
             L.regular_face = true;
             const int o[3] = {borg[3 * b], borg[3 * b + 1], borg[3 * b + 2]};
             bool lo[3], hi[3];
@@ -397,6 +403,7 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, bool
         }
     }
     L.n_face = (uint32_t)H.face_cells.size();
+    L.n_regular_face = (uint32_t)H.blist_rf.size();
     L.n_regular = (uint32_t)H.blist_rg.size();
     L.n_interior = (uint32_t)H.blist_in.size();
     L.n_boundary_regular = (uint32_t)H.blist_br.size();
@@ -870,7 +877,7 @@ int upload_level(OalPlan* p, Level& L, const HostLevel& H) {
         return rc;
     if (!H.solid.empty() && (rc = upload(L.solid, H.solid))) return rc;
     if ((rc = upload(L.blist_rg, H.blist_rg)) || (rc = upload(L.blist_in, H.blist_in)) ||
-        (rc = upload(L.blist_br, H.blist_br)) || (rc = upload(L.blist_bd, H.blist_bd)) || (rc = upload(L.blist_pr, H.blist_pr)) ||
+        (rc = upload(L.blist_br, H.blist_br)) || (rc = upload(L.blist_bd, H.blist_bd)) || (rc = upload(L.blist_pr, H.blist_pr)) || (rc = upload(L.blist_rp, H.blist_rp)) || (rc = upload(L.blist_rf, H.blist_rf)) ||
         (rc = upload(L.lslot, H.lslot)) || (rc = upload(L.lmask, H.lmask)) || (rc = upload(L.face_cells, H.face_cells)))
         return rc;
     if (p->stream_mode == OAL_STREAM_AA) {
@@ -918,6 +925,10 @@ StepArgs make_args(OalPlan* p, Level& L, int32_t time_step, bool want_force) {
     a.blist_boundary_regular = L.blist_br.p;
     a.blist_boundary = L.blist_bd.p;
     a.n_regular = L.n_regular;
+    a.blist_regular_plain = L.blist_rp.p;
+    a.n_regular_plain = L.n_regular - L.n_regular_face;
+    a.blist_regular_face = L.blist_rf.p;
+    a.n_regular_face = L.n_regular_face;
     a.blist_partial = L.blist_pr.p;
     a.n_partial = L.n_partial;
     a.n_interior = L.n_interior;
@@ -951,6 +962,10 @@ StepArgs make_args(OalPlan* p, Level& L, int32_t time_step, bool want_force) {
     }
     a.parity = p->stream_mode == OAL_STREAM_AA ? (L.aa_phase == 0 ? 1 : 0) : 0;
     a.pipe_planes = p->pipe_planes;
+    // (two cells per lane where the level's block grid fits its 32-bit offsets)
+    bool pair_ok = p->pair_kernel != 0;
+    for (const DevDomain& dm : L.doms_h) pair_ok = pair_ok && pair_fits(dm.nbx, dm.nby);
+    a.pair_kernel = pair_ok ? p->pair_kernel : 0;
     a.qmap = L.qmap.p;
     a.q_min = L.q_min;
     a.bfl_fallback = L.bfl_fallback;
@@ -1224,6 +1239,7 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
     p->cp.nu_sgs_bg = c.nu_sgs_background;
     p->cp.inlet_turb = c.inlet_turbulence;
     p->cp.dither = desc->rounding == OAL_ROUND_STOCHASTIC ? 1 : 0;
+    p->cp.unit = (c.omega_bulk == 1.0f && c.omega_3 == 1.0f && c.omega_4 == 1.0f) ? 1 : 0;
 
     auto bail = [&](int rc) {
         oal_plan_destroy(p);
@@ -1828,6 +1844,7 @@ int oal_get_tuning(const OalPlan* p, OalTuning* t) {
     t->overlap = p->overlap ? 1 : 0;
     t->face_kernel = p->face_kernel ? 1 : 0;
     t->graph = p->graph ? 1 : 0;
+    t->pair_kernel = p->pair_kernel;
     return OAL_OK;
 }
 
@@ -1838,6 +1855,7 @@ int oal_set_tuning(OalPlan* p, const OalTuning* t) {
     if (t->overlap != 0 && t->overlap != 1) return fail(OAL_ERR_ARG, "overlap must be 0 or 1");
     if (t->face_kernel != 0 && t->face_kernel != 1) return fail(OAL_ERR_ARG, "face_kernel must be 0 or 1");
     if (t->graph != 0 && t->graph != 1) return fail(OAL_ERR_ARG, "graph must be 0 or 1");
+    if (t->pair_kernel < 0 || t->pair_kernel > 2) return fail(OAL_ERR_ARG, "pair_kernel must be 0, 1 or 2");
     OAL_HIP(hipSetDevice(p->device));
     OAL_HIP(hipStreamSynchronize(p->stream));
     drop_graphs(p);
@@ -1845,6 +1863,7 @@ int oal_set_tuning(OalPlan* p, const OalTuning* t) {
     p->overlap = t->overlap != 0;
     p->face_kernel = t->face_kernel != 0;
     p->graph = t->graph != 0;
+    p->pair_kernel = t->pair_kernel;
     return OAL_OK;
 }
 
@@ -1903,6 +1922,7 @@ int oal_autotune(OalPlan* p, int level, int n_steps, OalTuning* best, double* ms
         best->overlap = besto ? 1 : 0;
         best->face_kernel = bestf ? 1 : 0;
         best->graph = p->graph ? 1 : 0;
+        best->pair_kernel = p->pair_kernel;
     }
     return rc;
 }

@@ -559,8 +559,16 @@ enum : int { SEL_ALL = 0, SEL_INTERIOR = 1, SEL_BOUNDARY = 2, SEL_REGULAR = 3, S
 #ifndef OAL_STEP_WAVES
 #define OAL_STEP_WAVES 3
 #endif
+// register budget of the one-plane kernels: 3 waves per SIMD, 2 for the forms that do not fit 168 VGPRs
+// without spilling (FP64 arithmetic, the AA pattern's two-sided slots, WALE's velocity gather), 1 for the
+// FP64 AA forms
+template <class SP, int PAT, bool WALE>
+constexpr int step_waves() {
+    return sizeof(typename SP::T) == 8 && PAT != PAT_AB ? 1 : sizeof(typename SP::T) == 8 || PAT != PAT_AB || WALE ? 2 : OAL_STEP_WAVES;
+}
+
 template <class SP, int PAT, bool WALE, int SEL = SEL_ALL, bool BFL = false>
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(OAL_STEP_WAVES))) k_step(const StepArgs a) {
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(step_waves<SP, PAT, WALE>()))) k_step(const StepArgs a) {
     const uint32_t litem = xcd_item(blockIdx.x, gridDim.x);
     const uint32_t lb = litem / PLANES;
     const uint32_t b = a.blist ? (uint32_t)a.blist[lb] : lb;
@@ -635,7 +643,7 @@ struct InletRule {
     int gy, gz;       // the cell's global y and z (the noise position)
 };
 
-template <class SP>
+template <class SP, bool UNIT>
 __device__ __forceinline__ void reg_collide_store(const StepArgs& a, const RegAddr<SP>& ra, const CellRates<typename SP::T>& q,
                                                   int z, uint32_t cell, const typename SP::raw (&r)[27],
                                                   const InletRule* inlet = nullptr, bool store = true) {
@@ -657,7 +665,7 @@ __device__ __forceinline__ void reg_collide_store(const StepArgs& a, const RegAd
         });
     }
     T rho_o, ux_o, uy_o, uz_o;
-    collide_cell<T>(f, q, a.cp, rho_o, ux_o, uy_o, uz_o);
+    collide_cell<T, UNIT>(f, q, a.cp, rho_o, ux_o, uy_o, uz_o);
     const bool dith = SP::dithered && a.cp.dither;
     const uint32_t hc = SP::dithered ? round_cell(round_key(a.step()), cell) : 0u;
     if (store) {
@@ -745,11 +753,14 @@ enum : int { FACE_NONE = 0, FACE_RULES = 1, FACE_SKIP = 2, FACE_PARTIAL = 3 };
 #ifndef OAL_FACE_RULES_WAVES
 #define OAL_FACE_RULES_WAVES 2
 #endif
-template <int FACE>
-constexpr int pipe_waves() { return FACE == FACE_RULES || FACE == FACE_PARTIAL ? OAL_FACE_RULES_WAVES : 3; }
+// (the 16-bit forms' conversions and rounding need more registers: 2 waves for every form of theirs)
+template <class SP, int FACE>
+constexpr int pipe_waves() { return FACE == FACE_RULES || FACE == FACE_PARTIAL || SP::dithered ? OAL_FACE_RULES_WAVES : 3; }
 
-template <class SP, int NP, int FACE>
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(pipe_waves<FACE>(), pipe_waves<FACE>())))
+// UNIT: the collision form for the unit rates omega_bulk = omega_3 = omega_4 = 1 (a separate kernel, so
+// that each form keeps its own register allocation)
+template <class SP, int NP, int FACE, bool UNIT>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(pipe_waves<SP, FACE>(), pipe_waves<SP, FACE>())))
 k_step_pipe(const StepArgs a) {
     using T = typename SP::T;
     using st = typename SP::st;
@@ -850,14 +861,14 @@ k_step_pipe(const StepArgs a) {
             if (z >= zmax) return;
             InletRule iz = in;
             iz.gz += z;
-            reg_collide_store<SP>(a, r, q, z, cell, src, &iz);
+            reg_collide_store<SP, UNIT>(a, r, q, z, cell, src, &iz);
         } else if constexpr (FACE == FACE_PARTIAL) {
             if (z >= zmax) return;
             InletRule iz = in;
             iz.gz += z;
-            reg_collide_store<SP>(a, r, q, z, cell, src, &iz, !fr.dead);
+            reg_collide_store<SP, UNIT>(a, r, q, z, cell, src, &iz, !fr.dead);
         } else if (live(z)) {
-            reg_collide_store<SP>(a, r, q, z, cell, src);
+            reg_collide_store<SP, UNIT>(a, r, q, z, cell, src);
         }
     };
     // the per-lane offsets stay those of plane z0; later planes shift the uniform base
@@ -877,7 +888,7 @@ k_step_pipe(const StepArgs a) {
 // Face cells of the regular-list blocks (AB pattern): one lane per listed cell, the general path with
 // the domain-face rules.  These blocks hold no solid cell and no solid link, so no force is summed.
 template <class SP, bool WALE>
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(OAL_STEP_WAVES))) k_face(const StepArgs a) {
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(step_waves<SP, PAT_AB, WALE>()))) k_face(const StepArgs a) {
     const uint32_t i = blockIdx.x * WG + threadIdx.x;
     if (i >= a.n_face) return;
     const uint32_t own = a.face_cells[i];
@@ -901,11 +912,25 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(OAL_STE
     step_cell<SP, PAT_AB, WALE, true>(a, c, dm, true, false, F0, F1, F2);
 }
 
+}  // namespace oal
+
+#include "oal_step_pair.h"
+
+namespace oal {
+
+template <class SP, int NP, bool UNIT>
+static void launch_pipe_u(int face, dim3 grid, hipStream_t s, const StepArgs& r) {
+    if (face == FACE_RULES) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_RULES, UNIT>), grid, dim3(WG), 0, s, r);
+    else if (face == FACE_SKIP) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_SKIP, UNIT>), grid, dim3(WG), 0, s, r);
+    else if (face == FACE_PARTIAL) {
+        if constexpr (NP <= 8) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_PARTIAL, UNIT>), grid, dim3(WG), 0, s, r);
+    }
+    else hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_NONE, UNIT>), grid, dim3(WG), 0, s, r);
+}
 template <class SP, int NP>
 static void launch_pipe(int face, dim3 grid, hipStream_t s, const StepArgs& r) {
-    if (face == FACE_RULES) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_RULES>), grid, dim3(WG), 0, s, r);
-    else if (face == FACE_SKIP) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_SKIP>), grid, dim3(WG), 0, s, r);
-    else hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_NONE>), grid, dim3(WG), 0, s, r);
+    if (r.cp.unit) launch_pipe_u<SP, NP, true>(face, grid, s, r);
+    else launch_pipe_u<SP, NP, false>(face, grid, s, r);
 }
 
 // One step of a level: launches over disjoint block lists (regular, other interior, regular boundary,
@@ -942,7 +967,7 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
     // leaves them to the face-cell kernel
     constexpr bool pipe_capable = PAT == PAT_AB && !WALE && sizeof(typename SP::T) == 4;
     if constexpr (PAT == PAT_AB) {
-        const bool piped = pipe_capable && a.pipe_planes > 1;
+        const bool piped = pipe_capable && a.pipe_planes > 1;   // (the pair kernel applies the rules too)
         if (a.n_face && (!piped || a.face_kernel))
             hipLaunchKernelGGL((k_face<SP, WALE>), dim3((a.n_face + WG - 1) / WG), dim3(WG), 0, bs, a);
     }
@@ -950,11 +975,21 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
         const hipError_t e = hipEventRecord(bulk0, stream);
         if (e != hipSuccess) return e;
     }
-    if (a.n_regular) {
-        r.blist = a.blist_regular;
-        const dim3 g(a.n_regular * (PLANES / (a.pipe_planes > 1 ? a.pipe_planes : 1)));
-        if constexpr (pipe_capable) {   // (no force on regular blocks)
-            const int fc = !a.regular_face ? FACE_NONE : (a.face_kernel ? FACE_SKIP : FACE_RULES);
+    // regular blocks (no force on them): one launch in lexicographic order, so that neighbouring blocks,
+    // which share halo rows, stay close in the sweep; the 16-bit storage forms step them two cells per
+    // lane (the face rules applied in the kernel)
+    auto launch_regular = [&](const int32_t* list, uint32_t n, bool face, bool pair) {
+        r.blist = list;
+        r.regular_face = face ? 1 : 0;
+        if constexpr (pipe_capable) {
+            const dim3 g(n * (PLANES / (a.pipe_planes > 1 ? a.pipe_planes : 1)));
+            if constexpr (SP::dithered) {
+                if (pair) {
+                    launch_pair<SP>(a.pipe_planes, face, g, stream, r);
+                    return;
+                }
+            }
+            const int fc = !face ? FACE_NONE : (a.face_kernel ? FACE_SKIP : FACE_RULES);
             switch (a.pipe_planes) {
                 case 2: launch_pipe<SP, 2>(fc, g, stream, r); break;
                 case 4: launch_pipe<SP, 4>(fc, g, stream, r); break;
@@ -963,8 +998,17 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
                 default: hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_REGULAR>), g, dim3(WG), 0, stream, r); break;
             }
         } else {
-            hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_REGULAR>), dim3(a.n_regular * PLANES), dim3(WG), 0, stream, r);
+            hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_REGULAR>), dim3(n * PLANES), dim3(WG), 0, stream, r);
         }
+    };
+    const bool pair = pipe_capable && SP::dithered && a.pair_kernel && a.pipe_planes > 1 &&
+                      !(a.regular_face && a.face_kernel);
+    if (pair && a.pair_kernel == 2) {
+        // the blocks on a domain face and the others apart (the latter with the smaller register budget)
+        if (a.n_regular_face) launch_regular(a.blist_regular_face, a.n_regular_face, true, true);
+        if (a.n_regular_plain) launch_regular(a.blist_regular_plain, a.n_regular_plain, false, true);
+    } else if (a.n_regular) {
+        launch_regular(a.blist_regular, a.n_regular, a.regular_face != 0, pair);
     }
     if (a.n_partial) {
         // regular blocks that overhang the extent (listed only where the pipelined kernel applies); at
@@ -976,9 +1020,9 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
             rp.pipe_planes = a.pipe_planes > 8 ? 8 : a.pipe_planes;
             const dim3 g(a.n_partial * (PLANES / (rp.pipe_planes > 1 ? rp.pipe_planes : 1)));
             switch (rp.pipe_planes) {
-                case 2: hipLaunchKernelGGL((k_step_pipe<SP, 2, FACE_PARTIAL>), g, dim3(WG), 0, bs, rp); break;
-                case 4: hipLaunchKernelGGL((k_step_pipe<SP, 4, FACE_PARTIAL>), g, dim3(WG), 0, bs, rp); break;
-                case 8: hipLaunchKernelGGL((k_step_pipe<SP, 8, FACE_PARTIAL>), g, dim3(WG), 0, bs, rp); break;
+                case 2: launch_pipe<SP, 2>(FACE_PARTIAL, g, bs, rp); break;
+                case 4: launch_pipe<SP, 4>(FACE_PARTIAL, g, bs, rp); break;
+                case 8: launch_pipe<SP, 8>(FACE_PARTIAL, g, bs, rp); break;
                 default: hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), g, dim3(WG), 0, bs, rp); break;
             }
         } else {

@@ -234,16 +234,18 @@ class Plan:
         step() replays a captured graph of one coarse step)."""
         t = _lib.OalTuning()
         check(self.lib.oal_get_tuning(self.h, C.byref(t)), "oal_get_tuning")
-        return {"pipe_planes": t.pipe_planes, "overlap": t.overlap, "face_kernel": t.face_kernel, "graph": t.graph}
+        return {"pipe_planes": t.pipe_planes, "overlap": t.overlap, "face_kernel": t.face_kernel, "graph": t.graph,
+                "pair_kernel": t.pair_kernel}
 
     def set_tuning(self, pipe_planes: int | None = None, overlap: int | None = None,
-                   face_kernel: int | None = None, graph: int | None = None) -> None:
+                   face_kernel: int | None = None, graph: int | None = None, pair_kernel: int | None = None) -> None:
         cur = self.tuning
         t = _lib.OalTuning()
         t.pipe_planes = int(cur["pipe_planes"] if pipe_planes is None else pipe_planes)
         t.overlap = int(cur["overlap"] if overlap is None else overlap)
         t.face_kernel = int(cur["face_kernel"] if face_kernel is None else face_kernel)
         t.graph = int(cur["graph"] if graph is None else graph)
+        t.pair_kernel = int(cur["pair_kernel"] if pair_kernel is None else pair_kernel)
         check(self.lib.oal_set_tuning(self.h, C.byref(t)), "oal_set_tuning")
 
     BFL_FALLBACKS = {"halfway_bb": _lib.OAL_BFL_FALLBACK_HALFWAY, "wall_eq": _lib.OAL_BFL_FALLBACK_WALL_EQ,
@@ -300,7 +302,7 @@ class Plan:
         ms = (C.c_double * _lib.OAL_TUNE_SHAPES)()
         check(self.lib.oal_autotune(self.h, level, n_steps, C.byref(best), ms), "oal_autotune")
         choice = {"pipe_planes": best.pipe_planes, "overlap": best.overlap, "face_kernel": best.face_kernel,
-                  "graph": best.graph}
+                  "graph": best.graph, "pair_kernel": best.pair_kernel}
         return choice, dict(zip(_lib.TUNE_KEYS, list(ms)))
 
     # -- results -----------------------------------------------------------------------------------

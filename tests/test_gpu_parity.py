@@ -159,6 +159,35 @@ def test_variant_parity(oracle, variant, storage, face_kernel):
     plan.close()
 
 
+PAIR_VARIANTS = {k: VARIANTS[k] for k in ("no_limiter_fixed_omega4", "positivity_limiter", "body_force",
+                                           "nearest_rounding", "inlet_noise")}
+PAIR_VARIANTS["unit_rates"] = {}
+
+
+@pytest.mark.parametrize("pair", [1, 2])
+@pytest.mark.parametrize("variant", list(PAIR_VARIANTS) + ["walls"])
+@pytest.mark.parametrize("storage", ["f32", "f16s", "bf16s"])
+def test_interior_blocks_rates_and_forcing(oracle, variant, storage, pair):
+    """A box with 16 interior blocks on the regular grid (6 x 4 x 4 blocks), stepped by the pipelined
+    kernels (the 16-bit forms two cells per lane) with the reference's unit rates and with others, a
+    body force, the positivity limiter and round-to-nearest storage: the oracle's bits."""
+    grid = uniform_grid((96, 64, 64))
+    grid.solid = sphere_mask(grid, (22, 30, 33), 4.5)
+    okw = dict(tau=0.53, u_inlet=0.05)
+    okw.update(VARIANTS[variant] if variant == "walls" else PAIR_VARIANTS[variant])
+    rho, u = fields(grid, seed=9, u0=(0.05, 0, 0), amp=0.03)
+    steps = 3
+    f_or, _, _ = oracle_run(oracle, grid, storage, steps, okw, rho, u)
+    plan = gpu_plan(grid, storage, okw)
+    assert plan.level_info()["n_regular"] >= 40
+    plan.set_tuning(pipe_planes=4, overlap=1, face_kernel=0, graph=0, pair_kernel=pair)
+    plan.init_equilibrium(rho, u)
+    for t in range(steps):
+        plan.level_step(0, t)
+    compare(oracle, plan, storage, f_or, grid)
+    plan.close()
+
+
 @pytest.mark.parametrize("storage", ["f32", "f16s", "bf16s", "f64"])
 def test_aa_pattern_equals_ab(storage):
     """The in-place AA pattern reproduces the pull/AB step bit for bit (after every even step),
@@ -302,12 +331,13 @@ def test_fp32_within_1e5_of_literal_reference(oracle, case):
     plan.close()
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16s", "bf16s"])
+@pytest.mark.parametrize("storage,pair", [("f32", 0), ("f16s", 0), ("f16s", 1), ("f16s", 2), ("bf16s", 0), ("bf16s", 1),
+                                          ("bf16s", 2)])
 @pytest.mark.parametrize("face_kernel", [0, 1])
 @pytest.mark.parametrize("planes", [1, 2, 4, 8, 16])
-def test_interior_pipeline_shapes(oracle, storage, planes, face_kernel):
+def test_interior_pipeline_shapes(oracle, storage, pair, planes, face_kernel):
     """Every launch shape of the pipelined interior kernel gives the same bits (regular interior
-    blocks present: 5 x 3 x 3 blocks)."""
+    blocks present: 5 x 3 x 3 blocks), the 16-bit forms with one and with two cells per lane."""
     grid = uniform_grid((80, 48, 48))
     grid.solid = sphere_mask(grid, (18, 24, 24), 4.0)
     okw = dict(tau=0.52, u_inlet=0.05)
@@ -315,7 +345,8 @@ def test_interior_pipeline_shapes(oracle, storage, planes, face_kernel):
     steps = 3
     f_or, _, _ = oracle_run(oracle, grid, storage, steps, okw, rho, u)
     plan = gpu_plan(grid, storage, okw)
-    shape = {"pipe_planes": planes, "overlap": int(planes % 4 == 0), "face_kernel": face_kernel, "graph": 0}
+    shape = {"pipe_planes": planes, "overlap": int(planes % 4 == 0), "face_kernel": face_kernel, "graph": 0,
+             "pair_kernel": pair}
     plan.set_tuning(**shape)
     assert plan.tuning == shape
     plan.init_equilibrium(rho, u)
