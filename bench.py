@@ -94,10 +94,11 @@ def copy_bandwidth_gbs() -> float:
     return v.value
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
-    (copy-calibrated), or None when the summary covers another kernel or launch shape."""
-    path = os.path.join(ROOT, "profiles", "step_f32_256_hbm_traffic.md")
+def measured_traffic(kernel: str, summary: str = "step_f32_256_hbm_traffic.md"):
+    """HBM bytes per launch of `kernel` (or per step of the launches joined by ' + ') from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None when the summary covers another
+    kernel or launch shape."""
+    path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
     text = open(path).read()
@@ -551,6 +552,9 @@ def main(argv=None) -> int:
     ap.add_argument("--tgv-cpu-steps", type=int, default=200, help="steps of the CPU oracle in --tgv")
     ap.add_argument("--amr-legs", default="f16s:1,f16s:0,f32:1",
                     help="block-AMR legs timed, storage:graph comma-separated; the first is the reported value")
+    ap.add_argument("--acc-steps", type=int, default=1500,
+                    help="steps of the accuracy leg of the default line (Cd and L2(u) of FP32 / FP16 / BF16 against "
+                         "FP64 runs of the same kernels); 0 skips it")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -571,7 +575,9 @@ def main(argv=None) -> int:
         return 0
     if args.bouzidi:
         args.no_cpu_baseline = True   # the CPU baseline times the halfway form
-    if args.steps % 2 or args.warmup % 2:
+    # the AA pattern steps in even/odd pairs: its counts are rounded up to even (and reported as run)
+    warmup_asked = args.warmup
+    if args.stream == "aa" and (args.steps % 2 or args.warmup % 2):
         args.steps += args.steps % 2
         args.warmup += args.warmup % 2
 
@@ -594,7 +600,8 @@ def main(argv=None) -> int:
     area = np.pi * (D / 2) ** 2
     cd = float(F[0] / (0.5 * U_IN ** 2 * area))
 
-    kernel = (f"k_step_pipe<StF32, {pp}, {2 if tun['face_kernel'] else 1}>" if args.stream == "ab" and pp > 1
+    # (the collision's unit-rate form: the reference's default rates omega_bulk = omega_3 = omega_4 = 1)
+    kernel = (f"k_step_pipe<StF32, {pp}, {2 if tun['face_kernel'] else 1}, true>" if args.stream == "ab" and pp > 1
               else "k_step<StF32, regular>")
     traffic = measured_traffic(kernel)
 
@@ -604,7 +611,7 @@ def main(argv=None) -> int:
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": warmup_asked,
         "ms_per_step": round(ms_total / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
@@ -634,10 +641,21 @@ def main(argv=None) -> int:
         bulk16 = (info["n_regular"] + info["n_interior"]) * 16 ** 3 - (0 if face16 else info["n_face_cells"])
         a16 = bytes_per_lup("f16s") * bulk16 / (ms_k16 * 1e-3) / 1e9
         s16 = bytes_per_lup("f16s") * n_fluid / (ms_t16 / args.steps * 1e-3) / 1e9
+        pp16 = tun16["pipe_planes"]
+        if args.stream == "ab" and pp16 > 1 and tun16.get("pair_kernel") and not tun16["face_kernel"]:
+            k16 = (f"k_step_pair<StF16S, {pp16}, true, 1> + k_step_pair<StF16S, {pp16}, true, 0>"
+                   if tun16["pair_kernel"] == 2 else f"k_step_pair<StF16S, {pp16}, true, 1>")
+        elif args.stream == "ab" and pp16 > 1:
+            k16 = f"k_step_pipe<StF16S, {pp16}, {2 if tun16['face_kernel'] else 1}, true>"
+        else:
+            k16 = "k_step<StF16S, regular>"
+        traffic16 = measured_traffic(k16, "step_f16_256_hbm_traffic.md")
         out["fp16_shifted"] = {"value": round(m16, 1), "unit": "MLUPS", "ms_per_step": round(ms_t16 / args.steps, 4),
                                "speedup_vs_f32": round(m16 / mlups, 3), "tuning": tun16,
                                "roofline": {"achieved": round(a16, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                            "frac": round(a16 / HBM_PEAK_GBS, 4), "kernel_ms": round(ms_k16, 5),
+                                            "frac": round(a16 / HBM_PEAK_GBS, 4), "traffic": traffic16,
+                                            "kernel": k16, "kernel_ms": round(ms_k16, 5),
+                                            "cells_per_launch": bulk16,
                                             "bytes_per_lup": bytes_per_lup("f16s"),
                                             "step_achieved_gbs": round(s16, 1),
                                             "step_frac": round(s16 / HBM_PEAK_GBS, 4)}}
@@ -645,6 +663,17 @@ def main(argv=None) -> int:
         bw = copy_bandwidth_gbs()
         out["roofline"]["measured_copy_gbs"] = round(bw, 1)
         out["roofline"]["frac_of_measured_copy"] = round(achieved / bw, 4)
+    if args.acc_steps > 0:
+        # bounded accuracy leg: the storage forms against FP64 runs of the same kernels after acc_steps steps
+        # (C2 FP32 from the uniform start, C3 FP16 / BF16 from the seeded perturbation; a transient state,
+        # the steady-state record is --validate)
+        acc = validation(args.acc_steps, args.seed, bouzidi=args.bouzidi)
+        out["accuracy"] = {"steps": acc["steps"], "seed": acc["seed"], "wall": acc["wall"],
+                           "cd_f64_C2": acc["C2"]["f64"]["cd"], "cd_f64_C3": acc["C3"]["f64"]["cd"],
+                           "cd_rel_err_vs_f64": {st: round(acc[c][st]["cd_rel_err_vs_f64"], 7)
+                                                 for c, st in (("C2", "f32"), ("C3", "f16s"), ("C3", "bf16s"))},
+                           "l2_u_vs_f64": {st: float(f"{acc[c][st]['l2_u_vs_f64']:.3e}")
+                                           for c, st in (("C2", "f32"), ("C3", "f16s"), ("C3", "bf16s"))}}
     if not args.no_cpu_baseline:
         v, cores, done, secs = cpu_baseline(grid)
         out["cpu_baseline"] = {"value": round(v, 2), "unit": "MLUPS", "cores": cores, "kind": "port",
