@@ -249,6 +249,16 @@ int oal_force(OalPlan* plan, int level, double* force);
  * Requires the AB pattern and OAL_BB_HALFWAY.  Drops captured step graphs. */
 int oal_set_bouzidi(OalPlan* plan, int level, const uint16_t* q_map, int32_t ff_fallback, float q_min);
 
+/* Sponge layer of a level: the blend of stream_collide_cumulant! (src/collision/kernel_cumulant.jl:428-446)
+ * with the per-cell strength sponge_arr (BlockLevel.sponge).  sponge [n_cells] in the reference layout
+ * [x,y,z,b], values in [0, 1]; NULL (or all zero) removes it.  A cell with sp > 0 has its density and
+ * velocity blended toward (1, u_inlet) before the collision uses them, rho (1 - sp) + sp and
+ * u (1 - sp) + (u_inlet, 0, 0) sp; with blend_distributions 1 (SPONGE_BLEND_DISTRIBUTIONS,
+ * src/io/config_loader.jl:1312) its populations are blended toward the inlet equilibrium too.  Blocks
+ * holding a nonzero strength are stepped by the general kernels.  Requires the AB pattern and a level
+ * without interpolated bounce-back; drops captured step graphs. */
+int oal_set_sponge(OalPlan* plan, int level, const float* sponge, int32_t blend_distributions);
+
 /* The raw level state of dump_raw_state (src/io/raw_state_dump.jl:10-47, the reference's "LBM1"
  * file, written per level as <output_dir>/raw_state/step_<NNNNNN>/level_<L>.bin), little-endian:
  *   Int32 magic 0x4C424D31; Int32 level_id, n_blocks, block_size, grid_dim_x, grid_dim_y, grid_dim_z;

@@ -29,7 +29,7 @@ EXPORTS = [
     "oal_level_cells", "oal_level_info", "oal_upload_f", "oal_download_f", "oal_init_equilibrium", "oal_level_step",
     "oal_step", "oal_force", "oal_mass", "oal_download_macro", "oal_guard", "oal_sync", "oal_time_steps",
     "oal_get_tuning", "oal_set_tuning", "oal_autotune", "oal_copy_bandwidth", "oal_set_bouzidi",
-    "oal_write_raw_state", "oal_read_raw_state", "oal_mass_noalloc",
+    "oal_write_raw_state", "oal_read_raw_state", "oal_mass_noalloc", "oal_set_sponge",
 ]
 
 
@@ -122,6 +122,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "oal_force": ([vp, C.c_int, dp], C.c_int),
         "oal_mass": ([vp, C.c_int, dp, vp], C.c_int),
         "oal_mass_noalloc": ([vp, C.c_int, vp, vp], C.c_int),
+        "oal_set_sponge": ([vp, C.c_int, vp, i32], C.c_int),
         "oal_download_macro": ([vp, C.c_int, dp, dp], C.c_int),
         "oal_guard": ([vp, C.c_int, C.c_float, C.c_float, i64p], C.c_int),
         "oal_sync": ([vp], C.c_int),

@@ -321,6 +321,10 @@ struct CellRates {
     T omega1, omega4_diag;
     T Fx, Fy, Fz;
     bool forced;
+    // sponge (read by the SPONGE form only): strength sp of the cell, the inlet velocity it blends
+    // toward, and whether the populations are blended too
+    T sponge, sponge_u;
+    int sponge_blend;
 };
 
 // The fluid-cell update after the gather (kernel_cumulant.jl:420-908): f holds the 27 arriving
@@ -338,7 +342,11 @@ struct CellRates {
 // terms they would contribute are left out, and the first backward pass uses the zeros it knows.  Each
 // term left out adds an exact zero in the general form, so both give the same values (only the sign of
 // a result that is exactly zero could differ).
-template <class T, bool UNIT = false>
+//
+// SPONGE: the sponge blend (kernel_cumulant.jl:428-446) of a cell with q.sponge > 0: rho and u blended
+// toward (1, u_inlet) before the collision uses them, and with q.sponge_blend the populations toward the
+// inlet equilibrium, the transform then summing the blended populations.
+template <class T, bool UNIT = false, bool SPONGE = false>
 __device__ __forceinline__ void collide_cell(T (&f)[27], const CellRates<T>& q, const CollParams& p,
                                              T& rho_o, T& ux_o, T& uy_o, T& uz_o) {
     T A[27];
@@ -366,14 +374,46 @@ __device__ __forceinline__ void collide_cell(T (&f)[27], const CellRates<T>& q, 
 
     const T rc = mx(rho, T(0.01f));
     const T inv = T(1) / rc;
-    const T ux = jx * inv, uy = jy * inv, uz = jz * inv;
+    T ux = jx * inv, uy = jy * inv, uz = jz * inv;
+    // rb: the density the collision reports and its limiter uses; rr, invr: the mass of the populations
+    // the transform sums and 1 / max(rr, 0.01) (the sponge's population blend changes them)
+    T rb = rc, rr = rho, invr = inv;
+    if constexpr (SPONGE) {
+        const T sp = q.sponge;
+        if (sp > T(0)) {
+            const T g = T(1) - sp;
+            rb = rc * g + T(1) * sp;
+            ux = ux * g + q.sponge_u * sp;
+            uy = uy * g;
+            uz = uz * g;
+            if (q.sponge_blend == 1) {
+#pragma unroll
+                for (int k = 0; k < 27; ++k)
+                    f[k] = f[k] * g + feq_auto(T(1), q.sponge_u, T(0), T(0), wgt<T>(k), T(dcx(k)), T(dcy(k)), T(dcz(k)),
+                                               p.compressible) * sp;
+#pragma unroll
+                for (int r = 0; r < 9; ++r) {
+                    const T s = f[3 * r] + f[3 * r + 2];
+                    const T d = f[3 * r + 2] - f[3 * r];
+                    A[3 * r] = s + f[3 * r + 1];
+                    A[3 * r + 1] = d;
+                    A[3 * r + 2] = s;
+                }
+                T z0[3];
+#pragma unroll
+                for (int iz = 0; iz < 3; ++iz) z0[iz] = (A[9 * iz] + A[6 + 9 * iz]) + A[3 + 9 * iz];
+                rr = (z0[0] + z0[2]) + z0[1];
+                invr = T(1) / mx(rr, T(0.01f));
+            }
+        }
+    }
     T uxe = ux, uye = uy, uze = uz;
     if (q.forced) {
         uxe = fmx(T(0.5) * q.Fx, inv, ux);
         uye = fmx(T(0.5) * q.Fy, inv, uy);
         uze = fmx(T(0.5) * q.Fz, inv, uz);
     }
-    rho_o = rc; ux_o = ux; uy_o = uy; uz_o = uz;
+    rho_o = rb; ux_o = ux; uy_o = uy; uz_o = uz;
 
     {
         const T u = uxe, uu = u * u, t = T(-2) * u;
@@ -400,9 +440,9 @@ __device__ __forceinline__ void collide_cell(T (&f)[27], const CellRates<T>& q, 
     for (int r = 0; r < 9; ++r) pass(r, r + 9, r + 18, uze);
 
     // normalised central moments (index a + 3b + 9c for order (a,b,c)) and 4th-order cumulants
-    const T c200 = A[2] * inv, c020 = A[6] * inv, c002 = A[18] * inv;
-    const T c110 = A[4] * inv, c101 = A[10] * inv, c011 = A[12] * inv;
-    const T c220n = A[8] * inv, c202n = A[20] * inv, c022n = A[24] * inv;
+    const T c200 = A[2] * invr, c020 = A[6] * invr, c002 = A[18] * invr;
+    const T c110 = A[4] * invr, c101 = A[10] * invr, c011 = A[12] * invr;
+    const T c220n = A[8] * invr, c202n = A[20] * invr, c022n = A[24] * invr;
     const T x110 = T(2) * c110, x101 = T(2) * c101, x011 = T(2) * c011;
     const T C220 = fmx(-x110, c110, fmx(-c200, c020, c220n));
     const T C202 = fmx(-x101, c101, fmx(-c200, c002, c202n));
@@ -433,7 +473,7 @@ __device__ __forceinline__ void collide_cell(T (&f)[27], const CellRates<T>& q, 
     const T s220 = fmx(y110, s110, fmx(s200, s020, S220));
     const T s202 = fmx(y101, s101, fmx(s200, s002, S202));
     const T s022 = fmx(y011, s011, fmx(s020, s002, S022));
-    const T r = rho;
+    const T r = rr;
     A[0] = r;
     A[1] = T(0); A[3] = T(0); A[9] = T(0);
     A[2] = r * s200; A[6] = r * s020; A[18] = r * s002;
@@ -455,10 +495,10 @@ __device__ __forceinline__ void collide_cell(T (&f)[27], const CellRates<T>& q, 
         A[14] = r * s211; A[16] = r * s121; A[22] = r * s112;
         A[26] = r * e;
     } else {
-        const T c210 = A[5] * inv, c120 = A[7] * inv, c201 = A[11] * inv;
-        const T c021 = A[15] * inv, c102 = A[19] * inv, c012 = A[21] * inv;
-        const T c111 = A[13] * inv;
-        const T c211n = A[14] * inv, c121n = A[16] * inv, c112n = A[22] * inv;
+        const T c210 = A[5] * invr, c120 = A[7] * invr, c201 = A[11] * invr;
+        const T c021 = A[15] * invr, c102 = A[19] * invr, c012 = A[21] * invr;
+        const T c111 = A[13] * invr;
+        const T c211n = A[14] * invr, c121n = A[16] * invr, c112n = A[22] * invr;
         const T C211 = fmx(-x110, c101, fmx(-c200, c011, c211n));
         const T C121 = fmx(-x110, c011, fmx(-c020, c101, c121n));
         const T C112 = fmx(-x101, c011, fmx(-c002, c110, c112n));
@@ -548,14 +588,14 @@ __device__ __forceinline__ void collide_cell(T (&f)[27], const CellRates<T>& q, 
             T alpha = T(1);
 #pragma unroll
             for (int k = 0; k < 27; ++k) {
-                const T fe = feq_auto(rc, uxe, uye, uze, wgt<T>(k), T(dcx(k)), T(dcy(k)), T(dcz(k)), p.compressible);
+                const T fe = feq_auto(rb, uxe, uye, uze, wgt<T>(k), T(dcx(k)), T(dcy(k)), T(dcz(k)), p.compressible);
                 const T fn = A[k] - fe;
                 if (fn < T(0) && fe > T(1e-15f)) alpha = mn(alpha, fe / (fe - fn + T(1e-15f)));
             }
             alpha = mx(alpha, T(0));
 #pragma unroll
             for (int k = 0; k < 27; ++k) {
-                const T fe = feq_auto(rc, uxe, uye, uze, wgt<T>(k), T(dcx(k)), T(dcy(k)), T(dcz(k)), p.compressible);
+                const T fe = feq_auto(rb, uxe, uye, uze, wgt<T>(k), T(dcx(k)), T(dcy(k)), T(dcz(k)), p.compressible);
                 A[k] = fe + alpha * (A[k] - fe);
             }
         }

@@ -20,8 +20,9 @@ enum : uint32_t {
     BF_SOLID = 2u,     // the block or its one-cell halo holds a solid cell
     BF_MISSING = 4u,   // some neighbour block is absent inside the extent (rest-weight fallback)
     BF_PARTIAL = 8u,   // some cells of the block lie beyond the extent
-    BF_INACTIVE = 16u  // refined level: a covered block (written by restriction) or a ghost block (written by
+    BF_INACTIVE = 16u, // refined level: a covered block (written by restriction) or a ghost block (written by
                        // the fill from the coarser level); never stepped
+    BF_SPONGE = 32u    // some cell of the block has a nonzero sponge strength
 };
 
 // one domain of a level on the device
@@ -90,6 +91,11 @@ struct StepArgs {
     // link from a cell along c_k (units of |c_k|); null = halfway bounce-back on every solid link
     const uint16_t* qmap;
     float q_min;                 // links with q <= q_min (or q > 1) keep halfway bounce-back
+    // sponge (blocks flagged BF_SPONGE): per sponge slot [BS3] strengths; slot of a block; 1 = the
+    // populations are blended too
+    const float* sponge;
+    const int32_t* sslot;
+    int32_t sponge_blend;
     int32_t bfl_fallback;        // q < 1/2 without a fluid cell behind: 0 halfway, 1 wall equilibrium, 2 own opposite
     CollParams cp;
     __device__ __forceinline__ int32_t step() const { return ts_dev ? ts_dev[0] * ts_mul + ts_add : time_step; }

@@ -143,6 +143,17 @@ struct Level {
     double restrict_scale = 0.0, restrict_keep = 0.0;
     bool has_solid = false;
     int64_t n_fluid = 0;
+    // host tables the block lists are built from (build_lists)
+    std::vector<uint32_t> flags_h;
+    std::vector<char> regular_h, near_h;
+    std::vector<int32_t> borg_h, bdom_h;
+    bool ab = true, pipe_ok = true;
+    // sponge (kernel_cumulant.jl:428-446): per sponge slot [BS3] strengths of the blocks that hold a
+    // nonzero one (empty: no sponge)
+    DevBuf<float> sponge;
+    DevBuf<int32_t> sslot;
+    int32_t sponge_blend = 0;
+    bool has_sponge = false;
     bool have_f = false;
     bool force_valid = false;
     int aa_phase = 0;                // AA: 0 = populations in stored form (next step odd), 1 = after an odd step
@@ -150,7 +161,7 @@ struct Level {
         nbr.release(); borg.release(); bdom.release(); dom_item0.release(); dom_block0.release();
         bflags.release(); solid.release(); doms.release(); fA.release(); fB.release(); velA.release(); velB.release();
         fpart.release(); mpart.release(); red_out.release(); guard.release(); oslot.release(); obuf.release(); blist_rg.release(); blist_in.release(); blist_br.release(); blist_bd.release(); blist_pr.release(); blist_rp.release(); blist_rf.release(); face_cells.release();
-        lslot.release(); lmask.release(); qmap.release();
+        lslot.release(); lmask.release(); qmap.release(); sponge.release(); sslot.release();
         g_src.release(); g_dst.release(); g_slot.release(); g_w.release(); g_m.release();
         r_src.release(); r_dst.release(); r_slot.release(); r_w.release(); r_m.release();
         rf_src.release(); rf_dst.release(); rf_slot.release(); rf_w.release();
@@ -215,6 +226,58 @@ struct HostLevel {
     std::vector<uint32_t> flags;
     std::vector<uint8_t> solid;
 };
+
+// The block lists of a level's step kernels from its block flags (L.flags_h) and the host tables kept
+// for that purpose; rebuilt when the flags change (a sponge field marks its blocks).
+void build_lists(Level& L, HostLevel& H) {
+    H.blist_rg.clear(); H.blist_rp.clear(); H.blist_rf.clear(); H.blist_in.clear();
+    H.blist_br.clear(); H.blist_bd.clear(); H.blist_pr.clear(); H.face_cells.clear();
+    L.regular_face = false;
+    const std::vector<uint32_t>& flags = L.flags_h;
+    for (uint32_t b = 0; b < L.nb; ++b) {
+        const DevDomain& dm = L.doms_h[L.bdom_h[b]];
+        if (flags[b] & BF_INACTIVE) continue;   // never stepped
+        const bool regular = L.regular_h[b] != 0;
+        if (flags[b] == 0) {
+            (regular ? H.blist_rg : H.blist_in).push_back((int32_t)b);
+            if (regular) H.blist_rp.push_back((int32_t)b);
+        } else if (L.ab && regular && flags[b] == BF_EDGE) {
+            // AB pattern: a whole block on a domain face, no solid near, joins the regular list; the
+            // cells on its faces (sources outside the extent) go to the face-cell list
+            H.blist_rg.push_back((int32_t)b);
+            H.blist_rf.push_back((int32_t)b);
+            L.regular_face = true;
+            const int o[3] = {L.borg_h[3 * b], L.borg_h[3 * b + 1], L.borg_h[3 * b + 2]};
+            bool lo[3], hi[3];
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = !dm.per[a] && o[a] == 0;
+                hi[a] = !dm.per[a] && o[a] + BS == dm.ext[a];
+            }
+            for (int z = 0; z < BS; ++z)
+                for (int y = 0; y < BS; ++y)
+                    for (int x = 0; x < BS; ++x) {
+                        const int l[3] = {x, y, z};
+                        bool face = false;
+                        for (int a = 0; a < 3; ++a) face = face || (lo[a] && l[a] == 0) || (hi[a] && l[a] == BS - 1);
+                        if (face) H.face_cells.push_back(b * BS3 + (uint32_t)(x + BS * (y + BS * z)));
+                    }
+        } else if (L.ab && L.pipe_ok && regular && flags[b] == (BF_EDGE | BF_PARTIAL) && L.near_h[L.bdom_h[b]]) {
+            // AB pattern: a block that overhangs the extent, no solid near: the pipelined kernel with the
+            // face rules at the extent's last cells (in a domain whose block layers span 2 GiB or more the
+            // one-plane boundary kernel, whose addresses are 64-bit)
+            H.blist_pr.push_back((int32_t)b);
+        } else {
+            (regular ? H.blist_br : H.blist_bd).push_back((int32_t)b);
+        }
+    }
+    L.n_face = (uint32_t)H.face_cells.size();
+    L.n_regular_face = (uint32_t)H.blist_rf.size();
+    L.n_regular = (uint32_t)H.blist_rg.size();
+    L.n_interior = (uint32_t)H.blist_in.size();
+    L.n_boundary_regular = (uint32_t)H.blist_br.size();
+    L.n_partial = (uint32_t)H.blist_pr.size();
+    L.n_boundary = (uint32_t)H.blist_bd.size();
+}
 
 // Host-side construction of one level's tables from the reference-convention descriptor.
 int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, bool pipe_ok, size_t sbytes, Level& L, HostLevel& H,
@@ -359,9 +422,9 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, bool
     }
     // (a boundary block qualifies when no neighbour inside the extent is absent and every present one is
     // at its lexicographic offset: its out-of-extent sources take the face rules and are never read)
+    L.regular_h.assign(nb, 0);
     for (uint32_t b = 0; b < nb; ++b) {
         const DevDomain& dm = L.doms_h[bdom[b]];
-        if (flags[b] & BF_INACTIVE) continue;   // never stepped
         bool regular = !(flags[b] & BF_MISSING);
         for (int k = 0; k < 27 && regular; ++k) {
             const int ox = k % 3 - 1, oy = (k / 3) % 3 - 1, oz = k / 9 - 1;
@@ -370,45 +433,15 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, bool
             if (have < 0) continue;   // absent: outside the extent (BF_MISSING is clear)
             regular = have == want;
         }
-        if (flags[b] == 0) {
-            (regular ? H.blist_rg : H.blist_in).push_back((int32_t)b);
-            if (regular) H.blist_rp.push_back((int32_t)b);
-        } else if (ab && regular && flags[b] == BF_EDGE) {
-            // AB pattern: a whole block on a domain face, no solid near, joins the regular list; the
-            // cells on its faces (sources outside the extent) go to the face-cell list
-            H.blist_rg.push_back((int32_t)b);
-            H.blist_rf.push_back((int32_t)b);
-            L.regular_face = true;
-            const int o[3] = {borg[3 * b], borg[3 * b + 1], borg[3 * b + 2]};
-            bool lo[3], hi[3];
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = !dm.per[a] && o[a] == 0;
-                hi[a] = !dm.per[a] && o[a] + BS == dm.ext[a];
-            }
-            for (int z = 0; z < BS; ++z)
-                for (int y = 0; y < BS; ++y)
-                    for (int x = 0; x < BS; ++x) {
-                        const int l[3] = {x, y, z};
-                        bool face = false;
-                        for (int a = 0; a < 3; ++a) face = face || (lo[a] && l[a] == 0) || (hi[a] && l[a] == BS - 1);
-                        if (face) H.face_cells.push_back(b * BS3 + (uint32_t)(x + BS * (y + BS * z)));
-                    }
-        } else if (ab && pipe_ok && regular && flags[b] == (BF_EDGE | BF_PARTIAL) && near[bdom[b]]) {
-            // AB pattern: a block that overhangs the extent, no solid near: the pipelined kernel with the
-            // face rules at the extent's last cells (in a domain whose block layers span 2 GiB or more the
-            // one-plane boundary kernel, whose addresses are 64-bit)
-            H.blist_pr.push_back((int32_t)b);
-        } else {
-            (regular ? H.blist_br : H.blist_bd).push_back((int32_t)b);
-        }
+        L.regular_h[b] = regular ? 1 : 0;
     }
-    L.n_face = (uint32_t)H.face_cells.size();
-    L.n_regular_face = (uint32_t)H.blist_rf.size();
-    L.n_regular = (uint32_t)H.blist_rg.size();
-    L.n_interior = (uint32_t)H.blist_in.size();
-    L.n_boundary_regular = (uint32_t)H.blist_br.size();
-    L.n_partial = (uint32_t)H.blist_pr.size();
-    L.n_boundary = (uint32_t)H.blist_bd.size();
+    L.near_h = near;
+    L.flags_h = flags;
+    L.borg_h = borg;
+    L.bdom_h = bdom;
+    L.ab = ab;
+    L.pipe_ok = pipe_ok;
+    build_lists(L, H);
     // block-major copy of the neighbour table: one 32-entry row per block (scalar loads)
     H.nbr_bm.assign((size_t)nb * 32, -1);
     for (uint32_t b = 0; b < nb; ++b)
@@ -967,6 +1000,9 @@ StepArgs make_args(OalPlan* p, Level& L, int32_t time_step, bool want_force) {
     for (const DevDomain& dm : L.doms_h) pair_ok = pair_ok && pair_fits(dm.nbx, dm.nby);
     a.pair_kernel = pair_ok ? p->pair_kernel : 0;
     a.qmap = L.qmap.p;
+    a.sponge = L.has_sponge ? L.sponge.p : nullptr;
+    a.sslot = L.sslot.p;
+    a.sponge_blend = L.sponge_blend;
     a.q_min = L.q_min;
     a.bfl_fallback = L.bfl_fallback;
     a.cp = p->cp;
@@ -1724,6 +1760,8 @@ int oal_copy_bandwidth(int device, int64_t bytes, int reps, double* gbs) {
 }
 
 int oal_set_bouzidi(OalPlan* p, int level, const uint16_t* q_map, int32_t ff_fallback, float q_min) {
+    if (p && level >= 0 && level < (int)p->levels.size() && p->levels[level].has_sponge && q_map)
+        return fail(OAL_ERR_UNSUPPORTED, "interpolated bounce-back on a level with a sponge is not supported");
     if (int rc = check_level(p, level)) return rc;
     if (p->stream_mode != OAL_STREAM_AB || p->bb_mode != OAL_BB_HALFWAY)
         return fail(OAL_ERR_ARG, "interpolated bounce-back needs the AB pattern and halfway bounce-back");
@@ -1751,6 +1789,47 @@ int oal_set_bouzidi(OalPlan* p, int level, const uint16_t* q_map, int32_t ff_fal
     }
     OAL_HIP(L.qmap.alloc(h.size()));
     OAL_HIP(hipMemcpy(L.qmap.p, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    return OAL_OK;
+}
+
+int oal_set_sponge(OalPlan* p, int level, const float* sponge, int32_t blend_distributions) {
+    if (int rc = check_level(p, level)) return rc;
+    if (blend_distributions != 0 && blend_distributions != 1) return fail(OAL_ERR_ARG, "blend_distributions must be 0 or 1");
+    Level& L = p->levels[level];
+    if (sponge && p->stream_mode != OAL_STREAM_AB) return fail(OAL_ERR_UNSUPPORTED, "the sponge needs the AB pattern");
+    if (sponge && L.qmap.p) return fail(OAL_ERR_UNSUPPORTED, "a sponge on a level with interpolated bounce-back is not supported");
+    const uint32_t nhost = L.ncell_host;
+    if (sponge) {
+        for (uint32_t i = 0; i < nhost; ++i)
+            if (!(sponge[i] >= 0.0f && sponge[i] <= 1.0f)) return fail(OAL_ERR_ARG, "sponge strengths must lie in [0, 1]");
+    }
+    // blocks with a nonzero strength: flagged, each with a slot of BS3 strengths
+    std::vector<int32_t> slot(L.nb, -1);
+    std::vector<float> vals;
+    for (uint32_t b = 0; b < L.nb; ++b) {
+        L.flags_h[b] &= ~BF_SPONGE;
+        if (!sponge || b >= L.n_caller_blocks || (L.flags_h[b] & BF_INACTIVE)) continue;
+        bool any = false;
+        for (int c = 0; c < BS3 && !any; ++c) any = sponge[(size_t)b * BS3 + c] > 0.0f;
+        if (!any) continue;
+        slot[b] = (int32_t)(vals.size() / BS3);
+        vals.insert(vals.end(), sponge + (size_t)b * BS3, sponge + (size_t)(b + 1) * BS3);
+        L.flags_h[b] |= BF_SPONGE;
+    }
+    OAL_HIP(hipSetDevice(p->device));
+    OAL_HIP(hipDeviceSynchronize());
+    drop_graphs(p);
+    HostLevel H;
+    build_lists(L, H);
+    auto re = [](auto& d, const auto& h) { d.release(); return upload(d, h); };
+    int rc;
+    if ((rc = re(L.bflags, L.flags_h)) || (rc = re(L.blist_rg, H.blist_rg)) || (rc = re(L.blist_in, H.blist_in)) ||
+        (rc = re(L.blist_br, H.blist_br)) || (rc = re(L.blist_bd, H.blist_bd)) || (rc = re(L.blist_pr, H.blist_pr)) ||
+        (rc = re(L.blist_rp, H.blist_rp)) || (rc = re(L.blist_rf, H.blist_rf)) || (rc = re(L.face_cells, H.face_cells)) ||
+        (rc = re(L.sponge, vals)) || (rc = re(L.sslot, slot)))
+        return rc;
+    L.has_sponge = !vals.empty();
+    L.sponge_blend = blend_distributions;
     return OAL_OK;
 }
 

@@ -358,7 +358,7 @@ __device__ __forceinline__ void bfl_links(const StepArgs& a, const CellCtx& c, t
 
 // BFL: the level has a q-map (interpolated bounce-back); a separate instantiation, so that the boundary
 // kernels of levels without one keep their register budget.
-template <class SP, int PAT, bool WALE, bool GEN, bool REG = false, bool BFL = false>
+template <class SP, int PAT, bool WALE, bool GEN, bool REG = false, bool BFL = false, bool SPONGE = false>
 __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, const DevDomain* dm, bool active,
                                           bool solid_own, double& F0, double& F1, double& F2) {
     using T = typename SP::T;
@@ -474,7 +474,12 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
         }
         q.forced = dm->forced != 0;
         q.Fx = T(dm->F[0]); q.Fy = T(dm->F[1]); q.Fz = T(dm->F[2]);
-        collide_cell<T>(f, q, a.cp, rho_o, ux_o, uy_o, uz_o);
+        if constexpr (SPONGE) {
+            q.sponge = (c.flags & BF_SPONGE) ? T(a.sponge[(size_t)a.sslot[b] * BS3 + (c.own - b * BS3)]) : T(0);
+            q.sponge_u = T(u_in);
+            q.sponge_blend = a.sponge_blend;
+        }
+        collide_cell<T, false, SPONGE>(f, q, a.cp, rho_o, ux_o, uy_o, uz_o);
     }
     if (a.vel_out) {
         T* vo = (T*)a.vel_out;
@@ -567,7 +572,7 @@ constexpr int step_waves() {
     return sizeof(typename SP::T) == 8 && PAT != PAT_AB ? 1 : sizeof(typename SP::T) == 8 || PAT != PAT_AB || WALE ? 2 : OAL_STEP_WAVES;
 }
 
-template <class SP, int PAT, bool WALE, int SEL = SEL_ALL, bool BFL = false>
+template <class SP, int PAT, bool WALE, int SEL = SEL_ALL, bool BFL = false, bool SPONGE = false>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(step_waves<SP, PAT, WALE>()))) k_step(const StepArgs a) {
     const uint32_t litem = xcd_item(blockIdx.x, gridDim.x);
     const uint32_t lb = litem / PLANES;
@@ -606,9 +611,9 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(step_wa
         const bool solid_own = (c.lmask >> 27) & 1u;
         const bool active = inside && !(solid_own && a.cp.bb_mode == 1);
         if constexpr (SEL == SEL_BOUNDARY_REGULAR)
-            step_cell<SP, PAT, WALE, true, true, BFL>(a, c, dm, active, solid_own, F0, F1, F2);
+            step_cell<SP, PAT, WALE, true, true, BFL, SPONGE>(a, c, dm, active, solid_own, F0, F1, F2);
         else
-            step_cell<SP, PAT, WALE, true, false, BFL>(a, c, dm, active, solid_own, F0, F1, F2);
+            step_cell<SP, PAT, WALE, true, false, BFL, SPONGE>(a, c, dm, active, solid_own, F0, F1, F2);
     }
     if (a.fpart) wg_sum3(F0 * dm->force_scale, F1 * dm->force_scale, F2 * dm->force_scale, a.fpart, item);
 }
@@ -948,6 +953,7 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
         const dim3 g(a.n_boundary_regular * PLANES);
         if constexpr (PAT == PAT_AB) {
             if (bfl) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, true>), g, dim3(WG), 0, bs, r);
+            else if (a.sponge) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, false, true>), g, dim3(WG), 0, bs, r);
             else hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), g, dim3(WG), 0, bs, r);
         } else {
             hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), g, dim3(WG), 0, bs, r);
@@ -958,6 +964,7 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
         const dim3 g(a.n_boundary * PLANES);
         if constexpr (PAT == PAT_AB) {
             if (bfl) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY, true>), g, dim3(WG), 0, bs, r);
+            else if (a.sponge) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY, false, true>), g, dim3(WG), 0, bs, r);
             else hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY>), g, dim3(WG), 0, bs, r);
         } else {
             hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY>), g, dim3(WG), 0, bs, r);

@@ -272,6 +272,18 @@ class Plan:
         check(self.lib.oal_set_bouzidi(self.h, level, ptr, self.BFL_FALLBACKS[ff_fallback], C.c_float(q_min)),
               "oal_set_bouzidi")
 
+    def set_sponge(self, sponge, blend_distributions: bool = False, level: int = 0) -> None:
+        """Sponge layer of a level (kernel_cumulant.jl:428-446): per-cell strengths [n_cells] in [0, 1] in the
+        reference layout, or None to remove it; blend_distributions also blends the populations
+        (SPONGE_BLEND_DISTRIBUTIONS)."""
+        if sponge is None:
+            check(self.lib.oal_set_sponge(self.h, level, None, 0), "oal_set_sponge")
+            return
+        sp = np.ascontiguousarray(sponge, dtype=np.float32)
+        if sp.shape != (self.n_cells(level),):
+            raise ValueError(f"sponge must be [{self.n_cells(level)}]")
+        check(self.lib.oal_set_sponge(self.h, level, sp.ctypes.data, int(bool(blend_distributions))), "oal_set_sponge")
+
     def write_raw_state(self, path, level: int = 0, level_id: int | None = None, dx: float = 1.0) -> None:
         """The level's density and velocity as the reference's LBM1 raw state file (dump_raw_state,
         src/io/raw_state_dump.jl:21-47); level_id defaults to the reference's 1-based level + 1."""

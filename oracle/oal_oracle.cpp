@@ -11,7 +11,8 @@
  *   inlet noise      src/collision/physics_utils.jl:5-16
  *   chimera          src/collision/kernel_cumulant.jl:4-26
  *   step (one cell)  src/collision/kernel_cumulant.jl:173-908 (pull gather, boundary rules,
- *                    solid reflection, WALE, cumulant relaxation, positivity limiter, Guo term)
+ *                    solid reflection, sponge blend, WALE, cumulant relaxation, positivity limiter,
+ *                    Guo term)
  *   WALE gradients   src/collision/physics_utils.jl:1981-2035, 2217-2233
  *   init             src/main.jl:546-572 (init_eq!)
  *   momentum exch.   src/octree_q19/src/stream.jl:248-256 (F += (f_i* + f_opp(i)(t+1)) c_i)
@@ -325,6 +326,8 @@ struct OrParams {
                               // 1 wall equilibrium (rho_f w_k), 2 own opposite population
                               // (BOUZIDI_FF_FALLBACK_MODE, config_loader.jl:2526-2527)
     double q_min;             // links with q <= q_min keep halfway bounce-back (Q_MIN_THRESHOLD)
+    int32_t sponge_blend;     // with a sponge field: 1 also blends the populations toward the inlet
+                              // equilibrium (SPONGE_BLEND_DISTRIBUTIONS, config_loader.jl:1312)
 };
 
 struct OrGrid {
@@ -337,6 +340,7 @@ struct OrGrid {
     const uint8_t* active;    // [nb]: 0 = not stepped (covered or ghost block of a refined level), null = all
     const uint16_t* qmap;     // [27*ncell] Float16 bits: q of the link from a cell along c_k, in units
                               // of |c_k| (bouzidi_setup.jl:139); null = halfway bounce-back only
+    const float* sponge;      // [ncell] sponge strength sp in [0, 1] per cell (sponge_arr), null = none
 };
 
 namespace {
@@ -344,12 +348,12 @@ namespace {
 template <class T>
 struct Coll {
     T tau, omega_bulk, omega3, omega4, lambda, c_wale, nu_sgs_bg, amr_scale, u_in, turb, F[3];
-    int adaptive, limiter, compressible;
+    int adaptive, limiter, compressible, sponge_blend;
     explicit Coll(const OrParams& p)
         : tau((T)p.tau), omega_bulk((T)p.omega_bulk), omega3((T)p.omega3), omega4((T)p.omega4),
           lambda((T)p.lambda), c_wale((T)p.c_wale), nu_sgs_bg((T)p.nu_sgs_bg), amr_scale((T)p.amr_sgs_scale),
           u_in((T)p.u_inlet), turb((T)p.inlet_turb), adaptive(p.adaptive_omega4), limiter(p.limiter),
-          compressible(p.compressible) {
+          compressible(p.compressible), sponge_blend(p.sponge_blend) {
         F[0] = (T)p.body_force[0]; F[1] = (T)p.body_force[1]; F[2] = (T)p.body_force[2];
     }
 };
@@ -388,11 +392,28 @@ T wale_nu(const T g[9], T c_wale) {
 // (rho, jx, jy, jz) their sums in gather order; nu_eddy the WALE viscosity before scaling.
 // Writes the post-collision populations into out and the macroscopic state into rho_o, u_o.
 template <class T>
-void collide(const T fin[27], T rho, T jx, T jy, T jz, T nu_eddy, const Coll<T>& c, T out[27], T& rho_o, T u_o[3]) {
+void collide(const T fin_in[27], T rho, T jx, T jy, T jz, T nu_eddy, const Coll<T>& c, T out[27], T& rho_o, T u_o[3],
+             T sp = T(0)) {
     rho = std::max(rho, T(0.01f));
     T inv_rho = T(1) / rho;
     T ux = jx * inv_rho, uy = jy * inv_rho, uz = jz * inv_rho;
-    // (sponge_arr = 0 everywhere: kernel_cumulant.jl:428-446 inactive)
+    T fin[27];
+    for (int k = 0; k < 27; ++k) fin[k] = fin_in[k];
+    // sponge (kernel_cumulant.jl:428-446): rho and u blended toward (1, u_inlet); with sponge_blend the
+    // populations toward the inlet equilibrium too
+    if (sp > T(0)) {
+        const T rho_target = T(1), ux_target = c.u_in;
+        rho = rho * (T(1) - sp) + rho_target * sp;
+        ux = ux * (T(1) - sp) + ux_target * sp;
+        uy = uy * (T(1) - sp);
+        uz = uz * (T(1) - sp);
+        if (c.sponge_blend == 1)
+            for (int k = 0; k < 27; ++k) {
+                const T feq_target = feq_auto(rho_target, ux_target, T(0), T(0), weight<T>(k), (T)LAT.cx[k], (T)LAT.cy[k],
+                                              (T)LAT.cz[k], c.compressible);
+                fin[k] = fin[k] * (T(1) - sp) + feq_target * sp;
+            }
+    }
     T Fx = c.F[0], Fy = c.F[1], Fz = c.F[2];
     T ux_eq = ux + T(0.5) * Fx * inv_rho;
     T uy_eq = uy + T(0.5) * Fy * inv_rho;
@@ -594,7 +615,7 @@ template <class T> inline T fma_(T a, T b, T c) { return std::fma(a, b, c); }
 // division fewer, no separate accumulation).  Every operation below is the one liboal executes, in
 // the same order; the result differs from the literal form by rounding only.
 template <class T>
-void collide_fast(T f[27], T nu_eddy, const Coll<T>& c, T& rho_o, T u_o[3]) {
+void collide_fast(T f[27], T nu_eddy, const Coll<T>& c, T& rho_o, T u_o[3], T sp = T(0)) {
     T A[27];
     // x-pass raw sums per row r = iy + 3 iz: (f-, f0, f+) -> (m0, d = f+ - f-, s = f- + f+)
     for (int r = 0; r < 9; ++r) {
@@ -621,7 +642,33 @@ void collide_fast(T f[27], T nu_eddy, const Coll<T>& c, T& rho_o, T u_o[3]) {
 
     const T rc = std::max(rho, T(0.01f));
     const T inv = T(1) / rc;
-    const T ux = jx * inv, uy = jy * inv, uz = jz * inv;
+    T ux = jx * inv, uy = jy * inv, uz = jz * inv;
+    // sponge: rho and u blended toward (1, u_inlet) as the literal form does; with sponge_blend the
+    // populations too, and the transform then sums the blended populations (their mass rr)
+    T rb = rc, rr = rho, invr = inv;
+    if (sp > T(0)) {
+        const T g = T(1) - sp;
+        rb = rc * g + T(1) * sp;
+        ux = ux * g + c.u_in * sp;
+        uy = uy * g;
+        uz = uz * g;
+        if (c.sponge_blend == 1) {
+            for (int k = 0; k < 27; ++k)
+                f[k] = f[k] * g + feq_auto(T(1), c.u_in, T(0), T(0), weight<T>(k), (T)LAT.cx[k], (T)LAT.cy[k], (T)LAT.cz[k],
+                                           c.compressible) * sp;
+            for (int r = 0; r < 9; ++r) {
+                const T s = f[3 * r] + f[3 * r + 2];
+                const T d = f[3 * r + 2] - f[3 * r];
+                A[3 * r] = s + f[3 * r + 1];
+                A[3 * r + 1] = d;
+                A[3 * r + 2] = s;
+            }
+            T z0[3];
+            for (int iz = 0; iz < 3; ++iz) z0[iz] = (A[9 * iz] + A[6 + 9 * iz]) + A[3 + 9 * iz];
+            rr = (z0[0] + z0[2]) + z0[1];
+            invr = T(1) / std::max(rr, T(0.01f));
+        }
+    }
     T uxe = ux, uye = uy, uze = uz;
     const bool forced = c.F[0] != T(0) || c.F[1] != T(0) || c.F[2] != T(0);
     if (forced) {
@@ -629,7 +676,7 @@ void collide_fast(T f[27], T nu_eddy, const Coll<T>& c, T& rho_o, T u_o[3]) {
         uye = fma_(T(0.5) * c.F[1], inv, uy);
         uze = fma_(T(0.5) * c.F[2], inv, uz);
     }
-    rho_o = rc; u_o[0] = ux; u_o[1] = uy; u_o[2] = uz;
+    rho_o = rb; u_o[0] = ux; u_o[1] = uy; u_o[2] = uz;
     T omega1, omega4_diag;
     rates(nu_eddy, c, omega1, omega4_diag);
 
@@ -655,14 +702,14 @@ void collide_fast(T f[27], T nu_eddy, const Coll<T>& c, T& rho_o, T u_o[3]) {
         for (int ma = 0; ma < 3; ++ma) pass(ma + 9 * iz, ma + 3 + 9 * iz, ma + 6 + 9 * iz, uye);
     for (int q = 0; q < 9; ++q) pass(q, q + 9, q + 18, uze);
 
-    // A[0] equals rho bit for bit (same sums in the same order): 1 / max(k000, 0.01) = inv
-    const T c200 = A[2] * inv, c020 = A[6] * inv, c002 = A[18] * inv;
-    const T c110 = A[4] * inv, c101 = A[10] * inv, c011 = A[12] * inv;
-    const T c210 = A[5] * inv, c120 = A[7] * inv, c201 = A[11] * inv;
-    const T c021 = A[15] * inv, c102 = A[19] * inv, c012 = A[21] * inv;
-    const T c111 = A[13] * inv;
-    const T c220n = A[8] * inv, c202n = A[20] * inv, c022n = A[24] * inv;
-    const T c211n = A[14] * inv, c121n = A[16] * inv, c112n = A[22] * inv;
+    // A[0] equals rr bit for bit (same sums in the same order): 1 / max(k000, 0.01) = invr
+    const T c200 = A[2] * invr, c020 = A[6] * invr, c002 = A[18] * invr;
+    const T c110 = A[4] * invr, c101 = A[10] * invr, c011 = A[12] * invr;
+    const T c210 = A[5] * invr, c120 = A[7] * invr, c201 = A[11] * invr;
+    const T c021 = A[15] * invr, c102 = A[19] * invr, c012 = A[21] * invr;
+    const T c111 = A[13] * invr;
+    const T c220n = A[8] * invr, c202n = A[20] * invr, c022n = A[24] * invr;
+    const T c211n = A[14] * invr, c121n = A[16] * invr, c112n = A[22] * invr;
     const T x110 = T(2) * c110, x101 = T(2) * c101, x011 = T(2) * c011;
     const T C220 = fma_(-x110, c110, fma_(-c200, c020, c220n));
     const T C202 = fma_(-x101, c101, fma_(-c200, c002, c202n));
@@ -720,7 +767,7 @@ void collide_fast(T f[27], T nu_eddy, const Coll<T>& c, T& rho_o, T u_o[3]) {
     e = fma_((T(g_plant == 2 ? 2 : 4) * y110) * s101, s011, e);
     const T s222 = e;
 
-    const T r = rho;
+    const T r = rr;
     A[0] = r;
     A[1] = T(0); A[3] = T(0); A[9] = T(0);
     A[2] = r * s200; A[6] = r * s020; A[18] = r * s002;
@@ -754,13 +801,13 @@ void collide_fast(T f[27], T nu_eddy, const Coll<T>& c, T& rho_o, T u_o[3]) {
         if (fmin < T(0)) {
             T alpha = T(1);
             for (int k = 0; k < 27; ++k) {
-                T feq = feq_auto(rc, uxe, uye, uze, weight<T>(k), (T)LAT.cx[k], (T)LAT.cy[k], (T)LAT.cz[k], c.compressible);
+                T feq = feq_auto(rb, uxe, uye, uze, weight<T>(k), (T)LAT.cx[k], (T)LAT.cy[k], (T)LAT.cz[k], c.compressible);
                 T fneq = A[k] - feq;
                 if (fneq < T(0) && feq > T(1e-15f)) alpha = std::min(alpha, feq / (feq - fneq + T(1e-15f)));
             }
             alpha = std::max(alpha, T(0));
             for (int k = 0; k < 27; ++k) {
-                T feq = feq_auto(rc, uxe, uye, uze, weight<T>(k), (T)LAT.cx[k], (T)LAT.cy[k], (T)LAT.cz[k], c.compressible);
+                T feq = feq_auto(rb, uxe, uye, uze, weight<T>(k), (T)LAT.cx[k], (T)LAT.cy[k], (T)LAT.cz[k], c.compressible);
                 A[k] = feq + alpha * (A[k] - feq);
             }
         }
@@ -964,11 +1011,12 @@ void step_impl(const OrGrid& g, const OrParams& prm, const typename P::st* fin, 
                     }
 
                     T out[27], rho_o, u_o[3];
+                    const T sp = g.sponge ? (T)g.sponge[own] : T(0);
                     if (prm.arith == 1) {
                         for (int k = 0; k < 27; ++k) out[k] = fs[k];
-                        collide_fast(out, nu_eddy, cp, rho_o, u_o);
+                        collide_fast(out, nu_eddy, cp, rho_o, u_o, sp);
                     } else {
-                        collide(fs, rho, jx, jy, jz, nu_eddy, cp, out, rho_o, u_o);
+                        collide(fs, rho, jx, jy, jz, nu_eddy, cp, out, rho_o, u_o, sp);
                     }
                     if (vel_out) { vel_out[own] = u_o[0]; vel_out[ncell + own] = u_o[1]; vel_out[2 * ncell + own] = u_o[2]; }
                     if (rho_out) rho_out[own] = rho_o;

@@ -79,6 +79,7 @@ class OrParams(C.Structure):
         ("adaptive_omega4", C.c_int32), ("limiter", C.c_int32), ("compressible", C.c_int32),
         ("ymin_wall_bc", C.c_int32), ("zwall_bc", C.c_int32), ("bb_mode", C.c_int32), ("time_step_seed", C.c_int32),
         ("arith", C.c_int32), ("rounding", C.c_int32), ("bfl_fallback", C.c_int32), ("q_min", C.c_double),
+        ("sponge_blend", C.c_int32),
     ]
 
 
@@ -86,7 +87,7 @@ class OrGrid(C.Structure):
     _fields_ = [
         ("bs", C.c_int32), ("nb", C.c_int32), ("bcoord", C.POINTER(C.c_int32)), ("nbr", C.POINTER(C.c_int32)),
         ("n", C.c_int32 * 3), ("periodic", C.c_int32 * 3), ("solid", C.POINTER(C.c_uint8)),
-        ("active", C.POINTER(C.c_uint8)), ("qmap", C.POINTER(C.c_uint16)),
+        ("active", C.POINTER(C.c_uint8)), ("qmap", C.POINTER(C.c_uint16)), ("sponge", C.POINTER(C.c_float)),
     ]
 
 
@@ -140,7 +141,8 @@ def f32(v: float) -> float:
 def make_params(tau, *, omega_bulk=1.0, omega_3=1.0, omega_4=1.0, lambda_=1.0 / 6.0, adaptive_omega4=1, limiter=1,
                 compressible=0, c_wale=0.0, nu_sgs_background=0.0, amr_sgs_scale=1.0, u_inlet=0.0,
                 inlet_turbulence=0.0, body_force=(0.0, 0.0, 0.0), ymin_wall_bc=0, zwall_bc=0, bb_mode=1,
-                time_step=0, arith=LITERAL, rounding=STOCHASTIC, bfl_fallback=0, q_min=0.001) -> OrParams:
+                time_step=0, arith=LITERAL, rounding=STOCHASTIC, bfl_fallback=0, q_min=0.001,
+                sponge_blend=0) -> OrParams:
     """Step parameters; arith LITERAL restates the reference arithmetic operation for operation, FAST the
     library's fused arithmetic (same mathematics, fused multiply-adds, rho and j from the transform's
     partial sums).  rounding: how the 16-bit forms store a step's populations (STOCHASTIC, the
@@ -158,14 +160,16 @@ def make_params(tau, *, omega_bulk=1.0, omega_3=1.0, omega_4=1.0, lambda_=1.0 / 
     p.rounding = rounding
     p.bfl_fallback = bfl_fallback
     p.q_min = f32(q_min)
+    p.sponge_blend = sponge_blend
     return p
 
 
 class Grid:
     """The oracle's view of a block grid (any object with n, periodic, coords, neighbor, solid, bs).
-    qmap: [27, n_cells] float16 wall distances of the links (interpolated bounce-back), or None."""
+    qmap: [27, n_cells] float16 wall distances of the links (interpolated bounce-back), or None.
+    sponge: [n_cells] sponge strength per cell (float32), or None."""
 
-    def __init__(self, g, qmap=None):
+    def __init__(self, g, qmap=None, sponge=None):
         self.src = g
         self.bs = int(g.bs)
         self.nb = int(g.coords.shape[0])
@@ -193,6 +197,10 @@ class Grid:
                 raise ValueError(f"qmap must be [27, {self.ncell}]")
             self.qmap = np.ascontiguousarray(q)
         s.qmap = None if self.qmap is None else self.qmap.ctypes.data_as(C.POINTER(C.c_uint16))
+        self.sponge = None if sponge is None else np.ascontiguousarray(sponge, dtype=np.float32)
+        if self.sponge is not None and self.sponge.shape != (self.ncell,):
+            raise ValueError(f"sponge must be [{self.ncell}]")
+        s.sponge = None if self.sponge is None else self.sponge.ctypes.data_as(C.POINTER(C.c_float))
         self.c = s
 
 

@@ -1,0 +1,82 @@
+"""The sponge blend (kernel_cumulant.jl:428-446) on the GPU against the oracle: a channel with an obstacle
+whose outlet quarter carries a tanh sponge ramp, in both blend modes and every storage form, bit for bit
+(the FP64 reductions to summation order); WALE reads the blended velocity of the step before."""
+import numpy as np
+import pytest
+
+from openaerolab_alpha_01_amd import _lib
+from openaerolab_alpha_01_amd.geometry import sphere_mask, uniform_grid
+
+from test_gpu_parity import compare, fields, gpu_plan
+
+pytestmark = pytest.mark.gpu
+
+
+def _sponge(grid):
+    """a tanh ramp over the last quarter of x (the reference's default profile shape), zero elsewhere"""
+    x = grid.cell_coords()[:, 0].astype(np.float64)
+    nx = grid.n[0]
+    x0 = 0.75 * nx
+    s = np.where(x >= x0, np.tanh(3.0 * (x - x0) / (nx - x0)), 0.0)
+    s[~grid.inside()] = 0.0
+    return s.astype(np.float32)
+
+
+def _run(oracle, storage, blend, okw, steps=5):
+    grid = uniform_grid((80, 32, 40))
+    grid.solid = sphere_mask(grid, (24, 16, 20), 5.0)
+    sp = _sponge(grid)
+    rho, u = fields(grid, seed=13, u0=(okw.get("u_inlet", 0.0), 0, 0), amp=0.03)
+    G = oracle.Grid(grid, sponge=sp)
+    f = oracle.init_eq(storage, G, rho, u)
+    wt = np.float64 if storage == "f64" else np.float32
+    vel = u.astype(wt) if okw.get("c_wale", 0) else None
+    for t in range(steps):
+        p = oracle.make_params(time_step=t, arith=oracle.FAST, sponge_blend=blend, **okw)
+        f, _, mac = oracle.step(storage, G, p, f, vel_in=vel, want_macro=vel is not None)
+        if vel is not None:
+            vel = mac[1]
+    plan = gpu_plan(grid, storage, okw)
+    before = plan.level_info()
+    plan.set_sponge(sp, blend_distributions=bool(blend))
+    after = plan.level_info()
+    # the blocks that hold a sponge left the regular list for the general kernels
+    assert after["n_regular"] < before["n_regular"]
+    assert after["n_boundary_regular"] + after["n_boundary"] > before["n_boundary_regular"] + before["n_boundary"]
+    plan.init_equilibrium(rho, u)
+    for t in range(steps):
+        plan.level_step(0, t)
+    compare(oracle, plan, storage, f, grid)
+    plan.close()
+
+
+@pytest.mark.parametrize("blend", [0, 1])
+@pytest.mark.parametrize("storage", ["f32", "f64", "f16s", "bf16s"])
+def test_sponge_parity(oracle, storage, blend):
+    _run(oracle, storage, blend, dict(tau=0.54, u_inlet=0.05))
+
+
+@pytest.mark.parametrize("blend", [0, 1])
+def test_sponge_with_wale_and_force(oracle, blend):
+    _run(oracle, "f32", blend, dict(tau=0.53, u_inlet=0.05, c_wale=0.2, body_force=(2e-6, 0.0, 1e-6)))
+
+
+def test_sponge_errors_and_removal(oracle):
+    grid = uniform_grid((48, 32, 32))
+    okw = dict(tau=0.55, u_inlet=0.04)
+    plan = gpu_plan(grid, "f32", okw)
+    base = plan.level_info()
+    bad = np.full(grid.n_cells, 1.5, dtype=np.float32)
+    with pytest.raises(_lib.OalError) as e:
+        plan.set_sponge(bad)
+    assert e.value.code == _lib.OAL_ERR_ARG
+    plan.set_sponge(_sponge(grid))
+    assert plan.level_info() != base
+    plan.set_sponge(None)
+    assert plan.level_info() == base
+    plan.close()
+    aa = gpu_plan(grid, "f32", okw, stream="aa")
+    with pytest.raises(_lib.OalError) as e:
+        aa.set_sponge(_sponge(grid))
+    assert e.value.code == _lib.OAL_ERR_UNSUPPORTED
+    aa.close()
