@@ -229,6 +229,61 @@ int oal_step(OalPlan* plan, int n_steps, int32_t first_time_step, int flags, voi
  * reference collects the same sum while gathering the next step. */
 int oal_force(OalPlan* plan, int level, double* force);
 
+/* ---- Surface-stress forces on a triangle mesh (src/forces/surface.jl) ----------------------------
+ * The reference's own drag path: each triangle probes density and velocity along its normal, turns
+ * them into a pressure and a wall-law shear stress, and the stresses are integrated over the mesh.
+ *
+ * oal_set_surface uploads the mesh of a level (gpu_mesh of the reference): n_tri triangles as Float32
+ * arrays centers[3 * n_tri] and normals[3 * n_tri] (x of every triangle, then y, then z; unit outward
+ * normals) and areas[n_tri]; NULL or n_tri 0 removes it.  The level must have one domain.
+ *
+ * oal_surface_forces maps the stresses (map_surface_stresses! with map_stresses_kernel!, the default
+ * Prandtl force method, surface.jl:636-1033 and 1333-1403, compute_stress_from_cell :180-269) and
+ * integrates them (integrate_surface_forces!, :1265-1331 and 1452-1548), on `stream`, and waits.
+ *   Probe field: the density and velocity of the level's current populations (the moments of
+ *   oal_download_macro, rho clamped to >= 0.01 as the reference's kernel clamps it, then rounded to
+ *   FP32); cells outside the extent, of absent blocks or solid are not sampled.  The mapping arithmetic
+ *   is FP32 in the reference's order; each triangle's force and moment and their sums are FP64, summed
+ *   in a fixed order (one result for one state, whatever the scheduling; the reference adds FP32
+ *   atomics).  The positions are in the units of dx with the level's cell (i, j, k) centred at
+ *   ((i + 1/2) dx, (j + 1/2) dx, (k + 1/2) dx) after mesh_offset is added to the centres.
+ * oal_download_surface copies the per-triangle maps of the last oal_surface_forces (force_data's
+ * pressure_map, shear_{x,y,z}_map, normal_sign_map and wall_dist_map; any pointer may be NULL):
+ * pressure NaN where no fluid probe was found. */
+typedef struct {
+    double dx;                 /* cell size of the level in mesh units (level.dx) */
+    double mesh_offset[3];     /* added to the triangle centres (params.mesh_offset) */
+    double pressure_scale;     /* rho_physical * velocity_scale^2 (surface.jl:1343); 1 = lattice units */
+    double stress_scale;       /* the same scale in the reference (surface.jl:1344) */
+    double tau;                /* molecular relaxation time of the level (level.tau) */
+    double c_wale;             /* C_SMAGO (src/io/config_loader.jl:82, 0.2): wall-law SGS estimate */
+    double nu_sgs_bg;          /* NU_SGS_BACKGROUND (config_loader.jl:88, 0) */
+    double cs2;                /* CS2_PHYSICS, 1/3 */
+    double moment_center[3];   /* force_data.moment_center */
+    double rho_ref, u_ref, area_ref, chord_ref;   /* coefficient references (force_data) */
+    int32_t search_radius;     /* probes along the normal (main.jl:1296: max(8, ceil(0.12 / dx))) */
+    int32_t recover_normal;    /* 1: probe the opposite side when the given one finds no fluid */
+    int32_t symmetric;         /* 1: half model about y = 0 (force_data.symmetric): Fx, Fz, My doubled */
+    int32_t reserved;
+} OalSurfaceParams;
+
+typedef struct {
+    double F_pressure[3];      /* Fx_pressure, Fy_pressure, Fz_pressure */
+    double F_viscous[3];       /* Fx_viscous, ... */
+    double F[3];               /* their sum */
+    double M[3];               /* moments about moment_center */
+    double Cd, Cl, Cs;         /* Fx, Fz, Fy over q_inf area_ref, q_inf = rho_ref u_ref^2 / 2 (0 when that is <= 1e-10) */
+    double Cm[3];              /* M over q_inf area_ref chord_ref */
+    int64_t n_mapped;          /* triangles with a finite pressure and stress (those integrated) */
+} OalSurfaceForces;
+
+int oal_set_surface(OalPlan* plan, int level, int64_t n_tri, const float* centers, const float* normals,
+                    const float* areas);
+int oal_surface_forces(OalPlan* plan, int level, const OalSurfaceParams* params, OalSurfaceForces* out,
+                       void* stream);
+int oal_download_surface(OalPlan* plan, int level, float* pressure, float* shear, float* normal_sign,
+                         float* wall_dist);
+
 /* Interpolated (curved-wall) bounce-back on the solid links of a level: the linear Bouzidi rule
  * (BFL) of bouzidi_correction_kernel_fixed! (src/geometry/bouzidi_kernel.jl:150-380, launched by
  * apply_bouzidi_correction!, :382-432, from src/collision/timestep.jl:1752-1770) with its D3Q27

@@ -30,6 +30,7 @@ EXPORTS = [
     "oal_step", "oal_force", "oal_mass", "oal_download_macro", "oal_guard", "oal_sync", "oal_time_steps",
     "oal_get_tuning", "oal_set_tuning", "oal_autotune", "oal_copy_bandwidth", "oal_set_bouzidi",
     "oal_write_raw_state", "oal_read_raw_state", "oal_mass_noalloc", "oal_set_sponge",
+    "oal_set_surface", "oal_surface_forces", "oal_download_surface",
 ]
 
 
@@ -78,6 +79,23 @@ class OalLevelInfo(C.Structure):
 class OalTuning(C.Structure):
     _fields_ = [("pipe_planes", C.c_int32), ("overlap", C.c_int32), ("face_kernel", C.c_int32),
                 ("graph", C.c_int32), ("pair_kernel", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
+class OalSurfaceParams(C.Structure):
+    _fields_ = [
+        ("dx", C.c_double), ("mesh_offset", C.c_double * 3), ("pressure_scale", C.c_double),
+        ("stress_scale", C.c_double), ("tau", C.c_double), ("c_wale", C.c_double), ("nu_sgs_bg", C.c_double),
+        ("cs2", C.c_double), ("moment_center", C.c_double * 3), ("rho_ref", C.c_double), ("u_ref", C.c_double),
+        ("area_ref", C.c_double), ("chord_ref", C.c_double), ("search_radius", C.c_int32),
+        ("recover_normal", C.c_int32), ("symmetric", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class OalSurfaceForces(C.Structure):
+    _fields_ = [
+        ("F_pressure", C.c_double * 3), ("F_viscous", C.c_double * 3), ("F", C.c_double * 3), ("M", C.c_double * 3),
+        ("Cd", C.c_double), ("Cl", C.c_double), ("Cs", C.c_double), ("Cm", C.c_double * 3), ("n_mapped", C.c_int64),
+    ]
 
 
 OAL_ALL_LEVELS = -1
@@ -134,6 +152,9 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         "oal_set_bouzidi": ([vp, C.c_int, vp, i32, C.c_float], C.c_int),
         "oal_write_raw_state": ([vp, C.c_int, i32, C.c_double, C.c_char_p], C.c_int),
         "oal_read_raw_state": ([vp, C.c_int, C.c_char_p, C.POINTER(C.c_int32), dp], C.c_int),
+        "oal_set_surface": ([vp, C.c_int, C.c_int64, vp, vp, vp], C.c_int),
+        "oal_surface_forces": ([vp, C.c_int, C.POINTER(OalSurfaceParams), C.POINTER(OalSurfaceForces), vp], C.c_int),
+        "oal_download_surface": ([vp, C.c_int, vp, vp, vp, vp], C.c_int),
     }
     for name, (argt, rest) in sig.items():
         fn = getattr(lib, name)

@@ -19,7 +19,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "liboal.so"
 # the step kernels of each storage form sit in their own translation unit and compile in parallel
 SOURCES = [CSRC / "oal_step_f32.hip", CSRC / "oal_step_f16s.hip", CSRC / "oal_step_bf16s.hip",
-           CSRC / "oal_step_f64.hip", CSRC / "oal_kernels.hip", CSRC / "oal_plan.hip"]
+           CSRC / "oal_step_f64.hip", CSRC / "oal_kernels.hip", CSRC / "oal_surface.hip", CSRC / "oal_plan.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

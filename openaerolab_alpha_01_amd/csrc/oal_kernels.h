@@ -166,4 +166,30 @@ hipError_t launch_guard(int storage, const void* f, const uint8_t* solid, const 
                         const uint32_t* bflags, const DevDomain* doms, uint32_t nb, float rho_min, float u_max,
                         unsigned long long* count, hipStream_t stream);
 
+// Surface-stress map and its integration over a triangle mesh (oal_surface.hip)
+constexpr int SURF_WG = 256;      // triangles per workgroup; one FP64 partial record per workgroup
+constexpr int SURF_SUMS = 10;     // Fp[3], Fv[3], M[3], mapped count
+struct SurfaceArgs {
+    const void* f;               // populations in stored form
+    int permute;                 // AA pattern: directions stored reversed
+    const uint8_t* solid;        // [ncell] or null
+    const int32_t* bmap;         // [nbx * nby * nbz] block of each block position of the domain, -1 = absent
+    int32_t nbx, nby, nbz;
+    int32_t ext[3];              // domain extent in cells
+    int64_t n_tri;
+    const float* cen;            // [3 * n_tri] centres
+    const float* nrm;            // [3 * n_tri] normals
+    const float* area;           // [n_tri]
+    float dx, off[3], pressure_scale, stress_scale, tau, c_wale, nu_sgs_bg, cs2;
+    double ref[3];               // moment centre
+    int32_t search_radius, recover_normal;
+    float* p_map;                // [n_tri]
+    float* s_map;                // [3 * n_tri]
+    float* sign_map;             // [n_tri]
+    float* wd_map;               // [n_tri]
+    double* part;                // [SURF_SUMS * n_groups]
+    double* out;                 // [SURF_SUMS]
+};
+hipError_t launch_surface(int storage, const SurfaceArgs& a, hipStream_t stream);
+
 }  // namespace oal

@@ -154,6 +154,19 @@ struct Level {
     DevBuf<int32_t> sslot;
     int32_t sponge_blend = 0;
     bool has_sponge = false;
+    // surface mesh of oal_set_surface (centres and normals [3][n_tri], areas), the block of each block
+    // position of the domain, and the per-triangle maps and FP64 records of the last oal_surface_forces
+    int64_t n_tri = 0;
+    bool surface_mapped = false;
+    DevBuf<float> s_cen, s_nrm, s_area, s_p, s_shear, s_sign, s_wd;
+    DevBuf<int32_t> s_bmap;
+    DevBuf<double> s_part, s_out;
+    int32_t s_nblk[3] = {0, 0, 0};
+    void release_surface() {
+        s_cen.release(); s_nrm.release(); s_area.release(); s_p.release(); s_shear.release(); s_sign.release();
+        s_wd.release(); s_bmap.release(); s_part.release(); s_out.release();
+        n_tri = 0; surface_mapped = false;
+    }
     bool have_f = false;
     bool force_valid = false;
     int aa_phase = 0;                // AA: 0 = populations in stored form (next step odd), 1 = after an odd step
@@ -165,6 +178,7 @@ struct Level {
         g_src.release(); g_dst.release(); g_slot.release(); g_w.release(); g_m.release();
         r_src.release(); r_dst.release(); r_slot.release(); r_w.release(); r_m.release();
         rf_src.release(); rf_dst.release(); rf_slot.release(); rf_w.release();
+        release_surface();
     }
 };
 
@@ -1830,6 +1844,127 @@ int oal_set_sponge(OalPlan* p, int level, const float* sponge, int32_t blend_dis
         return rc;
     L.has_sponge = !vals.empty();
     L.sponge_blend = blend_distributions;
+    return OAL_OK;
+}
+
+// ---- Surface-stress forces (src/forces/surface.jl) ---------------------------------------------------
+int oal_set_surface(OalPlan* p, int level, int64_t n_tri, const float* centers, const float* normals,
+                    const float* areas) {
+    if (int rc = check_level(p, level)) return rc;
+    Level& L = p->levels[level];
+    if (n_tri < 0) return fail(OAL_ERR_ARG, "negative triangle count");
+    OAL_HIP(hipSetDevice(p->device));
+    if (n_tri == 0 || !centers) {
+        OAL_HIP(hipDeviceSynchronize());
+        L.release_surface();
+        return OAL_OK;
+    }
+    if (!normals || !areas) return fail(OAL_ERR_ARG, "NULL normals or areas");
+    if (n_tri > ((int64_t)1 << 31) - 1) return fail(OAL_ERR_UNSUPPORTED, "more than 2^31 - 1 triangles");
+    if (L.n_domains != 1) return fail(OAL_ERR_UNSUPPORTED, "the surface forces need a level of one domain");
+    for (int64_t i = 0; i < 3 * n_tri; ++i)
+        if (!std::isfinite(centers[i]) || !std::isfinite(normals[i])) return fail(OAL_ERR_ARG, "non-finite triangle centre or normal");
+    for (int64_t i = 0; i < n_tri; ++i)
+        if (!std::isfinite(areas[i]) || areas[i] < 0.0f) return fail(OAL_ERR_ARG, "triangle areas must be finite and >= 0");
+    // block of each block position of the domain (the reference's block_ptr), caller's blocks only
+    const DevDomain& D = L.doms_h[0];
+    int32_t nblk[3];
+    for (int a = 0; a < 3; ++a) nblk[a] = (D.ext[a] + BS - 1) / BS;
+    std::vector<int32_t> bmap((size_t)nblk[0] * nblk[1] * nblk[2], -1);
+    for (uint32_t b = 0; b < L.n_caller_blocks; ++b) {
+        const int bx = L.map_h[3 * b] - 1, by = L.map_h[3 * b + 1] - 1, bz = L.map_h[3 * b + 2] - 1;
+        if (bx < 0 || by < 0 || bz < 0 || bx >= nblk[0] || by >= nblk[1] || bz >= nblk[2]) continue;
+        bmap[((size_t)bz * nblk[1] + by) * nblk[0] + bx] = (int32_t)b;
+    }
+    OAL_HIP(hipDeviceSynchronize());
+    L.release_surface();
+    const size_t n = (size_t)n_tri, ngroups = (n + SURF_WG - 1) / SURF_WG;
+    auto put = [](DevBuf<float>& d, const float* h, size_t count) -> hipError_t {
+        hipError_t e = d.alloc(count);
+        return e == hipSuccess ? hipMemcpy(d.p, h, count * sizeof(float), hipMemcpyHostToDevice) : e;
+    };
+    hipError_t e = put(L.s_cen, centers, 3 * n);
+    if (e == hipSuccess) e = put(L.s_nrm, normals, 3 * n);
+    if (e == hipSuccess) e = put(L.s_area, areas, n);
+    if (e == hipSuccess) e = L.s_p.alloc(n);
+    if (e == hipSuccess) e = L.s_shear.alloc(3 * n);
+    if (e == hipSuccess) e = L.s_sign.alloc(n);
+    if (e == hipSuccess) e = L.s_wd.alloc(n);
+    if (e == hipSuccess) e = L.s_part.alloc(SURF_SUMS * ngroups);
+    if (e == hipSuccess) e = L.s_out.alloc(SURF_SUMS);
+    if (e != hipSuccess) {
+        L.release_surface();
+        return fail(OAL_ERR_HIP, std::string("oal_set_surface: ") + hipGetErrorString(e));
+    }
+    if (int rc = upload(L.s_bmap, bmap)) { L.release_surface(); return rc; }
+    for (int a = 0; a < 3; ++a) L.s_nblk[a] = nblk[a];
+    L.n_tri = n_tri;
+    return OAL_OK;
+}
+
+int oal_surface_forces(OalPlan* p, int level, const OalSurfaceParams* sp, OalSurfaceForces* out, void* stream) {
+    if (int rc = check_level(p, level)) return rc;
+    if (!sp || !out) return fail(OAL_ERR_ARG, "NULL parameters or output");
+    Level& L = p->levels[level];
+    if (L.n_tri <= 0) return fail(OAL_ERR_STATE, "no surface mesh on this level (oal_set_surface)");
+    if (int rc = require_plain_state(p, L)) return rc;
+    if (!(sp->dx > 0.0 && std::isfinite(sp->dx))) return fail(OAL_ERR_ARG, "dx must be finite and > 0");
+    if (sp->search_radius < 1 || sp->search_radius > 64) return fail(OAL_ERR_ARG, "search_radius must lie in [1, 64]");
+    if (sp->recover_normal != 0 && sp->recover_normal != 1) return fail(OAL_ERR_ARG, "recover_normal must be 0 or 1");
+    if (sp->symmetric != 0 && sp->symmetric != 1) return fail(OAL_ERR_ARG, "symmetric must be 0 or 1");
+    OAL_HIP(hipSetDevice(p->device));
+    hipStream_t s = pick_stream(p, stream);
+    if (int rc = ml_sync_covered(p, s)) return rc;
+    const DevDomain& D = L.doms_h[0];
+    SurfaceArgs a{};
+    a.f = L.fA.p;
+    a.permute = permute_of(p);
+    a.solid = L.solid.p;
+    a.bmap = L.s_bmap.p;
+    a.nbx = L.s_nblk[0]; a.nby = L.s_nblk[1]; a.nbz = L.s_nblk[2];
+    for (int d = 0; d < 3; ++d) a.ext[d] = D.ext[d];
+    a.n_tri = L.n_tri;
+    a.cen = L.s_cen.p; a.nrm = L.s_nrm.p; a.area = L.s_area.p;
+    a.dx = (float)sp->dx;
+    for (int d = 0; d < 3; ++d) { a.off[d] = (float)sp->mesh_offset[d]; a.ref[d] = (double)(float)sp->moment_center[d]; }
+    a.pressure_scale = (float)sp->pressure_scale; a.stress_scale = (float)sp->stress_scale;
+    a.tau = (float)sp->tau; a.c_wale = (float)sp->c_wale; a.nu_sgs_bg = (float)sp->nu_sgs_bg; a.cs2 = (float)sp->cs2;
+    a.search_radius = sp->search_radius; a.recover_normal = sp->recover_normal;
+    a.p_map = L.s_p.p; a.s_map = L.s_shear.p; a.sign_map = L.s_sign.p; a.wd_map = L.s_wd.p;
+    a.part = L.s_part.p; a.out = L.s_out.p;
+    OAL_HIP(launch_surface(p->storage, a, s));
+    double h[SURF_SUMS];
+    OAL_HIP(hipMemcpyAsync(h, L.s_out.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    OAL_HIP(hipStreamSynchronize(s));
+    L.surface_mapped = true;
+    // integrate_surface_forces! (surface.jl:1485-1547): half model, totals and coefficients
+    double Fp[3] = {h[0], h[1], h[2]}, Fv[3] = {h[3], h[4], h[5]}, M[3] = {h[6], h[7], h[8]};
+    if (sp->symmetric) {
+        Fp[0] *= 2.0; Fp[2] *= 2.0; Fv[0] *= 2.0; Fv[2] *= 2.0; M[1] *= 2.0;
+        Fp[1] = 0.0; Fv[1] = 0.0; M[0] = 0.0; M[2] = 0.0;
+    }
+    *out = OalSurfaceForces{};
+    for (int d = 0; d < 3; ++d) {
+        out->F_pressure[d] = Fp[d]; out->F_viscous[d] = Fv[d]; out->F[d] = Fp[d] + Fv[d]; out->M[d] = M[d];
+    }
+    const double q_inf = 0.5 * sp->rho_ref * sp->u_ref * sp->u_ref;
+    const double F_ref = q_inf * sp->area_ref, M_ref = F_ref * sp->chord_ref;
+    if (F_ref > 1e-10) { out->Cd = out->F[0] / F_ref; out->Cl = out->F[2] / F_ref; out->Cs = out->F[1] / F_ref; }
+    if (M_ref > 1e-10) for (int d = 0; d < 3; ++d) out->Cm[d] = M[d] / M_ref;
+    out->n_mapped = (int64_t)h[9];
+    return OAL_OK;
+}
+
+int oal_download_surface(OalPlan* p, int level, float* pressure, float* shear, float* normal_sign, float* wall_dist) {
+    if (int rc = check_level(p, level)) return rc;
+    Level& L = p->levels[level];
+    if (!L.surface_mapped) return fail(OAL_ERR_STATE, "no surface map yet (oal_surface_forces)");
+    OAL_HIP(hipSetDevice(p->device));
+    const size_t n = (size_t)L.n_tri;
+    if (pressure) OAL_HIP(hipMemcpy(pressure, L.s_p.p, n * 4, hipMemcpyDeviceToHost));
+    if (shear) OAL_HIP(hipMemcpy(shear, L.s_shear.p, 3 * n * 4, hipMemcpyDeviceToHost));
+    if (normal_sign) OAL_HIP(hipMemcpy(normal_sign, L.s_sign.p, n * 4, hipMemcpyDeviceToHost));
+    if (wall_dist) OAL_HIP(hipMemcpy(wall_dist, L.s_wd.p, n * 4, hipMemcpyDeviceToHost));
     return OAL_OK;
 }
 

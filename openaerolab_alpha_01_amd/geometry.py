@@ -135,6 +135,47 @@ def sphere_mask(grid: BlockGrid, center, radius: float) -> np.ndarray:
     return (m & grid.inside()).astype(np.uint8)
 
 
+def icosphere(center, radius: float, subdivisions: int = 3):
+    """Triangulated sphere surface, the mesh the reference reads from an STL file (gpu_mesh of
+    src/forces/surface.jl): an icosahedron whose edges are split `subdivisions` times (20 * 4^s
+    triangles), vertices on the sphere.  Returns Float32 centres [3, n] (triangle centroids),
+    unit outward normals [3, n] and areas [n], computed in FP64."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    v = [np.array(p, dtype=np.float64) / np.linalg.norm(p) for p in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(int(subdivisions)):
+        mid: dict = {}
+
+        def midpoint(a: int, b: int) -> int:
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                m = v[a] + v[b]
+                v.append(m / np.linalg.norm(m))
+                mid[key] = len(v) - 1
+            return mid[key]
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = np.array(v) * float(radius)
+    F = np.array(faces)
+    p0, p1, p2 = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    cr = np.cross(p1 - p0, p2 - p0)
+    area = 0.5 * np.linalg.norm(cr, axis=1)
+    nrm = cr / np.linalg.norm(cr, axis=1)[:, None]
+    cen = (p0 + p1 + p2) / 3.0
+    flip = (nrm * cen).sum(1) < 0.0
+    nrm[flip] *= -1.0
+    cen = cen + np.asarray(center, dtype=np.float64)[None, :]
+    return (np.ascontiguousarray(cen.T, dtype=np.float32), np.ascontiguousarray(nrm.T, dtype=np.float32),
+            area.astype(np.float32))
+
+
 def link_solid(grid: BlockGrid) -> np.ndarray:
     """[27, n_cells] bool: the cell x + c_k is a solid cell of the level (inside the extent, in a present
     block; periodic axes wrap through the neighbour table).  Only blocks holding solid cells and their

@@ -339,6 +339,67 @@ class Plan:
               "oal_download_macro")
         return rho, u
 
+    # -- surface-stress forces (src/forces/surface.jl) ------------------------------------------------
+    def set_surface(self, centers, normals, areas, level: int = 0) -> None:
+        """Upload a triangle mesh (the reference's gpu_mesh): centers and normals [3, n_tri], areas
+        [n_tri], Float32; None removes it."""
+        if centers is None:
+            check(self.lib.oal_set_surface(self.h, level, 0, None, None, None), "oal_set_surface")
+            self._n_tri = getattr(self, "_n_tri", {})
+            self._n_tri.pop(level, None)
+            return
+        c = np.ascontiguousarray(centers, dtype=np.float32)
+        nrm = np.ascontiguousarray(normals, dtype=np.float32)
+        a = np.ascontiguousarray(areas, dtype=np.float32)
+        n = a.shape[0]
+        if a.ndim != 1 or c.shape != (3, n) or nrm.shape != (3, n):
+            raise ValueError("centers and normals must be [3, n_tri], areas [n_tri]")
+        check(self.lib.oal_set_surface(self.h, level, n, c.ctypes.data, nrm.ctypes.data, a.ctypes.data),
+              "oal_set_surface")
+        self._n_tri = getattr(self, "_n_tri", {})
+        self._n_tri[level] = n
+
+    def surface_forces(self, level: int = 0, *, dx: float = 1.0, mesh_offset=(0.0, 0.0, 0.0),
+                       pressure_scale: float = 1.0, stress_scale: float | None = None, tau: float | None = None,
+                       c_wale: float = 0.2, nu_sgs_bg: float = 0.0, cs2: float = 1.0 / 3.0,
+                       search_radius: int = 8, recover_normal: bool = True, symmetric: bool = False,
+                       moment_center=(0.0, 0.0, 0.0), rho_ref: float = 1.0, u_ref: float = 0.0,
+                       area_ref: float = 1.0, chord_ref: float = 1.0) -> dict:
+        """map_surface_stresses! + integrate_surface_forces! (src/forces/surface.jl:1333-1548) on the
+        level's current state: pressure and wall-law shear per triangle, integrated in FP64.  tau
+        defaults to the level's first domain's; stress_scale to pressure_scale (as the reference sets
+        it).  Returns the force_data fields: F_pressure, F_viscous, F, M (3-vectors), Cd, Cl, Cs, Cm
+        and n_mapped."""
+        p = _lib.OalSurfaceParams()
+        p.dx = dx
+        p.mesh_offset[:] = [float(v) for v in mesh_offset]
+        p.pressure_scale = pressure_scale
+        p.stress_scale = pressure_scale if stress_scale is None else stress_scale
+        p.tau = self.levels[level].domains[0].tau if tau is None else tau
+        p.c_wale, p.nu_sgs_bg, p.cs2 = c_wale, nu_sgs_bg, cs2
+        p.moment_center[:] = [float(v) for v in moment_center]
+        p.rho_ref, p.u_ref, p.area_ref, p.chord_ref = rho_ref, u_ref, area_ref, chord_ref
+        p.search_radius, p.recover_normal, p.symmetric = int(search_radius), int(bool(recover_normal)), int(bool(symmetric))
+        out = _lib.OalSurfaceForces()
+        check(self.lib.oal_surface_forces(self.h, level, C.byref(p), C.byref(out), None), "oal_surface_forces")
+        v3 = lambda a: np.array(list(a), dtype=np.float64)
+        return {"F_pressure": v3(out.F_pressure), "F_viscous": v3(out.F_viscous), "F": v3(out.F), "M": v3(out.M),
+                "Cd": out.Cd, "Cl": out.Cl, "Cs": out.Cs, "Cm": v3(out.Cm), "n_mapped": int(out.n_mapped)}
+
+    def download_surface(self, level: int = 0) -> dict:
+        """Per-triangle maps of the last surface_forces: pressure [n], shear [3, n], normal_sign [n],
+        wall_dist [n] (Float32; pressure NaN where no fluid probe was found)."""
+        n = getattr(self, "_n_tri", {}).get(level)
+        if n is None:
+            raise RuntimeError("no surface mesh on this level")
+        p = np.empty(n, np.float32)
+        s = np.empty((3, n), np.float32)
+        sg = np.empty(n, np.float32)
+        wd = np.empty(n, np.float32)
+        check(self.lib.oal_download_surface(self.h, level, p.ctypes.data, s.ctypes.data, sg.ctypes.data,
+                                            wd.ctypes.data), "oal_download_surface")
+        return {"pressure": p, "shear": s, "normal_sign": sg, "wall_dist": wd}
+
     def guard(self, rho_min: float = 0.2, u_max: float = 0.5, level: int = 0) -> int:
         """Stability guard (src/octree_q19/src/io_run.jl:31-65); raises OalError when a cell is bad."""
         bad = C.c_int64()

@@ -17,6 +17,8 @@
  *   init             src/main.jl:546-572 (init_eq!)
  *   momentum exch.   src/octree_q19/src/stream.jl:248-256 (F += (f_i* + f_opp(i)(t+1)) c_i)
  *   FP64 mass        src/octree_q19/src/state.jl:333-343
+ *   surface stress   src/forces/surface.jl:180-269, 271-298, 502-632, 636-1033, 1265-1331 (probe along the
+ *                    triangle normal, pressure and wall-law shear, force and moment integration)
  *
  * Layout (the reference's BlockLevel layout, src/geometry/blocks.jl:16-200, column-major
  * f[x,y,z,b,k]): population k of cell (x,y,z) of block b sits at
@@ -52,6 +54,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 namespace {
@@ -88,6 +91,10 @@ const Lattice LAT;
 //   6  WALE gradients: an absent neighbour contributes zero velocity instead of the cell's own
 //   7  BFL fallback 1: the wall equilibrium at unit density instead of the cell's density
 //   8  BFL fallback 2: the own population of the wall-ward link instead of the reflected one
+//   9  surface stress: the wall-law SGS estimate dropped
+//  10  surface stress: the shear divided by the unfloored probe distance
+//  11  surface stress: probes of any quality accepted
+//  12  surface stress: the z pressure force with the wrong sign
 int g_plant = 0;
 
 // weights as the reference builds them in the working precision (timestep.jl:2013)
@@ -1107,6 +1114,274 @@ void moments_impl(const OrGrid& g, const typename P::st* f, double* out4, double
     for (int a = 0; a < 4; ++a) out4[a] = s[a];
 }
 
+// ---- Surface-stress forces (src/forces/surface.jl, the default Prandtl force method) -----------------
+// Positions are 1-based as in the reference: cell (gx, gy, gz) has its centre at ((g - 1/2) dx, ...).
+// The probe field is the density and velocity of the state: FP64 sums of f_k - w_k (the rule of
+// moments_impl), rho clamped to >= 0.01 as stream_collide_cumulant! clamps what it writes
+// (kernel_cumulant.jl:421-425), rounded to FP32.  Arithmetic after that is FP32 in the reference's
+// order; the integration is FP64 (a deliberate change of the reference's Float32 atomics), summed in
+// triangle order.
+struct OrSurf {
+    double dx, mesh_offset[3], pressure_scale, stress_scale, tau, c_wale, nu_sgs_bg, cs2, moment_center[3];
+    int32_t search_radius, recover_normal;
+};
+
+struct SurfField {
+    const OrGrid& g;
+    int bs, dim[3];
+    std::vector<int32_t> block_ptr;        // [dim_x, dim_y, dim_z] 1-based block index, 0 = none
+    std::vector<float> rho, vel;           // [ncell], [3][ncell]
+    int64_t ncell;
+    SurfField(const OrGrid& gr, const double* f, bool w32) : g(gr), bs(gr.bs) {
+        for (int a = 0; a < 3; ++a) dim[a] = (g.n[a] + bs - 1) / bs;
+        block_ptr.assign((size_t)dim[0] * dim[1] * dim[2], 0);
+        for (int b = 0; b < g.nb; ++b) {
+            const int bx = g.bcoord[3 * b], by = g.bcoord[3 * b + 1], bz = g.bcoord[3 * b + 2];
+            if (bx < dim[0] && by < dim[1] && bz < dim[2]) block_ptr[((size_t)bz * dim[1] + by) * dim[0] + bx] = b + 1;
+        }
+        const int64_t bs3 = (int64_t)bs * bs * bs;
+        ncell = (int64_t)g.nb * bs3;
+        rho.assign(ncell, 1.0f);
+        vel.assign(3 * ncell, 0.0f);
+        for (int64_t c = 0; c < ncell; ++c) {
+            double dr = 0, jx = 0, jy = 0, jz = 0;
+            for (int k = 0; k < 27; ++k) {
+                const double w = w32 ? (double)weight<float>(k) : weight<double>(k);
+                const double s = f[(int64_t)k * ncell + c] - w;
+                dr += s; jx += s * LAT.cx[k]; jy += s * LAT.cy[k]; jz += s * LAT.cz[k];
+            }
+            double r = 1.0 + dr;
+            if (r < 0.01) r = 0.01;
+            rho[c] = (float)r;
+            vel[c] = (float)(jx / r); vel[ncell + c] = (float)(jy / r); vel[2 * ncell + c] = (float)(jz / r);
+        }
+    }
+    // get_cell_at_position (surface.jl:271-298) plus the domain extent, then the obstacle test:
+    // the cell index, or -1 for no fluid cell there
+    int64_t fluid(int gx, int gy, int gz) const {
+        if (gx < 1 || gy < 1 || gz < 1) return -1;
+        if (gx > g.n[0] || gy > g.n[1] || gz > g.n[2]) return -1;
+        const int bx = (gx - 1) / bs + 1, by = (gy - 1) / bs + 1, bz = (gz - 1) / bs + 1;
+        if (bx > dim[0] || by > dim[1] || bz > dim[2]) return -1;
+        const int32_t b_idx = block_ptr[((size_t)(bz - 1) * dim[1] + (by - 1)) * dim[0] + (bx - 1)];
+        if (b_idx <= 0) return -1;
+        const int lx = (gx - 1) % bs, ly = (gy - 1) % bs, lz = (gz - 1) % bs;
+        const int64_t c = (int64_t)(b_idx - 1) * bs * bs * bs + ((int64_t)lz * bs + ly) * bs + lx;
+        if (g.solid && g.solid[c]) return -1;
+        return c;
+    }
+};
+
+// floor(Int32, x) of the reference, for coordinates a cell index can reach
+static bool floor_i32(float x, int& i) {
+    if (!(x > -1.0e9f && x < 1.0e9f)) return false;
+    i = (int)std::floor(x);
+    return true;
+}
+
+// trilinear_sample_flow (surface.jl:502-632): returns ok; r, u*, w_q as the reference returns them
+static bool trilinear_sample_flow(const SurfField& F, float px, float py, float pz, float dx, float& r_out,
+                                  float& u_out, float& v_out, float& w_out, float& w_q) {
+    r_out = 1.0f; u_out = 0.0f; v_out = 0.0f; w_out = 0.0f; w_q = 0.0f;
+    const float inv_dx = 1.0f / dx;
+    const float fx = px * inv_dx - 0.5f, fy = py * inv_dx - 0.5f, fz = pz * inv_dx - 0.5f;
+    int fl_x, fl_y, fl_z;
+    if (!floor_i32(fx, fl_x) || !floor_i32(fy, fl_y) || !floor_i32(fz, fl_z)) return false;
+    const int ix0 = fl_x + 1, iy0 = fl_y + 1, iz0 = fl_z + 1;
+    auto clamp01 = [](float v) { return v > 1.0f ? 1.0f : (v < 0.0f ? 0.0f : v); };
+    const float wx = clamp01(fx - (float)(ix0 - 1));
+    const float wy = clamp01(fy - (float)(iy0 - 1));
+    const float wz = clamp01(fz - (float)(iz0 - 1));
+    const float w0x = 1.0f - wx, w1x = wx, w0y = 1.0f - wy, w1y = wy, w0z = 1.0f - wz, w1z = wz;
+    float rho_acc = 0.0f, ux_acc = 0.0f, uy_acc = 0.0f, uz_acc = 0.0f, w_acc = 0.0f;
+    // the eight corners in the reference's order
+    const int corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    for (int n = 0; n < 8; ++n) {
+        const float w_k = (corner[n][0] ? w1x : w0x) * (corner[n][1] ? w1y : w0y) * (corner[n][2] ? w1z : w0z);
+        const int64_t c = F.fluid(ix0 + corner[n][0], iy0 + corner[n][1], iz0 + corner[n][2]);
+        if (c < 0) continue;
+        rho_acc += w_k * F.rho[c];
+        ux_acc += w_k * F.vel[c];
+        uy_acc += w_k * F.vel[F.ncell + c];
+        uz_acc += w_k * F.vel[2 * F.ncell + c];
+        w_acc += w_k;
+    }
+    if (w_acc > 0.01f) {
+        const float inv_w = 1.0f / w_acc;
+        const float r = rho_acc * inv_w, u = ux_acc * inv_w, v = uy_acc * inv_w, w = uz_acc * inv_w;
+        if (!std::isfinite(r) || !std::isfinite(u) || !std::isfinite(v) || !std::isfinite(w)) return false;
+        r_out = r; u_out = u; v_out = v; w_out = w; w_q = w_acc;
+        return true;
+    }
+    return false;
+}
+
+static float jmax(float a, float b) { return a < b ? b : a; }
+static float jmin(float a, float b) { return b < a ? b : a; }
+
+// compute_stress_from_cell (surface.jl:180-269), T = 1
+static void compute_stress_from_cell(float rho_val, float ux, float uy, float uz, float nx, float ny, float nz,
+                                     float dist_to_wall, float tau_molecular, float pressure_scale,
+                                     float stress_scale, float c_wale, float nu_sgs_bg, float cs2_val, float out[4]) {
+    out[0] = out[1] = out[2] = out[3] = 0.0f;
+    if (!std::isfinite(rho_val) || rho_val < 0.01f || rho_val > 10.0f) return;
+    if (!std::isfinite(ux) || !std::isfinite(uy) || !std::isfinite(uz)) return;
+    const float u_mag_sq = ux * ux + uy * uy + uz * uz;
+    if (u_mag_sq > 100.0f) return;
+    const float p_gauge_lat = (rho_val * 1.0f - 1.0f) * cs2_val;
+    const float p_phys = p_gauge_lat * pressure_scale;
+    const float u_dot_n = ux * nx + uy * ny + uz * nz;
+    const float ut_x = ux - u_dot_n * nx, ut_y = uy - u_dot_n * ny, ut_z = uz - u_dot_n * nz;
+    const float u_tan_mag = std::sqrt(ut_x * ut_x + ut_y * ut_y + ut_z * ut_z);
+    const float nu_mol = (tau_molecular - 0.5f) * cs2_val;
+    float nu_sgs = 0.0f;
+    const float d_wall_eff = jmax(dist_to_wall, 1.0f);
+    if (c_wale > 0.0f && u_tan_mag > 1.0e-10f) {
+        const float S_mag = u_tan_mag / d_wall_eff;
+        nu_sgs = (c_wale * c_wale) * S_mag;
+    }
+    if (g_plant == 9) nu_sgs = 0.0f;                      // planted: SGS estimate dropped
+    nu_sgs = jmax(nu_sgs, nu_sgs_bg);
+    nu_sgs = jmin(nu_sgs, 20.0f * nu_mol);
+    const float nu_eff = nu_mol + nu_sgs;
+    float tau_x = 0.0f, tau_y = 0.0f, tau_z = 0.0f;
+    if (u_tan_mag > 1.0e-10f && dist_to_wall > 0.01f) {
+        const float tau_lat_mag = rho_val * nu_eff * u_tan_mag / (g_plant == 10 ? dist_to_wall : d_wall_eff);
+        const float tau_phys_mag = tau_lat_mag * stress_scale;
+        tau_x = (ut_x / u_tan_mag) * tau_phys_mag;
+        tau_y = (ut_y / u_tan_mag) * tau_phys_mag;
+        tau_z = (ut_z / u_tan_mag) * tau_phys_mag;
+    }
+    if (!std::isfinite(p_phys) || !std::isfinite(tau_x) || !std::isfinite(tau_y) || !std::isfinite(tau_z)) return;
+    out[0] = p_phys; out[1] = tau_x; out[2] = tau_y; out[3] = tau_z;
+}
+
+// map_stresses_kernel! (surface.jl:636-1033, FORCE_METHOD :prandtl, athermal, no periodic-Y seam) for
+// every triangle, then integrate_forces_kernel! (:1265-1331) in FP64: sums = Fp[3], Fv[3], M[3], count
+void surface_impl(const SurfField& F, const OrSurf& S, int64_t n_tri, const float* cen, const float* nrm,
+                  const float* area, float* p_map, float* s_map, float* sign_map, float* wd_map, double* sums) {
+    const float dx = (float)S.dx;
+    const float off[3] = {(float)S.mesh_offset[0], (float)S.mesh_offset[1], (float)S.mesh_offset[2]};
+    const float tau_molecular = (float)S.tau, pressure_scale = (float)S.pressure_scale,
+                stress_scale = (float)S.stress_scale, c_wale = (float)S.c_wale, nu_sgs_bg = (float)S.nu_sgs_bg,
+                cs2_val = (float)S.cs2;
+    const int search_radius = S.search_radius;
+    for (int j = 0; j < 10; ++j) sums[j] = 0.0;
+    for (int64_t i = 0; i < n_tri; ++i) {
+        const float tx = cen[i] + off[0], ty = cen[n_tri + i] + off[1], tz = cen[2 * n_tri + i] + off[2];
+        float n_x = nrm[i], n_y = nrm[n_tri + i], n_z = nrm[2 * n_tri + i];
+        float best_rho = 1.0f, best_ux = 0.0f, best_uy = 0.0f, best_uz = 0.0f;
+        float best_wall_dist = (float)(search_radius + 1), best_quality = 0.0f;
+        bool found_fluid = false;
+        float p_rho_accum = 0.0f, p_w_accum = 0.0f;
+        const float min_quality = g_plant == 11 ? 0.0f : 0.25f;   // planted: every probe accepted
+        const int pressure_probe_max = search_radius < 4 ? search_radius : 4;
+        for (int probe_i = 1; probe_i <= search_radius; ++probe_i) {
+            const float d_cells = (float)probe_i + 0.5f;
+            const float probe_x = tx + n_x * d_cells * dx, probe_y = ty + n_y * d_cells * dx, probe_z = tz + n_z * d_cells * dx;
+            float r, ux, uy, uz, w_q;
+            const bool ok = trilinear_sample_flow(F, probe_x, probe_y, probe_z, dx, r, ux, uy, uz, w_q);
+            if (ok && w_q >= min_quality && probe_i <= pressure_probe_max) {
+                p_rho_accum += w_q * r;
+                p_w_accum += w_q;
+            }
+            if (ok && w_q >= min_quality) {
+                if (!found_fluid) {
+                    best_rho = r; best_ux = ux; best_uy = uy; best_uz = uz;
+                    best_wall_dist = d_cells; best_quality = w_q; found_fluid = true;
+                }
+                if (probe_i >= pressure_probe_max) break;
+            } else if (ok && w_q > best_quality && !found_fluid) {
+                best_rho = r; best_ux = ux; best_uy = uy; best_uz = uz;
+                best_wall_dist = d_cells; best_quality = w_q; found_fluid = true;
+            }
+        }
+        float normal_sign = 1.0f;
+        if (!found_fluid && S.recover_normal) {
+            for (int probe_i = 1; probe_i <= pressure_probe_max; ++probe_i) {
+                const float d_cells = (float)probe_i + 0.5f;
+                const float px = tx - n_x * d_cells * dx, py = ty - n_y * d_cells * dx, pz = tz - n_z * d_cells * dx;
+                float r, ux, uy, uz, w_q;
+                const bool ok = trilinear_sample_flow(F, px, py, pz, dx, r, ux, uy, uz, w_q);
+                if (ok && w_q >= min_quality) {
+                    p_rho_accum += w_q * r;
+                    p_w_accum += w_q;
+                    if (!found_fluid) {
+                        best_rho = r; best_ux = ux; best_uy = uy; best_uz = uz;
+                        best_wall_dist = d_cells; best_quality = w_q; found_fluid = true; normal_sign = -1.0f;
+                    }
+                }
+            }
+            if (normal_sign < 0.0f) { n_x = -n_x; n_y = -n_y; n_z = -n_z; }
+        }
+        int fgx, fgy, fgz;
+        if (!found_fluid && floor_i32(tx / dx, fgx) && floor_i32(ty / dx, fgy) && floor_i32(tz / dx, fgz)) {
+            const int g_x = fgx + 1, g_y = fgy + 1, g_z = fgz + 1;
+            float best_dist_sq = 1.0e10f;
+            for (int radius = 1; radius <= search_radius; ++radius) {
+                if (found_fluid) break;
+                for (int dz = -radius; dz <= radius; ++dz)
+                    for (int dy = -radius; dy <= radius; ++dy)
+                        for (int ddx = -radius; ddx <= radius; ++ddx) {
+                            const bool at_shell = std::abs(ddx) == radius || std::abs(dy) == radius || std::abs(dz) == radius;
+                            if (!at_shell) continue;
+                            const int check_gx = g_x + ddx, check_gy = g_y + dy, check_gz = g_z + dz;
+                            const int64_t c = F.fluid(check_gx, check_gy, check_gz);
+                            if (c < 0) continue;
+                            const float cell_rho = F.rho[c];
+                            if (!std::isfinite(cell_rho) || cell_rho < 0.01f || cell_rho > 100.0f) continue;
+                            const float cell_cx = ((float)check_gx - 0.5f) * dx;
+                            const float cell_cy = ((float)check_gy - 0.5f) * dx;
+                            const float cell_cz = ((float)check_gz - 0.5f) * dx;
+                            const float disp_x = cell_cx - tx, disp_y = cell_cy - ty, disp_z = cell_cz - tz;
+                            const float dot_n = disp_x * n_x + disp_y * n_y + disp_z * n_z;
+                            if (dot_n <= 0.0f) continue;
+                            const float dist_sq = disp_x * disp_x + disp_y * disp_y + disp_z * disp_z;
+                            if (dist_sq < best_dist_sq) {
+                                best_dist_sq = dist_sq;
+                                best_rho = cell_rho;
+                                best_ux = F.vel[c]; best_uy = F.vel[F.ncell + c]; best_uz = F.vel[2 * F.ncell + c];
+                                best_wall_dist = dot_n / dx;
+                                found_fluid = true;
+                            }
+                        }
+            }
+        }
+        const float pressure_rho = p_w_accum > 0.5f ? p_rho_accum / p_w_accum : best_rho;
+        float p_val = std::numeric_limits<float>::quiet_NaN(), tau_x = 0.0f, tau_y = 0.0f, tau_z = 0.0f;
+        if (found_fluid) {
+            const float wall_dist = jmax(best_wall_dist, 0.5f);
+            float o[4];
+            compute_stress_from_cell(best_rho, best_ux, best_uy, best_uz, n_x, n_y, n_z, wall_dist, tau_molecular,
+                                     pressure_scale, stress_scale, c_wale, nu_sgs_bg, cs2_val, o);
+            p_val = o[0]; tau_x = o[1]; tau_y = o[2]; tau_z = o[3];
+            if (std::isfinite(pressure_rho) && pressure_rho >= 0.01f && pressure_rho <= 10.0f)
+                p_val = (pressure_rho * 1.0f - 1.0f) * cs2_val * pressure_scale;
+        }
+        p_map[i] = p_val;
+        s_map[i] = tau_x; s_map[n_tri + i] = tau_y; s_map[2 * n_tri + i] = tau_z;
+        sign_map[i] = normal_sign;
+        wd_map[i] = found_fluid ? best_wall_dist * dx : 0.0f;
+        // integrate_forces_kernel!, in FP64
+        if (std::isfinite(p_val) && std::isfinite(tau_x) && std::isfinite(tau_y) && std::isfinite(tau_z)) {
+            const double A = area[i], p = p_val;
+            const double nx = n_x, ny = n_y, nz = n_z;   // already carry the normal's sign
+            const double dFp_x = -p * nx * A, dFp_y = -p * ny * A, dFp_z = (g_plant == 12 ? p : -p) * nz * A;
+            const double dFv_x = (double)tau_x * A, dFv_y = (double)tau_y * A, dFv_z = (double)tau_z * A;
+            const double dFx = dFp_x + dFv_x, dFy = dFp_y + dFv_y, dFz = dFp_z + dFv_z;
+            const double rx = (double)tx - (double)(float)S.moment_center[0];
+            const double ry = (double)ty - (double)(float)S.moment_center[1];
+            const double rz = (double)tz - (double)(float)S.moment_center[2];
+            sums[0] += dFp_x; sums[1] += dFp_y; sums[2] += dFp_z;
+            sums[3] += dFv_x; sums[4] += dFv_y; sums[5] += dFv_z;
+            sums[6] += ry * dFz - rz * dFy;
+            sums[7] += rz * dFx - rx * dFz;
+            sums[8] += rx * dFy - ry * dFx;
+            sums[9] += 1.0;
+        }
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1232,6 +1507,17 @@ int oracle_moments(int storage, const OrGrid* g, const void* f, double* out4, do
         case 3: moments_impl<StBF16S>(*g, (const uint16_t*)f, out4, rho_c, u_c); return 0;
         default: return -1;
     }
+}
+
+// surface-stress map and forces of a state given as FP64 populations [27][ncell] (unshifted values;
+// w32: the state's working precision is FP32, so its f_k - w_k uses the FP32 weights)
+int oracle_surface(const OrGrid* g, const double* f, int w32, const OrSurf* s, int64_t n_tri, const float* cen,
+                   const float* nrm, const float* area, float* p_map, float* s_map, float* sign_map, float* wd_map,
+                   double* sums) {
+    if (!g || !f || !s || n_tri < 0) return -1;
+    const SurfField F(*g, f, w32 != 0);
+    surface_impl(F, *s, n_tri, cen, nrm, area, p_map, s_map, sign_map, wd_map, sums);
+    return 0;
 }
 
 }  // extern "C"

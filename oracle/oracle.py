@@ -91,6 +91,15 @@ class OrGrid(C.Structure):
     ]
 
 
+class OrSurf(C.Structure):
+    _fields_ = [
+        ("dx", C.c_double), ("mesh_offset", C.c_double * 3), ("pressure_scale", C.c_double),
+        ("stress_scale", C.c_double), ("tau", C.c_double), ("c_wale", C.c_double), ("nu_sgs_bg", C.c_double),
+        ("cs2", C.c_double), ("moment_center", C.c_double * 3), ("search_radius", C.c_int32),
+        ("recover_normal", C.c_int32),
+    ]
+
+
 _lib = None
 
 
@@ -129,6 +138,9 @@ def load() -> C.CDLL:
     lib.oracle_round_bits.argtypes = [vp, vp, C.c_int32, vp, C.c_int64]
     lib.oracle_set_plant.argtypes = [C.c_int]
     lib.oracle_vel_gradients.argtypes = [C.POINTER(OrGrid), vp, vp]
+    lib.oracle_surface.argtypes = [C.POINTER(OrGrid), vp, C.c_int, C.POINTER(OrSurf), C.c_int64, vp, vp, vp, vp, vp,
+                                   vp, vp, vp]
+    lib.oracle_surface.restype = C.c_int
     _lib = lib
     return lib
 
@@ -249,6 +261,41 @@ def vel_gradients(grid: Grid, vel: np.ndarray) -> np.ndarray:
     out = np.zeros((9, grid.ncell))
     load().oracle_vel_gradients(C.byref(grid.c), v.ctypes.data, out.ctypes.data)
     return out.reshape(3, 3, grid.ncell)
+
+
+def surface(grid: Grid, f: np.ndarray, centers, normals, areas, *, w32: bool = True, dx: float = 1.0,
+            mesh_offset=(0.0, 0.0, 0.0), pressure_scale: float = 1.0, stress_scale: float | None = None,
+            tau: float = 0.6, c_wale: float = 0.2, nu_sgs_bg: float = 0.0, cs2: float = 1.0 / 3.0,
+            search_radius: int = 8, recover_normal: bool = True, moment_center=(0.0, 0.0, 0.0)) -> dict:
+    """Surface-stress map and forces (src/forces/surface.jl) of a state given as unshifted populations
+    f [27, ncell] (FP64 values; w32: the state's working precision is FP32).  Returns the per-triangle
+    maps (pressure, shear [3, n], normal_sign, wall_dist; Float32) and the FP64 sums F_pressure,
+    F_viscous, M (3-vectors) and n_mapped."""
+    lib = load()
+    ff = np.ascontiguousarray(f, dtype=np.float64)
+    c = np.ascontiguousarray(centers, dtype=np.float32)
+    nrm = np.ascontiguousarray(normals, dtype=np.float32)
+    a = np.ascontiguousarray(areas, dtype=np.float32)
+    n = a.shape[0]
+    s = OrSurf()
+    s.dx = dx
+    s.mesh_offset[:] = [float(v) for v in mesh_offset]
+    s.pressure_scale = pressure_scale
+    s.stress_scale = pressure_scale if stress_scale is None else stress_scale
+    s.tau, s.c_wale, s.nu_sgs_bg, s.cs2 = tau, c_wale, nu_sgs_bg, cs2
+    s.moment_center[:] = [float(v) for v in moment_center]
+    s.search_radius, s.recover_normal = int(search_radius), int(bool(recover_normal))
+    p = np.empty(n, np.float32)
+    sh = np.empty((3, n), np.float32)
+    sg = np.empty(n, np.float32)
+    wd = np.empty(n, np.float32)
+    sums = np.zeros(10)
+    rc = lib.oracle_surface(C.byref(grid.c), ff.ctypes.data, int(bool(w32)), C.byref(s), n, c.ctypes.data,
+                            nrm.ctypes.data, a.ctypes.data, p.ctypes.data, sh.ctypes.data, sg.ctypes.data,
+                            wd.ctypes.data, sums.ctypes.data)
+    assert rc == 0
+    return {"pressure": p, "shear": sh, "normal_sign": sg, "wall_dist": wd, "F_pressure": sums[0:3].copy(),
+            "F_viscous": sums[3:6].copy(), "M": sums[6:9].copy(), "n_mapped": int(sums[9])}
 
 
 class planted:
