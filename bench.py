@@ -48,14 +48,19 @@ def tau_of():
     return 0.5 + 3.0 * nu
 
 
-def bytes_per_lup(storage: str) -> int:
-    return 2 * 27 * {"f32": 4, "f64": 8, "f16s": 2, "bf16s": 2}[storage]
+def bytes_per_lup(storage: str, wale: bool = False) -> int:
+    """algorithmic HBM bytes per cell update: 27 populations read and written, plus with WALE the
+    previous step's velocity read and this step's written (3 values of the working precision each)"""
+    pops = 2 * 27 * {"f32": 4, "f64": 8, "f16s": 2, "bf16s": 2}[storage]
+    return pops + (2 * 3 * (8 if storage == "f64" else 4) if wale else 0)
 
 
 def gpu_leg(grid, storage: str, stream: str, steps: int, warmup: int, autotune: bool = True, fixed=None,
-            qmap=None):
+            qmap=None, wale: bool = False):
+    """wale: the reference's production collision (WALE c_wale 0.2 with inlet noise 0.01,
+    src/io/config_loader.jl:82-88) instead of the plain cumulant"""
     from openaerolab_alpha_01_amd import solver
-    params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
+    params = solver.StepParams(c_wale=0.2 if wale else 0.0, inlet_turbulence=0.01 if wale else 0.0)
     plan = solver.uniform_plan(grid, tau_of(), storage=storage, stream=stream, params=params, u_inlet=U_IN)
     if qmap is not None:
         plan.set_bouzidi(qmap)
@@ -140,10 +145,15 @@ def validation(steps: int, seed: int, chunk: int = 1000, bouzidi: bool = False) 
     velocity carries a seeded perturbation, uniform in [-1e-3, 1e-3] per component (numpy PCG64,
     `seed`), with the FP64 run started from the same field.  After `steps` steps (the last one
     collecting the momentum exchange): Cd = F_x / (1/2 rho0 u_in^2 pi R^2) and
-    L2(u) = |u - u_64| / |u_64| over the fluid cells.  bouzidi: the sphere's links take interpolated
-    bounce-back (every storage form, the FP64 run included)."""
+    L2(u) = |u - u_64| / |u_64| over the fluid cells.  Beside the momentum exchange, the reference
+    application's own drag estimate: the surface-stress map of the sphere's triangulated surface
+    (5120 triangles of radius R; pressure and wall-law shear, src/forces/surface.jl, C_SMAGO 0.2) on the
+    same final state, cd_surface.  bouzidi: the sphere's links take interpolated bounce-back (every
+    storage form, the FP64 run included)."""
     from openaerolab_alpha_01_amd import solver
+    from openaerolab_alpha_01_amd.geometry import icosphere
     grid = workload_grid()
+    mesh = icosphere((96.0, 128.0, 128.0), D / 2, 4)
     qmap = wall_qmap(grid) if bouzidi else None
     fluid = grid.solid == 0
     n = grid.n_cells
@@ -175,6 +185,11 @@ def validation(steps: int, seed: int, chunk: int = 1000, bouzidi: bool = False) 
                       flush=True)
             plan.level_step(0, t, force=True)
             F = plan.force()[0]
+            plan.set_surface(*mesh)
+            ts = time.perf_counter()
+            surf = plan.surface_forces(tau=tau_of(), c_wale=0.2, rho_ref=1.0, u_ref=U_IN, area_ref=area,
+                                       chord_ref=float(D), moment_center=(96.0, 128.0, 128.0))
+            surf_ms = (time.perf_counter() - ts) * 1e3
             _, u = plan.macro()
             plan.guard(0.2, 0.5)
             plan.close()
@@ -182,12 +197,16 @@ def validation(steps: int, seed: int, chunk: int = 1000, bouzidi: bool = False) 
             del u
             cd = float(F[0] / (0.5 * U_IN ** 2 * area))
             entry = {"cd": round(cd, 6), "cl_y": round(float(F[1] / (0.5 * U_IN ** 2 * area)), 7),
-                     "cl_z": round(float(F[2] / (0.5 * U_IN ** 2 * area)), 7)}
+                     "cl_z": round(float(F[2] / (0.5 * U_IN ** 2 * area)), 7),
+                     "cd_surface": round(surf["Cd"], 6),
+                     "cd_surface_pressure": round(float(surf["F_pressure"][0] / (0.5 * U_IN ** 2 * area)), 6),
+                     "surface_mapped": surf["n_mapped"], "surface_ms": round(surf_ms, 3)}
             if ref is None:
-                ref = (cd, uf)
+                ref = (cd, uf, surf["Cd"])
             else:
                 entry["cd_rel_err_vs_f64"] = float((cd - ref[0]) / ref[0])
                 entry["l2_u_vs_f64"] = float(np.linalg.norm(uf - ref[1]) / np.linalg.norm(ref[1]))
+                entry["cd_surface_rel_err_vs_f64"] = float((surf["Cd"] - ref[2]) / ref[2])
             res[st] = entry
         out[case] = res
     return out
@@ -552,6 +571,8 @@ def main(argv=None) -> int:
     ap.add_argument("--tgv-cpu-steps", type=int, default=200, help="steps of the CPU oracle in --tgv")
     ap.add_argument("--amr-legs", default="f16s:1,f16s:0,f32:1",
                     help="block-AMR legs timed, storage:graph comma-separated; the first is the reported value")
+    ap.add_argument("--no-wale-leg", action="store_true",
+                    help="skip the WALE-on leg (FP32, c_wale 0.2, inlet noise 0.01) of the default line")
     ap.add_argument("--acc-steps", type=int, default=1500,
                     help="steps of the accuracy leg of the default line (Cd and L2(u) of FP32 / FP16 / BF16 against "
                          "FP64 runs of the same kernels); 0 skips it")
@@ -663,6 +684,24 @@ def main(argv=None) -> int:
         bw = copy_bandwidth_gbs()
         out["roofline"]["measured_copy_gbs"] = round(bw, 1)
         out["roofline"]["frac_of_measured_copy"] = round(achieved / bw, 4)
+    if not args.no_wale_leg:
+        # the reference's production collision on the same workload: WALE (c_wale 0.2) and inlet noise (0.01)
+        ms_tw, ms_kw, _, Fw, tunw, _ = gpu_leg(grid, "f32", args.stream, args.steps, args.warmup,
+                                               not args.no_autotune, fixed, qmap, wale=True)
+        mw = n_cells * args.steps / (ms_tw * 1e-3) / 1e6
+        facew = args.stream == "ab" and tunw["pipe_planes"] > 1
+        bulkw = (info["n_regular"] + info["n_interior"]) * 16 ** 3 - (0 if facew else info["n_face_cells"])
+        bw = bytes_per_lup("f32", wale=True)
+        aw = bw * bulkw / (ms_kw * 1e-3) / 1e9
+        ppw = min(tunw["pipe_planes"], 8)
+        out["wale_on"] = {"value": round(mw, 1), "unit": "MLUPS", "ms_per_step": round(ms_tw / args.steps, 4),
+                          "vs_wale_off": round(mw / mlups, 3), "c_wale": 0.2, "inlet_turbulence": 0.01,
+                          "tuning": tunw, "cd_transient": round(float(Fw[0] / (0.5 * U_IN ** 2 * area)), 5),
+                          "roofline": {"achieved": round(aw, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": round(aw / HBM_PEAK_GBS, 4), "bytes_per_lup": bw,
+                                       "kernel": (f"k_step_pipe<StF32, {ppw}, 1, true, true>" if facew
+                                                  else "k_step<StF32, regular, wale>"),
+                                       "kernel_ms": round(ms_kw, 5), "cells_per_launch": bulkw}}
     if args.acc_steps > 0:
         # bounded accuracy leg: the storage forms against FP64 runs of the same kernels after acc_steps steps
         # (C2 FP32 from the uniform start, C3 FP16 / BF16 from the seeded perturbation; a transient state,
@@ -673,7 +712,14 @@ def main(argv=None) -> int:
                            "cd_rel_err_vs_f64": {st: round(acc[c][st]["cd_rel_err_vs_f64"], 7)
                                                  for c, st in (("C2", "f32"), ("C3", "f16s"), ("C3", "bf16s"))},
                            "l2_u_vs_f64": {st: float(f"{acc[c][st]['l2_u_vs_f64']:.3e}")
-                                           for c, st in (("C2", "f32"), ("C3", "f16s"), ("C3", "bf16s"))}}
+                                           for c, st in (("C2", "f32"), ("C3", "f16s"), ("C3", "bf16s"))},
+                           # the reference application's drag convention on the same states (surface.jl)
+                           "cd_surface_f64_C2": acc["C2"]["f64"]["cd_surface"],
+                           "cd_surface_f64_C3": acc["C3"]["f64"]["cd_surface"],
+                           "cd_surface_rel_err_vs_f64": {st: round(acc[c][st]["cd_surface_rel_err_vs_f64"], 7)
+                                                         for c, st in (("C2", "f32"), ("C3", "f16s"), ("C3", "bf16s"))},
+                           "surface_mapped_triangles": int(acc["C2"]["f64"]["surface_mapped"]),
+                           "surface_map_ms": acc["C2"]["f32"]["surface_ms"]}
     if not args.no_cpu_baseline:
         v, cores, done, secs = cpu_baseline(grid)
         out["cpu_baseline"] = {"value": round(v, 2), "unit": "MLUPS", "cores": cores, "kind": "port",

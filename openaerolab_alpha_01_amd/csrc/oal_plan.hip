@@ -1315,8 +1315,8 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
             inactive.resize(o.nb());
             for (int b = 0; b < o.nb(); ++b) inactive[b] = o.status[b] != OCT_STEPPED;
         }
-        // the pipelined kernels cover FP32-arithmetic storage without WALE
-        const bool pipe_ok = desc->storage != OAL_F64 && !p->wale;
+        // the pipelined kernels cover FP32-arithmetic storage
+        const bool pipe_ok = desc->storage != OAL_F64;
         const int rc = prepare_level(d, desc->collision, desc->stream_mode == OAL_STREAM_AB, pipe_ok,
                                      storage_bytes(desc->storage), p->levels[l],
                                      host[l], ml ? &inactive : nullptr);

@@ -648,12 +648,37 @@ struct InletRule {
     int gy, gz;       // the cell's global y and z (the noise position)
 };
 
-template <class SP, bool UNIT>
-__device__ __forceinline__ void reg_collide_store(const StepArgs& a, const RegAddr<SP>& ra, const CellRates<typename SP::T>& q,
+// WALE in the pipelined kernel (AB pattern, regular blocks: no solid cell in the block or its halo, the
+// neighbour blocks at lexicographic offsets).  The previous step's velocity at the six face neighbours
+// (x+, x-, y+, y-, z+, z-) of a lane's cell is loaded before the next plane's populations, so that the
+// collision waits for these loads only; a neighbour across a non-periodic domain face is the cell
+// itself (physics_utils.jl:1981-2035, 2217-2233, the rule of wale_of).
+template <class T>
+struct PipeWale {
+    T u[6][3];
+};
+template <class T>
+struct PipeWaleRates {
+    T tau, amr_scale;           // of the domain, read before the first store
+};
+template <class SP, bool UNIT, bool WALE = false>
+__device__ __forceinline__ void reg_collide_store(const StepArgs& a, const RegAddr<SP>& ra, const CellRates<typename SP::T>& q0,
                                                   int z, uint32_t cell, const typename SP::raw (&r)[27],
-                                                  const InletRule* inlet = nullptr, bool store = true) {
+                                                  const InletRule* inlet = nullptr, bool store = true,
+                                                  const PipeWale<typename SP::T>* wv = nullptr,
+                                                  const PipeWaleRates<typename SP::T>* wr = nullptr, uint32_t vidx = 0) {
     using T = typename SP::T;
     using st = typename SP::st;
+    CellRates<T> q = q0;
+    if constexpr (WALE) {
+        const T h = T(0.5);
+        const auto& u = wv->u;
+        const T nu_eddy = wale_nu(h * (u[0][0] - u[1][0]), h * (u[2][0] - u[3][0]), h * (u[4][0] - u[5][0]),
+                                  h * (u[0][1] - u[1][1]), h * (u[2][1] - u[3][1]), h * (u[4][1] - u[5][1]),
+                                  h * (u[0][2] - u[1][2]), h * (u[2][2] - u[3][2]), h * (u[4][2] - u[5][2]), T(a.cp.c_wale));
+        cell_rates<T>(nu_eddy, wr->tau, wr->amr_scale, T(a.cp.nu_sgs_bg), a.cp.adaptive_omega4, T(a.cp.lambda),
+                      T(a.cp.omega_4), q.omega1, q.omega4_diag);
+    }
     T f[27];
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
@@ -678,6 +703,10 @@ __device__ __forceinline__ void reg_collide_store(const StepArgs& a, const RegAd
             constexpr int K = decltype(kc)::value;
             SP::sto_r((st*)(ra.base(a.fout, K, z, 0) + ra.v[1][1]), f[K], wgt<T>(K), round_bits(hc, K), dith);
         });
+        if constexpr (WALE) {
+            T* vo = (T*)a.vel_out;
+            vo[vidx] = ux_o; vo[a.ncell + vidx] = uy_o; vo[2 * (size_t)a.ncell + vidx] = uz_o;
+        }
     }
 }
 
@@ -759,13 +788,17 @@ enum : int { FACE_NONE = 0, FACE_RULES = 1, FACE_SKIP = 2, FACE_PARTIAL = 3 };
 #define OAL_FACE_RULES_WAVES 2
 #endif
 // (the 16-bit forms' conversions and rounding need more registers: 2 waves for every form of theirs)
-template <class SP, int FACE>
-constexpr int pipe_waves() { return FACE == FACE_RULES || FACE == FACE_PARTIAL || SP::dithered ? OAL_FACE_RULES_WAVES : 3; }
+// (WALE's velocity gather as well)
+template <class SP, int FACE, bool WALE = false>
+constexpr int pipe_waves() {
+    return FACE == FACE_RULES || FACE == FACE_PARTIAL || SP::dithered || WALE ? OAL_FACE_RULES_WAVES : 3;
+}
 
 // UNIT: the collision form for the unit rates omega_bulk = omega_3 = omega_4 = 1 (a separate kernel, so
-// that each form keeps its own register allocation)
-template <class SP, int NP, int FACE, bool UNIT>
-__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(pipe_waves<SP, FACE>(), pipe_waves<SP, FACE>())))
+// that each form keeps its own register allocation).  WALE: the eddy viscosity from the previous step's
+// velocity, and this step's velocity written.
+template <class SP, int NP, int FACE, bool UNIT, bool WALE = false>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(pipe_waves<SP, FACE, WALE>(), pipe_waves<SP, FACE, WALE>())))
 k_step_pipe(const StepArgs a) {
     using T = typename SP::T;
     using st = typename SP::st;
@@ -854,39 +887,83 @@ k_step_pipe(const StepArgs a) {
         in.gy = a.borg[3 * b + 1] + c.ly;
         in.gz = a.borg[3 * b + 2];      // + z per plane
     }
+    // WALE: the x / y neighbours' cell offsets of the lane (0: across a domain face, the cell itself),
+    // the block layer, and the domain's rates
+    int32_t wxp = 0, wxm = 0, wyp = 0, wym = 0, wlayer = 0;
+    PipeWaleRates<T> wr{};
+    if constexpr (WALE) {
+        const int32_t row = dm->nbx * BS3 - (BS - 1) * BS;
+        wxp = fr.xh ? 0 : (c.lx < BS - 1 ? 1 : BS3 - (BS - 1));
+        wxm = fr.xl ? 0 : (c.lx > 0 ? -1 : -(BS3 - (BS - 1)));
+        wyp = fr.yh ? 0 : (c.ly < BS - 1 ? BS : row);
+        wym = fr.yl ? 0 : (c.ly > 0 ? -BS : -row);
+        if constexpr (FACE == FACE_PARTIAL) {
+            if (fr.dead) { wxp = 0; wxm = 0; wyp = 0; wym = 0; }
+        }
+        wlayer = dm->nbx * dm->nby * BS3;
+        wr.tau = T(dm->tau);
+        wr.amr_scale = T(dm->amr_scale);
+    }
+    // the six neighbours' velocities of the lane's cell in plane z
+    auto wale_load = [&](int z, PipeWale<T>& w) {
+        const T* v = (const T*)a.vel_in;
+        const uint32_t own = c.own + (uint32_t)(z - z0) * BS2;
+        bool zl = false, zh = false;
+        if constexpr (FACE == FACE_RULES) { zl = fr.zlo && z == 0; zh = fr.zhi && z == BS - 1; }
+        if constexpr (FACE == FACE_PARTIAL) { zl = z == fr.zlo_plane; zh = z == fr.zhi_plane; }
+        const int32_t zp = zh ? 0 : (z < BS - 1 ? BS2 : wlayer - (BS - 1) * BS2);
+        const int32_t zm = zl ? 0 : (z > 0 ? -BS2 : -(wlayer - (BS - 1) * BS2));
+        const int32_t off[6] = {wxp, wxm, wyp, wym, zp, zm};
+#pragma unroll
+        for (int q6 = 0; q6 < 6; ++q6) {
+            const uint32_t i = (uint32_t)((int32_t)own + off[q6]);
+            w.u[q6][0] = v[i]; w.u[q6][1] = v[a.ncell + i]; w.u[q6][2] = v[2 * (size_t)a.ncell + i];
+        }
+    };
     auto load = [&](const RegAddr<SP>& r, int z, typename SP::raw (&dst)[27]) {
         if constexpr (FACE == FACE_RULES) { if (z < zmax) reg_load_raw_face<SP>(a, r, fr, z, dst); }
         else if constexpr (FACE == FACE_PARTIAL) { if (z < zmax) reg_load_raw_face<SP, true>(a, r, fr, z, dst); }
         else if (live(z)) reg_load_raw<SP>(a, r, z, dst);
     };
     const uint32_t cell0 = dm->cell0;   // read before the first store, as every domain parameter
-    auto collide = [&](const RegAddr<SP>& r, int z, const typename SP::raw (&src)[27]) {
-        const uint32_t cell = c.own + (uint32_t)(z - z0) * BS2 - cell0;
+    auto collide = [&](const RegAddr<SP>& r, int z, const typename SP::raw (&src)[27], const PipeWale<T>& w) {
+        const uint32_t vidx = c.own + (uint32_t)(z - z0) * BS2;
+        const uint32_t cell = vidx - cell0;
         if constexpr (FACE == FACE_RULES) {
             if (z >= zmax) return;
             InletRule iz = in;
             iz.gz += z;
-            reg_collide_store<SP, UNIT>(a, r, q, z, cell, src, &iz);
+            reg_collide_store<SP, UNIT, WALE>(a, r, q, z, cell, src, &iz, true, &w, &wr, vidx);
         } else if constexpr (FACE == FACE_PARTIAL) {
             if (z >= zmax) return;
             InletRule iz = in;
             iz.gz += z;
-            reg_collide_store<SP, UNIT>(a, r, q, z, cell, src, &iz, !fr.dead);
+            reg_collide_store<SP, UNIT, WALE>(a, r, q, z, cell, src, &iz, !fr.dead, &w, &wr, vidx);
         } else if (live(z)) {
-            reg_collide_store<SP, UNIT>(a, r, q, z, cell, src);
+            reg_collide_store<SP, UNIT, WALE>(a, r, q, z, cell, src, nullptr, true, &w, &wr, vidx);
+        }
+    };
+    // WALE: the plane's neighbour velocities, issued before the next plane's populations
+    auto wale_pre = [&](int z, PipeWale<T>& w) {
+        if constexpr (WALE) {
+            if (z < zmax && live(z)) wale_load(z, w);
+            __builtin_amdgcn_sched_barrier(0);
         }
     };
     // the per-lane offsets stay those of plane z0; later planes shift the uniform base
     typename SP::raw fa[27], fb[27];
+    PipeWale<T> wv;
     load(ra, z0, fa);
     for (int z = z0; z < z0 + np; z += 2) {
         wait_all_vmem();
+        wale_pre(z, wv);
         if (z + 1 < z0 + np) load(ra, z + 1, fb);
-        collide(ra, z, fa);
+        collide(ra, z, fa, wv);
         if (z + 1 >= z0 + np) break;
         wait_all_vmem();
+        wale_pre(z + 1, wv);
         if (z + 2 < z0 + np) load(ra, z + 2, fa);
-        collide(ra, z + 1, fb);
+        collide(ra, z + 1, fb, wv);
     }
 }
 
@@ -923,19 +1000,31 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(step_wa
 
 namespace oal {
 
-template <class SP, int NP, bool UNIT>
+// (WALE: the face-rule form for every whole block; the forms without rules spill with WALE's gather)
+template <class SP, int NP, bool UNIT, bool WALE>
 static void launch_pipe_u(int face, dim3 grid, hipStream_t s, const StepArgs& r) {
-    if (face == FACE_RULES) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_RULES, UNIT>), grid, dim3(WG), 0, s, r);
-    else if (face == FACE_SKIP) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_SKIP, UNIT>), grid, dim3(WG), 0, s, r);
-    else if (face == FACE_PARTIAL) {
-        if constexpr (NP <= 8) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_PARTIAL, UNIT>), grid, dim3(WG), 0, s, r);
+    if constexpr (WALE) {
+        if (face == FACE_PARTIAL) {
+            if constexpr (NP <= 8) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_PARTIAL, UNIT, true>), grid, dim3(WG), 0, s, r);
+        } else {
+            hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_RULES, UNIT, true>), grid, dim3(WG), 0, s, r);
+        }
+    } else {
+        if (face == FACE_RULES) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_RULES, UNIT, false>), grid, dim3(WG), 0, s, r);
+        else if (face == FACE_SKIP) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_SKIP, UNIT, false>), grid, dim3(WG), 0, s, r);
+        else if (face == FACE_PARTIAL) {
+            if constexpr (NP <= 8) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_PARTIAL, UNIT, false>), grid, dim3(WG), 0, s, r);
+        }
+        else hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_NONE, UNIT, false>), grid, dim3(WG), 0, s, r);
     }
-    else hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_NONE, UNIT>), grid, dim3(WG), 0, s, r);
 }
-template <class SP, int NP>
+// (the WALE forms come in 2, 4 and 8 planes; the launch caps the planes at 8)
+template <class SP, int NP, bool WALE = false>
 static void launch_pipe(int face, dim3 grid, hipStream_t s, const StepArgs& r) {
-    if (r.cp.unit) launch_pipe_u<SP, NP, true>(face, grid, s, r);
-    else launch_pipe_u<SP, NP, false>(face, grid, s, r);
+    if constexpr (!WALE || NP <= 8) {
+        if (r.cp.unit) launch_pipe_u<SP, NP, true, WALE>(face, grid, s, r);
+        else launch_pipe_u<SP, NP, false, WALE>(face, grid, s, r);
+    }
 }
 
 // One step of a level: launches over disjoint block lists (regular, other interior, regular boundary,
@@ -972,10 +1061,11 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
     }
     // face cells: the pipelined kernel applies the face rules itself, the one-plane regular kernel
     // leaves them to the face-cell kernel
-    constexpr bool pipe_capable = PAT == PAT_AB && !WALE && sizeof(typename SP::T) == 4;
+    constexpr bool pipe_capable = PAT == PAT_AB && sizeof(typename SP::T) == 4;
     if constexpr (PAT == PAT_AB) {
-        const bool piped = pipe_capable && a.pipe_planes > 1;   // (the pair kernel applies the rules too)
-        if (a.n_face && (!piped || a.face_kernel))
+        // (the pair kernel applies the rules too; with WALE the pipelined kernel always does)
+        const bool piped = pipe_capable && a.pipe_planes > 1;
+        if (a.n_face && (!piped || (a.face_kernel && !WALE)))
             hipLaunchKernelGGL((k_face<SP, WALE>), dim3((a.n_face + WG - 1) / WG), dim3(WG), 0, bs, a);
     }
     if (bulk0) {
@@ -989,26 +1079,28 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
         r.blist = list;
         r.regular_face = face ? 1 : 0;
         if constexpr (pipe_capable) {
-            const dim3 g(n * (PLANES / (a.pipe_planes > 1 ? a.pipe_planes : 1)));
-            if constexpr (SP::dithered) {
+            // (WALE: at most 8 planes per workgroup)
+            r.pipe_planes = WALE && a.pipe_planes > 8 ? 8 : a.pipe_planes;
+            const dim3 g(n * (PLANES / (r.pipe_planes > 1 ? r.pipe_planes : 1)));
+            if constexpr (SP::dithered && !WALE) {
                 if (pair) {
                     launch_pair<SP>(a.pipe_planes, face, g, stream, r);
                     return;
                 }
             }
-            const int fc = !face ? FACE_NONE : (a.face_kernel ? FACE_SKIP : FACE_RULES);
-            switch (a.pipe_planes) {
-                case 2: launch_pipe<SP, 2>(fc, g, stream, r); break;
-                case 4: launch_pipe<SP, 4>(fc, g, stream, r); break;
-                case 8: launch_pipe<SP, 8>(fc, g, stream, r); break;
-                case 16: launch_pipe<SP, 16>(fc, g, stream, r); break;
+            const int fc = !face ? FACE_NONE : (a.face_kernel && !WALE ? FACE_SKIP : FACE_RULES);
+            switch (r.pipe_planes) {
+                case 2: launch_pipe<SP, 2, WALE>(fc, g, stream, r); break;
+                case 4: launch_pipe<SP, 4, WALE>(fc, g, stream, r); break;
+                case 8: launch_pipe<SP, 8, WALE>(fc, g, stream, r); break;
+                case 16: launch_pipe<SP, 16, WALE>(fc, g, stream, r); break;
                 default: hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_REGULAR>), g, dim3(WG), 0, stream, r); break;
             }
         } else {
             hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_REGULAR>), dim3(n * PLANES), dim3(WG), 0, stream, r);
         }
     };
-    const bool pair = pipe_capable && SP::dithered && a.pair_kernel && a.pipe_planes > 1 &&
+    const bool pair = pipe_capable && SP::dithered && !WALE && a.pair_kernel && a.pipe_planes > 1 &&
                       !(a.regular_face && a.face_kernel);
     if (pair && a.pair_kernel == 2) {
         // the blocks on a domain face and the others apart (the latter with the smaller register budget)
@@ -1027,9 +1119,9 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
             rp.pipe_planes = a.pipe_planes > 8 ? 8 : a.pipe_planes;
             const dim3 g(a.n_partial * (PLANES / (rp.pipe_planes > 1 ? rp.pipe_planes : 1)));
             switch (rp.pipe_planes) {
-                case 2: launch_pipe<SP, 2>(FACE_PARTIAL, g, bs, rp); break;
-                case 4: launch_pipe<SP, 4>(FACE_PARTIAL, g, bs, rp); break;
-                case 8: launch_pipe<SP, 8>(FACE_PARTIAL, g, bs, rp); break;
+                case 2: launch_pipe<SP, 2, WALE>(FACE_PARTIAL, g, bs, rp); break;
+                case 4: launch_pipe<SP, 4, WALE>(FACE_PARTIAL, g, bs, rp); break;
+                case 8: launch_pipe<SP, 8, WALE>(FACE_PARTIAL, g, bs, rp); break;
                 default: hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), g, dim3(WG), 0, bs, rp); break;
             }
         } else {
