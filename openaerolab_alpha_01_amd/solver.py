@@ -436,13 +436,41 @@ def acoustic_taus(tau_finest: float, n_levels: int) -> list[float]:
     return [0.5 + (tau_finest - 0.5) / 2.0 ** (n_levels - 1 - l) for l in range(n_levels)]
 
 
+TAU_MIN = 0.501   # advanced.numerics.tau_min (src/io/config_loader.jl:83)
+
+
+def level_taus(tau_finest: float, n_levels: int, rule: str = "legacy", tau_min: float = TAU_MIN) -> list[float]:
+    """Relaxation times of the levels (coarsest first) by the GPU solver's rules
+    (src/solver/physics_scaling.jl:654-682, advanced.refinement.level_viscosity):
+      legacy (the default):  tau_l = 1/2 + (tau_f - 1/2) 2^(L-1-l);
+      physical:              tau_l = max(1/2 + (tau_f - 1/2) / 2^(L-1-l), tau_min) below the finest level;
+    the finest level keeps tau_f; each value is rounded to Float32 as the reference stores it."""
+    if rule not in ("legacy", "physical"):
+        raise ValueError("rule must be 'legacy' or 'physical'")
+    out = []
+    for l in range(n_levels):
+        k = n_levels - 1 - l
+        if k == 0:
+            t = tau_finest
+        elif rule == "legacy":
+            t = 0.5 + (tau_finest - 0.5) * 2.0 ** k
+        else:
+            t = max(0.5 + (tau_finest - 0.5) / 2.0 ** k, float(np.float32(tau_min)))
+        out.append(float(np.float32(t)))
+    return out
+
+
 def octree_plan(grids: list[BlockGrid], tau_finest: float, storage: str = "f32", params: StepParams | None = None,
                 u_inlet: float = 0.0, body_force=(0.0, 0.0, 0.0), rounding: str = "stochastic",
-                bb_mode: str = "halfway") -> Plan:
+                bb_mode: str = "halfway", taus: list[float] | None = None) -> Plan:
     """A refined plan over the levels of a block octree (coarsest first, e.g. geometry.sphere_octree):
-    one domain per level with the acoustic-scaling relaxation times; the inlet velocity is the same in
-    every level's lattice units."""
-    taus = acoustic_taus(tau_finest, len(grids))
+    one domain per level, by default with the acoustic-scaling relaxation times (taus: explicit per-level
+    values instead, e.g. level_taus(..., "legacy")); the inlet velocity is the same in every level's
+    lattice units."""
+    if taus is None:
+        taus = acoustic_taus(tau_finest, len(grids))
+    elif len(taus) != len(grids):
+        raise ValueError("one tau per level")
     levels = [Level(g, [Domain(extent=g.n, tau=t, periodic=g.periodic, u_inlet=u_inlet, body_force=tuple(body_force))])
               for g, t in zip(grids, taus)]
     return Plan(levels, storage=storage, stream="ab", bb_mode=bb_mode, params=params, rounding=rounding)

@@ -33,10 +33,20 @@ def _three():
     return block_octree((64, 16, 64), [{(1, 0, 1)}, {(2, 0, 2)}], periodic=(True, True, True))
 
 
+def _sphere4():
+    # the block-AMR sphere case at reduced extent: 4 levels, inlet / outlet channel, the sphere (radius 6
+    # finest cells) on the finest level, refinement by distance to its surface (2, 16, 64, 32 blocks)
+    return sphere_octree((32, 16, 16), 4, (80.0, 64.0, 64.0), 6.0, (8, 6, 3))
+
+
 CASES = {
     "slab_periodic": dict(grids=_slab, tau=0.56, u_inlet=0.0, u0=(0.03, 0.0, 0.0)),
     "sphere_channel": dict(grids=_sphere2, tau=0.54, u_inlet=0.04, u0=(0.04, 0.0, 0.0)),
     "three_levels": dict(grids=_three, tau=0.58, u_inlet=0.0, u0=(0.02, 0.01, 0.0)),
+}
+FOUR = {
+    "acoustic": dict(grids=_sphere4, tau=0.52, u_inlet=0.04, u0=(0.04, 0.0, 0.0)),
+    "legacy": dict(grids=_sphere4, tau=0.52, u_inlet=0.04, u0=(0.04, 0.0, 0.0), rule="legacy"),
 }
 
 
@@ -56,18 +66,19 @@ def level_fields(g, seed, u0, amp=0.02):
     return rho, u
 
 
-def run_both(oracle, octree, case, storage, steps, rounding="stochastic", force=False):
-    cfg = CASES[case]
+def run_both(oracle, octree, case, storage, steps, rounding="stochastic", force=False, cfg=None):
+    cfg = cfg or CASES[case]
     grids = cfg["grids"]()
     fields = [level_fields(g, 7 + l, cfg["u0"]) for l, g in enumerate(grids)]
-    taus = solver.acoustic_taus(cfg["tau"], len(grids))
+    rule = cfg.get("rule")
+    taus = solver.level_taus(cfg["tau"], len(grids), rule) if rule else solver.acoustic_taus(cfg["tau"], len(grids))
     rcode = {"stochastic": oracle.STOCHASTIC, "nearest": oracle.NEAREST}[rounding]
     run = octree.OctreeRun(grids, storage, taus, u_inlet=cfg["u_inlet"], rounding=rcode)
     run.init([f[0] for f in fields], [f[1] for f in fields])
     run.step(steps, force=force)
     params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
     plan = solver.octree_plan(grids, cfg["tau"], storage=storage, params=params, u_inlet=cfg["u_inlet"],
-                              rounding=rounding)
+                              rounding=rounding, taus=taus)
     for l, (rho, u) in enumerate(fields):
         plan.init_equilibrium(rho, u, level=l)
     print(f"octree case {case} storage {storage} rounding {rounding} levels {len(grids)} "
@@ -98,6 +109,21 @@ def test_octree_parity(oracle, octree, case, storage):
     with plan:
         for l in range(len(grids)):
             compare_level(oracle, plan, run, storage, l)
+
+
+@pytest.mark.parametrize("rule,storage", [("acoustic", "f32"), ("legacy", "f16s")])
+def test_octree_four_levels_sphere(oracle, octree, rule, storage):
+    """The block-AMR sphere case's depth: 4 levels with inlet, outlet and the sphere on the finest level,
+    one coarse step (8 finest sub-steps), with the acoustic relaxation times and with the GPU solver's
+    legacy per-level rule (src/solver/physics_scaling.jl:654-682): every level's caller cells bit for bit."""
+    grids, run, plan = run_both(oracle, octree, rule, storage, steps=1, force=True, cfg=FOUR[rule])
+    with plan:
+        assert len(grids) == 4
+        for l in range(4):
+            compare_level(oracle, plan, run, storage, l)
+        fine = 3
+        np.testing.assert_allclose(plan.force(level=fine)[0], np.asarray(run.force[fine], dtype=np.float64),
+                                   rtol=1e-11, atol=1e-14)
 
 
 def test_octree_parity_nearest_rounding(oracle, octree):
