@@ -571,6 +571,8 @@ def main(argv=None) -> int:
     ap.add_argument("--tgv-cpu-steps", type=int, default=200, help="steps of the CPU oracle in --tgv")
     ap.add_argument("--amr-legs", default="f16s:1,f16s:0,f32:1",
                     help="block-AMR legs timed, storage:graph comma-separated; the first is the reported value")
+    ap.add_argument("--no-aa-leg", action="store_true",
+                    help="skip the in-place AA-pattern leg (FP32, one population buffer) of the default line")
     ap.add_argument("--no-wale-leg", action="store_true",
                     help="skip the WALE-on leg (FP32, c_wale 0.2, inlet noise 0.01) of the default line")
     ap.add_argument("--acc-steps", type=int, default=1500,
@@ -684,6 +686,20 @@ def main(argv=None) -> int:
         bw = copy_bandwidth_gbs()
         out["roofline"]["measured_copy_gbs"] = round(bw, 1)
         out["roofline"]["frac_of_measured_copy"] = round(achieved / bw, 4)
+    if not args.no_aa_leg and args.stream == "ab":
+        # the in-place AA pattern on the same workload: one population buffer instead of two
+        aa_steps, aa_warm = args.steps + args.steps % 2, args.warmup + args.warmup % 2
+        ms_ta, ms_ka, _, Fa, tuna, infoa = gpu_leg(grid, "f32", "aa", aa_steps, aa_warm, not args.no_autotune, fixed, qmap)
+        ma = n_cells * aa_steps / (ms_ta * 1e-3) / 1e6
+        sa = bytes_per_lup("f32") * n_fluid / (ms_ta / aa_steps * 1e-3) / 1e9
+        out["aa_pattern"] = {"value": round(ma, 1), "unit": "MLUPS", "steps": aa_steps,
+                             "ms_per_step": round(ms_ta / aa_steps, 4), "vs_ab": round(ma / mlups, 3),
+                             "population_bytes": 27 * 4 * n_cells, "population_bytes_ab": 2 * 27 * 4 * n_cells,
+                             "step_achieved_gbs": round(sa, 1), "step_frac": round(sa / HBM_PEAK_GBS, 4),
+                             "tuning": {k: tuna[k] for k in ("pipe_planes", "overlap", "face_kernel")},
+                             "blocks": {"pipelined_regular": infoa["n_regular"],
+                                        "one_plane_boundary": infoa["n_boundary_regular"] + infoa["n_boundary"]},
+                             "cd_transient": round(float(Fa[0] / (0.5 * U_IN ** 2 * area)), 5)}
     if not args.no_wale_leg:
         # the reference's production collision on the same workload: WALE (c_wale 0.2) and inlet noise (0.01)
         ms_tw, ms_kw, _, Fw, tunw, _ = gpu_leg(grid, "f32", args.stream, args.steps, args.warmup,

@@ -967,6 +967,88 @@ k_step_pipe(const StepArgs a) {
     }
 }
 
+// The in-place AA pattern on the regular blocks (no domain face, no solid near; neighbour blocks at
+// lexicographic offsets), software-pipelined over z as k_step_pipe.  The even half-step reads each
+// population from the cell's own slot k and writes f*_k to its own slot opp k; the odd half-step reads
+// slot opp k of x - c_k and writes f*_k to slot k of x + c_k.  Either way a lane reads and writes the
+// same 27 slots and no other lane touches them, so the next plane's prefetch never meets a store of the
+// current plane.  The storage rounding of f*_k takes the bits of (cell, k, step) as in every pattern.
+// (the odd half-step's two-sided addressing needs the 2-wave register budget)
+template <class SP, int PAT>
+constexpr int pipe_aa_waves() { return PAT == PAT_AA_ODD ? OAL_FACE_RULES_WAVES : pipe_waves<SP, FACE_NONE>(); }
+
+template <class SP, int NP, int PAT, bool UNIT>
+__global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(pipe_aa_waves<SP, PAT>(), pipe_aa_waves<SP, PAT>())))
+k_step_pipe_aa(const StepArgs a) {
+    using T = typename SP::T;
+    using st = typename SP::st;
+    const int np = a.pipe_planes;   // (a run-time bound, as in k_step_pipe)
+    constexpr uint32_t groups = (uint32_t)(PLANES / NP);
+    const uint32_t litem = xcd_item(blockIdx.x, gridDim.x);
+    const uint32_t b = (uint32_t)a.blist[litem / groups];
+    const int z0 = (int)(litem % groups) * np;
+    CellCtx c;
+    c.b = b;
+    c.lx = threadIdx.x & (BS - 1);
+    c.ly = threadIdx.x >> 4;
+    c.z = z0;
+    c.own = b * BS3 + (uint32_t)z0 * BS2 + (uint32_t)c.ly * BS + (uint32_t)c.lx;
+    const DevDomain* dm = a.doms + a.bdom[b];
+    RegAddr<SP> ra;
+    ra.init(c, dm->nbx, dm->nby);
+    CellRates<T> q;
+    if constexpr (sizeof(T) == 8) { q.omega1 = dm->omega1_d; q.omega4_diag = dm->omega4d_d; }
+    else { q.omega1 = dm->omega1; q.omega4_diag = dm->omega4d; }
+    q.forced = dm->forced != 0;
+    q.Fx = T(dm->F[0]); q.Fy = T(dm->F[1]); q.Fz = T(dm->F[2]);
+    const uint32_t cell0 = dm->cell0;
+    const uint32_t key = SP::dithered ? round_key(a.step()) : 0u;
+    auto load = [&](int z, typename SP::raw (&r)[27]) {
+        for_dirs([&](auto kc) {
+            constexpr int K = decltype(kc)::value;
+            if constexpr (PAT == PAT_AA_EVEN)
+                r[K] = SP::ld_raw((const st*)(ra.base(a.fin, K, z, 0) + ra.v[1][1]));
+            else
+                r[K] = SP::ld_raw((const st*)(ra.base(a.fin, dopp(K), z, dcz(K)) + ra.v[dcx(K) + 1][dcy(K) + 1]));
+        });
+    };
+    auto collide = [&](int z, const typename SP::raw (&r)[27]) {
+        T f[27];
+        for_dirs([&](auto kc) {
+            constexpr int K = decltype(kc)::value;
+            f[K] = SP::from_raw(r[K], wgt<T>(K));
+        });
+        T rho_o, ux_o, uy_o, uz_o;
+        collide_cell<T, UNIT>(f, q, a.cp, rho_o, ux_o, uy_o, uz_o);
+        const bool dith = SP::dithered && a.cp.dither;
+        const uint32_t hc = SP::dithered ? round_cell(key, c.own + (uint32_t)(z - z0) * BS2 - cell0) : 0u;
+        for_dirs([&](auto kc) {
+            constexpr int K = decltype(kc)::value;
+            st* p;
+            if constexpr (PAT == PAT_AA_EVEN) p = (st*)(ra.base(a.fout, dopp(K), z, 0) + ra.v[1][1]);
+            else p = (st*)(ra.base(a.fout, K, z, -dcz(K)) + ra.v[1 - dcx(K)][1 - dcy(K)]);
+            SP::sto_r(p, f[K], wgt<T>(K), round_bits(hc, K), dith);
+        });
+    };
+    typename SP::raw fa[27], fb[27];
+    load(z0, fa);
+    for (int z = z0; z < z0 + np; z += 2) {
+        wait_all_vmem();
+        if (z + 1 < z0 + np) load(z + 1, fb);
+        collide(z, fa);
+        if (z + 1 >= z0 + np) break;
+        wait_all_vmem();
+        if (z + 2 < z0 + np) load(z + 2, fa);
+        collide(z + 1, fb);
+    }
+}
+
+template <class SP, int NP, int PAT>
+static void launch_pipe_aa(dim3 grid, hipStream_t s, const StepArgs& r) {
+    if (r.cp.unit) hipLaunchKernelGGL((k_step_pipe_aa<SP, NP, PAT, true>), grid, dim3(WG), 0, s, r);
+    else hipLaunchKernelGGL((k_step_pipe_aa<SP, NP, PAT, false>), grid, dim3(WG), 0, s, r);
+}
+
 // Face cells of the regular-list blocks (AB pattern): one lane per listed cell, the general path with
 // the domain-face rules.  These blocks hold no solid cell and no solid link, so no force is summed.
 template <class SP, bool WALE>
@@ -1094,6 +1176,16 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
                 case 4: launch_pipe<SP, 4, WALE>(fc, g, stream, r); break;
                 case 8: launch_pipe<SP, 8, WALE>(fc, g, stream, r); break;
                 case 16: launch_pipe<SP, 16, WALE>(fc, g, stream, r); break;
+                default: hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_REGULAR>), g, dim3(WG), 0, stream, r); break;
+            }
+        } else if constexpr (PAT != PAT_AB && !WALE && sizeof(typename SP::T) == 4) {
+            // the AA pattern's regular blocks (none on a domain face): pipelined over z, 2 to 8 planes
+            r.pipe_planes = a.pipe_planes > 8 ? 8 : a.pipe_planes;
+            const dim3 g(n * (PLANES / (r.pipe_planes > 1 ? r.pipe_planes : 1)));
+            switch (r.pipe_planes) {
+                case 2: launch_pipe_aa<SP, 2, PAT>(g, stream, r); break;
+                case 4: launch_pipe_aa<SP, 4, PAT>(g, stream, r); break;
+                case 8: launch_pipe_aa<SP, 8, PAT>(g, stream, r); break;
                 default: hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_REGULAR>), g, dim3(WG), 0, stream, r); break;
             }
         } else {

@@ -220,6 +220,31 @@ def test_wale_pipelined(oracle, storage, where, planes, rates):
     plan.close()
 
 
+@pytest.mark.parametrize("rates", ["unit", "fixed"])
+@pytest.mark.parametrize("planes", [1, 2, 4, 8])
+@pytest.mark.parametrize("storage", ["f32", "f16s", "bf16s"])
+def test_aa_pipelined(oracle, storage, planes, rates):
+    """The AA pattern's regular blocks (8 of 6 x 4 x 4, off every face and the sphere) on the pipelined
+    even and odd half-step kernels: after an even number of steps the state is the pull step's, the
+    oracle's bits."""
+    grid = uniform_grid((96, 64, 64))
+    grid.solid = sphere_mask(grid, (22, 30, 33), 4.5)
+    okw = dict(tau=0.53, u_inlet=0.05, inlet_turbulence=0.01)
+    if rates == "fixed":
+        okw.update(omega_bulk=1.2, omega_3=0.9, omega_4=1.1, adaptive_omega4=0)
+    rho, u = fields(grid, seed=17, u0=(0.05, 0, 0), amp=0.03)
+    steps = 4
+    f_or, _, _ = oracle_run(oracle, grid, storage, steps, okw, rho, u)
+    plan = gpu_plan(grid, storage, okw, stream="aa")
+    assert plan.level_info()["n_regular"] >= 8
+    plan.set_tuning(pipe_planes=planes, overlap=1, face_kernel=0, graph=0, pair_kernel=2)
+    plan.init_equilibrium(rho, u)
+    for t in range(steps):
+        plan.level_step(0, t)
+    compare(oracle, plan, storage, f_or, grid)
+    plan.close()
+
+
 @pytest.mark.parametrize("storage", ["f32", "f16s", "bf16s", "f64"])
 def test_aa_pattern_equals_ab(storage):
     """The in-place AA pattern reproduces the pull/AB step bit for bit (after every even step),
