@@ -133,7 +133,14 @@ typedef struct {
 typedef struct {
     int32_t abi_version;       /* OAL_ABI_VERSION */
     int32_t storage;           /* OAL_F32 / OAL_F64 / OAL_F16S / OAL_BF16S */
-    int32_t block_size;        /* cells per block edge: 16 */
+    int32_t block_size;        /* cells per block edge: 16, or 8, the reference's BlockLevel
+                                  (src/geometry/blocks.jl:14, BLOCK_SIZE = 8), for one-level plans:
+                                  the library regroups each 2 x 2 x 2 of them into one of its 16^3
+                                  blocks, which must then hold every octant that reaches into the
+                                  domain extent (OAL_ERR_UNSUPPORTED otherwise); every per-cell host
+                                  array (f, rho, u, q-map, sponge, LBM1 files) stays in the caller's
+                                  8^3 layout.  The 16-bit forms' stochastic rounding draws its bits by
+                                  the library's cell numbering. */
     int32_t stream_mode;       /* OAL_STREAM_AB or OAL_STREAM_AA */
     int32_t bb_mode;           /* OAL_BB_REFERENCE or OAL_BB_HALFWAY */
     int32_t n_levels;          /* 1 for a uniform grid */
@@ -167,7 +174,8 @@ int oal_level_cells(const OalPlan* plan, int level, int64_t* n_cells);
  * that overhang the extent (n_partial), launched on a second stream that overlaps the bulk launches.
  * The bulk launches that oal_time_steps times (ms_kernel) cover the n_regular and n_interior blocks. */
 typedef struct OalLevelInfo {
-    int64_t n_cells;         /* n_blocks * bs^3 */
+    int64_t n_cells;         /* n_blocks * bs^3 (of the caller's blocks, 8^3 ones included; the
+                                kernel counts below are of the library's 16^3 blocks) */
     int64_t n_fluid_cells;   /* inside the domain extent and not solid */
     int32_t n_blocks;
     int32_t n_regular;

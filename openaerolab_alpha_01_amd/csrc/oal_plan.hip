@@ -109,6 +109,11 @@ struct Level {
     DevBuf<uint32_t> lmask;            // per-cell solid-link masks of blocks near solid cells
     std::vector<int32_t> lslot_h;      // host copy of lslot
     std::vector<int32_t> map_h;        // [3 * nb] 1-based block coordinates (caller's blocks first)
+    // a level the caller gave in 8^3 blocks: caller cell (b8 * 512 + local) -> library cell, and the
+    // caller's 1-based 8^3 block coordinates and obstacle (empty for 16^3 levels)
+    std::vector<uint32_t> c8map;
+    std::vector<int32_t> map8_h;
+    std::vector<uint8_t> obst8_h;
     std::vector<uint8_t> obst_h;       // [nb * BS3] obstacle of the caller's layout, empty = none
     int32_t n_lslots = 0;
     // interpolated bounce-back: per link-mask slot [27][BS3] Float16 q codes (empty: halfway only)
@@ -1229,6 +1234,146 @@ int refresh_velocity(OalPlan* p, Level& L) {
     return OAL_OK;
 }
 
+// ---- The reference's 8^3 BlockLevel (src/geometry/blocks.jl:14, BLOCK_SIZE = 8) -----------------------
+// A level given in 8^3 blocks is regrouped into the library's 16^3 blocks: each 16^3 block holds the
+// eight 8^3 blocks (octants) 2B + (0|1, 0|1, 0|1).  Every octant of a 16^3 block that reaches into the
+// domain extent must be present (no holes: the kernels' neighbour rules work per 16^3 block); octants
+// wholly beyond the extent may be absent and their cells are never stepped or read through.  The 16^3
+// neighbour table follows the caller's 8^3 table (so periodic wraps and separate domains carry over).
+constexpr int BS8 = 8, BS8_3 = 512;
+struct Regroup8 {
+    std::vector<int32_t> mx, my, mz, nbr, bdom;   // the 16^3 level; nbr column-major nb x 27, 1-based
+    std::vector<uint8_t> obst;                    // [nb * BS3] or empty
+    std::vector<uint32_t> cmap;                   // caller cell -> library cell
+};
+
+int regroup8(const OalLevelDesc& d, Regroup8& R) {
+    if (d.n_blocks <= 0) return fail(OAL_ERR_ARG, "level with no blocks");
+    if (!d.map_x || !d.map_y || !d.map_z || !d.neighbor_table) return fail(OAL_ERR_ARG, "level block arrays missing");
+    if (d.n_domains <= 0 || !d.domains) return fail(OAL_ERR_ARG, "level has no domain descriptors");
+    const int n8 = d.n_blocks;
+    struct Key { int x, y, z, dom; };
+    std::unordered_map<uint64_t, int> idx16;     // 16^3 block coordinate -> 16^3 block
+    std::vector<Key> keys;
+    std::vector<std::array<int32_t, 8>> oct;      // 8^3 block of each octant, -1 = absent
+    auto pack = [](int x, int y, int z) { return ((uint64_t)(uint32_t)x << 42) | ((uint64_t)(uint32_t)y << 21) | (uint64_t)(uint32_t)z; };
+    std::vector<int32_t> b16(n8), o8(n8);
+    for (int b = 0; b < n8; ++b) {
+        if (d.map_x[b] < 1 || d.map_y[b] < 1 || d.map_z[b] < 1) return fail(OAL_ERR_ARG, "block coordinates are 1-based");
+        if (d.map_x[b] > (1 << 20) || d.map_y[b] > (1 << 20) || d.map_z[b] > (1 << 20)) return fail(OAL_ERR_ARG, "block coordinate too large");
+        const int cx = d.map_x[b] - 1, cy = d.map_y[b] - 1, cz = d.map_z[b] - 1;
+        const int dom = d.block_domain ? d.block_domain[b] : 0;
+        if (dom < 0 || dom >= d.n_domains) return fail(OAL_ERR_ARG, "block_domain out of range");
+        const uint64_t k = pack(cx >> 1, cy >> 1, cz >> 1);
+        auto it = idx16.find(k);
+        int B;
+        if (it == idx16.end()) {
+            B = (int)keys.size();
+            idx16.emplace(k, B);
+            keys.push_back({cx >> 1, cy >> 1, cz >> 1, dom});
+            std::array<int32_t, 8> a;
+            a.fill(-1);
+            oct.push_back(a);
+        } else {
+            B = it->second;
+            if (keys[B].dom != dom) return fail(OAL_ERR_UNSUPPORTED, "8^3 blocks: the octants of a 16^3 block must share one domain");
+        }
+        const int o = (cx & 1) + 2 * (cy & 1) + 4 * (cz & 1);
+        if (oct[B][o] >= 0) return fail(OAL_ERR_ARG, "two blocks with the same coordinates");
+        oct[B][o] = b;
+        b16[b] = B;
+        o8[b] = o;
+    }
+    const int n16 = (int)keys.size();
+    // no holes inside the extent
+    for (int B = 0; B < n16; ++B) {
+        const OalDomainDesc& dm = d.domains[keys[B].dom];
+        for (int o = 0; o < 8; ++o) {
+            if (oct[B][o] >= 0) continue;
+            const int ox = (2 * keys[B].x + (o & 1)) * BS8, oy = (2 * keys[B].y + ((o >> 1) & 1)) * BS8,
+                      oz = (2 * keys[B].z + (o >> 2)) * BS8;
+            if (ox < dm.extent[0] && oy < dm.extent[1] && oz < dm.extent[2])
+                return fail(OAL_ERR_UNSUPPORTED, "8^3 blocks: a 16^3 block of the library would miss an octant inside the "
+                                                 "domain extent (levels with holes at that scale are not supported)");
+        }
+    }
+    // library order: by domain, then z, y, x
+    std::vector<int> order(n16);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+        const Key &p = keys[a], &q = keys[b];
+        if (p.dom != q.dom) return p.dom < q.dom;
+        if (p.z != q.z) return p.z < q.z;
+        if (p.y != q.y) return p.y < q.y;
+        return p.x < q.x;
+    });
+    std::vector<int> rank(n16);
+    for (int i = 0; i < n16; ++i) rank[order[i]] = i;
+    R.mx.resize(n16); R.my.resize(n16); R.mz.resize(n16); R.bdom.resize(n16);
+    for (int B = 0; B < n16; ++B) {
+        const int i = rank[B];
+        R.mx[i] = keys[B].x + 1; R.my[i] = keys[B].y + 1; R.mz[i] = keys[B].z + 1; R.bdom[i] = keys[B].dom;
+    }
+    // neighbour table: for direction k, the octants on that side of the block must agree on the 16^3
+    // block their 8^3 neighbours fall in (or all have none)
+    R.nbr.assign((size_t)27 * n16, 0);
+    for (int B = 0; B < n16; ++B) {
+        for (int k = 0; k < 27; ++k) {
+            const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
+            if (k == 13) { R.nbr[(size_t)13 * n16 + rank[B]] = rank[B] + 1; continue; }
+            int found = -2;   // -2: no octant looked at yet
+            for (int o = 0; o < 8; ++o) {
+                const int ox = o & 1, oy = (o >> 1) & 1, oz = o >> 2;
+                if ((dx > 0 && ox != 1) || (dx < 0 && ox != 0) || (dy > 0 && oy != 1) || (dy < 0 && oy != 0) ||
+                    (dz > 0 && oz != 1) || (dz < 0 && oz != 0))
+                    continue;
+                const int b = oct[B][o];
+                if (b < 0) continue;
+                const int32_t v = d.neighbor_table[(size_t)k * n8 + b];
+                if (v < 0 || v > n8) return fail(OAL_ERR_ARG, "neighbor_table entry out of range");
+                const int nb16 = v == 0 ? -1 : rank[b16[v - 1]];
+                if (found == -2) found = nb16;
+                else if (found != nb16)
+                    return fail(OAL_ERR_UNSUPPORTED, "8^3 blocks: the neighbour table does not regroup into 16^3 blocks");
+            }
+            R.nbr[(size_t)k * n16 + rank[B]] = found >= 0 ? found + 1 : 0;
+        }
+    }
+    // cells
+    R.cmap.resize((size_t)n8 * BS8_3);
+    if (d.obstacle) R.obst.assign((size_t)n16 * BS3, 0);
+    for (int b = 0; b < n8; ++b) {
+        const int o = o8[b], i = rank[b16[b]];
+        const int x0 = (o & 1) * BS8, y0 = ((o >> 1) & 1) * BS8, z0 = (o >> 2) * BS8;
+        for (int z = 0; z < BS8; ++z)
+            for (int y = 0; y < BS8; ++y)
+                for (int x = 0; x < BS8; ++x) {
+                    const size_t c8 = (size_t)b * BS8_3 + (size_t)(z * BS8 + y) * BS8 + x;
+                    const uint32_t c16 = (uint32_t)i * BS3 + (uint32_t)((z0 + z) * BS + (y0 + y)) * BS + (uint32_t)(x0 + x);
+                    R.cmap[c8] = c16;
+                    if (d.obstacle) R.obst[c16] = d.obstacle[c8] ? 1 : 0;
+                }
+    }
+    return OAL_OK;
+}
+
+// per-cell host arrays between the caller's 8^3 layout (comps x n8 cells) and the library's
+// (comps x ncell_host cells; cells outside the caller's blocks take `fill`)
+template <class T>
+std::vector<T> cells_to_lib(const Level& L, const T* src, int comps, T fill) {
+    const size_t n8 = L.c8map.size(), n = L.ncell_host;
+    std::vector<T> out((size_t)comps * n, fill);
+    for (int c = 0; c < comps; ++c)
+        for (size_t i = 0; i < n8; ++i) out[(size_t)c * n + L.c8map[i]] = src[(size_t)c * n8 + i];
+    return out;
+}
+template <class T>
+void cells_from_lib(const Level& L, const T* src, T* dst, int comps) {
+    const size_t n8 = L.c8map.size(), n = L.ncell_host;
+    for (int c = 0; c < comps; ++c)
+        for (size_t i = 0; i < n8; ++i) dst[(size_t)c * n8 + i] = src[(size_t)c * n + L.c8map[i]];
+}
+
 }  // namespace
 
 extern "C" {
@@ -1255,7 +1400,9 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
     *out = nullptr;
     if (desc->abi_version != OAL_ABI_VERSION) return fail(OAL_ERR_ARG, "ABI version mismatch");
     if (desc->storage < OAL_F32 || desc->storage > OAL_BF16S) return fail(OAL_ERR_ARG, "unknown storage");
-    if (desc->block_size != BS) return fail(OAL_ERR_UNSUPPORTED, "block_size must be 16");
+    if (desc->block_size != BS && desc->block_size != BS8) return fail(OAL_ERR_UNSUPPORTED, "block_size must be 16 or 8");
+    if (desc->block_size == BS8 && desc->n_levels != 1)
+        return fail(OAL_ERR_UNSUPPORTED, "8^3 blocks: one level per plan (refined plans take 16^3 blocks)");
     if (desc->stream_mode != OAL_STREAM_AB && desc->stream_mode != OAL_STREAM_AA) return fail(OAL_ERR_ARG, "unknown stream_mode");
     if (desc->bb_mode != OAL_BB_REFERENCE && desc->bb_mode != OAL_BB_HALFWAY) return fail(OAL_ERR_ARG, "unknown bb_mode");
     if (desc->stream_mode == OAL_STREAM_AA && desc->bb_mode != OAL_BB_HALFWAY)
@@ -1306,6 +1453,15 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
     for (int l = 0; l < desc->n_levels; ++l) {
         OalLevelDesc d = desc->levels[l];
         std::vector<uint8_t> inactive;
+        Regroup8 r8;
+        if (desc->block_size == BS8) {
+            if (int rc = regroup8(d, r8)) return bail(rc);
+            d.n_blocks = (int32_t)r8.mx.size();
+            d.map_x = r8.mx.data(); d.map_y = r8.my.data(); d.map_z = r8.mz.data();
+            d.neighbor_table = r8.nbr.data();
+            d.block_domain = d.block_domain ? r8.bdom.data() : nullptr;
+            d.obstacle = d.obstacle ? r8.obst.data() : nullptr;
+        }
         if (ml) {
             const OctLevel& o = oct[l];
             d.n_blocks = o.nb();
@@ -1321,6 +1477,17 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
                                      storage_bytes(desc->storage), p->levels[l],
                                      host[l], ml ? &inactive : nullptr);
         if (rc) return bail(rc);
+        if (desc->block_size == BS8) {
+            Level& L = p->levels[l];
+            const OalLevelDesc& d8 = desc->levels[l];
+            L.c8map = std::move(r8.cmap);
+            L.map8_h.resize(3 * (size_t)d8.n_blocks);
+            for (int b = 0; b < d8.n_blocks; ++b) {
+                L.map8_h[3 * b] = d8.map_x[b]; L.map8_h[3 * b + 1] = d8.map_y[b]; L.map8_h[3 * b + 2] = d8.map_z[b];
+            }
+            if (d8.obstacle) L.obst8_h.assign(d8.obstacle, d8.obstacle + (size_t)d8.n_blocks * BS8_3);
+            for (auto& v : L.obst8_h) v = v ? 1 : 0;
+        }
         if (ml) {
             Level& L = p->levels[l];
             const OctLevel& o = oct[l];
@@ -1471,7 +1638,8 @@ int oal_plan_destroy(OalPlan* p) {
 int oal_level_cells(const OalPlan* p, int level, int64_t* n) {
     if (int rc = check_level(p, level)) return rc;
     if (!n) return fail(OAL_ERR_ARG, "NULL output");
-    *n = p->levels[level].ncell_host;
+    const Level& L = p->levels[level];
+    *n = L.c8map.empty() ? (int64_t)L.ncell_host : (int64_t)L.c8map.size();
     return OAL_OK;
 }
 
@@ -1480,9 +1648,9 @@ int oal_level_info(const OalPlan* p, int level, OalLevelInfo* info) {
     if (!info) return fail(OAL_ERR_ARG, "NULL output");
     const Level& L = p->levels[level];
     *info = OalLevelInfo{};
-    info->n_cells = L.ncell_host;
+    info->n_cells = L.c8map.empty() ? (int64_t)L.ncell_host : (int64_t)L.c8map.size();
     info->n_fluid_cells = L.n_fluid;
-    info->n_blocks = (int32_t)L.n_caller_blocks;
+    info->n_blocks = L.c8map.empty() ? (int32_t)L.n_caller_blocks : (int32_t)(L.map8_h.size() / 3);
     info->n_ghost_blocks = (int32_t)L.n_ghost_blocks;
     info->n_covered_blocks = (int32_t)L.n_covered_blocks;
     info->n_regular = (int32_t)L.n_regular;
@@ -1494,7 +1662,7 @@ int oal_level_info(const OalPlan* p, int level, OalLevelInfo* info) {
     return OAL_OK;
 }
 
-int oal_upload_f(OalPlan* p, int level, const void* host_f, int dt) {
+static int upload_f_lib(OalPlan* p, int level, const void* host_f, int dt) {
     if (int rc = check_level(p, level)) return rc;
     if (!host_f || (dt != OAL_F32 && dt != OAL_F64)) return fail(OAL_ERR_ARG, "host_f must be OAL_F32 or OAL_F64 data");
     OAL_HIP(hipSetDevice(p->device));
@@ -1520,7 +1688,7 @@ int oal_upload_f(OalPlan* p, int level, const void* host_f, int dt) {
     return refresh_velocity(p, L);
 }
 
-int oal_download_f(OalPlan* p, int level, void* host_f, int dt) {
+static int download_f_lib(OalPlan* p, int level, void* host_f, int dt) {
     if (int rc = check_level(p, level)) return rc;
     if (!host_f || (dt != OAL_F32 && dt != OAL_F64)) return fail(OAL_ERR_ARG, "host_f must be OAL_F32 or OAL_F64 data");
     Level& L = p->levels[level];
@@ -1538,7 +1706,7 @@ int oal_download_f(OalPlan* p, int level, void* host_f, int dt) {
     return OAL_OK;
 }
 
-int oal_init_equilibrium(OalPlan* p, int level, const double* rho, const double* u, double rho0, const double u0[3]) {
+static int init_equilibrium_lib(OalPlan* p, int level, const double* rho, const double* u, double rho0, const double u0[3]) {
     if (int rc = check_level(p, level)) return rc;
     OAL_HIP(hipSetDevice(p->device));
     Level& L = p->levels[level];
@@ -1653,7 +1821,7 @@ int oal_mass_noalloc(OalPlan* p, int level, double* out_dev, void* stream) {
     return OAL_OK;
 }
 
-int oal_download_macro(OalPlan* p, int level, double* rho, double* u) {
+static int download_macro_lib(OalPlan* p, int level, double* rho, double* u) {
     if (int rc = check_level(p, level)) return rc;
     Level& L = p->levels[level];
     if (int rc = require_plain_state(p, L)) return rc;
@@ -1773,7 +1941,7 @@ int oal_copy_bandwidth(int device, int64_t bytes, int reps, double* gbs) {
     return OAL_OK;
 }
 
-int oal_set_bouzidi(OalPlan* p, int level, const uint16_t* q_map, int32_t ff_fallback, float q_min) {
+static int set_bouzidi_lib(OalPlan* p, int level, const uint16_t* q_map, int32_t ff_fallback, float q_min) {
     if (p && level >= 0 && level < (int)p->levels.size() && p->levels[level].has_sponge && q_map)
         return fail(OAL_ERR_UNSUPPORTED, "interpolated bounce-back on a level with a sponge is not supported");
     if (int rc = check_level(p, level)) return rc;
@@ -1806,7 +1974,7 @@ int oal_set_bouzidi(OalPlan* p, int level, const uint16_t* q_map, int32_t ff_fal
     return OAL_OK;
 }
 
-int oal_set_sponge(OalPlan* p, int level, const float* sponge, int32_t blend_distributions) {
+static int set_sponge_lib(OalPlan* p, int level, const float* sponge, int32_t blend_distributions) {
     if (int rc = check_level(p, level)) return rc;
     if (blend_distributions != 0 && blend_distributions != 1) return fail(OAL_ERR_ARG, "blend_distributions must be 0 or 1");
     Level& L = p->levels[level];
@@ -1968,6 +2136,67 @@ int oal_download_surface(OalPlan* p, int level, float* pressure, float* shear, f
     return OAL_OK;
 }
 
+// ---- Per-cell host arrays: the caller's layout (8^3 levels regrouped) ---------------------------------
+int oal_upload_f(OalPlan* p, int level, const void* host_f, int dt) {
+    if (int rc = check_level(p, level)) return rc;
+    const Level& L = p->levels[level];
+    if (L.c8map.empty() || !host_f) return upload_f_lib(p, level, host_f, dt);
+    if (dt == OAL_F64) return upload_f_lib(p, level, cells_to_lib(L, (const double*)host_f, 27, 0.0).data(), dt);
+    if (dt == OAL_F32) return upload_f_lib(p, level, cells_to_lib(L, (const float*)host_f, 27, 0.0f).data(), dt);
+    return upload_f_lib(p, level, host_f, dt);
+}
+
+int oal_download_f(OalPlan* p, int level, void* host_f, int dt) {
+    if (int rc = check_level(p, level)) return rc;
+    const Level& L = p->levels[level];
+    if (L.c8map.empty() || !host_f || (dt != OAL_F32 && dt != OAL_F64)) return download_f_lib(p, level, host_f, dt);
+    if (dt == OAL_F64) {
+        std::vector<double> t((size_t)27 * L.ncell_host);
+        if (int rc = download_f_lib(p, level, t.data(), dt)) return rc;
+        cells_from_lib(L, t.data(), (double*)host_f, 27);
+    } else {
+        std::vector<float> t((size_t)27 * L.ncell_host);
+        if (int rc = download_f_lib(p, level, t.data(), dt)) return rc;
+        cells_from_lib(L, t.data(), (float*)host_f, 27);
+    }
+    return OAL_OK;
+}
+
+int oal_init_equilibrium(OalPlan* p, int level, const double* rho, const double* u, double rho0, const double u0[3]) {
+    if (int rc = check_level(p, level)) return rc;
+    const Level& L = p->levels[level];
+    if (L.c8map.empty()) return init_equilibrium_lib(p, level, rho, u, rho0, u0);
+    std::vector<double> r, v;
+    if (rho) r = cells_to_lib(L, rho, 1, rho0);
+    if (u) v = cells_to_lib(L, u, 3, 0.0);
+    return init_equilibrium_lib(p, level, rho ? r.data() : nullptr, u ? v.data() : nullptr, rho0, u0);
+}
+
+int oal_download_macro(OalPlan* p, int level, double* rho, double* u) {
+    if (int rc = check_level(p, level)) return rc;
+    const Level& L = p->levels[level];
+    if (L.c8map.empty()) return download_macro_lib(p, level, rho, u);
+    std::vector<double> r(rho ? L.ncell_host : 0), v(u ? 3 * (size_t)L.ncell_host : 0);
+    if (int rc = download_macro_lib(p, level, rho ? r.data() : nullptr, u ? v.data() : nullptr)) return rc;
+    if (rho) cells_from_lib(L, r.data(), rho, 1);
+    if (u) cells_from_lib(L, v.data(), u, 3);
+    return OAL_OK;
+}
+
+int oal_set_bouzidi(OalPlan* p, int level, const uint16_t* q_map, int32_t ff_fallback, float q_min) {
+    if (int rc = check_level(p, level)) return rc;
+    const Level& L = p->levels[level];
+    if (L.c8map.empty() || !q_map) return set_bouzidi_lib(p, level, q_map, ff_fallback, q_min);
+    return set_bouzidi_lib(p, level, cells_to_lib(L, q_map, 27, (uint16_t)0).data(), ff_fallback, q_min);
+}
+
+int oal_set_sponge(OalPlan* p, int level, const float* sponge, int32_t blend_distributions) {
+    if (int rc = check_level(p, level)) return rc;
+    const Level& L = p->levels[level];
+    if (L.c8map.empty() || !sponge) return set_sponge_lib(p, level, sponge, blend_distributions);
+    return set_sponge_lib(p, level, cells_to_lib(L, sponge, 1, 0.0f).data(), blend_distributions);
+}
+
 // ---- LBM1 raw level state (src/io/raw_state_dump.jl:10-47) ------------------------------------------
 namespace {
 constexpr int32_t LBM1_MAGIC = 0x4C424D31;   // "LBM1"
@@ -1983,26 +2212,30 @@ int oal_write_raw_state(OalPlan* p, int level, int32_t level_id, double dx, cons
     if (!path) return fail(OAL_ERR_ARG, "NULL path");
     Level& L = p->levels[level];
     if (!L.have_f) return fail(OAL_ERR_STATE, "no populations on this level");
-    const size_t nc = L.ncell_host;
-    const uint32_t nb = L.n_caller_blocks;
+    // in the caller's blocks (8^3 levels: the caller's 8^3 blocks)
+    const bool c8 = !L.c8map.empty();
+    const size_t nc = c8 ? L.c8map.size() : L.ncell_host;
+    const uint32_t nb = c8 ? (uint32_t)(L.map8_h.size() / 3) : L.n_caller_blocks;
+    const std::vector<int32_t>& map = c8 ? L.map8_h : L.map_h;
+    const std::vector<uint8_t>& obst = c8 ? L.obst8_h : L.obst_h;
     std::vector<double> rho(nc), u(3 * nc);
     if (int rc = oal_download_macro(p, level, rho.data(), u.data())) return rc;
     // the reference's obstacle cells carry rho = 1, u = 0 (kernel_cumulant.jl:405-410)
     std::vector<float> fr(nc), fv(3 * nc);
     std::vector<uint8_t> ob(nc, 0);
     for (size_t c = 0; c < nc; ++c) {
-        const bool solid = !L.obst_h.empty() && L.obst_h[c];
+        const bool solid = !obst.empty() && obst[c];
         ob[c] = solid ? 1 : 0;
         fr[c] = solid ? 1.0f : (float)rho[c];
         for (int a = 0; a < 3; ++a) fv[a * nc + c] = solid ? 0.0f : (float)u[a * nc + c];
     }
-    int32_t head[7] = {LBM1_MAGIC, level_id, (int32_t)nb, BS, 0, 0, 0};
+    int32_t head[7] = {LBM1_MAGIC, level_id, (int32_t)nb, c8 ? BS8 : BS, 0, 0, 0};
     for (uint32_t b = 0; b < nb; ++b)
-        for (int a = 0; a < 3; ++a) head[4 + a] = std::max(head[4 + a], L.map_h[3 * b + a]);
+        for (int a = 0; a < 3; ++a) head[4 + a] = std::max(head[4 + a], map[3 * b + a]);
     FileCloser fc{std::fopen(path, "wb")};
     if (!fc.f) return fail(OAL_ERR_ARG, std::string("cannot open for writing: ") + path);
     bool ok = std::fwrite(head, sizeof(head), 1, fc.f) == 1 && std::fwrite(&dx, sizeof(dx), 1, fc.f) == 1 &&
-              std::fwrite(L.map_h.data(), sizeof(int32_t), 3 * (size_t)nb, fc.f) == 3 * (size_t)nb &&
+              std::fwrite(map.data(), sizeof(int32_t), 3 * (size_t)nb, fc.f) == 3 * (size_t)nb &&
               std::fwrite(fr.data(), sizeof(float), nc, fc.f) == nc &&
               std::fwrite(fv.data(), sizeof(float), 3 * nc, fc.f) == 3 * nc &&
               std::fwrite(ob.data(), 1, nc, fc.f) == nc;
@@ -2016,8 +2249,11 @@ int oal_read_raw_state(OalPlan* p, int level, const char* path, int32_t* level_i
     if (int rc = check_level(p, level)) return rc;
     if (!path) return fail(OAL_ERR_ARG, "NULL path");
     Level& L = p->levels[level];
-    const size_t nc = L.ncell_host;
-    const uint32_t nb = L.n_caller_blocks;
+    const bool c8 = !L.c8map.empty();
+    const size_t nc = c8 ? L.c8map.size() : L.ncell_host;
+    const uint32_t nb = c8 ? (uint32_t)(L.map8_h.size() / 3) : L.n_caller_blocks;
+    const std::vector<int32_t>& map = c8 ? L.map8_h : L.map_h;
+    const std::vector<uint8_t>& obst = c8 ? L.obst8_h : L.obst_h;
     FileCloser fc{std::fopen(path, "rb")};
     if (!fc.f) return fail(OAL_ERR_ARG, std::string("cannot open: ") + path);
     int32_t head[7];
@@ -2025,7 +2261,7 @@ int oal_read_raw_state(OalPlan* p, int level, const char* path, int32_t* level_i
     if (std::fread(head, sizeof(head), 1, fc.f) != 1 || head[0] != LBM1_MAGIC)
         return fail(OAL_ERR_ARG, "not an LBM1 raw state file");
     if (std::fread(&h_dx, sizeof(h_dx), 1, fc.f) != 1) return fail(OAL_ERR_ARG, "truncated LBM1 header");
-    if (head[2] != (int32_t)nb || head[3] != BS)
+    if (head[2] != (int32_t)nb || head[3] != (c8 ? BS8 : BS))
         return fail(OAL_ERR_ARG, "LBM1 file: block count or block size differs from the level's");
     std::vector<int32_t> coords(3 * (size_t)nb);
     std::vector<float> fr(nc), fv(3 * nc);
@@ -2036,10 +2272,10 @@ int oal_read_raw_state(OalPlan* p, int level, const char* path, int32_t* level_i
         return fail(OAL_ERR_ARG, "truncated LBM1 file");
     unsigned char extra;
     if (std::fread(&extra, 1, 1, fc.f) != 0) return fail(OAL_ERR_ARG, "LBM1 file has trailing data");
-    if (!std::equal(coords.begin(), coords.end(), L.map_h.begin()))
+    if (!std::equal(coords.begin(), coords.end(), map.begin()))
         return fail(OAL_ERR_ARG, "LBM1 file: block coordinates differ from the level's");
     for (size_t c = 0; c < nc; ++c) {
-        const uint8_t want = !L.obst_h.empty() && L.obst_h[c] ? 1 : 0;
+        const uint8_t want = !obst.empty() && obst[c] ? 1 : 0;
         if ((ob[c] != 0) != (want != 0)) return fail(OAL_ERR_ARG, "LBM1 file: obstacle mask differs from the level's");
     }
     std::vector<double> rho(nc), u(3 * nc);

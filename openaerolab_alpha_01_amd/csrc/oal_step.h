@@ -197,9 +197,10 @@ __device__ __forceinline__ T boundary_value(int kind, const StepArgs& a, const C
                                             OWN&& own) {
     switch (kind) {
         case SRC_INLET: {
+            // the noise amplitude in the working precision (FP32 for the FP32-arithmetic forms, as the reference)
             const float turb = a.cp.inlet_turb;
-            const float noise = turb > 0.0f ? inlet_noise(c.gy + 1, c.gz + 1, step, 1234) * turb * u_in : 0.0f;
-            const T u_inst = T(u_in) + T(noise);
+            const T noise = turb > 0.0f ? T(inlet_noise(c.gy + 1, c.gz + 1, step, 1234)) * T(turb) * T(u_in) : T(0);
+            const T u_inst = T(u_in) + noise;
             return feq_auto(T(1), u_inst, T(0), T(0), wgt<T>(K), T(dcx(K)), T(dcy(K)), T(dcz(K)), a.cp.compressible);
         }
         case SRC_OWN: return own(dir_c<K>{});
