@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define OAL_ABI_VERSION 1
+#define OAL_ABI_VERSION 2
 
 enum {
     OAL_OK = 0,
@@ -93,6 +93,10 @@ typedef struct {
     float c_wale;              /* WALE constant; 0 disables the subgrid model */
     float nu_sgs_background;   /* NU_SGS_BACKGROUND (0) */
     float inlet_turbulence;    /* INLET_TURBULENCE_INTENSITY; 0 = steady inlet */
+    int32_t transverse_freestream;       /* TRANSVERSE_FREESTREAM_BOUNDARY (0): 1 = a population arriving
+                                            through a y or z face is (1 - b) f_k + b feq(1, u_inlet)
+                                            (kernel_cumulant.jl:323-339); excludes zwall_bc != 0 */
+    float transverse_freestream_blend;   /* b, TRANSVERSE_FREESTREAM_BLEND (1), in [0, 1] */
 } OalCollision;
 
 /* One domain of a level: an extent in cells with its boundary conditions.  A level holds one
@@ -105,7 +109,9 @@ typedef struct {
     float body_force[3];       /* uniform force density, lattice units of this level (Guo term) */
     float amr_sgs_scale;       /* _amr_sgs_scale (1 on a single level) */
     int32_t ymin_wall_bc;      /* 1: no-slip bounce-back at y_min (GROUND_ANALYSIS) */
-    int32_t zwall_bc;          /* 0: legacy mirror, 2: no-slip bounce-back; 1 is not supported */
+    int32_t zwall_bc;          /* z faces (advanced.boundary.z_walls): 0 legacy mirror, 1 exact specular
+                                  slip (the image cell's mirror_z population, kernel_cumulant.jl:272-321;
+                                  one-level plans, AB pattern), 2 no-slip bounce-back */
     float force_scale;         /* factor on this domain's momentum-exchange sum (8^(1-l) on octree levels) */
     int32_t reserved;
 } OalDomainDesc;

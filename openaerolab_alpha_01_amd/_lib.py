@@ -11,7 +11,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "liboal.so"
 
-OAL_ABI_VERSION = 1
+OAL_ABI_VERSION = 2
 OAL_OK, OAL_ERR_ARG, OAL_ERR_HIP, OAL_ERR_NOMEM, OAL_ERR_UNSUPPORTED, OAL_ERR_STATE, OAL_ERR_NONFINITE = 0, -1, -2, -3, -4, -5, -6
 OAL_F32, OAL_F64, OAL_F16S, OAL_BF16S = 0, 1, 2, 3
 OAL_STREAM_AB, OAL_STREAM_AA = 0, 1
@@ -39,6 +39,7 @@ class OalCollision(C.Structure):
         ("omega_bulk", C.c_float), ("omega_3", C.c_float), ("omega_4", C.c_float), ("lambda_", C.c_float),
         ("adaptive_omega4", C.c_int32), ("limiter", C.c_int32), ("compressible", C.c_int32),
         ("c_wale", C.c_float), ("nu_sgs_background", C.c_float), ("inlet_turbulence", C.c_float),
+        ("transverse_freestream", C.c_int32), ("transverse_freestream_blend", C.c_float),
     ]
 
 

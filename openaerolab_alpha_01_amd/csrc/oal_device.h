@@ -236,6 +236,8 @@ struct CollParams {
     float c_wale, nu_sgs_bg, inlet_turb;
     int32_t dither;   // 16-bit storage forms: stochastic rounding of the step's stores
     int32_t unit;     // omega_bulk = omega_3 = omega_4 = 1: the pipelined kernels use the UNIT collision
+    int32_t tfs;      // transverse freestream at the y and z faces
+    float tfs_blend;
 };
 
 // WALE eddy viscosity of the gradient g_ij = d u_i / d x_j (kernel_cumulant.jl:500-535)

@@ -153,6 +153,7 @@ struct Level {
     std::vector<char> regular_h, near_h;
     std::vector<int32_t> borg_h, bdom_h;
     bool ab = true, pipe_ok = true;
+    bool tfs = false;                // transverse freestream on the y and z faces
     // sponge (kernel_cumulant.jl:428-446): per sponge slot [BS3] strengths of the blocks that hold a
     // nonzero one (empty: no sponge)
     DevBuf<float> sponge;
@@ -253,14 +254,27 @@ void build_lists(Level& L, HostLevel& H) {
     H.blist_br.clear(); H.blist_bd.clear(); H.blist_pr.clear(); H.face_cells.clear();
     L.regular_face = false;
     const std::vector<uint32_t>& flags = L.flags_h;
+    // blocks on a face whose rule the pipelined kernels do not apply (the exact specular z wall reads
+    // another cell of the wall row; the transverse freestream blends): the general kernels step them
+    auto special_face = [&](uint32_t b, const DevDomain& dm) {
+        bool lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            const int o = L.borg_h[3 * b + a];
+            lo[a] = !dm.per[a] && o == 0;
+            hi[a] = !dm.per[a] && o + BS >= dm.ext[a];
+        }
+        const bool zface = lo[2] || hi[2], yface = lo[1] || hi[1];
+        return (dm.zwall == 1 && zface) || (L.tfs && (zface || yface));
+    };
     for (uint32_t b = 0; b < L.nb; ++b) {
         const DevDomain& dm = L.doms_h[L.bdom_h[b]];
         if (flags[b] & BF_INACTIVE) continue;   // never stepped
         const bool regular = L.regular_h[b] != 0;
+        const bool special = (flags[b] & BF_EDGE) && special_face(b, dm);
         if (flags[b] == 0) {
             (regular ? H.blist_rg : H.blist_in).push_back((int32_t)b);
             if (regular) H.blist_rp.push_back((int32_t)b);
-        } else if (L.ab && regular && flags[b] == BF_EDGE) {
+        } else if (L.ab && regular && flags[b] == BF_EDGE && !special) {
             // AB pattern: a whole block on a domain face, no solid near, joins the regular list; the
             // cells on its faces (sources outside the extent) go to the face-cell list
             H.blist_rg.push_back((int32_t)b);
@@ -280,7 +294,7 @@ void build_lists(Level& L, HostLevel& H) {
                         for (int a = 0; a < 3; ++a) face = face || (lo[a] && l[a] == 0) || (hi[a] && l[a] == BS - 1);
                         if (face) H.face_cells.push_back(b * BS3 + (uint32_t)(x + BS * (y + BS * z)));
                     }
-        } else if (L.ab && L.pipe_ok && regular && flags[b] == (BF_EDGE | BF_PARTIAL) && L.near_h[L.bdom_h[b]]) {
+        } else if (L.ab && L.pipe_ok && regular && flags[b] == (BF_EDGE | BF_PARTIAL) && L.near_h[L.bdom_h[b]] && !special) {
             // AB pattern: a block that overhangs the extent, no solid near: the pipelined kernel with the
             // face rules at the extent's last cells (in a domain whose block layers span 2 GiB or more the
             // one-plane boundary kernel, whose addresses are 64-bit)
@@ -360,7 +374,10 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, bool
             t.F[a] = s.body_force[a];
         }
         if (!(s.tau > 0.5f)) return fail(OAL_ERR_ARG, "tau must exceed 1/2");
-        if (s.zwall_bc != 0 && s.zwall_bc != 2) return fail(OAL_ERR_UNSUPPORTED, "zwall_bc 1 (exact specular slip) is not supported");
+        if (s.zwall_bc < 0 || s.zwall_bc > 2) return fail(OAL_ERR_ARG, "zwall_bc must be 0, 1 or 2");
+        if (s.zwall_bc == 1 && !ab) return fail(OAL_ERR_UNSUPPORTED, "zwall_bc 1 (exact specular slip) needs the AB pattern");
+        if (s.zwall_bc != 0 && coll.transverse_freestream)
+            return fail(OAL_ERR_ARG, "z walls and the transverse freestream act on the same faces (the reference rejects the pair)");
         t.tau = s.tau;
         t.u_in = s.u_inlet;
         t.amr_scale = s.amr_sgs_scale;
@@ -460,6 +477,7 @@ int prepare_level(const OalLevelDesc& d, const OalCollision& coll, bool ab, bool
     L.bdom_h = bdom;
     L.ab = ab;
     L.pipe_ok = pipe_ok;
+    L.tfs = coll.transverse_freestream != 0;
     build_lists(L, H);
     // block-major copy of the neighbour table: one 32-entry row per block (scalar loads)
     H.nbr_bm.assign((size_t)nb * 32, -1);
@@ -1413,6 +1431,14 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
     const bool ml = desc->n_levels > 1;
     if (ml && desc->stream_mode != OAL_STREAM_AB) return fail(OAL_ERR_UNSUPPORTED, "refined plans use the AB pattern");
     if (ml && c.c_wale != 0.0f) return fail(OAL_ERR_UNSUPPORTED, "WALE is not supported on refined plans");
+    if (c.transverse_freestream != 0 && c.transverse_freestream != 1) return fail(OAL_ERR_ARG, "transverse_freestream must be 0 or 1");
+    if (!(c.transverse_freestream_blend >= 0.0f && c.transverse_freestream_blend <= 1.0f))
+        return fail(OAL_ERR_ARG, "transverse_freestream_blend must lie in [0, 1]");
+    if (ml)
+        for (int l = 0; l < desc->n_levels; ++l)
+            for (int i = 0; i < desc->levels[l].n_domains && desc->levels[l].domains; ++i)
+                if (desc->levels[l].domains[i].zwall_bc == 1)
+                    return fail(OAL_ERR_UNSUPPORTED, "zwall_bc 1 on a refined plan (the image cell may lie on the parent level)");
     if (c.limiter < 0 || c.limiter > 2) return fail(OAL_ERR_ARG, "unknown limiter");
     if (desc->rounding != OAL_ROUND_STOCHASTIC && desc->rounding != OAL_ROUND_NEAREST)
         return fail(OAL_ERR_ARG, "rounding must be OAL_ROUND_STOCHASTIC or OAL_ROUND_NEAREST");
@@ -1437,6 +1463,8 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
     p->cp.inlet_turb = c.inlet_turbulence;
     p->cp.dither = desc->rounding == OAL_ROUND_STOCHASTIC ? 1 : 0;
     p->cp.unit = (c.omega_bulk == 1.0f && c.omega_3 == 1.0f && c.omega_4 == 1.0f) ? 1 : 0;
+    p->cp.tfs = c.transverse_freestream;
+    p->cp.tfs_blend = c.transverse_freestream_blend;
 
     auto bail = [&](int rc) {
         oal_plan_destroy(p);

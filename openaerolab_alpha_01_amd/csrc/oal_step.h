@@ -33,7 +33,9 @@ enum : int {
     SRC_MIRY = 4,    // own y-mirrored population: free slip             kernel_cumulant.jl:342-343
     SRC_MIRZ = 5,    // own z-mirrored population: free slip             kernel_cumulant.jl:344-345
     SRC_REST = 6,    // rest weight (no neighbour block on level 1)      kernel_cumulant.jl:389-390
-    SRC_SOLID = 7    // halfway bounce-back from a solid source cell
+    SRC_SOLID = 7,   // halfway bounce-back from a solid source cell
+    SRC_ZIMG = 8,    // exact specular z wall: the image cell's mirror_z population  kernel_cumulant.jl:272-321
+    SRC_TFS = 9      // transverse freestream: own population blended toward feq(1, u_inlet)  :323-339
 };
 
 enum : int { PAT_AB = 0, PAT_AA_EVEN = 1, PAT_AA_ODD = 2 };
@@ -64,7 +66,20 @@ __device__ __forceinline__ int classify(const StepArgs& a, const CellCtx& c, uin
             const bool zh = cz < 0 && !c.per2 && sgz >= c.ext2;
             if (xl) return SRC_INLET;
             if (xh) return SRC_OWN;
+            if (c.zwall == 1 && (zl || zh)) {
+                // the source beyond the wall mirrors the wall-row cell (x - cx, y - cy, z): that cell's
+                // population mirror_z(K) arrives; with no such cell on the level (beyond a non-periodic
+                // y face, or an absent block) the cell's own mirror_z(K)
+                const int ix = c.lx - cx, iy = c.ly - cy, igy = c.gy - cy;
+                const int ox = ix < 0 ? -1 : (ix >= BS ? 1 : 0), oy = iy < 0 ? -1 : (iy >= BS ? 1 : 0);
+                int32_t nbk = (int32_t)c.b;
+                if (ox | oy) nbk = a.nbr_bm[(size_t)c.b * 32 + (ox + 1) + 3 * (oy + 1) + 9];
+                if ((!c.per1 && (igy < 0 || igy >= c.ext1)) || nbk < 0) return SRC_MIRZ;
+                src = (uint32_t)nbk * BS3 + ((uint32_t)c.z << 8) + ((uint32_t)(iy & (BS - 1)) << 4) + (uint32_t)(ix & (BS - 1));
+                return SRC_ZIMG;
+            }
             if (c.zwall != 0 && (zl || zh)) return SRC_OPP;
+            if (a.cp.tfs && (yl || yh || zl || zh)) return SRC_TFS;
             if (yl && c.ymin_wall == 1) return SRC_OPP;
             if (yl || yh) return SRC_MIRY;
             if (zl || zh) return SRC_MIRZ;
@@ -190,6 +205,14 @@ __device__ __forceinline__ typename SP::T own_prev(const StepArgs& a, const Cell
 template <int J>
 using dir_c = std::integral_constant<int, J>;
 
+// transverse freestream: (1 - b) f_K + b feq_K(1, u_inlet) (kernel_cumulant.jl:333-339)
+template <int K, class T>
+__device__ __forceinline__ T tfs_value(const StepArgs& a, T donor, float u_in) {
+    const T b = T(a.cp.tfs_blend);
+    const T feq = feq_auto(T(1), T(u_in), T(0), T(0), wgt<T>(K), T(dcx(K)), T(dcy(K)), T(dcz(K)), a.cp.compressible);
+    return (T(1) - b) * donor + b * feq;
+}
+
 // The value of a boundary rule for direction K.  `own(dir_c<j>)` yields population j of the cell's
 // own post-collision state of the step the rule refers to.
 template <int K, class T, class OWN>
@@ -207,7 +230,9 @@ __device__ __forceinline__ T boundary_value(int kind, const StepArgs& a, const C
         case SRC_OPP:
         case SRC_SOLID: return own(dir_c<dopp(K)>{});
         case SRC_MIRY: return own(dir_c<dmir_y(K)>{});
-        case SRC_MIRZ: return own(dir_c<dmir_z(K)>{});
+        case SRC_MIRZ:
+        case SRC_ZIMG: return own(dir_c<dmir_z(K)>{});   // (the pull gather reads the image cell itself)
+        case SRC_TFS: return tfs_value<K, T>(a, own(dir_c<K>{}), u_in);
         default: return wgt<T>(K);
     }
 }
@@ -220,7 +245,8 @@ __device__ __forceinline__ int rule_dir(int kind) {
         case SRC_OPP:
         case SRC_SOLID: return dopp(K);
         case SRC_MIRY: return dmir_y(K);
-        case SRC_MIRZ: return dmir_z(K);
+        case SRC_MIRZ:
+        case SRC_ZIMG: return dmir_z(K);
         default: return K;
     }
 }
@@ -375,7 +401,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
 
     T f[27];
     typename SP::raw raw[27];     // AB pattern: the loaded populations, decoded after all loads
-    uint32_t inlet_links = 0, rest_links = 0, solid_links = 0, reg_links = 0;
+    uint32_t inlet_links = 0, rest_links = 0, solid_links = 0, reg_links = 0, tfs_links = 0;
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
         constexpr T w = wgt<T>(K);
@@ -405,12 +431,13 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
             const st* p;
             if constexpr (REG)
                 p = kind == SRC_REG ? (const st*)(ra.base(fi, K, c.z, dcz(K)) + ra.v[dcx(K) + 1][dcy(K) + 1])
-                                    : fi + pop_index(c.own, jd);
+                                    : fi + pop_index(kind == SRC_ZIMG ? src : c.own, jd);
             else
-                p = fi + pop_index(kind == SRC_REG ? src : c.own, jd);
+                p = fi + pop_index(kind == SRC_REG || kind == SRC_ZIMG ? src : c.own, jd);
             raw[K] = SP::ld_raw(p);
             inlet_links |= (kind == SRC_INLET ? 1u : 0u) << K;
             rest_links |= (kind == SRC_REST ? 1u : 0u) << K;
+            tfs_links |= (kind == SRC_TFS ? 1u : 0u) << K;
             solid_links |= (kind == SRC_SOLID ? 1u : 0u) << K;
             reg_links |= (kind == SRC_REG ? 1u : 0u) << K;
             val = T(0);   // set below
@@ -444,6 +471,8 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
                 if constexpr (dcx(K) > 0)
                     if ((inlet_links >> K) & 1u)
                         val = boundary_value<K, T>(SRC_INLET, a, c, u_in, a.step(), [&](auto) { return T(0); });
+                if constexpr (dcy(K) != 0 || dcz(K) != 0)
+                    if ((tfs_links >> K) & 1u) val = tfs_value<K, T>(a, val, u_in);
             }
             f[K] = val;
         });

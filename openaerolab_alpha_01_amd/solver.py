@@ -37,11 +37,14 @@ class StepParams:
     c_wale: float = 0.20
     nu_sgs_background: float = 0.0
     inlet_turbulence: float = 0.01
+    transverse_freestream: int = 0          # advanced.boundary.transverse_freestream (config_loader.jl:2425)
+    transverse_freestream_blend: float = 1.0
 
     def to_c(self) -> _lib.OalCollision:
         return _lib.OalCollision(self.omega_bulk, self.omega_3, self.omega_4, self.lambda_, self.adaptive_omega4,
                                  self.limiter, self.compressible, self.c_wale, self.nu_sgs_background,
-                                 self.inlet_turbulence)
+                                 self.inlet_turbulence, int(self.transverse_freestream),
+                                 self.transverse_freestream_blend)
 
 
 @dataclass

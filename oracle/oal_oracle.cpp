@@ -95,6 +95,8 @@ const Lattice LAT;
 //  10  surface stress: the shear divided by the unfloored probe distance
 //  11  surface stress: probes of any quality accepted
 //  12  surface stress: the z pressure force with the wrong sign
+//  13  exact specular z wall: the cell's own mirror_z population instead of the image cell's
+//  14  transverse freestream: the own population mirrored instead of the own population
 int g_plant = 0;
 
 // weights as the reference builds them in the working precision (timestep.jl:2013)
@@ -334,6 +336,8 @@ struct OrParams {
                               // (BOUZIDI_FF_FALLBACK_MODE, config_loader.jl:2526-2527)
     double q_min;             // links with q <= q_min keep halfway bounce-back (Q_MIN_THRESHOLD)
     int32_t sponge_blend;     // with a sponge field: 1 also blends the populations toward the inlet
+    int32_t transverse_freestream;   // 1: y and z faces blend the own population toward feq(1, u_inlet)
+    double tfs_blend;                // its blend b (TRANSVERSE_FREESTREAM_BLEND)
                               // equilibrium (SPONGE_BLEND_DISTRIBUTIONS, config_loader.jl:1312)
 };
 
@@ -965,8 +969,21 @@ void step_impl(const OrGrid& g, const OrParams& prm, const typename P::st* fin, 
                                 val = feq_auto(T(1), u_inst, T(0), T(0), w[k], (T)cx, (T)cy, (T)cz, cp.compressible);
                             } else if (xh) {
                                 val = own_f(k);
+                            } else if (prm.zwall_bc == 1 && (zl || zh)) {
+                                // exact specular slip (kernel_cumulant.jl:272-321): the source beyond the wall is
+                                // the image of the wall-row cell (x - cx, y - cy, z), whose population mirror_z(k)
+                                // arrives; with no such cell on the level, the cell's own mirror_z(k)
+                                const int mk = LAT.mir_z[k];
+                                const int64_t img = geo.outside(1, gy - cy) || g_plant == 13 ? -1 : geo.locate(b, x - cx, y - cy, z);
+                                val = img >= 0 ? P::ld(fin[(int64_t)mk * ncell + img], w[mk]) : own_f(mk);
                             } else if (prm.zwall_bc != 0 && (zl || zh)) {
                                 val = own_f(LAT.opp[k]);   // zwall_bc 2: halfway bounce-back
+                            } else if (prm.transverse_freestream == 1 && (yl || yh || zl || zh)) {
+                                // transverse freestream (kernel_cumulant.jl:323-339)
+                                const T beta = (T)(float)prm.tfs_blend;
+                                const T feq_fs = feq_auto(T(1), cp.u_in, T(0), T(0), w[k], (T)cx, (T)cy, (T)cz, cp.compressible);
+                                const int dk = g_plant == 14 ? ((yl || yh) ? LAT.mir_y[k] : LAT.mir_z[k]) : k;
+                                val = (T(1) - beta) * own_f(dk) + beta * feq_fs;
                             } else if (yl && prm.ymin_wall_bc == 1) {
                                 val = own_f(LAT.opp[k]);
                             } else if (yl || yh) {

@@ -79,7 +79,7 @@ class OrParams(C.Structure):
         ("adaptive_omega4", C.c_int32), ("limiter", C.c_int32), ("compressible", C.c_int32),
         ("ymin_wall_bc", C.c_int32), ("zwall_bc", C.c_int32), ("bb_mode", C.c_int32), ("time_step_seed", C.c_int32),
         ("arith", C.c_int32), ("rounding", C.c_int32), ("bfl_fallback", C.c_int32), ("q_min", C.c_double),
-        ("sponge_blend", C.c_int32),
+        ("sponge_blend", C.c_int32), ("transverse_freestream", C.c_int32), ("tfs_blend", C.c_double),
     ]
 
 
@@ -154,7 +154,7 @@ def make_params(tau, *, omega_bulk=1.0, omega_3=1.0, omega_4=1.0, lambda_=1.0 / 
                 compressible=0, c_wale=0.0, nu_sgs_background=0.0, amr_sgs_scale=1.0, u_inlet=0.0,
                 inlet_turbulence=0.0, body_force=(0.0, 0.0, 0.0), ymin_wall_bc=0, zwall_bc=0, bb_mode=1,
                 time_step=0, arith=LITERAL, rounding=STOCHASTIC, bfl_fallback=0, q_min=0.001,
-                sponge_blend=0) -> OrParams:
+                sponge_blend=0, transverse_freestream=0, transverse_freestream_blend=1.0) -> OrParams:
     """Step parameters; arith LITERAL restates the reference arithmetic operation for operation, FAST the
     library's fused arithmetic (same mathematics, fused multiply-adds, rho and j from the transform's
     partial sums).  rounding: how the 16-bit forms store a step's populations (STOCHASTIC, the
@@ -173,6 +173,7 @@ def make_params(tau, *, omega_bulk=1.0, omega_3=1.0, omega_4=1.0, lambda_=1.0 / 
     p.bfl_fallback = bfl_fallback
     p.q_min = f32(q_min)
     p.sponge_blend = sponge_blend
+    p.transverse_freestream, p.tfs_blend = transverse_freestream, f32(transverse_freestream_blend)
     return p
 
 

@@ -44,10 +44,23 @@ def test_descriptor_validation_without_gpu():
     with pytest.raises(_lib.OalError) as e:
         solver.Plan(solver.Level(g, [dom]))
     assert e.value.code == _lib.OAL_ERR_ARG
+    # the exact specular z wall reads the image cell at the gather: the AB pattern only
     dom = solver.Domain(extent=g.n, tau=0.6, zwall_bc=1)
     with pytest.raises(_lib.OalError) as e:
-        solver.Plan(solver.Level(g, [dom]))
+        solver.Plan(solver.Level(g, [dom]), stream="aa")
     assert e.value.code == _lib.OAL_ERR_UNSUPPORTED
+    with pytest.raises(_lib.OalError) as e:
+        solver.Plan(solver.Level(g, [solver.Domain(extent=g.n, tau=0.6, zwall_bc=3)]))
+    assert e.value.code == _lib.OAL_ERR_ARG
+    # z walls and the transverse freestream act on the same faces; the blend lies in [0, 1]
+    tfs = solver.StepParams(transverse_freestream=1)
+    with pytest.raises(_lib.OalError) as e:
+        solver.Plan(solver.Level(g, [solver.Domain(extent=g.n, tau=0.6, zwall_bc=2)]), params=tfs)
+    assert e.value.code == _lib.OAL_ERR_ARG
+    with pytest.raises(_lib.OalError) as e:
+        solver.Plan(solver.Level(g, [solver.Domain(extent=g.n, tau=0.6)]),
+                    params=solver.StepParams(transverse_freestream=1, transverse_freestream_blend=1.5))
+    assert e.value.code == _lib.OAL_ERR_ARG
     with pytest.raises(_lib.OalError) as e:
         solver.Plan(solver.Level(g, [solver.Domain(extent=g.n, tau=0.6)]), stream="aa", bb_mode="reference")
     assert e.value.code == _lib.OAL_ERR_UNSUPPORTED

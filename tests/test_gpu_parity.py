@@ -58,7 +58,9 @@ def gpu_plan(grid, storage, okw, stream="ab"):
                                adaptive_omega4=okw.get("adaptive_omega4", 1), limiter=okw.get("limiter", 1),
                                compressible=okw.get("compressible", 0), c_wale=okw.get("c_wale", 0.0),
                                nu_sgs_background=okw.get("nu_sgs_background", 0.0),
-                               inlet_turbulence=okw.get("inlet_turbulence", 0.0))
+                               inlet_turbulence=okw.get("inlet_turbulence", 0.0),
+                               transverse_freestream=okw.get("transverse_freestream", 0),
+                               transverse_freestream_blend=okw.get("transverse_freestream_blend", 1.0))
     dom = solver.Domain(extent=grid.n, tau=okw["tau"], periodic=grid.periodic, u_inlet=okw.get("u_inlet", 0.0),
                         body_force=tuple(okw.get("body_force", (0.0, 0.0, 0.0))),
                         ymin_wall_bc=okw.get("ymin_wall_bc", 0), zwall_bc=okw.get("zwall_bc", 0))
@@ -136,6 +138,8 @@ VARIANTS = {
     "wale": dict(c_wale=0.2, nu_sgs_background=1e-4),
     "body_force": dict(body_force=(1e-5, -2e-6, 3e-6)),
     "nearest_rounding": dict(rounding=1),
+    "specular_zwall": dict(zwall_bc=1),
+    "transverse_freestream": dict(transverse_freestream=1, transverse_freestream_blend=0.7),
 }
 
 
@@ -155,6 +159,21 @@ def test_variant_parity(oracle, variant, storage, face_kernel):
     f_or, F_or, _ = oracle_run(oracle, grid, storage, steps, okw, rho, u, force_steps=fst)
     plan, F_gpu = gpu_run(grid, storage, steps, okw, rho, u, force_steps=fst, tuning=dict(face_kernel=face_kernel))
     compare(oracle, plan, storage, f_or, grid)
+    np.testing.assert_allclose(F_gpu[0], F_or[0], rtol=1e-11, atol=1e-14)
+    plan.close()
+
+
+@pytest.mark.parametrize("variant", ["specular_zwall", "transverse_freestream", "inlet_noise", "walls"])
+def test_variant_parity_f64(oracle, variant):
+    """The FP64 path of the optional face rules and the inlet noise (the same case as test_variant_parity)."""
+    grid = uniform_grid((64, 32, 32))
+    grid.solid = sphere_mask(grid, (31, 15, 17), 5.5)
+    okw = dict(tau=0.53, u_inlet=0.05)
+    okw.update(VARIANTS[variant])
+    rho, u = fields(grid, seed=3, u0=(0.05, 0, 0), amp=0.03)
+    f_or, F_or, _ = oracle_run(oracle, grid, "f64", 5, okw, rho, u, force_steps=(4,))
+    plan, F_gpu = gpu_run(grid, "f64", 5, okw, rho, u, force_steps=(4,))
+    compare(oracle, plan, "f64", f_or, grid)
     np.testing.assert_allclose(F_gpu[0], F_or[0], rtol=1e-11, atol=1e-14)
     plan.close()
 
