@@ -295,8 +295,32 @@ def tgv_main(args) -> int:
 
 # ---- many-instance form: 48 independent boxes with ellipsoids, one launch per step ------------------
 MANY_COUNT = 48
-MANY_TAU = 0.60      # fixed relaxation time of every box (the configuration leaves nu open; at 0.52 and 0.56
-                     # boxes whose wake meets the zero-gradient outlet early diverge within 5000 steps, in FP64 too)
+MANY_RE = 20.0       # Reynolds number of every box, u_in D / nu with D the ellipsoid's largest diameter (the
+                     # configuration leaves nu open; a fixed tau of 0.52 to 0.60 let the faster, larger boxes'
+                     # wakes reach the zero-gradient outlet unsteady, and some diverged within 5000 steps)
+
+
+MANY_SPONGE = 0.15   # outlet sponge thickness in units of the reference length (advanced.domain.sponge_thickness)
+MANY_SPONGE_SIGMA = 0.25   # its tanh transition width (sponge_tanh_sigma)
+
+
+def many_sponge(inst) -> np.ndarray:
+    """the outlet sponge of apply_sponge! (src/geometry/domain_generation.jl:296-349) for one box:
+    thickness T = D max(sponge_thickness, 0.08) with D the ellipsoid's largest diameter, the tanh profile
+    of tanh_sponge_profile (:265-275) in the distance from the outlet, outlet strength 1"""
+    g = inst["grid"]
+    nx = float(inst["n"][0])
+    T = min(nx, 2.0 * float(np.max(inst["semi"])) * max(MANY_SPONGE, 0.08))
+    xx = nx - (g.cell_coords()[:, 0] + 0.5)            # distance from the outlet
+    s = np.where(xx <= 0.0, 1.0, np.where(xx >= T, 0.0,
+                 0.5 * (1.0 - np.tanh((xx - 0.5 * T) / (MANY_SPONGE_SIGMA * T)))))
+    s[~g.inside()] = 0.0
+    return s.astype(np.float32)
+
+
+def many_tau(inst) -> float:
+    """relaxation time of a box at Re = MANY_RE: tau = 1/2 + 3 u_in D / Re"""
+    return 0.5 + 3.0 * inst["u_in"] * 2.0 * float(np.max(inst["semi"])) / MANY_RE
 
 
 def many_instances(seed: int):
@@ -322,11 +346,12 @@ def many_instances(seed: int):
 def many_plan(inst, storage):
     from openaerolab_alpha_01_amd import solver
     params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
-    plan = solver.batched_plan([i["grid"] for i in inst], [MANY_TAU] * len(inst), [i["u_in"] for i in inst],
+    plan = solver.batched_plan([i["grid"] for i in inst], [many_tau(i) for i in inst], [i["u_in"] for i in inst],
                                storage=storage, params=params)
     rho = np.concatenate([np.ones(i["grid"].n_cells) for i in inst])
     u = np.concatenate([np.stack([np.where(i["grid"].solid != 0, 0.0, i["u_in"]), np.zeros(i["grid"].n_cells),
                                   np.zeros(i["grid"].n_cells)]) for i in inst], axis=1)
+    plan.set_sponge(np.concatenate([many_sponge(i) for i in inst]))
     plan.init_equilibrium(rho, u)
     return plan
 
@@ -339,9 +364,12 @@ def many_main(args) -> int:
     cells = sum(int(np.prod(i["n"])) for i in inst)
     stored = sum(i["grid"].n_cells for i in inst)
     out = {"workload": f"{MANY_COUNT} independent inlet/outlet/free-slip boxes, edges U[48,160], ellipsoids with "
-                       f"semi-axes U[6,20] at random orientation, u_in U[0.02,0.08], tau={MANY_TAU}, seed {args.seed}; "
+                       f"semi-axes U[6,20] at random orientation, u_in U[0.02,0.08], Re={MANY_RE:g} per box "
+                       f"(tau = 1/2 + 3 u_in D / Re, D the largest diameter), outlet sponge of the reference's form "
+                       f"(thickness {MANY_SPONGE:g} D, tanh sigma {MANY_SPONGE_SIGMA:g}), seed {args.seed}; "
                        "all boxes in one block list, one launch per kernel kind per step",
-           "cells": cells, "stored_cells": stored}
+           "cells": cells, "stored_cells": stored,
+           "tau_range": [round(min(many_tau(i) for i in inst), 4), round(max(many_tau(i) for i in inst), 4)]}
     if args.validate:
         res = {}
         for st in ("f64", "f16s"):
@@ -365,6 +393,13 @@ def many_main(args) -> int:
                                    for k in range(len(inst))]
         ok = [a and b for a, b in zip(res["f64_finite"], res["f16s_finite"])]
         err = [(a - b) / b for a, b, o in zip(res["f16s"], res["f64"], ok) if o]
+        # boxes that did not stay bounded, with what sets them apart: the gap between the body's far
+        # end and the zero-gradient outlet, the relaxation time and the blockage
+        out["unstable_boxes"] = [{"box": k, "f64_bounded": res["f64_finite"][k], "f16s_bounded": res["f16s_finite"][k],
+                                  "tau": round(many_tau(i), 4), "u_in": round(i["u_in"], 4),
+                                  "outlet_gap": round(i["n"][0] - i["center"][0] - float(np.max(i["semi"])), 1),
+                                  "blockage": round(2 * float(np.max(i["semi"])) / min(i["n"][1], i["n"][2]), 3)}
+                                 for k, (i, o) in enumerate(zip(inst, ok)) if not o]
         out["validation"] = {"steps": args.many_steps, "cd_f64": [round(v, 5) for v in res["f64"]],
                              "cd_f16s": [round(v, 5) for v in res["f16s"]],
                              "stable_boxes": int(sum(ok)),
