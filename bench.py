@@ -659,7 +659,7 @@ def main(argv=None) -> int:
     cd = float(F[0] / (0.5 * U_IN ** 2 * area))
 
     # (the collision's unit-rate form: the reference's default rates omega_bulk = omega_3 = omega_4 = 1)
-    kernel = (f"k_step_pipe<StF32, {pp}, {2 if tun['face_kernel'] else 1}, true>" if args.stream == "ab" and pp > 1
+    kernel = (f"k_step_pipe<StF32, {pp}, {2 if tun['face_kernel'] else 1}, true, false>" if args.stream == "ab" and pp > 1
               else "k_step<StF32, regular>")
     traffic = measured_traffic(kernel)
 
@@ -704,7 +704,7 @@ def main(argv=None) -> int:
             k16 = (f"k_step_pair<StF16S, {pp16}, true, 1> + k_step_pair<StF16S, {pp16}, true, 0>"
                    if tun16["pair_kernel"] == 2 else f"k_step_pair<StF16S, {pp16}, true, 1>")
         elif args.stream == "ab" and pp16 > 1:
-            k16 = f"k_step_pipe<StF16S, {pp16}, {2 if tun16['face_kernel'] else 1}, true>"
+            k16 = f"k_step_pipe<StF16S, {pp16}, {2 if tun16['face_kernel'] else 1}, true, false>"
         else:
             k16 = "k_step<StF16S, regular>"
         traffic16 = measured_traffic(k16, "step_f16_256_hbm_traffic.md")
@@ -745,13 +745,14 @@ def main(argv=None) -> int:
         bw = bytes_per_lup("f32", wale=True)
         aw = bw * bulkw / (ms_kw * 1e-3) / 1e9
         ppw = min(tunw["pipe_planes"], 8)
+        kw = f"k_step_pipe<StF32, {ppw}, 1, true, true>" if facew else "k_step<StF32, regular, wale>"
         out["wale_on"] = {"value": round(mw, 1), "unit": "MLUPS", "ms_per_step": round(ms_tw / args.steps, 4),
                           "vs_wale_off": round(mw / mlups, 3), "c_wale": 0.2, "inlet_turbulence": 0.01,
                           "tuning": tunw, "cd_transient": round(float(Fw[0] / (0.5 * U_IN ** 2 * area)), 5),
                           "roofline": {"achieved": round(aw, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                        "frac": round(aw / HBM_PEAK_GBS, 4), "bytes_per_lup": bw,
-                                       "kernel": (f"k_step_pipe<StF32, {ppw}, 1, true, true>" if facew
-                                                  else "k_step<StF32, regular, wale>"),
+                                       "traffic": measured_traffic(kw, "step_wale_256_hbm_traffic.md"),
+                                       "kernel": kw,
                                        "kernel_ms": round(ms_kw, 5), "cells_per_launch": bulkw}}
     if args.acc_steps > 0:
         # bounded accuracy leg: the storage forms against FP64 runs of the same kernels after acc_steps steps
