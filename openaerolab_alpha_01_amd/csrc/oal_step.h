@@ -2,10 +2,11 @@
 // units oal_step_<storage>.hip so that the four storage forms compile in parallel.
 //
 // Work decomposition of the step kernels: one work-item per cell, one workgroup per 16 x 16 plane of
-// a 16^3 block (256 lanes = 4 wavefronts of 16 x 4 cells).  Populations are SoA per direction with
-// every block's 4096 cells contiguous (the reference's f[x,y,z,b,k] layout), so the 64 lanes of a
-// wavefront touch 4 consecutive 64-byte rows of every direction array.  A workgroup's block, plane,
-// domain and neighbour-block indices are uniform, which keeps them in scalar registers.
+// a 16^3 block (256 lanes = 4 wavefronts of 16 x 4 cells), or per several planes in the pipelined
+// kernels.  Populations are block-major: each block holds its 27 direction arrays of 4096 cells
+// contiguously (pop_index), so the 64 lanes of a wavefront touch 4 consecutive 64-byte rows of each
+// of the block's direction arrays.  A workgroup's block, plane, domain and neighbour-block indices are
+// uniform, which keeps them in scalar registers.
 //
 // Reference: src/collision/kernel_cumulant.jl:29-912 (stream_collide_cumulant!).
 #pragma once

@@ -50,6 +50,18 @@ same check is shown to catch.
                         q-placed walls; rest state kept under every fallback; fallbacks 0, 1 and 2 away
                         from rest on a single cut link, the arriving value read back from the link's
                         momentum exchange and derived by hand (test_oracle_pins.py)
+  sponge blend          pinned (test_oracle_sponge.py): mass and momentum after the collision equal the
+                        blended state's (rho (1 - sp) + sp, and the blended velocity), in both blend modes
+                        and both arithmetic forms; sp = 0 is no sponge bit for bit
+  face-rule variants    pinned (test_oracle_faces.py): exact specular z walls equal one step of the box
+                        doubled in z by its mirror image (bit for bit); the transverse freestream's face
+                        cells hold the state's moments plus b times the change of the crossing
+                        populations (b = 0, 0.3, 1) and keep the freestream equilibrium
+  surface stress        pinned (test_oracle_surface.py): uniform pressure gives zero net force and
+                        moment on a closed mesh; uniform flow's wall-law shear integrates to the sphere's
+                        closed form with and without the SGS estimate; a linear density gives the
+                        divergence-theorem force and the moment arm; hand-derived single triangles for
+                        the quality gate, the opposite-side recovery and the shell search
   golden fixtures       tests/golden/ (written from this oracle; regression pins, test_golden.py)
 """
 from __future__ import annotations
