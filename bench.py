@@ -659,8 +659,13 @@ def main(argv=None) -> int:
     cd = float(F[0] / (0.5 * U_IN ** 2 * area))
 
     # (the collision's unit-rate form: the reference's default rates omega_bulk = omega_3 = omega_4 = 1)
-    kernel = (f"k_step_pipe<StF32, {pp}, {2 if tun['face_kernel'] else 1}, true, false>" if args.stream == "ab" and pp > 1
-              else "k_step<StF32, regular>")
+    if args.stream == "ab" and pp > 1 and tun.get("pair_kernel") and not tun["face_kernel"]:
+        kernel = (f"k_step_pair32<{pp}, true, 1> + k_step_pair32<{pp}, true, 0>" if tun["pair_kernel"] == 2
+                  else f"k_step_pair32<{pp}, true, 1>")
+    elif args.stream == "ab" and pp > 1:
+        kernel = f"k_step_pipe<StF32, {pp}, {2 if tun['face_kernel'] else 1}, true, false>"
+    else:
+        kernel = "k_step<StF32, regular>"
     traffic = measured_traffic(kernel)
 
     out = {

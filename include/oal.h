@@ -398,10 +398,11 @@ typedef struct {
     int32_t overlap;
     int32_t face_kernel;
     int32_t graph;
-    int32_t pair_kernel;   /* 16-bit storage (OAL_F16S, OAL_BF16S), pipe_planes > 1, face_kernel 0: the
-                              regular blocks step two cells per lane (32-bit words holding both): 1 in one
-                              launch, 2 (default) with the blocks on a domain face in a launch of their own;
-                              0 = one cell per lane; the same bits every way */
+    int32_t pair_kernel;   /* FP32 and 16-bit storage, AB pattern, pipe_planes > 1, face_kernel 0, no WALE: the
+                              regular blocks step two cells per lane (8-byte FP32 / 4-byte 16-bit words holding
+                              both): 1 in one launch (the FP32 default), 2 with the blocks on a domain face in a
+                              launch of their own (the 16-bit default); 0 = one cell per lane; the same bits
+                              every way */
     int32_t reserved[3];
 } OalTuning;
 

@@ -86,7 +86,7 @@ struct StepArgs {
     int32_t ts_mul, ts_add;
     int32_t parity;              // AA pattern: 1 odd step (neighbour slots), 0 even step (own slots)
     int32_t pipe_planes;         // planes per workgroup of the pipelined interior kernel (1 = off; divides 16)
-    int32_t pair_kernel;         // 16-bit storage: interior regular blocks two cells per lane (pipelined shapes)
+    int32_t pair_kernel;         // FP32 / 16-bit storage: regular blocks two cells per lane (pipelined shapes)
     // interpolated bounce-back: per link-mask slot, [27][BS3] Float16 bits of the wall distance q of the
     // link from a cell along c_k (units of |c_k|); null = halfway bounce-back on every solid link
     const uint16_t* qmap;
@@ -101,11 +101,11 @@ struct StepArgs {
     __device__ __forceinline__ int32_t step() const { return ts_dev ? ts_dev[0] * ts_mul + ts_add : time_step; }
 };
 
-// Whether a domain's block grid fits the 32-bit offsets of the two-cells-per-lane kernel of the 16-bit
-// forms (16-bit populations, one buffer resource per block): two layers of blocks, two rows and three
-// blocks below 4 GiB
-inline bool pair_fits(int nbx, int nby) {
-    const uint64_t blk = 27ull * BS3 * 2;
+// Whether a domain's block grid fits the 32-bit offsets of the two-cells-per-lane kernels (populations
+// of `esize` bytes, one buffer resource per block): two layers of blocks, two rows and three blocks
+// below 4 GiB
+inline bool pair_fits(int nbx, int nby, int esize = 2) {
+    const uint64_t blk = 27ull * BS3 * (uint64_t)esize;
     return 2ull * (uint64_t)nbx * (uint64_t)nby * blk + (2ull * (uint64_t)nbx + 3ull) * blk < (1ull << 32);
 }
 

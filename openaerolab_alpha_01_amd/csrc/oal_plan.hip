@@ -195,7 +195,7 @@ struct OalPlan {
     int pipe_planes = 4;   // planes per workgroup of the pipelined interior kernel
     bool overlap = true;   // boundary and face launches on the side stream
     bool face_kernel = false;   // face cells by the face-cell kernel instead of rules in the pipelined one
-    int pair_kernel = 2;        // 16-bit storage: regular blocks two cells per lane (1 one launch, 2 face blocks apart)
+    int pair_kernel = 2;        // FP32 / 16-bit storage: regular blocks two cells per lane (1 one launch, 2 face blocks apart)
     int storage = OAL_F32;
     int stream_mode = OAL_STREAM_AB;
     int bb_mode = OAL_BB_HALFWAY;
@@ -1034,7 +1034,7 @@ StepArgs make_args(OalPlan* p, Level& L, int32_t time_step, bool want_force) {
     a.pipe_planes = p->pipe_planes;
     // (two cells per lane where the level's block grid fits its 32-bit offsets)
     bool pair_ok = p->pair_kernel != 0;
-    for (const DevDomain& dm : L.doms_h) pair_ok = pair_ok && pair_fits(dm.nbx, dm.nby);
+    for (const DevDomain& dm : L.doms_h) pair_ok = pair_ok && pair_fits(dm.nbx, dm.nby, p->storage == OAL_F32 ? 4 : 2);
     a.pair_kernel = pair_ok ? p->pair_kernel : 0;
     a.qmap = L.qmap.p;
     a.sponge = L.has_sponge ? L.sponge.p : nullptr;
@@ -1447,6 +1447,7 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
     if (!p) return fail(OAL_ERR_NOMEM, "host allocation failed");
     p->device = desc->device;
     p->storage = desc->storage;
+    p->pair_kernel = desc->storage == OAL_F32 ? 1 : 2;   // (the measured faster split per storage form)
     p->stream_mode = desc->stream_mode;
     p->bb_mode = desc->bb_mode;
     p->wale = c.c_wale != 0.0f;

@@ -1110,6 +1110,7 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(step_wa
 }  // namespace oal
 
 #include "oal_step_pair.h"
+#include "oal_step_pair32.h"
 
 namespace oal {
 
@@ -1200,6 +1201,11 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
                     launch_pair<SP>(a.pipe_planes, face, g, stream, r);
                     return;
                 }
+            } else if constexpr (SP::code == StF32::code && !WALE) {
+                if (pair) {
+                    launch_pair32(a.pipe_planes, face, g, stream, r);
+                    return;
+                }
             }
             const int fc = !face ? FACE_NONE : (a.face_kernel && !WALE ? FACE_SKIP : FACE_RULES);
             switch (r.pipe_planes) {
@@ -1223,8 +1229,8 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
             hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_REGULAR>), dim3(n * PLANES), dim3(WG), 0, stream, r);
         }
     };
-    const bool pair = pipe_capable && SP::dithered && !WALE && a.pair_kernel && a.pipe_planes > 1 &&
-                      !(a.regular_face && a.face_kernel);
+    const bool pair = pipe_capable && (SP::dithered || SP::code == StF32::code) && !WALE && a.pair_kernel &&
+                      a.pipe_planes > 1 && !(a.regular_face && a.face_kernel);
     if (pair && a.pair_kernel == 2) {
         // the blocks on a domain face and the others apart (the latter with the smaller register budget)
         if (a.n_regular_face) launch_regular(a.blist_regular_face, a.n_regular_face, true, true);
