@@ -294,6 +294,9 @@ __device__ __forceinline__ void chim_b(T& a, T& b, T& c, T u) {
 template <class T> __device__ __forceinline__ T fmx(T a, T b, T c);
 template <> __device__ __forceinline__ float fmx<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 template <> __device__ __forceinline__ double fmx<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// IEEE minNum: the smaller operand, the other one where one is NaN
+__device__ __forceinline__ float fmn(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double fmn(double a, double b) { return __builtin_fmin(a, b); }
 
 // omega1 and the diagonal 4th-order rate of a cell (kernel_cumulant.jl:592-610); computed per cell
 // only with WALE, otherwise once per domain on the host with the same operations
@@ -465,9 +468,13 @@ __device__ __forceinline__ void collide_cell(T (&f)[27], const CellRates<T>& q, 
     T s110 = g1 * c110, s101 = g1 * c101, s011 = g1 * c011;
     if (p.limiter == 1) {
         const T q110 = mx(s200 * s020, T(0)), q101 = mx(s200 * s002, T(0)), q011 = mx(s020 * s002, T(0));
-        if (s110 * s110 > q110) s110 = copysign(sqrt(q110), s110);
-        if (s101 * s101 > q101) s101 = copysign(sqrt(q101), s101);
-        if (s011 * s011 > q011) s011 = copysign(sqrt(q011), s011);
+        const bool l110 = s110 * s110 > q110, l101 = s101 * s101 > q101, l011 = s011 * s011 > q011;
+        // the clamp is rare: a wave takes the square roots only when one of its lanes needs one
+        if (__builtin_amdgcn_ballot_w64(l110 || l101 || l011) != 0) {
+            if (l110) s110 = copysign(sqrt(q110), s110);
+            if (l101) s101 = copysign(sqrt(q101), s101);
+            if (l011) s011 = copysign(sqrt(q011), s011);
+        }
     }
     const T g4d = T(1) - q.omega4_diag;
     const T S220 = g4d * C220, S202 = g4d * C202, S022 = g4d * C022;
@@ -583,10 +590,15 @@ __device__ __forceinline__ void collide_cell(T (&f)[27], const CellRates<T>& q, 
     for (int r2 = 0; r2 < 9; ++r2) back(3 * r2, 3 * r2 + 1, 3 * r2 + 2, uxe);
 
     if (p.limiter == 2) {   // positivity (kernel_cumulant.jl:849-884)
-        T fmin = A[0];
+        // The running minimum m = mn(m, A[k]) from m = A[0] is negative exactly when A[0] is not NaN
+        // (a NaN first value stays) and some A[k] is negative (a NaN A[k] is passed over): the same
+        // test as a min tree (fmin passes over NaN) with A[0] checked, 13 three-way minima.
+        T m9[9];
 #pragma unroll
-        for (int k = 1; k < 27; ++k) fmin = mn(fmin, A[k]);
-        if (fmin < T(0)) {
+        for (int i = 0; i < 9; ++i) m9[i] = fmn(fmn(A[3 * i], A[3 * i + 1]), A[3 * i + 2]);
+        const T fmin = fmn(fmn(fmn(fmn(m9[0], m9[1]), m9[2]), fmn(fmn(m9[3], m9[4]), m9[5])),
+                           fmn(fmn(m9[6], m9[7]), m9[8]));
+        if (fmin < T(0) && A[0] == A[0]) {
             T alpha = T(1);
 #pragma unroll
             for (int k = 0; k < 27; ++k) {

@@ -1203,7 +1203,7 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
                 }
             } else if constexpr (SP::code == StF32::code && !WALE) {
                 if (pair) {
-                    launch_pair32(a.pipe_planes, face, g, stream, r);
+                    launch_pair32<SP>(a.pipe_planes, face, g, stream, r);
                     return;
                 }
             }

@@ -204,19 +204,34 @@ __device__ __forceinline__ void pair_collide(const StepArgs& a, const CellRates<
         });
     }
     collide_cell<float, UNIT>(f, q, a.cp, rho_o, ux_o, uy_o, uz_o);
-    for_dirs([&](auto kc) {
-        constexpr int K = decltype(kc)::value;
-        s[K] = SP::template put<false>(s[K], f[K], wgt<float>(K), round_bits(hca, K), dith);
-    });
+    // the rounding mode is uniform: one branch for the 27 stores rather than one per direction
+    if (dith) {
+        for_dirs([&](auto kc) {
+            constexpr int K = decltype(kc)::value;
+            s[K] = SP::template put<false>(s[K], f[K], wgt<float>(K), round_bits(hca, K), true);
+        });
+    } else {
+        for_dirs([&](auto kc) {
+            constexpr int K = decltype(kc)::value;
+            s[K] = SP::template put<false>(s[K], f[K], wgt<float>(K), 0u, false);
+        });
+    }
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
         f[K] = SP::dec_hi(s[K], wgt<float>(K));
     });
     collide_cell<float, UNIT>(f, q, a.cp, rho_o, ux_o, uy_o, uz_o);
-    for_dirs([&](auto kc) {
-        constexpr int K = decltype(kc)::value;
-        s[K] = SP::template put<true>(s[K], f[K], wgt<float>(K), round_bits(hcb, K), dith);
-    });
+    if (dith) {
+        for_dirs([&](auto kc) {
+            constexpr int K = decltype(kc)::value;
+            s[K] = SP::template put<true>(s[K], f[K], wgt<float>(K), round_bits(hcb, K), true);
+        });
+    } else {
+        for_dirs([&](auto kc) {
+            constexpr int K = decltype(kc)::value;
+            s[K] = SP::template put<true>(s[K], f[K], wgt<float>(K), 0u, false);
+        });
+    }
 }
 
 // register budget: 3 waves per SIMD (168 VGPRs) where the kernel fits it without spilling, 2 otherwise

@@ -250,7 +250,10 @@ static void launch_pair32_u(int np, dim3 grid, hipStream_t s, const StepArgs& r)
         default: hipLaunchKernelGGL((k_step_pair32<16, UNIT, FACE>), grid, dim3(PWG), 0, s, r); break;
     }
 }
+// (a template, so that only the FP32 translation unit instantiates the kernels)
+template <class SP>
 static void launch_pair32(int np, bool face, dim3 grid, hipStream_t s, const StepArgs& r) {
+    static_assert(SP::code == StF32::code, "the two-cells-per-lane FP32 kernel steps FP32 storage");
     if (r.cp.unit) {
         if (face) launch_pair32_u<true, FACE_RULES>(np, grid, s, r);
         else launch_pair32_u<true, FACE_NONE>(np, grid, s, r);

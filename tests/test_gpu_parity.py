@@ -207,6 +207,61 @@ def test_interior_blocks_rates_and_forcing(oracle, variant, storage, pair):
     plan.close()
 
 
+def limiter_state(oracle, storage, grid, which):
+    """Far-from-equilibrium populations (reference layout, exactly representable in `storage`) on which
+    a limiter acts in some cells and not in others: random populations for the positivity limiter,
+    random populations plus a strong random shear (Pi_xy, Pi_xz, Pi_yz) for the realizability clamp."""
+    cx, cy, cz, w = (oracle.lattice()[i] for i in (0, 1, 2, 7))
+    n = grid.n_blocks * 4096
+    rng = np.random.default_rng(7)
+    if which == "clamp":
+        sh = 4.0 * rng.uniform(-1, 1, (3, n))
+        f = w[:, None] * (1 + 0.3 * rng.uniform(-1, 1, (27, n)) + (cx * cy)[:, None] * sh[0]
+                          + (cx * cz)[:, None] * sh[1] + (cy * cz)[:, None] * sh[2])
+    else:
+        f = w[:, None] * (1 + 0.4 * rng.uniform(-1, 1, (27, n)))
+    return oracle.to_unshifted(storage, oracle.from_unshifted(storage, f))
+
+
+LIMITER_OKW = {"clamp": dict(tau=0.52, limiter=1), "positivity": dict(tau=0.5005, limiter=2, omega_3=1.9, omega_4=1.9)}
+
+
+@pytest.mark.parametrize("tuning", ["pair", "pipe", "one_plane"])
+@pytest.mark.parametrize("which", list(LIMITER_OKW))
+@pytest.mark.parametrize("storage", ["f32", "f64", "f16s", "bf16s"])
+def test_limiters_far_from_equilibrium(oracle, storage, which, tuning):
+    """The realizability clamp of the relaxed second-order cumulants (kernel_cumulant.jl:715-735, the
+    square roots taken only by waves with a lane that needs one) and the positivity limiter (:849-884,
+    its test as a minimum tree) on states where each acts in a few per cent of the cells (checked on
+    the oracle against the same step without the limiter): the oracle's bits for two steps, in the two-
+    cells-per-lane, one-cell pipelined and one-plane kernels."""
+    # 4 x 4 x 4 periodic blocks: the inner 8 have their neighbours at lexicographic offsets (no wrap)
+    # and take the pipelined kernels, the others the boundary kernel
+    grid = uniform_grid((64, 64, 64), periodic=(True, True, True))
+    okw = LIMITER_OKW[which]
+    f0 = limiter_state(oracle, storage, grid, which)
+    G = oracle.Grid(grid)
+    fs = oracle.from_unshifted(storage, f0)
+    plain = dict(okw, limiter=0)
+    once = oracle.step(storage, G, oracle.make_params(time_step=0, arith=oracle.FAST, **okw), fs)[0]
+    free = oracle.step(storage, G, oracle.make_params(time_step=0, arith=oracle.FAST, **plain), fs)[0]
+    acted = np.mean(np.any(once != free, axis=0))
+    assert 0.002 < acted < 0.5, acted
+    f_or = fs
+    for t in range(2):
+        f_or = oracle.step(storage, G, oracle.make_params(time_step=t, arith=oracle.FAST, **okw), f_or)[0]
+    plan = gpu_plan(grid, storage, okw)
+    assert plan.level_info()["n_regular"] == 8
+    shape = {"pair": dict(pipe_planes=2, pair_kernel=1), "pipe": dict(pipe_planes=4, pair_kernel=0),
+             "one_plane": dict(pipe_planes=1, pair_kernel=0)}[tuning]
+    plan.set_tuning(overlap=1, face_kernel=0, graph=0, **shape)
+    plan.upload_f(f0 if storage == "f64" else f0.astype(np.float32))
+    for t in range(2):
+        plan.level_step(0, t)
+    compare(oracle, plan, storage, f_or, grid)
+    plan.close()
+
+
 WALE_GRIDS = {"interior": ((96, 64, 64), (22, 30, 33), 4.5), "partial": ((72, 40, 36), (14, 14, 14), 3.0)}
 
 
