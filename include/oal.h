@@ -133,9 +133,12 @@ typedef struct {
  * extent of level l, and a level-l block with children is refined into all 8 of them.  Requirements:
  * one domain per level (its tau in that level's lattice units; the acoustic rule is
  * tau_{l+1} - 1/2 = 2 (tau_l - 1/2), src/octree_q19/src/config.jl:328-335), extents multiples of 16,
- * the AB pattern, no WALE, and a margin of one coarser block around every refined region.  The
- * library derives the neighbour tables of refined plans from the block coordinates, and adds a layer of
- * ghost blocks around every finer level. */
+ * the AB pattern, and a margin of one coarser block around every refined region.  The library derives
+ * the neighbour tables of refined plans from the block coordinates, and adds a layer of ghost blocks
+ * around every finer level.  With WALE, the ghost-cell fill and the restriction also write the velocity
+ * of each cell they write, which the receiving level's gradient gather reads; each level's
+ * amr_sgs_scale multiplies its eddy and background viscosity (the reference now uses 1 on every level,
+ * src/collision/timestep.jl:1128). */
 typedef struct {
     int32_t abi_version;       /* OAL_ABI_VERSION */
     int32_t storage;           /* OAL_F32 / OAL_F64 / OAL_F16S / OAL_BF16S */

@@ -152,11 +152,12 @@ hipError_t launch_macro(int storage, const void* f, double* rho, double* u, uint
 // records of XFER_STRIDE values; for the 16-bit forms without fold, records of the stored codes
 // (raw, decoded by the rebuild, which must then be told `raw`).  rebuild: for each target cell, the weighted sum of the gathered populations over its
 // 64 stencil nodes (slot[n * ntarget + i], -1 = unused; weight w[...]), its second-order
-// non-equilibrium moments set to scale * keep times their value, written to f.
+// non-equilibrium moments set to scale * keep times their value, written to f; with vel (WALE), the
+// velocity of the interpolated state written to vel ([3][ncell], working precision) too.
 hipError_t launch_xfer_gather(int storage, const void* fnew, const void* fold, const uint32_t* cells, uint32_t n,
                                uint32_t ncell, double w1, void* out, hipStream_t stream);
 hipError_t launch_xfer_rebuild(int storage, bool raw, void* f, uint32_t ncell, const uint32_t* tcells, uint32_t n,
-                               const int32_t* slot, const float* w, const void* m, double scale, double keep,
+                               const int32_t* slot, const float* w, const void* m, double scale, double keep, void* vel,
                                hipStream_t stream);
 hipError_t launch_aa_outlet_prep(int storage, const StepArgs& a, hipStream_t stream);
 hipError_t launch_mass(int storage, const void* f, const uint8_t* solid, const int32_t* borg, const int32_t* bdom,

@@ -418,7 +418,7 @@ __global__ void __launch_bounds__(WG) k_xfer_gather(const typename SP::st* fnew,
 template <class SP, bool RAW>
 __global__ void __launch_bounds__(WG) k_xfer_rebuild(typename SP::st* f, uint32_t ncell, const uint32_t* tcells, uint32_t n,
                                                      const int32_t* slot, const float* wt, const typename SP::T* m,
-                                                     typename SP::T scale, typename SP::T keep) {
+                                                     typename SP::T scale, typename SP::T keep, typename SP::T* vel) {
     using T = typename SP::T;
     const uint32_t i = blockIdx.x * WG + threadIdx.x;
     if (i >= n) return;
@@ -537,6 +537,13 @@ __global__ void __launch_bounds__(WG) k_xfer_rebuild(typename SP::st* f, uint32_
         constexpr int K = decltype(kc)::value;
         SP::sto(f + pop_index(c, K), bad ? eq[K] : out[K], wgt<T>(K));
     });
+    // WALE: the written cell's velocity (that of the interpolated state, which the rescale and the
+    // equilibrium fallback keep) for the receiving level's gradient gather
+    if (vel) {
+        vel[c] = ux;
+        vel[ncell + c] = uy;
+        vel[2 * (size_t)ncell + c] = uz;
+    }
 }
 
 __global__ void k_set_i32(int32_t* p, int32_t v) {
@@ -560,11 +567,11 @@ static hipError_t xfer_sp(const void* fnew, const void* fold, const uint32_t* ce
 }
 template <class SP, bool RAW>
 static hipError_t rebuild_sp(void* f, uint32_t ncell, const uint32_t* tcells, uint32_t n, const int32_t* slot, const float* w,
-                             const void* m, double scale, double keep, hipStream_t s) {
+                             const void* m, double scale, double keep, void* vel, hipStream_t s) {
     using T = typename SP::T;
     if (!n) return hipSuccess;
     hipLaunchKernelGGL((k_xfer_rebuild<SP, RAW>), dim3((n + WG - 1) / WG), dim3(WG), 0, s, (typename SP::st*)f, ncell, tcells, n,
-                       slot, w, (const T*)m, T(scale), T(keep));
+                       slot, w, (const T*)m, T(scale), T(keep), (T*)vel);
     return hipGetLastError();
 }
 
@@ -583,14 +590,15 @@ hipError_t launch_xfer_gather(int storage, const void* fnew, const void* fold, c
     }
 }
 hipError_t launch_xfer_rebuild(int storage, bool raw, void* f, uint32_t ncell, const uint32_t* tcells, uint32_t n,
-                               const int32_t* slot, const float* w, const void* m, double scale, double keep, hipStream_t s) {
+                               const int32_t* slot, const float* w, const void* m, double scale, double keep, void* vel,
+                               hipStream_t s) {
     switch (storage) {
-        case 0: return rebuild_sp<StF32, false>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
-        case 1: return rebuild_sp<StF64, false>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
-        case 2: return raw ? rebuild_sp<StF16S, true>(f, ncell, tcells, n, slot, w, m, scale, keep, s)
-                           : rebuild_sp<StF16S, false>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
-        case 3: return raw ? rebuild_sp<StBF16S, true>(f, ncell, tcells, n, slot, w, m, scale, keep, s)
-                           : rebuild_sp<StBF16S, false>(f, ncell, tcells, n, slot, w, m, scale, keep, s);
+        case 0: return rebuild_sp<StF32, false>(f, ncell, tcells, n, slot, w, m, scale, keep, vel, s);
+        case 1: return rebuild_sp<StF64, false>(f, ncell, tcells, n, slot, w, m, scale, keep, vel, s);
+        case 2: return raw ? rebuild_sp<StF16S, true>(f, ncell, tcells, n, slot, w, m, scale, keep, vel, s)
+                           : rebuild_sp<StF16S, false>(f, ncell, tcells, n, slot, w, m, scale, keep, vel, s);
+        case 3: return raw ? rebuild_sp<StBF16S, true>(f, ncell, tcells, n, slot, w, m, scale, keep, vel, s)
+                           : rebuild_sp<StBF16S, false>(f, ncell, tcells, n, slot, w, m, scale, keep, vel, s);
         default: return hipErrorInvalidValue;
     }
 }

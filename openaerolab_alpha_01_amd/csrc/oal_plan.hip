@@ -1092,7 +1092,7 @@ int ml_fill(OalPlan* p, int l, double w1, hipStream_t s) {
     OAL_HIP(launch_xfer_gather(p->storage, C.fA.p, w1 == 1.0 ? nullptr : C.fB.p, F.g_src.p, F.n_gsrc, C.ncell, w1,
                                 F.g_m.p, s));
     OAL_HIP(launch_xfer_rebuild(p->storage, w1 == 1.0, F.fA.p, F.ncell, F.g_dst.p, F.n_gdst, F.g_slot.p, F.g_w.p, F.g_m.p,
-                                F.fill_scale, F.fill_keep, s));
+                                F.fill_scale, F.fill_keep, p->wale ? (void*)F.velA.p : nullptr, s));
     return OAL_OK;
 }
 
@@ -1106,7 +1106,7 @@ int ml_restrict(OalPlan* p, int l, hipStream_t s, bool full = false) {
     OAL_HIP(launch_xfer_gather(p->storage, F.fA.p, nullptr, full ? C.rf_src.p : C.r_src.p, ns, F.ncell, 1.0, C.r_m.p, s));
     OAL_HIP(launch_xfer_rebuild(p->storage, true, C.fA.p, C.ncell, full ? C.rf_dst.p : C.r_dst.p, nd,
                                 full ? C.rf_slot.p : C.r_slot.p, full ? C.rf_w.p : C.r_w.p, C.r_m.p, C.restrict_scale,
-                                C.restrict_keep, s));
+                                C.restrict_keep, p->wale ? (void*)C.velA.p : nullptr, s));
     return OAL_OK;
 }
 
@@ -1430,7 +1430,6 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
     const OalCollision& c = desc->collision;
     const bool ml = desc->n_levels > 1;
     if (ml && desc->stream_mode != OAL_STREAM_AB) return fail(OAL_ERR_UNSUPPORTED, "refined plans use the AB pattern");
-    if (ml && c.c_wale != 0.0f) return fail(OAL_ERR_UNSUPPORTED, "WALE is not supported on refined plans");
     if (c.transverse_freestream != 0 && c.transverse_freestream != 1) return fail(OAL_ERR_ARG, "transverse_freestream must be 0 or 1");
     if (!(c.transverse_freestream_blend >= 0.0f && c.transverse_freestream_blend <= 1.0f))
         return fail(OAL_ERR_ARG, "transverse_freestream_blend must lie in [0, 1]");
@@ -1576,32 +1575,43 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
                 return bail(fail(OAL_ERR_ARG, "restriction transfer list out of bounds"));
         }
         // rescale of the non-equilibrium moments between levels (coupling.jl:134-178, 188-193): tau of
-        // each level in its own lattice units, 1/(1 - omega) of the sending level (0 near omega = 1)
+        // each level in its own lattice units, 1/(1 - omega) of the sending level (0 near omega = 1), with
+        // each level's molecular rate 1/tau (the subgrid viscosity, background or WALE, left out: the
+        // octree's rescale is between the levels' relaxation times)
         const bool dbl = p->storage == OAL_F64;
+        std::vector<float> om(nl);
+        std::vector<double> om_d(nl);
+        for (int l = 0; l < nl; ++l) {
+            const DevDomain& d = p->levels[l].doms_h[0];
+            float o4;
+            double o4d;
+            cell_rates<float>(0.0f, d.tau, 1.0f, 0.0f, 0, 0.0f, 0.0f, om[l], o4);
+            cell_rates<double>(0.0, (double)d.tau, 1.0, 0.0, 0, 0.0, 0.0, om_d[l], o4d);
+        }
         for (int l = 0; l < nl; ++l) {
             const DevDomain& d = p->levels[l].doms_h[0];
             if (l > 0) {
                 const DevDomain& dc = p->levels[l - 1].doms_h[0];
                 if (dbl) {
-                    const double pc = std::fabs(1.0 - dc.omega1_d) > 0.05 ? 1.0 / (1.0 - dc.omega1_d) : 0.0;
+                    const double pc = std::fabs(1.0 - om_d[l - 1]) > 0.05 ? 1.0 / (1.0 - om_d[l - 1]) : 0.0;
                     p->levels[l].fill_scale = ((double)d.tau / (2.0 * (double)dc.tau)) * pc;
-                    p->levels[l].fill_keep = 1.0 - d.omega1_d;
+                    p->levels[l].fill_keep = 1.0 - om_d[l];
                 } else {
-                    const float pc = std::fabs(1.0f - dc.omega1) > 0.05f ? 1.0f / (1.0f - dc.omega1) : 0.0f;
+                    const float pc = std::fabs(1.0f - om[l - 1]) > 0.05f ? 1.0f / (1.0f - om[l - 1]) : 0.0f;
                     p->levels[l].fill_scale = (double)((d.tau / (2.0f * dc.tau)) * pc);
-                    p->levels[l].fill_keep = (double)(1.0f - d.omega1);
+                    p->levels[l].fill_keep = (double)(1.0f - om[l]);
                 }
             }
             if (l + 1 < nl) {
                 const DevDomain& df = p->levels[l + 1].doms_h[0];
                 if (dbl) {
-                    const double pf = std::fabs(1.0 - df.omega1_d) > 0.05 ? 1.0 / (1.0 - df.omega1_d) : 0.0;
+                    const double pf = std::fabs(1.0 - om_d[l + 1]) > 0.05 ? 1.0 / (1.0 - om_d[l + 1]) : 0.0;
                     p->levels[l].restrict_scale = ((2.0 * (double)d.tau) / (double)df.tau) * pf;
-                    p->levels[l].restrict_keep = 1.0 - d.omega1_d;
+                    p->levels[l].restrict_keep = 1.0 - om_d[l];
                 } else {
-                    const float pf = std::fabs(1.0f - df.omega1) > 0.05f ? 1.0f / (1.0f - df.omega1) : 0.0f;
+                    const float pf = std::fabs(1.0f - om[l + 1]) > 0.05f ? 1.0f / (1.0f - om[l + 1]) : 0.0f;
                     p->levels[l].restrict_scale = (double)(((2.0f * d.tau) / df.tau) * pf);
-                    p->levels[l].restrict_keep = (double)(1.0f - d.omega1);
+                    p->levels[l].restrict_keep = (double)(1.0f - om[l]);
                 }
             }
         }

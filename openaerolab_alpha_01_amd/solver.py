@@ -465,17 +465,24 @@ def level_taus(tau_finest: float, n_levels: int, rule: str = "legacy", tau_min: 
 
 def octree_plan(grids: list[BlockGrid], tau_finest: float, storage: str = "f32", params: StepParams | None = None,
                 u_inlet: float = 0.0, body_force=(0.0, 0.0, 0.0), rounding: str = "stochastic",
-                bb_mode: str = "halfway", taus: list[float] | None = None) -> Plan:
+                bb_mode: str = "halfway", taus: list[float] | None = None,
+                amr_sgs_scales: list[float] | None = None) -> Plan:
     """A refined plan over the levels of a block octree (coarsest first, e.g. geometry.sphere_octree):
     one domain per level, by default with the acoustic-scaling relaxation times (taus: explicit per-level
     values instead, e.g. level_taus(..., "legacy")); the inlet velocity is the same in every level's
-    lattice units."""
+    lattice units.  amr_sgs_scales: each level's factor on the WALE and background eddy viscosity (by
+    default 1 on every level, as the reference now sets it, src/collision/timestep.jl:1128; its earlier
+    rule was 2^-(L-1-l))."""
     if taus is None:
         taus = acoustic_taus(tau_finest, len(grids))
     elif len(taus) != len(grids):
         raise ValueError("one tau per level")
-    levels = [Level(g, [Domain(extent=g.n, tau=t, periodic=g.periodic, u_inlet=u_inlet, body_force=tuple(body_force))])
-              for g, t in zip(grids, taus)]
+    scales = [1.0] * len(grids) if amr_sgs_scales is None else list(amr_sgs_scales)
+    if len(scales) != len(grids):
+        raise ValueError("one amr_sgs_scale per level")
+    levels = [Level(g, [Domain(extent=g.n, tau=t, periodic=g.periodic, u_inlet=u_inlet, body_force=tuple(body_force),
+                               amr_sgs_scale=a)])
+              for g, t, a in zip(grids, taus, scales)]
     return Plan(levels, storage=storage, stream="ab", bb_mode=bb_mode, params=params, rounding=rounding)
 
 

@@ -17,6 +17,12 @@ What it restates:
                                           impose the polynomial equilibrium's higher moments)
   recursive step, initial coupling        coupling.jl:236-268 (step_recursive!, couple_initial!)
   level relaxation times                  src/octree_q19/src/config.jl:328-335 (acoustic scaling)
+  WALE on refined levels                  kernel_cumulant.jl:496-595 per level (amr_sgs_scale per level,
+                                          src/collision/timestep.jl:1088-1128); a transferred cell's
+                                          velocity is that of its interpolated state, the role the
+                                          parent-level velocities play in the reference's refined gather
+                                          (pinned: tests/test_oracle_octree.py, the recorded velocity
+                                          equals j / rho of the stored populations)
 
 Floating-point operations are written one by one in numpy scalars of the working type (float32, or
 float64 for the FP64 form), in the order the library's kernels use, so that results agree bit for bit.
@@ -337,13 +343,14 @@ def _moments(T, acc):
     return rho, ux, uy, uz, q
 
 
-def apply(storage, f_dst: np.ndarray, X: Xfer, m: np.ndarray, scale, keep) -> None:
+def apply(storage, f_dst: np.ndarray, X: Xfer, m: np.ndarray, scale, keep):
     """stencil sums of the gathered populations m [n_src, 27], the second-order non-equilibrium moments set
-    to scale * keep times theirs, stored into f_dst (stored form, to nearest)"""
+    to scale * keep times theirs, stored into f_dst (stored form, to nearest); returns the velocity
+    (ux, uy, uz) of each written cell's interpolated state (what a WALE run records for the cell)"""
     T = m.dtype.type
     n = len(X.dst)
     if n == 0:
-        return
+        return None
     acc = np.zeros((n, 27), dtype=T)
     for q in range(64):
         sl = X.slot[:, q]
@@ -374,6 +381,7 @@ def apply(storage, f_dst: np.ndarray, X: Xfer, m: np.ndarray, scale, keep) -> No
         f_dst[:, X.dst] = out.astype(f_dst.dtype)
     else:
         f_dst[:, X.dst] = O.from_unshifted(storage, out.astype(np.float64))
+    return ux, uy, uz
 
 
 # -------------------------------------------------------------------------------------------------
@@ -383,10 +391,15 @@ class OctreeRun:
     """An oracle run of a refined plan: per level the two buffers (A = read, B = other) of the
     augmented grid, the transfer lists and scales, and the recursive step."""
 
-    def __init__(self, grids, storage: str, taus, u_inlet=0.0, **kw):
+    def __init__(self, grids, storage: str, taus, u_inlet=0.0, amr_scales=None, **kw):
         self.storage = storage
         self.T = O.WORK_DTYPE[storage]
         self.levels = build_levels(grids)
+        # WALE: each level keeps the velocity its gradient gather reads (the last step's for stepped
+        # cells, the transfer's for the ghost and covered cells it writes) and its amr_sgs_scale
+        self.wale = kw.get("c_wale", 0.0) != 0.0
+        self.amr = [1.0] * len(grids) if amr_scales is None else [float(a) for a in amr_scales]
+        self.V = [None] * len(grids)
         # a level collects the force when its caller gave an obstacle array (the library's rule)
         self.has_obstacle = [g.solid is not None for g in grids]
         self.G = [O.Grid(A) for A in self.levels]
@@ -435,10 +448,14 @@ class OctreeRun:
                 u[:, :nc] = u_list[l]
             self.A[l] = O.init_eq(self.storage, self.G[l], rho, u)
             self.B[l] = self.A[l].copy()
+            if self.wale:
+                self.V[l] = u.astype(self.T)
         self.coupled = False
 
     def set_f(self, f_list):
         """unshifted populations [27, n_caller * 4096] (float64) of every level; ghost cells start at rest"""
+        if self.wale:
+            raise ValueError("a WALE run starts from init(): its velocity field is the initial one")
         w = O.lattice()[7]
         for l, A in enumerate(self.levels):
             ncell = A.coords.shape[0] * BS ** 3
@@ -455,7 +472,9 @@ class OctreeRun:
         if X is None or len(X.dst) == 0:
             return
         m = gather(_work(self.storage, self.A[l + 1]), X.src)
-        apply(self.storage, self.A[l], X, m, self.rest_scale[l], self.rest_keep[l])
+        u = apply(self.storage, self.A[l], X, m, self.rest_scale[l], self.rest_keep[l])
+        if self.wale:
+            self.V[l][:, X.dst] = np.stack(u)
 
     def _fill(self, l, w1):
         X = self.fill[l]
@@ -471,7 +490,9 @@ class OctreeRun:
             m = np.where(X.nob[:, None], mn, blend)
         else:
             m = mn
-        apply(self.storage, self.A[l], X, m, self.fill_scale[l], self.fill_keep[l])
+        u = apply(self.storage, self.A[l], X, m, self.fill_scale[l], self.fill_keep[l])
+        if self.wale:
+            self.V[l][:, X.dst] = np.stack(u)
 
     def couple(self):
         L = len(self.levels)
@@ -484,9 +505,13 @@ class OctreeRun:
         self.coupled = True
 
     def _step_level(self, l, t, want_force):
-        p = O.make_params(self.taus[l], time_step=t, arith=O.FAST, **self.kw)
-        _, F, _ = O.step(self.storage, self.G[l], p, self.A[l], out=self.B[l], want_force=want_force)
+        p = O.make_params(self.taus[l], time_step=t, arith=O.FAST, amr_sgs_scale=self.amr[l], **self.kw)
+        _, F, mac = O.step(self.storage, self.G[l], p, self.A[l], out=self.B[l], want_force=want_force,
+                           vel_in=self.V[l], want_macro=self.wale)
         self.A[l], self.B[l] = self.B[l], self.A[l]
+        if self.wale:
+            # the stepped cells' velocities; the others are rewritten by a transfer before they are read
+            self.V[l] = mac[1]
         # ghost cells of the new read buffer are refilled before the level's next sub-step reads them
         return F
 

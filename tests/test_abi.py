@@ -76,7 +76,7 @@ def test_operator_registry():
 
 
 def test_refined_plan_validation_without_gpu():
-    """Refined-plan descriptors are checked before any device call: nesting, margin, pattern, WALE."""
+    """Refined-plan descriptors are checked before any device call: nesting, margin, pattern."""
     import numpy as np
     from openaerolab_alpha_01_amd import _lib, solver
     from openaerolab_alpha_01_amd.geometry import BlockGrid, block_octree, build_neighbor_table
@@ -102,8 +102,7 @@ def test_refined_plan_validation_without_gpu():
     # a child without its 7 siblings
     c2 = (np.asarray(grids[2].coords) - 1).tolist()
     assert code_of([grids[0], grids[1], level_from(grids[2], list(range(1, len(c2))))]) == _lib.OAL_ERR_ARG
-    # WALE and the AA pattern are not offered on refined plans
-    assert code_of(grids, params=solver.StepParams(c_wale=0.2, inlet_turbulence=0.0)) == _lib.OAL_ERR_UNSUPPORTED
+    # the AA pattern is not offered on refined plans
     levels = [solver.Level(g, [solver.Domain(extent=g.n, tau=t, periodic=g.periodic)])
               for g, t in zip(grids, solver.acoustic_taus(0.56, 3))]
     with pytest.raises(_lib.OalError) as e:

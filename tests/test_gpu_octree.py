@@ -73,12 +73,18 @@ def run_both(oracle, octree, case, storage, steps, rounding="stochastic", force=
     rule = cfg.get("rule")
     taus = solver.level_taus(cfg["tau"], len(grids), rule) if rule else solver.acoustic_taus(cfg["tau"], len(grids))
     rcode = {"stochastic": oracle.STOCHASTIC, "nearest": oracle.NEAREST}[rounding]
-    run = octree.OctreeRun(grids, storage, taus, u_inlet=cfg["u_inlet"], rounding=rcode)
+    wale = dict(cfg.get("wale", {}))
+    if wale.get("amr_scales") == "halving":      # the reference's earlier per-level rule, 2^-(L-1-l)
+        wale["amr_scales"] = [0.5 ** (len(grids) - 1 - l) for l in range(len(grids))]
+    run = octree.OctreeRun(grids, storage, taus, u_inlet=cfg["u_inlet"], rounding=rcode,
+                           amr_scales=wale.get("amr_scales"), c_wale=wale.get("c_wale", 0.0),
+                           nu_sgs_background=wale.get("nu_sgs_background", 0.0))
     run.init([f[0] for f in fields], [f[1] for f in fields])
     run.step(steps, force=force)
-    params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
+    params = solver.StepParams(c_wale=wale.get("c_wale", 0.0), nu_sgs_background=wale.get("nu_sgs_background", 0.0),
+                               inlet_turbulence=0.0)
     plan = solver.octree_plan(grids, cfg["tau"], storage=storage, params=params, u_inlet=cfg["u_inlet"],
-                              rounding=rounding, taus=taus)
+                              rounding=rounding, taus=taus, amr_sgs_scales=wale.get("amr_scales"))
     for l, (rho, u) in enumerate(fields):
         plan.init_equilibrium(rho, u, level=l)
     print(f"octree case {case} storage {storage} rounding {rounding} levels {len(grids)} "
@@ -126,6 +132,30 @@ def test_octree_four_levels_sphere(oracle, octree, rule, storage):
                                    rtol=1e-11, atol=1e-14)
 
 
+WALE = dict(c_wale=0.2, nu_sgs_background=1e-4, amr_scales="halving")
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16s", "bf16s"])
+@pytest.mark.parametrize("case", ["sphere_channel", "three_levels"])
+def test_octree_wale(oracle, octree, case, storage):
+    """WALE on refined plans (kernel_cumulant.jl:496-595 on every level, each level's amr_sgs_scale on
+    the eddy and background viscosity): the fill and the restriction record the velocity of each cell
+    they write, which the receiving level's gradient gather reads; every level's caller cells bit for
+    bit after 3 coarse steps."""
+    grids, run, plan = run_both(oracle, octree, case, storage, steps=3, cfg=dict(CASES[case], wale=WALE))
+    with plan:
+        for l in range(len(grids)):
+            compare_level(oracle, plan, run, storage, l)
+
+
+def test_octree_wale_four_levels_sphere(oracle, octree):
+    """WALE at the block-AMR sphere case's depth (4 levels, FP16, one coarse step): bit for bit."""
+    grids, run, plan = run_both(oracle, octree, "acoustic", "f16s", steps=1, cfg=dict(FOUR["acoustic"], wale=WALE))
+    with plan:
+        for l in range(4):
+            compare_level(oracle, plan, run, "f16s", l)
+
+
 def test_octree_parity_nearest_rounding(oracle, octree):
     grids, run, plan = run_both(oracle, octree, "slab_periodic", "f16s", steps=3, rounding="nearest")
     with plan:
@@ -164,16 +194,17 @@ def _states(plan, n_levels):
     return [plan.download_f(level=l, dtype=np.float64) for l in range(n_levels)]
 
 
-@pytest.mark.parametrize("case,storage,overlap", [("sphere_channel", "f16s", 1), ("three_levels", "bf16s", 0),
-                                                  ("sphere_channel", "f32", 0)])
-def test_graph_replay_matches_eager(case, storage, overlap):
+@pytest.mark.parametrize("case,storage,overlap,c_wale", [("sphere_channel", "f16s", 1, 0.0), ("three_levels", "bf16s", 0, 0.0),
+                                                         ("sphere_channel", "f32", 0, 0.0), ("three_levels", "f32", 1, 0.2)])
+def test_graph_replay_matches_eager(case, storage, overlap, c_wale):
     """oal_step replaying captured coarse-step graphs gives the eager launches' bits, call after call
     (with downloads and odd step counts between the replays, so both buffer arrangements of level 0
-    are captured), and the force of a last step that collects it."""
+    are captured), and the force of a last step that collects it; with WALE the velocity buffers swap
+    with the populations."""
     cfg = CASES[case]
     grids = cfg["grids"]()
     fields = [level_fields(g, 3 + l, cfg["u0"]) for l, g in enumerate(grids)]
-    params = solver.StepParams(c_wale=0.0, inlet_turbulence=0.0)
+    params = solver.StepParams(c_wale=c_wale, inlet_turbulence=0.0)
     plans = []
     for graph in (1, 0):
         p = solver.octree_plan(grids, cfg["tau"], storage=storage, params=params, u_inlet=cfg["u_inlet"])

@@ -203,3 +203,70 @@ def test_shear_wave_catches_wrong_fill_scale(oracle, octree):
         run.fill_scale[1] = run.T(1) / (run.T(1) - run.omega[0])
     rel = _shear_run(oracle, octree, plant)
     assert np.abs(rel).max() > 6e-3 or np.ptp(rel) > 4e-3
+
+
+def _wale_shear_run(oracle, octree, c_wale, plant=None, steps=6):
+    """the shear-wave slab in FP64 with WALE at c_wale (and a level-dependent amr_sgs_scale)"""
+    grids = slab()
+    run = octree.OctreeRun(grids, "f64", acoustic_taus(0.6, 2), amr_scales=[0.5, 1.0], adaptive_omega4=0,
+                           c_wale=c_wale)
+    rho, vel = [], []
+    for g in grids:
+        z = g.cell_coords()[:, 2] + 0.5
+        u = np.zeros((3, g.n_cells))
+        u[0] = 0.01 * np.sin(2 * np.pi * z / g.n[2])
+        rho.append(np.ones(g.n_cells))
+        vel.append(u)
+    run.init(rho, vel)
+    if plant:
+        plant(run)
+    run.step(steps)
+    return run, grids
+
+
+def _swap_recorded_velocity(octree, monkeypatch):
+    """plant: the transfers record (uy, ux, uz) for the cells they write"""
+    real = octree.apply
+
+    def swapped(*a, **k):
+        u = real(*a, **k)
+        return None if u is None else (u[1], u[0], u[2])
+    monkeypatch.setattr(octree, "apply", swapped)
+
+
+@pytest.mark.parametrize("plant", [False, True])
+def test_wale_transfers_record_the_written_cells_velocity(oracle, octree, monkeypatch, plant):
+    """WALE on a refined plan: the velocity a transfer records for each ghost (fill) and covered
+    (restriction) cell it writes is j / rho of the populations it stored there, as an independent
+    moment sum of the stored state gives it; swapping two components (planted) fails the check."""
+    if plant:
+        _swap_recorded_velocity(octree, monkeypatch)
+    run, grids = _wale_shear_run(oracle, octree, 0.2)
+    worst = 0.0
+    for l, X in ((1, run.fill[1]), (0, run.restr[0])):
+        G = oracle.Grid(run.levels[l])
+        _, _, v = oracle.moments("f64", G, run.A[l])
+        worst = max(worst, float(np.abs(run.V[l][:, X.dst] - v[:, X.dst]).max()))
+    if plant:
+        assert worst > 1e-4, worst
+    else:
+        assert worst < 1e-15, worst
+
+
+@pytest.mark.parametrize("plant", [False, True])
+def test_wale_refined_pure_shear_wave(oracle, octree, monkeypatch, plant):
+    """WALE's eddy viscosity vanishes in pure shear (its operator is the traceless symmetric part of g^2,
+    zero for g = d u_x / d z alone): the shear wave crossing the refined slab with WALE on gives the run
+    without it to rounding, through the interfaces where the gradient gather reads the ghost and covered
+    cells' recorded velocities; with those velocities' components swapped (planted) the interfaces see a
+    spurious gradient and the runs part."""
+    if plant:
+        _swap_recorded_velocity(octree, monkeypatch)
+    run_w, grids = _wale_shear_run(oracle, octree, 0.2)
+    monkeypatch.undo()
+    run_0, _ = _wale_shear_run(oracle, octree, 0.0)
+    diff = max(float(np.abs(run_w.caller_f(l) - run_0.caller_f(l)).max()) for l in range(len(grids)))
+    if plant:
+        assert diff > 1e-9, diff
+    else:
+        assert diff < 1e-14, diff
