@@ -32,6 +32,14 @@ __host__ __device__ constexpr T wgt(int k) {
     return dnorm2(k) == 0 ? T(8) / T(27) : dnorm2(k) == 1 ? T(2) / T(27) : dnorm2(k) == 2 ? T(1) / T(54) : T(1) / T(216);
 }
 
+// the same weights for a direction known at run time only (a select among the four constants)
+template <class T>
+__device__ __forceinline__ T wgt_rt(int k) {
+    constexpr T w0 = wgt<T>(13), w1 = wgt<T>(12), w2 = wgt<T>(9), w3 = wgt<T>(0);
+    const int n2 = (k % 3 - 1) * (k % 3 - 1) + ((k / 3) % 3 - 1) * ((k / 3) % 3 - 1) + (k / 9 - 1) * (k / 9 - 1);
+    return n2 == 0 ? w0 : n2 == 1 ? w1 : n2 == 2 ? w2 : w3;
+}
+
 template <int... Ks, class F>
 __device__ __forceinline__ void for_each_dir(std::integer_sequence<int, Ks...>, F&& f) {
     (f(std::integral_constant<int, Ks>{}), ...);

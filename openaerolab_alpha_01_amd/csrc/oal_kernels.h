@@ -86,6 +86,7 @@ struct StepArgs {
     int32_t ts_mul, ts_add;
     int32_t parity;              // AA pattern: 1 odd step (neighbour slots), 0 even step (own slots)
     int32_t pipe_planes;         // planes per workgroup of the pipelined interior kernel (1 = off; divides 16)
+    int32_t face_x;              // a domain of the level has the exact specular z wall, or the transverse freestream is on
     int32_t pair_kernel;         // FP32 / 16-bit storage: regular blocks two cells per lane (pipelined shapes)
     // interpolated bounce-back: per link-mask slot, [27][BS3] Float16 bits of the wall distance q of the
     // link from a cell along c_k (units of |c_k|); null = halfway bounce-back on every solid link
@@ -151,7 +152,7 @@ hipError_t launch_macro(int storage, const void* f, double* rho, double* u, uint
 // (bit 31 set = not blended in time) of fnew, or of (1 - w1) fold + w1 fnew when fold is given, in
 // records of XFER_STRIDE values; for the 16-bit forms without fold, records of the stored codes
 // (raw, decoded by the rebuild, which must then be told `raw`).  rebuild: for each target cell, the weighted sum of the gathered populations over its
-// 64 stencil nodes (slot[n * ntarget + i], -1 = unused; weight w[...]), its second-order
+// 64 stencil nodes (slot[64 * i + node], -1 = unused; weight w[...]: cell-major), its second-order
 // non-equilibrium moments set to scale * keep times their value, written to f; with vel (WALE), the
 // velocity of the interpolated state written to vel ([3][ncell], working precision) too.
 hipError_t launch_xfer_gather(int storage, const void* fnew, const void* fold, const uint32_t* cells, uint32_t n,

@@ -368,125 +368,154 @@ template <> struct Vec16<double> {
 // the rebuild; the decoded values are the ones a float record would hold.
 constexpr int XFER_RAW_STRIDE = 32;
 
+// gather: a few lanes per donor, each writing one 16-byte piece of its record (the pieces of a
+// wavefront's donors are contiguous): RAW, 4 lanes of 8 codes; float records, 7 lanes of 4 values;
+// double records, 14 lanes of 2
+template <class SP, bool RAW>
+constexpr int xfer_gather_lanes() {
+    return RAW ? XFER_RAW_STRIDE / 8 : XFER_STRIDE / Vec16<typename SP::T>::n;
+}
 template <class SP, bool RAW>
 __global__ void __launch_bounds__(WG) k_xfer_gather(const typename SP::st* fnew, const typename SP::st* fold,
                                                     const uint32_t* cells, uint32_t n, uint32_t ncell, typename SP::T w1,
                                                     typename SP::T* out) {
     using T = typename SP::T;
-    const uint32_t i = blockIdx.x * WG + threadIdx.x;
-    if (i >= n) return;
+    constexpr int Q = xfer_gather_lanes<SP, RAW>();
+    const uint32_t t = blockIdx.x * WG + threadIdx.x;
+    if (t >= n * (uint32_t)Q) return;
+    const uint32_t i = t / Q;
+    const int q = (int)(t - i * Q);
     const uint32_t e = cells[i];
     const uint32_t c = e & 0x7fffffffu;
     if constexpr (RAW) {
-        uint16_t codes[XFER_RAW_STRIDE];
+        uint32_t h[8];
 #pragma unroll
-        for (int k = 27; k < XFER_RAW_STRIDE; ++k) codes[k] = 0;
-        for_dirs([&](auto kc) {
-            constexpr int K = decltype(kc)::value;
-            codes[K] = (uint16_t)SP::ld_raw(fnew + pop_index(c, K));
-        });
-        uint4* o = reinterpret_cast<uint4*>(reinterpret_cast<uint16_t*>(out) + (size_t)i * XFER_RAW_STRIDE);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            o[j] = make_uint4(codes[8 * j] | ((uint32_t)codes[8 * j + 1] << 16), codes[8 * j + 2] | ((uint32_t)codes[8 * j + 3] << 16),
-                              codes[8 * j + 4] | ((uint32_t)codes[8 * j + 5] << 16), codes[8 * j + 6] | ((uint32_t)codes[8 * j + 7] << 16));
+        for (int r = 0; r < 8; ++r) {
+            const int k = 8 * q + r;
+            h[r] = k < 27 ? (uint32_t)SP::ld_raw(fnew + pop_index(c, k)) : 0u;
+        }
+        reinterpret_cast<uint4*>(reinterpret_cast<uint16_t*>(out) + (size_t)i * XFER_RAW_STRIDE)[q] =
+            make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
         return;
     }
-    const bool blend = fold && !(e >> 31);
-    const T w0 = T(1) - w1;
-    T vals[XFER_STRIDE];
-    vals[XFER_STRIDE - 1] = T(0);
-    for_dirs([&](auto kc) {
-        constexpr int K = decltype(kc)::value;
-        T v = SP::ld(fnew + pop_index(c, K), wgt<T>(K));
-        if (blend) v = SP::ld(fold + pop_index(c, K), wgt<T>(K)) * w0 + v * w1;
-        vals[K] = v;
-    });
-    // one 16-byte-aligned record per donor, written as whole 16-byte vectors
     using V = typename Vec16<T>::type;
     constexpr int VN = Vec16<T>::n;
-    V* o = reinterpret_cast<V*>(out + (size_t)i * XFER_STRIDE);
+    const bool blend = fold && !(e >> 31);
+    V v;
 #pragma unroll
-    for (int j = 0; j < XFER_STRIDE / VN; ++j) {
-        V v;
-#pragma unroll
-        for (int e = 0; e < VN; ++e) v[e] = vals[j * VN + e];
-        o[j] = v;
+    for (int r = 0; r < VN; ++r) {
+        const int k = VN * q + r;
+        T x = T(0);
+        if (k < 27) {
+            const T w = wgt_rt<T>(k);
+            x = SP::ld(fnew + pop_index(c, k), w);
+            if (blend) x = SP::ld(fold + pop_index(c, k), w) * (T(1) - w1) + x * w1;
+        }
+        v[r] = x;
     }
+    reinterpret_cast<V*>(out + (size_t)i * XFER_STRIDE)[q] = v;
 }
 
+// sum of a value over the 16 lanes of a DPP row, as a fixed pairwise tree: lanes l and l xor 1, then
+// l xor 2 (quad permutations), then the two quads of each half-row (row_half_mirror: every lane of a
+// quad holds its quad's sum, so the mirror pairs the quad sums) and the two half-rows (row_mirror).
+// Every lane ends with ((p0 + p1) + (p2 + p3)) + ((p4 + p5) + (p6 + p7)) + ((p8 + ...)) in the same
+// bits (addition commutes).
+template <int CTRL>
+__device__ __forceinline__ float dpp_get(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_get(double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)b, CTRL, 0xf, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+template <class T>
+__device__ __forceinline__ T row_sum(T v) {
+    v = v + dpp_get<0xb1>(v);    // quad_perm [1, 0, 3, 2]
+    v = v + dpp_get<0x4e>(v);    // quad_perm [2, 3, 0, 1]
+    v = v + dpp_get<0x141>(v);   // row_half_mirror
+    v = v + dpp_get<0x140>(v);   // row_mirror
+    return v;
+}
+// any lane of the lane's DPP row
+__device__ __forceinline__ bool row_any(bool b) {
+    const uint64_t m = __builtin_amdgcn_ballot_w64(b);
+    return ((m >> (threadIdx.x & 48u)) & 0xffffull) != 0;
+}
+
+// rebuild: 16 lanes (one DPP row) per written cell, lane j summing stencil nodes 4j .. 4j + 3 in node
+// order from zero, the 16 partial sums then added as a pairwise tree (row_sum); every lane forms the
+// moments, lane j rebuilds and stores directions j and j + 16.  Stencils are cell-major: slot[i][64],
+// wt[i][64].
 template <class SP, bool RAW>
 __global__ void __launch_bounds__(WG) k_xfer_rebuild(typename SP::st* f, uint32_t ncell, const uint32_t* tcells, uint32_t n,
                                                      const int32_t* slot, const float* wt, const typename SP::T* m,
                                                      typename SP::T scale, typename SP::T keep, typename SP::T* vel) {
     using T = typename SP::T;
-    const uint32_t i = blockIdx.x * WG + threadIdx.x;
-    if (i >= n) return;
+    const uint32_t i = blockIdx.x * (WG / 16) + (threadIdx.x >> 4);
+    const int j = (int)(threadIdx.x & 15);
+    if (i >= n) return;   // whole rows: the lanes of a live cell stay together
+    const int4 s4 = *reinterpret_cast<const int4*>(slot + (size_t)i * 64 + 4 * j);
+    const float4 w4 = *reinterpret_cast<const float4*>(wt + (size_t)i * 64 + 4 * j);
+    const int32_t sr[4] = {s4.x, s4.y, s4.z, s4.w};
+    const T wn[4] = {T(w4.x), T(w4.y), T(w4.z), T(w4.w)};
     T acc[27];
 #pragma unroll
     for (int k = 0; k < 27; ++k) acc[k] = T(0);
-    using V = typename Vec16<T>::type;
-    constexpr int VN = Vec16<T>::n;
-    // the 64 nodes in groups of G: the group's slots and weights, then all of its records, are in
-    // flight at once (the compiler would otherwise wait for each node's loads in turn); an unused node
-    // reads record 0 and leaves acc unchanged through a select; the sums are formed in node order
-    constexpr int G = RAW ? 8 : (sizeof(T) == 8 ? 2 : 4);
-    for (int nb = 0; nb < 64; nb += G) {
-        int32_t sr[G];
-        T wn[G];
+    if constexpr (RAW) {
+        uint4 rec[4][4];
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            sr[g] = slot[(size_t)(nb + g) * n + i];
-            wn[g] = T(wt[(size_t)(nb + g) * n + i]);
+        for (int g = 0; g < 4; ++g) {
+            const uint4* row = reinterpret_cast<const uint4*>(reinterpret_cast<const uint16_t*>(m) +
+                                                              (size_t)(sr[g] >= 0 ? sr[g] : 0) * XFER_RAW_STRIDE);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) rec[g][q] = row[q];
         }
-        if constexpr (RAW) {
-            uint4 rec[G][4];
 #pragma unroll
-            for (int g = 0; g < G; ++g) {
-                const uint4* row = reinterpret_cast<const uint4*>(reinterpret_cast<const uint16_t*>(m) +
-                                                                  (size_t)(sr[g] >= 0 ? sr[g] : 0) * XFER_RAW_STRIDE);
+        for (int g = 0; g < 4; ++g) {
+            const bool use = sr[g] >= 0;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) rec[g][j] = row[j];
-            }
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t wd[4] = {rec[g][q].x, rec[g][q].y, rec[g][q].z, rec[g][q].w};
 #pragma unroll
-            for (int g = 0; g < G; ++g) {
-                const bool use = sr[g] >= 0;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const uint32_t wd[4] = {rec[g][j].x, rec[g][j].y, rec[g][j].z, rec[g][j].w};
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const int k = 8 * j + e;
-                        if (k < 27) {
-                            const T add = acc[k] + wn[g] * SP::from_raw((wd[e >> 1] >> (16 * (e & 1))) & 0xffffu, wgt<T>(k));
-                            acc[k] = use ? add : acc[k];
-                        }
+                for (int e = 0; e < 8; ++e) {
+                    const int k = 8 * q + e;
+                    if (k < 27) {
+                        const T add = acc[k] + wn[g] * SP::from_raw((wd[e >> 1] >> (16 * (e & 1))) & 0xffffu, wgt<T>(k));
+                        acc[k] = use ? add : acc[k];
                     }
                 }
             }
-        } else {
-            V rec[G][XFER_STRIDE / VN];
+        }
+    } else {
+        using V = typename Vec16<T>::type;
+        constexpr int VN = Vec16<T>::n;
+        V rec[4][XFER_STRIDE / VN];
 #pragma unroll
-            for (int g = 0; g < G; ++g) {
-                const V* row = reinterpret_cast<const V*>(m + (size_t)(sr[g] >= 0 ? sr[g] : 0) * XFER_STRIDE);
+        for (int g = 0; g < 4; ++g) {
+            const V* row = reinterpret_cast<const V*>(m + (size_t)(sr[g] >= 0 ? sr[g] : 0) * XFER_STRIDE);
 #pragma unroll
-                for (int j = 0; j < XFER_STRIDE / VN; ++j) rec[g][j] = row[j];
-            }
+            for (int q = 0; q < XFER_STRIDE / VN; ++q) rec[g][q] = row[q];
+        }
 #pragma unroll
-            for (int g = 0; g < G; ++g) {
-                const bool use = sr[g] >= 0;
+        for (int g = 0; g < 4; ++g) {
+            const bool use = sr[g] >= 0;
 #pragma unroll
-                for (int j = 0; j < XFER_STRIDE / VN; ++j)
+            for (int q = 0; q < XFER_STRIDE / VN; ++q)
 #pragma unroll
-                    for (int e = 0; e < VN; ++e)
-                        if (j * VN + e < 27) {
-                            const T add = acc[j * VN + e] + wn[g] * rec[g][j][e];
-                            acc[j * VN + e] = use ? add : acc[j * VN + e];
-                        }
-            }
+                for (int e = 0; e < VN; ++e)
+                    if (q * VN + e < 27) {
+                        const T add = acc[q * VN + e] + wn[g] * rec[g][q][e];
+                        acc[q * VN + e] = use ? add : acc[q * VN + e];
+                    }
         }
     }
-    // moments of the interpolated state
+#pragma unroll
+    for (int k = 0; k < 27; ++k) acc[k] = row_sum(acc[k]);
+    // moments of the interpolated state (every lane)
     T rho = T(0), jx = T(0), jy = T(0), jz = T(0), pxx = T(0), pyy = T(0), pzz = T(0), pxy = T(0), pxz = T(0), pyz = T(0);
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
@@ -520,26 +549,35 @@ __global__ void __launch_bounds__(WG) k_xfer_rebuild(typename SP::st* f, uint32_
     const T dxy = sk * qxy - qxy, dxz = sk * qxz - qxz, dyz = sk * qyz - qyz;
     const T tr = (dxx + dyy) + dzz;
     const T usq = (ux * ux + uy * uy) + uz * uz;
-    T out[27], eq[27];
+    // lane j: directions j and j + 16
+    T out[2], eq[2];
     bool bad = false;
-    for_dirs([&](auto kc) {
-        constexpr int K = decltype(kc)::value;
-        const T cx = T(dcx(K)), cy = T(dcy(K)), cz = T(dcz(K));
-        constexpr T w = wgt<T>(K);
-        const T cac = ((cx * cx * dxx + cy * cy * dyy) + cz * cz * dzz) + T(2) * ((cx * cy * dxy + cx * cz * dxz) + cy * cz * dyz);
-        out[K] = acc[K] + (w * T(4.5)) * (cac - tr * third);
-        const T cu = (cx * ux + cy * uy) + cz * uz;
-        eq[K] = (w * rho) * (((T(1) + T(3) * cu) + T(4.5) * cu * cu) - T(1.5) * usq);
-        bad = bad || out[K] < T(1e-6) * w;
-    });
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int k = j + 16 * h;
+        if (k < 27) {
+            const T cx = T(k % 3 - 1), cy = T((k / 3) % 3 - 1), cz = T(k / 9 - 1);
+            const T w = wgt_rt<T>(k);
+            T a = acc[0];
+#pragma unroll
+            for (int kk = 1; kk < 27; ++kk) a = kk == k ? acc[kk] : a;
+            const T cac = ((cx * cx * dxx + cy * cy * dyy) + cz * cz * dzz) + T(2) * ((cx * cy * dxy + cx * cz * dxz) + cy * cz * dyz);
+            out[h] = a + (w * T(4.5)) * (cac - tr * third);
+            const T cu = (cx * ux + cy * uy) + cz * uz;
+            eq[h] = (w * rho) * (((T(1) + T(3) * cu) + T(4.5) * cu * cu) - T(1.5) * usq);
+            bad = bad || out[h] < T(1e-6) * w;
+        }
+    }
+    bad = row_any(bad);
     const uint32_t c = tcells[i];
-    for_dirs([&](auto kc) {
-        constexpr int K = decltype(kc)::value;
-        SP::sto(f + pop_index(c, K), bad ? eq[K] : out[K], wgt<T>(K));
-    });
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int k = j + 16 * h;
+        if (k < 27) SP::sto(f + pop_index(c, k), bad ? eq[h] : out[h], wgt_rt<T>(k));
+    }
     // WALE: the written cell's velocity (that of the interpolated state, which the rescale and the
     // equilibrium fallback keep) for the receiving level's gradient gather
-    if (vel) {
+    if (vel && j == 0) {
         vel[c] = ux;
         vel[ncell + c] = uy;
         vel[2 * (size_t)ncell + c] = uz;
@@ -561,7 +599,8 @@ static hipError_t xfer_sp(const void* fnew, const void* fold, const uint32_t* ce
                           void* out, hipStream_t s) {
     using T = typename SP::T;
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL((k_xfer_gather<SP, RAW>), dim3((n + WG - 1) / WG), dim3(WG), 0, s, (const typename SP::st*)fnew,
+    constexpr uint32_t Q = (uint32_t)xfer_gather_lanes<SP, RAW>();
+    hipLaunchKernelGGL((k_xfer_gather<SP, RAW>), dim3((n * Q + WG - 1) / WG), dim3(WG), 0, s, (const typename SP::st*)fnew,
                        (const typename SP::st*)fold, cells, n, ncell, T(w1), (T*)out);
     return hipGetLastError();
 }
@@ -570,7 +609,7 @@ static hipError_t rebuild_sp(void* f, uint32_t ncell, const uint32_t* tcells, ui
                              const void* m, double scale, double keep, void* vel, hipStream_t s) {
     using T = typename SP::T;
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL((k_xfer_rebuild<SP, RAW>), dim3((n + WG - 1) / WG), dim3(WG), 0, s, (typename SP::st*)f, ncell, tcells, n,
+    hipLaunchKernelGGL((k_xfer_rebuild<SP, RAW>), dim3((n + WG / 16 - 1) / (WG / 16)), dim3(WG), 0, s, (typename SP::st*)f, ncell, tcells, n,
                        slot, w, (const T*)m, T(scale), T(keep), (T*)vel);
     return hipGetLastError();
 }

@@ -126,7 +126,7 @@ struct Level {
     uint32_t ncell_host = 0;
     uint32_t n_caller_blocks = 0, n_ghost_blocks = 0, n_covered_blocks = 0;
     // fill of this level's ghost cells from the coarser level: moments of g_src cells (coarser level),
-    // stencils (g_slot, g_w: [64][n_gdst]) onto g_dst cells (this level)
+    // stencils (g_slot, g_w: [n_gdst][64], cell-major) onto g_dst cells (this level)
     DevBuf<uint32_t> g_src, g_dst;
     DevBuf<int32_t> g_slot;
     DevBuf<float> g_w;
@@ -600,6 +600,15 @@ struct XferHost {
     std::vector<float> w;             // [64][n_dst]
 };
 
+// [64][n] (node-major, as the stencils are built) -> [n][64] (cell-major, as the rebuild reads them)
+template <class V>
+static std::vector<V> node_to_cell_major(const std::vector<V>& a, size_t n) {
+    std::vector<V> b(a.size());
+    for (size_t q = 0; q < 64; ++q)
+        for (size_t i = 0; i < n; ++i) b[i * 64 + q] = a[q * n + i];
+    return b;
+}
+
 // written cells inside the destination level, donors inside the source level, slots inside the donor list
 static bool xfer_in_bounds(const XferHost& X, uint64_t ncell_dst, uint64_t ncell_src) {
     for (uint32_t c : X.dst)
@@ -1015,6 +1024,8 @@ StepArgs make_args(OalPlan* p, Level& L, int32_t time_step, bool want_force) {
     for (const DevDomain& dm : L.doms_h)
         near = near && ((uint64_t)dm.nbx * dm.nby + dm.nbx + 1) * 27u * BS3 * (uint64_t)storage_bytes(p->storage) < (1ull << 31);
     a.face_kernel = (p->face_kernel || !near) ? 1 : 0;
+    a.face_x = p->cp.tfs ? 1 : 0;
+    for (const DevDomain& dm : L.doms_h) a.face_x |= dm.zwall == 1 ? 1 : 0;
     a.obuf = L.obuf.p;
     a.ncell = L.ncell;
     a.nb = L.nb;
@@ -1632,6 +1643,11 @@ int oal_plan_create(const OalPlanDesc* desc, OalPlan** out) {
         if (rc) return bail(rc);
         if (ml) {
             Level& L = p->levels[l];
+            // the rebuild kernel reads each written cell's 64 stencil nodes as one row: cell-major
+            for (XferHost* X : {&fillX[l], &restX[l], &restFull[l]}) {
+                X->slot = node_to_cell_major(X->slot, X->dst.size());
+                X->w = node_to_cell_major(X->w, X->dst.size());
+            }
             const XferHost& F = fillX[l];
             const XferHost& R = restX[l];
             const XferHost& RF = restFull[l];

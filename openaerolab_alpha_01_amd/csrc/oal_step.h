@@ -53,7 +53,9 @@ struct CellCtx {
 // The source of direction K for this cell: classification and, for SRC_REG / SRC_SOLID, the cell index.
 // GEN = false is the interior path (block flags all clear): every source is a regular cell of this
 // block or of one of its 26 neighbours, which all exist.
-template <int K, bool GEN, bool REG = false>
+// FX: the exact specular z wall and the transverse freestream are compiled in (levels that use neither
+// run a form without them)
+template <int K, bool GEN, bool REG = false, bool FX = true>
 __device__ __forceinline__ int classify(const StepArgs& a, const CellCtx& c, uint32_t& src) {
     constexpr int cx = dcx(K), cy = dcy(K), cz = dcz(K);
     if constexpr (GEN) {
@@ -67,7 +69,7 @@ __device__ __forceinline__ int classify(const StepArgs& a, const CellCtx& c, uin
             const bool zh = cz < 0 && !c.per2 && sgz >= c.ext2;
             if (xl) return SRC_INLET;
             if (xh) return SRC_OWN;
-            if (c.zwall == 1 && (zl || zh)) {
+            if (FX && c.zwall == 1 && (zl || zh)) {
                 // the source beyond the wall mirrors the wall-row cell (x - cx, y - cy, z): that cell's
                 // population mirror_z(K) arrives; with no such cell on the level (beyond a non-periodic
                 // y face, or an absent block) the cell's own mirror_z(K)
@@ -80,7 +82,7 @@ __device__ __forceinline__ int classify(const StepArgs& a, const CellCtx& c, uin
                 return SRC_ZIMG;
             }
             if (c.zwall != 0 && (zl || zh)) return SRC_OPP;
-            if (a.cp.tfs && (yl || yh || zl || zh)) return SRC_TFS;
+            if (FX && a.cp.tfs && (yl || yh || zl || zh)) return SRC_TFS;
             if (yl && c.ymin_wall == 1) return SRC_OPP;
             if (yl || yh) return SRC_MIRY;
             if (zl || zh) return SRC_MIRZ;
@@ -386,7 +388,7 @@ __device__ __forceinline__ void bfl_links(const StepArgs& a, const CellCtx& c, t
 
 // BFL: the level has a q-map (interpolated bounce-back); a separate instantiation, so that the boundary
 // kernels of levels without one keep their register budget.
-template <class SP, int PAT, bool WALE, bool GEN, bool REG = false, bool BFL = false, bool SPONGE = false>
+template <class SP, int PAT, bool WALE, bool GEN, bool REG = false, bool BFL = false, bool SPONGE = false, bool FX = true>
 __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, const DevDomain* dm, bool active,
                                           bool solid_own, double& F0, double& F1, double& F2) {
     using T = typename SP::T;
@@ -427,24 +429,24 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
             // so the 27 loads are all in flight together; the inlet and rest rules replace the value
             // afterwards.  The rules' directions (opposite, mirrors) have the weight of K.
             uint32_t src = 0;
-            const int kind = classify<K, GEN, REG>(a, c, src);
+            const int kind = classify<K, GEN, REG, FX>(a, c, src);
             const int jd = kind == SRC_REG ? K : rule_dir<K>(kind);
             const st* p;
             if constexpr (REG)
                 p = kind == SRC_REG ? (const st*)(ra.base(fi, K, c.z, dcz(K)) + ra.v[dcx(K) + 1][dcy(K) + 1])
-                                    : fi + pop_index(kind == SRC_ZIMG ? src : c.own, jd);
+                                    : fi + pop_index(FX && kind == SRC_ZIMG ? src : c.own, jd);
             else
-                p = fi + pop_index(kind == SRC_REG || kind == SRC_ZIMG ? src : c.own, jd);
+                p = fi + pop_index(kind == SRC_REG || (FX && kind == SRC_ZIMG) ? src : c.own, jd);
             raw[K] = SP::ld_raw(p);
             inlet_links |= (kind == SRC_INLET ? 1u : 0u) << K;
             rest_links |= (kind == SRC_REST ? 1u : 0u) << K;
-            tfs_links |= (kind == SRC_TFS ? 1u : 0u) << K;
+            if constexpr (FX) tfs_links |= (kind == SRC_TFS ? 1u : 0u) << K;
             solid_links |= (kind == SRC_SOLID ? 1u : 0u) << K;
             reg_links |= (kind == SRC_REG ? 1u : 0u) << K;
             val = T(0);   // set below
         } else {
             uint32_t src = 0;
-            const int kind = classify<K, GEN, REG>(a, c, src);
+            const int kind = classify<K, GEN, REG, FX>(a, c, src);
             if (kind == SRC_REG) {
                 constexpr int slot = PAT == PAT_AB ? K : dopp(K);
                 if constexpr (REG)
@@ -472,7 +474,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
                 if constexpr (dcx(K) > 0)
                     if ((inlet_links >> K) & 1u)
                         val = boundary_value<K, T>(SRC_INLET, a, c, u_in, a.step(), [&](auto) { return T(0); });
-                if constexpr (dcy(K) != 0 || dcz(K) != 0)
+                if constexpr (FX && (dcy(K) != 0 || dcz(K) != 0))
                     if ((tfs_links >> K) & 1u) val = tfs_value<K, T>(a, val, u_in);
             }
             f[K] = val;
@@ -543,7 +545,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
             SP::sto_r((st*)(ra.base(fo, K, c.z, -dcz(K)) + ra.v[1 - dcx(K)][1 - dcy(K)]), f[K], wK, rb(K), dith);
         } else if constexpr (REG) {
             uint32_t dst = 0;
-            const int kind = classify<J, GEN, REG>(a, c, dst);
+            const int kind = classify<J, GEN, REG, FX>(a, c, dst);
             if (kind == SRC_REG) {
                 SP::sto_r((st*)(ra.base(fo, K, c.z, -dcz(K)) + ra.v[1 - dcx(K)][1 - dcy(K)]), f[K], wK, rb(K), dith);
             } else if (kind != SRC_INLET) {
@@ -561,7 +563,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
             }
         } else {
             uint32_t dst = 0;
-            const int kind = classify<J, GEN>(a, c, dst);
+            const int kind = classify<J, GEN, false, FX>(a, c, dst);
             if (kind == SRC_REG) {
                 SP::sto_r(fo + pop_index(dst, K), f[K], wK, rb(K), dith);
             } else if (kind != SRC_INLET) {   // inlet links are evaluated by the even step itself
@@ -573,7 +575,7 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
         if constexpr (GEN) {
             if (link_force) {
                 uint32_t dst = 0;
-                const int kind = classify<J, GEN, REG>(a, c, dst);
+                const int kind = classify<J, GEN, REG, FX>(a, c, dst);
                 double m = 0.0;
                 if (kind == SRC_SOLID) {
                     m = 2.0 * ((double)SP::rt_r(f[K], wK, rb(K), dith) - (double)wK);
@@ -603,7 +605,7 @@ constexpr int step_waves() {
     return sizeof(typename SP::T) == 8 && PAT != PAT_AB ? 1 : sizeof(typename SP::T) == 8 || PAT != PAT_AB || WALE ? 2 : OAL_STEP_WAVES;
 }
 
-template <class SP, int PAT, bool WALE, int SEL = SEL_ALL, bool BFL = false, bool SPONGE = false>
+template <class SP, int PAT, bool WALE, int SEL = SEL_ALL, bool BFL = false, bool SPONGE = false, bool FX = true>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(step_waves<SP, PAT, WALE>()))) k_step(const StepArgs a) {
     const uint32_t litem = xcd_item(blockIdx.x, gridDim.x);
     const uint32_t lb = litem / PLANES;
@@ -642,9 +644,9 @@ __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(step_wa
         const bool solid_own = (c.lmask >> 27) & 1u;
         const bool active = inside && !(solid_own && a.cp.bb_mode == 1);
         if constexpr (SEL == SEL_BOUNDARY_REGULAR)
-            step_cell<SP, PAT, WALE, true, true, BFL, SPONGE>(a, c, dm, active, solid_own, F0, F1, F2);
+            step_cell<SP, PAT, WALE, true, true, BFL, SPONGE, FX>(a, c, dm, active, solid_own, F0, F1, F2);
         else
-            step_cell<SP, PAT, WALE, true, false, BFL, SPONGE>(a, c, dm, active, solid_own, F0, F1, F2);
+            step_cell<SP, PAT, WALE, true, false, BFL, SPONGE, FX>(a, c, dm, active, solid_own, F0, F1, F2);
     }
     if (a.fpart) wg_sum3(F0 * dm->force_scale, F1 * dm->force_scale, F2 * dm->force_scale, a.fpart, item);
 }
@@ -1157,7 +1159,8 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
         if constexpr (PAT == PAT_AB) {
             if (bfl) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, true>), g, dim3(WG), 0, bs, r);
             else if (a.sponge) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, false, true>), g, dim3(WG), 0, bs, r);
-            else hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), g, dim3(WG), 0, bs, r);
+            else if (a.face_x) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), g, dim3(WG), 0, bs, r);
+            else hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, false, false, false>), g, dim3(WG), 0, bs, r);
         } else {
             hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), g, dim3(WG), 0, bs, r);
         }
@@ -1168,7 +1171,8 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
         if constexpr (PAT == PAT_AB) {
             if (bfl) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY, true>), g, dim3(WG), 0, bs, r);
             else if (a.sponge) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY, false, true>), g, dim3(WG), 0, bs, r);
-            else hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY>), g, dim3(WG), 0, bs, r);
+            else if (a.face_x) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY>), g, dim3(WG), 0, bs, r);
+            else hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY, false, false, false>), g, dim3(WG), 0, bs, r);
         } else {
             hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY>), g, dim3(WG), 0, bs, r);
         }

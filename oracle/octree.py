@@ -351,11 +351,17 @@ def apply(storage, f_dst: np.ndarray, X: Xfer, m: np.ndarray, scale, keep):
     n = len(X.dst)
     if n == 0:
         return None
-    acc = np.zeros((n, 27), dtype=T)
-    for q in range(64):
-        sl = X.slot[:, q]
-        ok = sl >= 0
-        acc[ok] = acc[ok] + X.w[ok, q].astype(T)[:, None] * m[sl[ok]]
+    # the library's order: 16 partial sums of 4 nodes each (nodes 4j .. 4j + 3 from zero, in node
+    # order), then added as a pairwise tree (partial j plus partial j xor d for d = 1, 2, 4, 8)
+    part = np.zeros((16, n, 27), dtype=T)
+    for j in range(16):
+        for q in range(4 * j, 4 * j + 4):
+            sl = X.slot[:, q]
+            ok = sl >= 0
+            part[j, ok] = part[j, ok] + X.w[ok, q].astype(T)[:, None] * m[sl[ok]]
+    for d in (1, 2, 4, 8):
+        part = part + part[np.arange(16) ^ d]
+    acc = part[0]
     rho, ux, uy, uz, (qxx, qyy, qzz, qxy, qxz, qyz) = _moments(T, acc)
     sk = T(scale) * T(keep)
     dxx, dyy, dzz = sk * qxx - qxx, sk * qyy - qyy, sk * qzz - qzz
