@@ -1162,7 +1162,8 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
             else if (a.face_x) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), g, dim3(WG), 0, bs, r);
             else hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, false, false, false>), g, dim3(WG), 0, bs, r);
         } else {
-            hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), g, dim3(WG), 0, bs, r);
+            if (a.face_x) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), g, dim3(WG), 0, bs, r);
+            else hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, false, false, false>), g, dim3(WG), 0, bs, r);
         }
     }
     if (a.n_boundary) {
@@ -1174,7 +1175,8 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
             else if (a.face_x) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY>), g, dim3(WG), 0, bs, r);
             else hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY, false, false, false>), g, dim3(WG), 0, bs, r);
         } else {
-            hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY>), g, dim3(WG), 0, bs, r);
+            if (a.face_x) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY>), g, dim3(WG), 0, bs, r);
+            else hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY, false, false, false>), g, dim3(WG), 0, bs, r);
         }
     }
     // face cells: the pipelined kernel applies the face rules itself, the one-plane regular kernel
