@@ -74,15 +74,18 @@ def run_both(oracle, octree, case, storage, steps, rounding="stochastic", force=
     taus = solver.level_taus(cfg["tau"], len(grids), rule) if rule else solver.acoustic_taus(cfg["tau"], len(grids))
     rcode = {"stochastic": oracle.STOCHASTIC, "nearest": oracle.NEAREST}[rounding]
     wale = dict(cfg.get("wale", {}))
+    tfs = cfg.get("tfs", 0.0)   # transverse freestream blend on every level (0: off)
     if wale.get("amr_scales") == "halving":      # the reference's earlier per-level rule, 2^-(L-1-l)
         wale["amr_scales"] = [0.5 ** (len(grids) - 1 - l) for l in range(len(grids))]
     run = octree.OctreeRun(grids, storage, taus, u_inlet=cfg["u_inlet"], rounding=rcode,
                            amr_scales=wale.get("amr_scales"), c_wale=wale.get("c_wale", 0.0),
-                           nu_sgs_background=wale.get("nu_sgs_background", 0.0))
+                           nu_sgs_background=wale.get("nu_sgs_background", 0.0),
+                           transverse_freestream=1 if tfs else 0, transverse_freestream_blend=tfs or 1.0)
     run.init([f[0] for f in fields], [f[1] for f in fields])
     run.step(steps, force=force)
     params = solver.StepParams(c_wale=wale.get("c_wale", 0.0), nu_sgs_background=wale.get("nu_sgs_background", 0.0),
-                               inlet_turbulence=0.0)
+                               inlet_turbulence=0.0, transverse_freestream=1 if tfs else 0,
+                               transverse_freestream_blend=tfs or 1.0)
     plan = solver.octree_plan(grids, cfg["tau"], storage=storage, params=params, u_inlet=cfg["u_inlet"],
                               rounding=rounding, taus=taus, amr_sgs_scales=wale.get("amr_scales"))
     for l, (rho, u) in enumerate(fields):
@@ -154,6 +157,17 @@ def test_octree_wale_four_levels_sphere(oracle, octree):
     with plan:
         for l in range(4):
             compare_level(oracle, plan, run, "f16s", l)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16s"])
+def test_octree_transverse_freestream(oracle, octree, storage):
+    """The transverse freestream blend (kernel_cumulant.jl:323-339) on every level of the 2-level sphere
+    channel, whose fine level reaches the y and z faces: every level bit for bit after 3 coarse steps."""
+    grids, run, plan = run_both(oracle, octree, "sphere_channel", storage, steps=3,
+                                cfg=dict(CASES["sphere_channel"], tfs=0.6))
+    with plan:
+        for l in range(len(grids)):
+            compare_level(oracle, plan, run, storage, l)
 
 
 def test_octree_parity_nearest_rounding(oracle, octree):
