@@ -726,8 +726,9 @@ def main(argv=None) -> int:
         bw = copy_bandwidth_gbs()
         out["roofline"]["measured_copy_gbs"] = round(bw, 1)
         out["roofline"]["frac_of_measured_copy"] = round(achieved / bw, 4)
-    if not args.no_aa_leg and args.stream == "ab":
-        # the in-place AA pattern on the same workload: one population buffer instead of two
+    if not args.no_aa_leg and args.stream == "ab" and qmap is None:
+        # the in-place AA pattern on the same workload: one population buffer instead of two (halfway
+        # bounce-back only: interpolated bounce-back needs the AB pattern)
         aa_steps, aa_warm = args.steps + args.steps % 2, args.warmup + args.warmup % 2
         ms_ta, ms_ka, _, Fa, tuna, infoa = gpu_leg(grid, "f32", "aa", aa_steps, aa_warm, not args.no_autotune, fixed, qmap)
         ma = n_cells * aa_steps / (ms_ta * 1e-3) / 1e6

@@ -1157,8 +1157,10 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
         r.blist = a.blist_boundary_regular;
         const dim3 g(a.n_boundary_regular * PLANES);
         if constexpr (PAT == PAT_AB) {
-            if (bfl) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, true>), g, dim3(WG), 0, bs, r);
-            else if (a.sponge) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, false, true>), g, dim3(WG), 0, bs, r);
+            if (bfl && a.face_x) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, true>), g, dim3(WG), 0, bs, r);
+            else if (bfl) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, true, false, false>), g, dim3(WG), 0, bs, r);
+            else if (a.sponge && a.face_x) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, false, true>), g, dim3(WG), 0, bs, r);
+            else if (a.sponge) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, false, true, false>), g, dim3(WG), 0, bs, r);
             else if (a.face_x) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), g, dim3(WG), 0, bs, r);
             else hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, false, false, false>), g, dim3(WG), 0, bs, r);
         } else {
@@ -1170,8 +1172,10 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
         r.blist = a.blist_boundary;
         const dim3 g(a.n_boundary * PLANES);
         if constexpr (PAT == PAT_AB) {
-            if (bfl) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY, true>), g, dim3(WG), 0, bs, r);
-            else if (a.sponge) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY, false, true>), g, dim3(WG), 0, bs, r);
+            if (bfl && a.face_x) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY, true>), g, dim3(WG), 0, bs, r);
+            else if (bfl) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY, true, false, false>), g, dim3(WG), 0, bs, r);
+            else if (a.sponge && a.face_x) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY, false, true>), g, dim3(WG), 0, bs, r);
+            else if (a.sponge) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY, false, true, false>), g, dim3(WG), 0, bs, r);
             else if (a.face_x) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY>), g, dim3(WG), 0, bs, r);
             else hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY, false, false, false>), g, dim3(WG), 0, bs, r);
         } else {
