@@ -202,6 +202,8 @@ typedef struct OalLevelInfo {
                                 pipelined kernel with the face rules at the extent's last cells, on the
                                 second stream; in a domain whose block layers span 2 GiB or more they
                                 are boundary blocks) */
+    int32_t n_sponge;        /* AB pattern: regular blocks with sponge cells and no solid near (the
+                                pipelined kernel's sponge form, on the second stream) */
 } OalLevelInfo;
 int oal_level_info(const OalPlan* plan, int level, OalLevelInfo* info);
 

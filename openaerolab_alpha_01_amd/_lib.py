@@ -73,7 +73,7 @@ class OalLevelInfo(C.Structure):
         ("n_cells", C.c_int64), ("n_fluid_cells", C.c_int64), ("n_blocks", C.c_int32), ("n_regular", C.c_int32),
         ("n_interior", C.c_int32), ("n_boundary_regular", C.c_int32), ("n_boundary", C.c_int32),
         ("n_face_cells", C.c_int32), ("n_ghost_blocks", C.c_int32), ("n_covered_blocks", C.c_int32),
-        ("n_partial", C.c_int32),
+        ("n_partial", C.c_int32), ("n_sponge", C.c_int32),
     ]
 
 

@@ -62,6 +62,8 @@ struct StepArgs {
     uint32_t n_regular, n_interior, n_boundary_regular, n_boundary;
     const int32_t* blist_partial;    // [n_partial] AB pattern: regular blocks that overhang the extent, no solid near
     uint32_t n_partial;
+    const int32_t* blist_sponge;     // [n_sponge] AB pattern: regular blocks with sponge cells, no solid near
+    uint32_t n_sponge;
     // the regular list split (AB pattern): blocks on no domain face, and whole blocks on a domain face
     const int32_t* blist_regular_plain;
     uint32_t n_regular_plain;

@@ -98,6 +98,8 @@ struct Level {
     DevBuf<int32_t> blist_rg, blist_in, blist_br, blist_bd;  // block lists per kernel kind
     uint32_t n_regular = 0, n_interior = 0, n_boundary_regular = 0, n_boundary = 0;
     DevBuf<int32_t> blist_pr;          // regular blocks that overhang the extent (pipelined, AB)
+    DevBuf<int32_t> blist_sp;          // regular blocks with sponge cells, no solid near (pipelined, AB)
+    uint32_t n_sponge = 0;
     uint32_t n_partial = 0;
     // the regular blocks split in two (AB), for the kernels that step the ones on a domain face apart
     DevBuf<int32_t> blist_rp, blist_rf;   // plain (no face) and on a domain face, each in lexicographic order
@@ -179,7 +181,7 @@ struct Level {
     void release() {
         nbr.release(); borg.release(); bdom.release(); dom_item0.release(); dom_block0.release();
         bflags.release(); solid.release(); doms.release(); fA.release(); fB.release(); velA.release(); velB.release();
-        fpart.release(); mpart.release(); red_out.release(); guard.release(); oslot.release(); obuf.release(); blist_rg.release(); blist_in.release(); blist_br.release(); blist_bd.release(); blist_pr.release(); blist_rp.release(); blist_rf.release(); face_cells.release();
+        fpart.release(); mpart.release(); red_out.release(); guard.release(); oslot.release(); obuf.release(); blist_rg.release(); blist_in.release(); blist_br.release(); blist_bd.release(); blist_pr.release(); blist_sp.release(); blist_rp.release(); blist_rf.release(); face_cells.release();
         lslot.release(); lmask.release(); qmap.release(); sponge.release(); sslot.release();
         g_src.release(); g_dst.release(); g_slot.release(); g_w.release(); g_m.release();
         r_src.release(); r_dst.release(); r_slot.release(); r_w.release(); r_m.release();
@@ -239,7 +241,8 @@ int upload(DevBuf<T>& d, const std::vector<T>& h) {
 
 // Host tables of one level, built and validated before any device call
 struct HostLevel {
-    std::vector<int32_t> borg, bdom, nbr, nbr_bm, oslot, blist_rg, blist_in, blist_br, blist_bd, blist_pr, blist_rp, blist_rf, lslot;
+    std::vector<int32_t> borg, bdom, nbr, nbr_bm, oslot, blist_rg, blist_in, blist_br, blist_bd, blist_pr, blist_sp, blist_rp, blist_rf,
+        lslot;
     std::vector<uint32_t> lmask;
     std::vector<uint32_t> face_cells;
     int32_t n_oslots = 0;
@@ -251,7 +254,7 @@ struct HostLevel {
 // for that purpose; rebuilt when the flags change (a sponge field marks its blocks).
 void build_lists(Level& L, HostLevel& H) {
     H.blist_rg.clear(); H.blist_rp.clear(); H.blist_rf.clear(); H.blist_in.clear();
-    H.blist_br.clear(); H.blist_bd.clear(); H.blist_pr.clear(); H.face_cells.clear();
+    H.blist_br.clear(); H.blist_bd.clear(); H.blist_pr.clear(); H.blist_sp.clear(); H.face_cells.clear();
     L.regular_face = false;
     const std::vector<uint32_t>& flags = L.flags_h;
     // blocks on a face whose rule the pipelined kernels do not apply (the exact specular z wall reads
@@ -299,6 +302,11 @@ void build_lists(Level& L, HostLevel& H) {
             // face rules at the extent's last cells (in a domain whose block layers span 2 GiB or more the
             // one-plane boundary kernel, whose addresses are 64-bit)
             H.blist_pr.push_back((int32_t)b);
+        } else if (L.ab && L.pipe_ok && regular && (flags[b] & BF_SPONGE) && !(flags[b] & ~(BF_SPONGE | BF_EDGE | BF_PARTIAL)) &&
+                   L.near_h[L.bdom_h[b]] && !special) {
+            // AB pattern: a block with sponge cells and no solid near (whole or overhanging the extent):
+            // the pipelined kernel's sponge form, with the face rules at the extent's last cells
+            H.blist_sp.push_back((int32_t)b);
         } else {
             (regular ? H.blist_br : H.blist_bd).push_back((int32_t)b);
         }
@@ -309,6 +317,7 @@ void build_lists(Level& L, HostLevel& H) {
     L.n_interior = (uint32_t)H.blist_in.size();
     L.n_boundary_regular = (uint32_t)H.blist_br.size();
     L.n_partial = (uint32_t)H.blist_pr.size();
+    L.n_sponge = (uint32_t)H.blist_sp.size();
     L.n_boundary = (uint32_t)H.blist_bd.size();
 }
 
@@ -956,7 +965,7 @@ int upload_level(OalPlan* p, Level& L, const HostLevel& H) {
         return rc;
     if (!H.solid.empty() && (rc = upload(L.solid, H.solid))) return rc;
     if ((rc = upload(L.blist_rg, H.blist_rg)) || (rc = upload(L.blist_in, H.blist_in)) ||
-        (rc = upload(L.blist_br, H.blist_br)) || (rc = upload(L.blist_bd, H.blist_bd)) || (rc = upload(L.blist_pr, H.blist_pr)) || (rc = upload(L.blist_rp, H.blist_rp)) || (rc = upload(L.blist_rf, H.blist_rf)) ||
+        (rc = upload(L.blist_br, H.blist_br)) || (rc = upload(L.blist_bd, H.blist_bd)) || (rc = upload(L.blist_pr, H.blist_pr)) || (rc = upload(L.blist_sp, H.blist_sp)) || (rc = upload(L.blist_rp, H.blist_rp)) || (rc = upload(L.blist_rf, H.blist_rf)) ||
         (rc = upload(L.lslot, H.lslot)) || (rc = upload(L.lmask, H.lmask)) || (rc = upload(L.face_cells, H.face_cells)))
         return rc;
     if (p->stream_mode == OAL_STREAM_AA) {
@@ -1009,6 +1018,8 @@ StepArgs make_args(OalPlan* p, Level& L, int32_t time_step, bool want_force) {
     a.blist_regular_face = L.blist_rf.p;
     a.n_regular_face = L.n_regular_face;
     a.blist_partial = L.blist_pr.p;
+    a.blist_sponge = L.blist_sp.p;
+    a.n_sponge = L.n_sponge;
     a.n_partial = L.n_partial;
     a.n_interior = L.n_interior;
     a.n_boundary_regular = L.n_boundary_regular;
@@ -1066,7 +1077,7 @@ int level_step(OalPlan* p, int level, int32_t time_step, int flags, hipStream_t 
     StepArgs a = make_args(p, L, time_step, want_force);
     // side-stream work: boundary blocks, face cells and the blocks that overhang the extent
     const bool split = p->overlap && L.n_regular + L.n_interior > 0 &&
-                       L.n_boundary_regular + L.n_boundary + L.n_face + L.n_partial > 0;
+                       L.n_boundary_regular + L.n_boundary + L.n_face + L.n_partial + L.n_sponge > 0;
     if (split) {
         // fork: the side stream starts after everything queued on s, then s waits for the side stream
         OAL_HIP(hipEventRecord(p->fork, s));
@@ -1710,6 +1721,7 @@ int oal_level_info(const OalPlan* p, int level, OalLevelInfo* info) {
     info->n_covered_blocks = (int32_t)L.n_covered_blocks;
     info->n_regular = (int32_t)L.n_regular;
     info->n_partial = (int32_t)L.n_partial;
+    info->n_sponge = (int32_t)L.n_sponge;
     info->n_interior = (int32_t)L.n_interior;
     info->n_boundary_regular = (int32_t)L.n_boundary_regular;
     info->n_boundary = (int32_t)L.n_boundary;
@@ -2061,7 +2073,7 @@ static int set_sponge_lib(OalPlan* p, int level, const float* sponge, int32_t bl
     auto re = [](auto& d, const auto& h) { d.release(); return upload(d, h); };
     int rc;
     if ((rc = re(L.bflags, L.flags_h)) || (rc = re(L.blist_rg, H.blist_rg)) || (rc = re(L.blist_in, H.blist_in)) ||
-        (rc = re(L.blist_br, H.blist_br)) || (rc = re(L.blist_bd, H.blist_bd)) || (rc = re(L.blist_pr, H.blist_pr)) ||
+        (rc = re(L.blist_br, H.blist_br)) || (rc = re(L.blist_bd, H.blist_bd)) || (rc = re(L.blist_pr, H.blist_pr)) || (rc = re(L.blist_sp, H.blist_sp)) ||
         (rc = re(L.blist_rp, H.blist_rp)) || (rc = re(L.blist_rf, H.blist_rf)) || (rc = re(L.face_cells, H.face_cells)) ||
         (rc = re(L.sponge, vals)) || (rc = re(L.sslot, slot)))
         return rc;

@@ -694,15 +694,17 @@ template <class T>
 struct PipeWaleRates {
     T tau, amr_scale;           // of the domain, read before the first store
 };
-template <class SP, bool UNIT, bool WALE = false>
+template <class SP, bool UNIT, bool WALE = false, bool SPONGE = false>
 __device__ __forceinline__ void reg_collide_store(const StepArgs& a, const RegAddr<SP>& ra, const CellRates<typename SP::T>& q0,
                                                   int z, uint32_t cell, const typename SP::raw (&r)[27],
                                                   const InletRule* inlet = nullptr, bool store = true,
                                                   const PipeWale<typename SP::T>* wv = nullptr,
-                                                  const PipeWaleRates<typename SP::T>* wr = nullptr, uint32_t vidx = 0) {
+                                                  const PipeWaleRates<typename SP::T>* wr = nullptr, uint32_t vidx = 0,
+                                                  typename SP::T sponge = typename SP::T(0)) {
     using T = typename SP::T;
     using st = typename SP::st;
     CellRates<T> q = q0;
+    if constexpr (SPONGE) q.sponge = sponge;
     if constexpr (WALE) {
         const T h = T(0.5);
         const auto& u = wv->u;
@@ -728,7 +730,7 @@ __device__ __forceinline__ void reg_collide_store(const StepArgs& a, const RegAd
         });
     }
     T rho_o, ux_o, uy_o, uz_o;
-    collide_cell<T, UNIT>(f, q, a.cp, rho_o, ux_o, uy_o, uz_o);
+    collide_cell<T, UNIT, SPONGE>(f, q, a.cp, rho_o, ux_o, uy_o, uz_o);
     const bool dith = SP::dithered && a.cp.dither;
     const uint32_t hc = SP::dithered ? round_cell(round_key(a.step()), cell) : 0u;
     if (store) {
@@ -830,7 +832,9 @@ constexpr int pipe_waves() {
 // UNIT: the collision form for the unit rates omega_bulk = omega_3 = omega_4 = 1 (a separate kernel, so
 // that each form keeps its own register allocation).  WALE: the eddy viscosity from the previous step's
 // velocity, and this step's velocity written.
-template <class SP, int NP, int FACE, bool UNIT, bool WALE = false>
+// SPONGE: blocks with sponge cells (kernel_cumulant.jl:428-446), each cell's strength loaded with its
+// populations
+template <class SP, int NP, int FACE, bool UNIT, bool WALE = false, bool SPONGE = false>
 __global__ void __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(pipe_waves<SP, FACE, WALE>(), pipe_waves<SP, FACE, WALE>())))
 k_step_pipe(const StepArgs a) {
     using T = typename SP::T;
@@ -869,6 +873,18 @@ k_step_pipe(const StepArgs a) {
     else { q.omega1 = dm->omega1; q.omega4_diag = dm->omega4d; }
     q.forced = dm->forced != 0;
     q.Fx = T(dm->F[0]); q.Fy = T(dm->F[1]); q.Fz = T(dm->F[2]);
+    const float* sp_cells = nullptr;
+    if constexpr (SPONGE) {
+        q.sponge = T(0);
+        q.sponge_u = T(dm->u_in);
+        q.sponge_blend = a.sponge_blend;
+        if (a.bflags[b] & BF_SPONGE) sp_cells = a.sponge + (size_t)a.sslot[b] * BS3 + (c.own - b * BS3 - (uint32_t)z0 * BS2);
+    }
+    // the strength of the lane's cell in plane z (0 outside the sponge)
+    auto sp_load = [&](int z) -> T {
+        if constexpr (SPONGE) return sp_cells ? T(sp_cells[(uint32_t)z * BS2]) : T(0);
+        else return T(0);
+    };
     // blocks on a domain face: FACE_RULES applies the face rules here, FACE_SKIP leaves the face cells
     // to the face-cell kernel
     FaceRules fr{};
@@ -959,21 +975,21 @@ k_step_pipe(const StepArgs a) {
         else if (live(z)) reg_load_raw<SP>(a, r, z, dst);
     };
     const uint32_t cell0 = dm->cell0;   // read before the first store, as every domain parameter
-    auto collide = [&](const RegAddr<SP>& r, int z, const typename SP::raw (&src)[27], const PipeWale<T>& w) {
+    auto collide = [&](const RegAddr<SP>& r, int z, const typename SP::raw (&src)[27], const PipeWale<T>& w, T sp) {
         const uint32_t vidx = c.own + (uint32_t)(z - z0) * BS2;
         const uint32_t cell = vidx - cell0;
         if constexpr (FACE == FACE_RULES) {
             if (z >= zmax) return;
             InletRule iz = in;
             iz.gz += z;
-            reg_collide_store<SP, UNIT, WALE>(a, r, q, z, cell, src, &iz, true, &w, &wr, vidx);
+            reg_collide_store<SP, UNIT, WALE, SPONGE>(a, r, q, z, cell, src, &iz, true, &w, &wr, vidx, sp);
         } else if constexpr (FACE == FACE_PARTIAL) {
             if (z >= zmax) return;
             InletRule iz = in;
             iz.gz += z;
-            reg_collide_store<SP, UNIT, WALE>(a, r, q, z, cell, src, &iz, !fr.dead, &w, &wr, vidx);
+            reg_collide_store<SP, UNIT, WALE, SPONGE>(a, r, q, z, cell, src, &iz, !fr.dead, &w, &wr, vidx, sp);
         } else if (live(z)) {
-            reg_collide_store<SP, UNIT, WALE>(a, r, q, z, cell, src, nullptr, true, &w, &wr, vidx);
+            reg_collide_store<SP, UNIT, WALE, SPONGE>(a, r, q, z, cell, src, nullptr, true, &w, &wr, vidx, sp);
         }
     };
     // WALE: the plane's neighbour velocities, issued before the next plane's populations
@@ -985,18 +1001,26 @@ k_step_pipe(const StepArgs a) {
     };
     // the per-lane offsets stay those of plane z0; later planes shift the uniform base
     typename SP::raw fa[27], fb[27];
+    T spa = T(0), spb = T(0);
     PipeWale<T> wv;
     load(ra, z0, fa);
+    if constexpr (SPONGE) spa = sp_load(z0);
     for (int z = z0; z < z0 + np; z += 2) {
         wait_all_vmem();
         wale_pre(z, wv);
-        if (z + 1 < z0 + np) load(ra, z + 1, fb);
-        collide(ra, z, fa, wv);
+        if (z + 1 < z0 + np) {
+            load(ra, z + 1, fb);
+            if constexpr (SPONGE) spb = sp_load(z + 1);
+        }
+        collide(ra, z, fa, wv, spa);
         if (z + 1 >= z0 + np) break;
         wait_all_vmem();
         wale_pre(z + 1, wv);
-        if (z + 2 < z0 + np) load(ra, z + 2, fa);
-        collide(ra, z + 1, fb, wv);
+        if (z + 2 < z0 + np) {
+            load(ra, z + 2, fa);
+            if constexpr (SPONGE) spa = sp_load(z + 2);
+        }
+        collide(ra, z + 1, fb, wv, spb);
     }
 }
 
@@ -1135,6 +1159,12 @@ static void launch_pipe_u(int face, dim3 grid, hipStream_t s, const StepArgs& r)
     }
 }
 // (the WALE forms come in 2, 4 and 8 planes; the launch caps the planes at 8)
+template <class SP, int NP>
+static void launch_pipe_sponge(dim3 grid, hipStream_t s, const StepArgs& r) {
+    if (r.cp.unit) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_PARTIAL, true, false, true>), grid, dim3(WG), 0, s, r);
+    else hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_PARTIAL, false, false, true>), grid, dim3(WG), 0, s, r);
+}
+
 template <class SP, int NP, bool WALE = false>
 static void launch_pipe(int face, dim3 grid, hipStream_t s, const StepArgs& r) {
     if constexpr (!WALE || NP <= 8) {
@@ -1266,6 +1296,31 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
         } else {
             hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR>), dim3(a.n_partial * PLANES), dim3(WG), 0,
                                bs, r);
+        }
+    }
+    if (a.n_sponge) {
+        // regular blocks with sponge cells and no solid near (AB pattern): the pipelined kernel's sponge
+        // form (without WALE; at most 8 planes per workgroup), else the one-plane kernel's; on the side
+        // stream with the boundary launches
+        StepArgs rs = r;
+        rs.blist = a.blist_sponge;
+        bool piped = false;
+        if constexpr (pipe_capable && !WALE) {
+            if (a.pipe_planes > 1) {
+                rs.pipe_planes = a.pipe_planes > 8 ? 8 : a.pipe_planes;
+                const dim3 g(a.n_sponge * (PLANES / rs.pipe_planes));
+                switch (rs.pipe_planes) {
+                    case 2: launch_pipe_sponge<SP, 2>(g, bs, rs); break;
+                    case 4: launch_pipe_sponge<SP, 4>(g, bs, rs); break;
+                    default: launch_pipe_sponge<SP, 8>(g, bs, rs); break;
+                }
+                piped = true;
+            }
+        }
+        if (!piped) {
+            const dim3 g(a.n_sponge * PLANES);
+            if (a.face_x) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, false, true>), g, dim3(WG), 0, bs, rs);
+            else hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, false, true, false>), g, dim3(WG), 0, bs, rs);
         }
     }
     if (a.n_interior) {

@@ -77,3 +77,25 @@ def test_the_comparison_catches_a_wrong_type(bad):
     field = bad.split("::")[0]
     jl[name] = [(n, bad.split("::")[1] if n == field else t) for n, t in jl[name]]
     assert jl[name] != c[name]
+
+
+def test_julia_constants_match_the_header():
+    """the ABI version and the enum values the Julia shim passes are the header's"""
+    h = open(os.path.join(ROOT, "include", "oal.h")).read()
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    code = "\n".join(re.findall(r"```julia\n(.*?)```", text, flags=re.S))
+    m = re.search(r"const OAL_ABI_VERSION = Int32\((\d+)\)", code)
+    assert m, "the Julia shim defines OAL_ABI_VERSION"
+    assert int(m.group(1)) == int(re.search(r"#define OAL_ABI_VERSION (\d+)", h).group(1))
+    # every plan descriptor the shim builds passes that constant
+    assert re.findall(r"OalPlanDesc\(([^,]+),", code) and all(
+        a.strip() == "OAL_ABI_VERSION" for a in re.findall(r"OalPlanDesc\(([^,]+),", code))
+    for name in ("OAL_F32", "OAL_F64", "OAL_F16S", "OAL_BF16S", "OAL_STREAM_AB", "OAL_BB_HALFWAY", "OAL_STEP_FORCE",
+                 "OAL_ROUND_STOCHASTIC", "OAL_ROUND_NEAREST"):
+        cv = re.search(rf"\b{name}\s*=\s*(\d+)", h)
+        assert cv, name
+        names = re.search(rf"const ([\w, ]*\b{name}\b[\w, ]*) = ([^\n#]+)", code)
+        assert names, name
+        lhs = [x.strip() for x in names.group(1).split(",")]
+        rhs = re.findall(r"\((\d+)\)", names.group(2))
+        assert int(rhs[lhs.index(name)]) == int(cv.group(1)), name
