@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define OAL_ABI_VERSION 2
+#define OAL_ABI_VERSION 3
 
 enum {
     OAL_OK = 0,
@@ -204,6 +204,10 @@ typedef struct OalLevelInfo {
                                 are boundary blocks) */
     int32_t n_sponge;        /* AB pattern: regular blocks with sponge cells and no solid near (the
                                 pipelined kernel's sponge form, on the second stream) */
+    int32_t n_solid_regular; /* of n_boundary_regular: blocks whose only flag is a solid cell near, away
+                                from the domain faces (FP32 arithmetic, halfway bounce-back: the
+                                pipelined kernel's solid-link form on steps that do not collect the
+                                force) */
 } OalLevelInfo;
 int oal_level_info(const OalPlan* plan, int level, OalLevelInfo* info);
 

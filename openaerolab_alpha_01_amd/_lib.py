@@ -11,7 +11,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "liboal.so"
 
-OAL_ABI_VERSION = 2
+OAL_ABI_VERSION = 3
 OAL_OK, OAL_ERR_ARG, OAL_ERR_HIP, OAL_ERR_NOMEM, OAL_ERR_UNSUPPORTED, OAL_ERR_STATE, OAL_ERR_NONFINITE = 0, -1, -2, -3, -4, -5, -6
 OAL_F32, OAL_F64, OAL_F16S, OAL_BF16S = 0, 1, 2, 3
 OAL_STREAM_AB, OAL_STREAM_AA = 0, 1
@@ -73,7 +73,7 @@ class OalLevelInfo(C.Structure):
         ("n_cells", C.c_int64), ("n_fluid_cells", C.c_int64), ("n_blocks", C.c_int32), ("n_regular", C.c_int32),
         ("n_interior", C.c_int32), ("n_boundary_regular", C.c_int32), ("n_boundary", C.c_int32),
         ("n_face_cells", C.c_int32), ("n_ghost_blocks", C.c_int32), ("n_covered_blocks", C.c_int32),
-        ("n_partial", C.c_int32), ("n_sponge", C.c_int32),
+        ("n_partial", C.c_int32), ("n_sponge", C.c_int32), ("n_solid_regular", C.c_int32),
     ]
 
 

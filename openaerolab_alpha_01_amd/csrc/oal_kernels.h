@@ -60,6 +60,7 @@ struct StepArgs {
     const int32_t* blist_boundary_regular;  // [n_boundary_regular] boundary blocks with lexicographic neighbours
     const int32_t* blist_boundary;   // [n_boundary] the other blocks
     uint32_t n_regular, n_interior, n_boundary_regular, n_boundary;
+    uint32_t n_boundary_regular_solid;   // blocks at the head of blist_boundary_regular whose only flag is BF_SOLID
     const int32_t* blist_partial;    // [n_partial] AB pattern: regular blocks that overhang the extent, no solid near
     uint32_t n_partial;
     const int32_t* blist_sponge;     // [n_sponge] AB pattern: regular blocks with sponge cells, no solid near

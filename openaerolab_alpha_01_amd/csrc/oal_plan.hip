@@ -97,6 +97,7 @@ struct Level {
     DevBuf<int32_t> oslot;             // AA: outlet side-buffer slot per block
     DevBuf<int32_t> blist_rg, blist_in, blist_br, blist_bd;  // block lists per kernel kind
     uint32_t n_regular = 0, n_interior = 0, n_boundary_regular = 0, n_boundary = 0;
+    uint32_t n_br_solid = 0;   // blocks at the head of the regular boundary list whose only flag is BF_SOLID
     DevBuf<int32_t> blist_pr;          // regular blocks that overhang the extent (pipelined, AB)
     DevBuf<int32_t> blist_sp;          // regular blocks with sponge cells, no solid near (pipelined, AB)
     uint32_t n_sponge = 0;
@@ -310,6 +311,13 @@ void build_lists(Level& L, HostLevel& H) {
         } else {
             (regular ? H.blist_br : H.blist_bd).push_back((int32_t)b);
         }
+    }
+    // the regular boundary blocks whose only flag is a solid cell near first (the pipelined kernel's
+    // solid-link form takes them where it applies)
+    {
+        const auto mid = std::stable_partition(H.blist_br.begin(), H.blist_br.end(),
+                                               [&](int32_t b) { return flags[b] == BF_SOLID; });
+        L.n_br_solid = (uint32_t)(mid - H.blist_br.begin());
     }
     L.n_face = (uint32_t)H.face_cells.size();
     L.n_regular_face = (uint32_t)H.blist_rf.size();
@@ -1024,6 +1032,7 @@ StepArgs make_args(OalPlan* p, Level& L, int32_t time_step, bool want_force) {
     a.n_interior = L.n_interior;
     a.n_boundary_regular = L.n_boundary_regular;
     a.n_boundary = L.n_boundary;
+    a.n_boundary_regular_solid = L.pipe_ok ? L.n_br_solid : 0;
     a.lslot = L.lslot.p;
     a.lmask = L.lmask.p;
     a.face_cells = L.face_cells.p;
@@ -1722,6 +1731,7 @@ int oal_level_info(const OalPlan* p, int level, OalLevelInfo* info) {
     info->n_regular = (int32_t)L.n_regular;
     info->n_partial = (int32_t)L.n_partial;
     info->n_sponge = (int32_t)L.n_sponge;
+    info->n_solid_regular = L.pipe_ok ? (int32_t)L.n_br_solid : 0;
     info->n_interior = (int32_t)L.n_interior;
     info->n_boundary_regular = (int32_t)L.n_boundary_regular;
     info->n_boundary = (int32_t)L.n_boundary;

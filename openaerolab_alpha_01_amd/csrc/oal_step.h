@@ -814,7 +814,10 @@ __device__ __forceinline__ void reg_load_raw_face(const StepArgs& a, const RegAd
 
 // FACE: how blocks on a domain face are handled.  FACE_PARTIAL: blocks that overhang the extent, with
 // the face rules at the extent's last cells and the cells beyond it neither read through nor stored.
-enum : int { FACE_NONE = 0, FACE_RULES = 1, FACE_SKIP = 2, FACE_PARTIAL = 3 };
+// FACE_SOLID: blocks of the regular grid whose only flag is a solid cell in or next to them, away from
+// the domain faces: halfway bounce-back (the source of a solid link is the cell's own slot opp K of the
+// previous step) chosen per lane by address, the link masks loaded two planes ahead
+enum : int { FACE_NONE = 0, FACE_RULES = 1, FACE_SKIP = 2, FACE_PARTIAL = 3, FACE_SOLID = 4 };
 
 // Register budget: with one plane of populations in flight beside the collision of another the kernel
 // fits the 168 VGPRs of 3 waves per SIMD; the in-kernel face rules add per-lane address selects and
@@ -826,7 +829,7 @@ enum : int { FACE_NONE = 0, FACE_RULES = 1, FACE_SKIP = 2, FACE_PARTIAL = 3 };
 // (WALE's velocity gather as well)
 template <class SP, int FACE, bool WALE = false>
 constexpr int pipe_waves() {
-    return FACE == FACE_RULES || FACE == FACE_PARTIAL || SP::dithered || WALE ? OAL_FACE_RULES_WAVES : 3;
+    return FACE == FACE_RULES || FACE == FACE_PARTIAL || FACE == FACE_SOLID || SP::dithered || WALE ? OAL_FACE_RULES_WAVES : 3;
 }
 
 // UNIT: the collision form for the unit rates omega_bulk = omega_3 = omega_4 = 1 (a separate kernel, so
@@ -969,15 +972,39 @@ k_step_pipe(const StepArgs a) {
             w.u[q6][0] = v[i]; w.u[q6][1] = v[a.ncell + i]; w.u[q6][2] = v[2 * (size_t)a.ncell + i];
         }
     };
+    // FACE_SOLID: the solid-link masks of the lane's cell (bit 27: the cell itself is solid)
+    const uint32_t* lm_cells = nullptr;
+    if constexpr (FACE == FACE_SOLID) lm_cells = a.lmask + (size_t)a.lslot[b] * BS3 + (c.own - b * BS3 - (uint32_t)z0 * BS2);
+    auto lm_load = [&](int z) -> uint32_t {
+        if constexpr (FACE == FACE_SOLID) return z < z0 + np ? lm_cells[(uint32_t)z * BS2] : 0u;
+        else return 0u;
+    };
+    uint32_t lm_cur = 0;   // FACE_SOLID: the mask of the plane the next load() reads
     auto load = [&](const RegAddr<SP>& r, int z, typename SP::raw (&dst)[27]) {
         if constexpr (FACE == FACE_RULES) { if (z < zmax) reg_load_raw_face<SP>(a, r, fr, z, dst); }
         else if constexpr (FACE == FACE_PARTIAL) { if (z < zmax) reg_load_raw_face<SP, true>(a, r, fr, z, dst); }
+        else if constexpr (FACE == FACE_SOLID) {
+            using st = typename SP::st;
+            const uint32_t m = lm_cur;
+            for_dirs([&](auto kc) {
+                constexpr int K = decltype(kc)::value;
+                const char* pr = r.base(a.fin, K, z, dcz(K)) + r.v[dcx(K) + 1][dcy(K) + 1];
+                const char* ps = r.base(a.fin, dopp(K), z, 0) + r.v[1][1];
+                dst[K] = SP::ld_raw((const st*)(((m >> K) & 1u) ? ps : pr));
+            });
+        }
         else if (live(z)) reg_load_raw<SP>(a, r, z, dst);
     };
     const uint32_t cell0 = dm->cell0;   // read before the first store, as every domain parameter
-    auto collide = [&](const RegAddr<SP>& r, int z, const typename SP::raw (&src)[27], const PipeWale<T>& w, T sp) {
+    auto collide = [&](const RegAddr<SP>& r, int z, const typename SP::raw (&src)[27], const PipeWale<T>& w, T sp,
+                       uint32_t lm) {
         const uint32_t vidx = c.own + (uint32_t)(z - z0) * BS2;
         const uint32_t cell = vidx - cell0;
+        if constexpr (FACE == FACE_SOLID) {
+            // a solid cell is not stepped (halfway bounce-back)
+            reg_collide_store<SP, UNIT, WALE, SPONGE>(a, r, q, z, cell, src, nullptr, !((lm >> 27) & 1u), &w, &wr, vidx, sp);
+            return;
+        }
         if constexpr (FACE == FACE_RULES) {
             if (z >= zmax) return;
             InletRule iz = in;
@@ -1002,25 +1029,36 @@ k_step_pipe(const StepArgs a) {
     // the per-lane offsets stay those of plane z0; later planes shift the uniform base
     typename SP::raw fa[27], fb[27];
     T spa = T(0), spb = T(0);
+    uint32_t lma = 0, lmb = 0, lmn = 0;   // FACE_SOLID: the masks of the planes in fa / fb, and the next one's
     PipeWale<T> wv;
+    if constexpr (FACE == FACE_SOLID) {
+        lma = lm_load(z0);
+        wait_all_vmem();
+        lm_cur = lma;
+    }
     load(ra, z0, fa);
     if constexpr (SPONGE) spa = sp_load(z0);
+    if constexpr (FACE == FACE_SOLID) lmn = lm_load(z0 + 1);
     for (int z = z0; z < z0 + np; z += 2) {
         wait_all_vmem();
         wale_pre(z, wv);
         if (z + 1 < z0 + np) {
+            if constexpr (FACE == FACE_SOLID) { lmb = lmn; lm_cur = lmb; }
             load(ra, z + 1, fb);
             if constexpr (SPONGE) spb = sp_load(z + 1);
+            if constexpr (FACE == FACE_SOLID) lmn = lm_load(z + 2);
         }
-        collide(ra, z, fa, wv, spa);
+        collide(ra, z, fa, wv, spa, lma);
         if (z + 1 >= z0 + np) break;
         wait_all_vmem();
         wale_pre(z + 1, wv);
         if (z + 2 < z0 + np) {
+            if constexpr (FACE == FACE_SOLID) { lma = lmn; lm_cur = lma; }
             load(ra, z + 2, fa);
             if constexpr (SPONGE) spa = sp_load(z + 2);
+            if constexpr (FACE == FACE_SOLID) lmn = lm_load(z + 3);
         }
-        collide(ra, z + 1, fb, wv, spb);
+        collide(ra, z + 1, fb, wv, spb, lmb);
     }
 }
 
@@ -1155,6 +1193,9 @@ static void launch_pipe_u(int face, dim3 grid, hipStream_t s, const StepArgs& r)
         else if (face == FACE_PARTIAL) {
             if constexpr (NP <= 8) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_PARTIAL, UNIT, false>), grid, dim3(WG), 0, s, r);
         }
+        else if (face == FACE_SOLID) {
+            if constexpr (NP <= 8) hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_SOLID, UNIT, false>), grid, dim3(WG), 0, s, r);
+        }
         else hipLaunchKernelGGL((k_step_pipe<SP, NP, FACE_NONE, UNIT, false>), grid, dim3(WG), 0, s, r);
     }
 }
@@ -1183,9 +1224,27 @@ static hipError_t launch_pat(const StepArgs& a, hipStream_t stream, hipStream_t 
     hipStream_t bs = side ? side : stream;
     // (a q-map exists only in the AB pattern)
     const bool bfl = PAT == PAT_AB && a.qmap != nullptr;
-    if (a.n_boundary_regular) {
-        r.blist = a.blist_boundary_regular;
-        const dim3 g(a.n_boundary_regular * PLANES);
+    // regular-grid blocks whose only flag is a solid cell near (at the head of the list): the pipelined
+    // kernel's solid-link form where it applies (AB pattern, FP32 arithmetic, halfway bounce-back, no
+    // q-map, WALE or force), the rest on the one-plane kernel
+    uint32_t br_first = 0;
+    if constexpr (PAT == PAT_AB && !WALE && sizeof(typename SP::T) == 4) {
+        if (a.n_boundary_regular_solid && !bfl && !a.fpart && a.cp.bb_mode == 1 && a.pipe_planes > 1) {
+            StepArgs rs = r;
+            rs.blist = a.blist_boundary_regular;
+            rs.pipe_planes = a.pipe_planes > 8 ? 8 : a.pipe_planes;
+            const dim3 g(a.n_boundary_regular_solid * (PLANES / rs.pipe_planes));
+            switch (rs.pipe_planes) {
+                case 2: launch_pipe<SP, 2>(FACE_SOLID, g, bs, rs); break;
+                case 4: launch_pipe<SP, 4>(FACE_SOLID, g, bs, rs); break;
+                default: launch_pipe<SP, 8>(FACE_SOLID, g, bs, rs); break;
+            }
+            br_first = a.n_boundary_regular_solid;
+        }
+    }
+    if (a.n_boundary_regular > br_first) {
+        r.blist = a.blist_boundary_regular + br_first;
+        const dim3 g((a.n_boundary_regular - br_first) * PLANES);
         if constexpr (PAT == PAT_AB) {
             if (bfl && a.face_x) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, true>), g, dim3(WG), 0, bs, r);
             else if (bfl) hipLaunchKernelGGL((k_step<SP, PAT, WALE, SEL_BOUNDARY_REGULAR, true, false, false>), g, dim3(WG), 0, bs, r);

@@ -199,6 +199,8 @@ def test_interior_blocks_rates_and_forcing(oracle, variant, storage, pair):
     f_or, _, _ = oracle_run(oracle, grid, storage, steps, okw, rho, u)
     plan = gpu_plan(grid, storage, okw)
     assert plan.level_info()["n_regular"] >= 40
+    # the blocks around the sphere: the pipelined kernel's solid-link form (steps without the force)
+    assert plan.level_info()["n_solid_regular"] > 0
     plan.set_tuning(pipe_planes=4, overlap=1, face_kernel=0, graph=0, pair_kernel=pair)
     plan.init_equilibrium(rho, u)
     for t in range(steps):
