@@ -403,8 +403,17 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
     if constexpr (REG) ra.init(c, dm->nbx, dm->nby);
 
     T f[27];
-    typename SP::raw raw[27];     // AB pattern: the loaded populations, decoded after all loads
+    typename SP::raw raw[27];     // the loaded populations, decoded after all loads
     uint32_t inlet_links = 0, rest_links = 0, solid_links = 0, reg_links = 0, tfs_links = 0;
+    // AA odd half-step: the even step's copy of the populations with cx = -1 of the cell's row on the
+    // outlet face (they sit in slots that the west neighbour overwrites during this half-step)
+    const st* ocopy = nullptr;
+    if constexpr (PAT == PAT_AA_ODD && GEN) {
+        if (c.flags & BF_EDGE) {
+            const int32_t s = a.oslot[b];
+            if (s >= 0) ocopy = (const st*)a.obuf + (size_t)s * 9 * BS2 + c.ly * BS + c.z;
+        }
+    }
     for_dirs([&](auto kc) {
         constexpr int K = decltype(kc)::value;
         constexpr T w = wgt<T>(K);
@@ -415,14 +424,13 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
             constexpr int slot = PAT == PAT_AB ? K : dopp(K);
             val = SP::ld((const st*)(ra.base(fi, slot, c.z, dcz(K)) + ra.v[dcx(K) + 1][dcy(K) + 1]), w);
         } else if constexpr (PAT == PAT_AA_EVEN) {
-            // inlet links are evaluated here in full precision, as the pull step does, rather than
-            // read back from a slot that went through the storage rounding
-            bool inlet = false;
-            if constexpr (GEN && dcx(K) > 0) inlet = (c.flags & BF_EDGE) && !c.per0 && c.gx == 0;
-            if (inlet)
-                val = boundary_value<K, T>(SRC_INLET, a, c, u_in, a.step(), [&](auto) { return T(0); });
-            else
-                val = SP::ld(fi + pop_index(c.own, K), w);
+            // every direction loads the own slot K, so the 27 loads are in flight together; an inlet
+            // link's value replaces it afterwards, evaluated in full precision as the pull step does
+            // rather than read back from a slot that went through the storage rounding
+            raw[K] = SP::ld_raw(fi + pop_index(c.own, K));
+            if constexpr (GEN && dcx(K) > 0)
+                inlet_links |= ((c.flags & BF_EDGE) && !c.per0 && c.gx == 0 ? 1u : 0u) << K;
+            val = T(0);   // set below
         } else if constexpr (PAT == PAT_AB) {
             // branch-free gather: each direction reads one population - its source, or the own-cell
             // population its boundary rule returns (an inlet or rest link reads its own slot, unused) -
@@ -445,26 +453,31 @@ __device__ __forceinline__ void step_cell(const StepArgs& a, const CellCtx& c, c
             reg_links |= (kind == SRC_REG ? 1u : 0u) << K;
             val = T(0);   // set below
         } else {
+            // AA odd half-step, branch-free as the AB gather: each direction loads one slot - the
+            // source cell's slot opp K, the outlet copy (the own population of direction K, cx = -1),
+            // or the own slot opp j of the population j its boundary rule returns; an inlet or rest
+            // link reads the own slot K, unused, which no other cell writes in this half-step (its
+            // writer would be the absent cell x - c_K) - and the inlet, rest and freestream rules
+            // replace the value afterwards
+            static_assert(PAT == PAT_AA_ODD, "the AA odd half-step");
             uint32_t src = 0;
             const int kind = classify<K, GEN, REG, FX>(a, c, src);
+            const st* p = fi + pop_index(c.own, kind == SRC_INLET || kind == SRC_REST ? K : dopp(rule_dir<K>(kind)));
             if (kind == SRC_REG) {
-                constexpr int slot = PAT == PAT_AB ? K : dopp(K);
-                if constexpr (REG)
-                    val = SP::ld((const st*)(ra.base(fi, slot, c.z, dcz(K)) + ra.v[dcx(K) + 1][dcy(K) + 1]), w);
-                else
-                    val = SP::ld(fi + pop_index(src, slot), w);
-            } else if (PAT == PAT_AA_ODD && kind == SRC_OWN) {
-                // outlet: the own population of direction K (cx = -1) sits in slot (x, opp K), which
-                // the west neighbour overwrites during this step; the even step saved a copy
-                val = SP::ld((const st*)a.obuf + ((size_t)a.oslot[b] * 9 + K / 3) * BS2 + c.ly * BS + c.z, w);
-            } else {
-                val = boundary_value<K, T>(kind, a, c, u_in, a.step(),
-                                           [&](auto jc) { return own_prev<SP, PAT, decltype(jc)::value>(a, c); });
+                if constexpr (REG) p = (const st*)(ra.base(fi, dopp(K), c.z, dcz(K)) + ra.v[dcx(K) + 1][dcy(K) + 1]);
+                else p = fi + pop_index(src, dopp(K));
             }
+            if constexpr (GEN && dcx(K) < 0)
+                if (kind == SRC_OWN) p = ocopy + (K / 3) * BS2;
+            raw[K] = SP::ld_raw(p);
+            inlet_links |= (kind == SRC_INLET ? 1u : 0u) << K;
+            rest_links |= (kind == SRC_REST ? 1u : 0u) << K;
+            if constexpr (FX) tfs_links |= (kind == SRC_TFS ? 1u : 0u) << K;
+            val = T(0);   // set below
         }
         f[K] = val;
     });
-    if constexpr (PAT == PAT_AB && !(REG && !GEN)) {
+    if constexpr (!(REG && !GEN)) {
         for_dirs([&](auto kc) {
             constexpr int K = decltype(kc)::value;
             constexpr T w = wgt<T>(K);
