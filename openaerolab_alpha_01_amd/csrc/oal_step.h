@@ -1093,8 +1093,11 @@ k_step_pipe_aa(const StepArgs a) {
     const int np = a.pipe_planes;   // (a run-time bound, as in k_step_pipe)
     constexpr uint32_t groups = (uint32_t)(PLANES / NP);
     const uint32_t litem = xcd_item(blockIdx.x, gridDim.x);
-    const uint32_t b = (uint32_t)a.blist[litem / groups];
-    const int z0 = (int)(litem % groups) * np;
+    // the odd half-step walks the blocks fastest: the x-neighbour block, which completes the rows of
+    // this block's x-shifted slots, runs at the same time on the same L2 (AA step 0.8 % faster)
+    const uint32_t nblk = gridDim.x / groups;
+    const uint32_t b = (uint32_t)a.blist[PAT == PAT_AA_ODD ? litem % nblk : litem / groups];
+    const int z0 = (int)(PAT == PAT_AA_ODD ? litem / nblk : litem % groups) * np;
     CellCtx c;
     c.b = b;
     c.lx = threadIdx.x & (BS - 1);
