@@ -187,9 +187,9 @@ __attribute__((amdgpu_waves_per_eu(pair32_waves<UNIT, FACE>(), pair32_waves<UNIT
 k_step_pair32(const StepArgs a) {
     const int np = a.pipe_planes;   // run-time bound equal to NP (see k_step_pipe)
     constexpr uint32_t groups = (uint32_t)(PLANES / NP);
-    const uint32_t litem = xcd_item(blockIdx.x, gridDim.x);
-    const uint32_t b = (uint32_t)a.blist[litem / groups];
-    const int z0 = (int)(litem % groups) * np;
+    const PipeItem it = pipe_item<true>(groups, np);
+    const uint32_t b = (uint32_t)a.blist[it.pos];
+    const int z0 = it.z0;
     const int jx = (int)(threadIdx.x & 7u), ly = (int)(threadIdx.x >> 3);
     const DevDomain* dm = a.doms + a.bdom[b];
     Pair32Addr pa, po;   // the populations read and those written
